@@ -1,0 +1,17 @@
+"""charmlu_amd: MI355X-native dense LU with partial pivoting.
+
+A device-resident re-design of UIUC-PPL/charmlu's block LU: hand-written
+gfx950 HIP kernels behind the C ABI of ``include/charmlu_amd.h``
+(``libcharmlu_amd.so``), driven from Python through ctypes.  Importing the
+package loads the library and fails loudly if it has not been built.
+"""
+from . import _lib
+from ._lib import CharmluError, load
+
+load()
+
+from .lu import (LUConfig, gen_matrix, gen_rhs, getrf, getrs, hpl_gflops,  # noqa: E402
+                 residual, run)
+
+__all__ = ["CharmluError", "LUConfig", "gen_matrix", "gen_rhs", "getrf", "getrs",
+           "hpl_gflops", "residual", "run", "load"]

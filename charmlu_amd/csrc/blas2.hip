@@ -1,0 +1,93 @@
+// blas2.hip -- the rank-1 update and matrix-vector product of the
+// platformBlas.h surface on device pointers.
+//
+// dger replaces cblas_dger(RowMajor, M, N, alpha, x, incx, y, incy, A, lda)
+// (updateLsubBlock, lu.C:247-252): one thread per element of A, rows
+// contiguous across lanes.  dgemv replaces cblas_dgemv(RowMajor, NoTrans /
+// Trans, ...) (recvXvec, benchmark.C:494-496): 16 lanes per row for the
+// row-major no-transpose product, one lane per column for the transpose.
+// Negative increments follow the CBLAS convention (the vector is walked from
+// its last stored element).
+#include "common.h"
+#include "kernels.h"
+
+namespace clu {
+
+CLU_DEV long vidx(long i, long n, long inc) { return inc > 0 ? i * inc : (n - 1 - i) * (-inc); }
+
+__global__ void dger_kernel(int M, int N, double alpha, const double *__restrict__ x, long incx,
+                            const double *__restrict__ y, long incy, double *__restrict__ A,
+                            long lda) {
+  const long total = (long)M * N;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const long i = e / N, j = e % N;
+    A[i * lda + j] += (alpha * x[vidx(i, M, incx)]) * y[vidx(j, N, incy)];
+  }
+}
+
+int dger(int M, int N, double alpha, const double *x, long incx, const double *y, long incy,
+         double *A, long lda, hipStream_t s) {
+  if (M <= 0 || N <= 0 || alpha == 0.0) return CLU_OK;
+  long blocks = ((long)M * N + 255) / 256;
+  if (blocks > 65536L * 16) blocks = 65536L * 16;
+  hipLaunchKernelGGL(dger_kernel, dim3((unsigned)blocks), dim3(256), 0, s, M, N, alpha, x, incx,
+                     y, incy, A, lda);
+  CLU_LAUNCHED("dger_kernel");
+  return CLU_OK;
+}
+
+// y = alpha * A x + beta * y, A is M x N row-major
+__global__ __launch_bounds__(256) void dgemv_n_kernel(int M, int N, double alpha,
+                                                      const double *__restrict__ A, long lda,
+                                                      const double *__restrict__ x, long incx,
+                                                      double beta, double *__restrict__ y,
+                                                      long incy) {
+  const int row = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int sub = threadIdx.x & 15;
+  double s = 0.0;
+  if (row < M)
+    for (int k = sub; k < N; k += 16) s += A[(long)row * lda + k] * x[vidx(k, N, incx)];
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
+  if (row < M && sub == 0) {
+    double *yp = y + vidx(row, M, incy);
+    *yp = beta == 0.0 ? alpha * s : alpha * s + beta * *yp;
+  }
+}
+
+// y = alpha * A^T x + beta * y, A is M x N row-major, y has N entries
+__global__ __launch_bounds__(256) void dgemv_t_kernel(int M, int N, double alpha,
+                                                      const double *__restrict__ A, long lda,
+                                                      const double *__restrict__ x, long incx,
+                                                      double beta, double *__restrict__ y,
+                                                      long incy) {
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (j < N)
+    for (int k = wave; k < M; k += 4) s += A[(long)k * lda + j] * x[vidx(k, M, incx)];
+  part[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && j < N) {
+    const double t = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    double *yp = y + vidx(j, N, incy);
+    *yp = beta == 0.0 ? alpha * t : alpha * t + beta * *yp;
+  }
+}
+
+int dgemv(bool trans, int M, int N, double alpha, const double *A, long lda, const double *x,
+          long incx, double beta, double *y, long incy, hipStream_t s) {
+  if (M <= 0 || N <= 0) return CLU_OK;
+  if (!trans)
+    hipLaunchKernelGGL(dgemv_n_kernel, dim3((M + 15) / 16), dim3(256), 0, s, M, N, alpha, A, lda,
+                       x, incx, beta, y, incy);
+  else
+    hipLaunchKernelGGL(dgemv_t_kernel, dim3((N + 63) / 64), dim3(256), 0, s, M, N, alpha, A, lda,
+                       x, incx, beta, y, incy);
+  CLU_LAUNCHED("dgemv_kernel");
+  return CLU_OK;
+}
+
+}  // namespace clu

@@ -1,0 +1,95 @@
+// kernels.h -- host-side launchers of the charmlu_amd device kernels.
+// All matrices are row-major with a leading dimension in elements, as the
+// reference's blocks are (lu.h:151-153).  Every launcher is asynchronous on
+// the given stream and returns CLU_OK or a negative status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clu {
+
+// ---- GEMM: C = alpha * op(A) * op(B) + beta * C (FP64 MFMA) -------------
+// ta: A is stored K x M (op(A) = A^T); tb: B is stored N x K.
+int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A,
+          long lda, const double *B, long ldb, double beta, double *C, long ldc,
+          hipStream_t s);
+
+// Tile configuration dgemm() picks for an M x N product: 0 = 128x128 (the
+// trailing-update kernel), 1 = 256x32, 2 = 256x64, 3 = 32x256, 4 = 64x64.
+int dgemm_variant(int M, int N);
+
+// ---- pivoting -------------------------------------------------------------
+// Row move list of a run of sequential swaps ipiv[k1..k2) (global rows):
+// moves[i] = {dst, src} means "row dst receives old row src"; count[0] is
+// the number of moves.  Up to 2*(k2-k1) moves.
+int pivots_to_moves(const int *ipiv, int k1, int k2, int2 *moves, int *count,
+                    hipStream_t s);
+// Inverse direction: row src receives old row dst (undoes the permutation).
+// Apply a move list to columns [c0, c1) of a row-major matrix, skipping the
+// columns [skip0, skip1).
+template <typename T>
+int apply_moves(const int2 *moves, const int *count, int max_moves, T *A,
+                long lda, int c0, int c1, int skip0, int skip1, bool inverse,
+                hipStream_t s);
+
+// ---- panel factorization (FP64 / FP32) ------------------------------------
+struct PanelWork {
+  void *slots;        // candidate rows, 2 parities
+  void *top;          // the current top row, 2 parities
+  unsigned *counter;  // grid barrier counter
+  unsigned epoch;     // host-side running count of barrier arrivals
+  int max_groups;
+  int *info;          // device int, first near-singular column + 1
+};
+// Factor rows [j0, m_end) x columns [j0, j0+w) (w <= 16) of A in place with
+// partial pivoting, one column at a time; ipiv[j0..j0+w) get global rows.
+template <typename T>
+int panel_base(T *A, long lda, int j0, int m_end, int w, int *ipiv,
+               PanelWork &pw, hipStream_t s);
+
+// ---- triangular solves ----------------------------------------------------
+// B = L^{-1} B with L unit lower (M x M), B is M x N (M <= 256).
+template <typename T>
+int trsm_llnu(int M, int N, const T *L, long ldl, T *B, long ldb, hipStream_t s);
+// B = U^{-1} B with U non-unit upper (M x M), B is M x N (M <= 256).
+template <typename T>
+int trsm_lunn(int M, int N, const T *U, long ldu, T *B, long ldb, hipStream_t s);
+// Transposed solves: B = L^{-T} B (L unit lower) and B = U^{-T} B.
+template <typename T>
+int trsm_lltu(int M, int N, const T *L, long ldl, T *B, long ldb, hipStream_t s);
+template <typename T>
+int trsm_lutn(int M, int N, const T *U, long ldu, T *B, long ldb, hipStream_t s);
+
+// General small triangular solve, left side: B = alpha * op(F)^{-1} B
+// (M <= 256); upper/trans/unit select the triangle, op and diagonal.
+template <typename T>
+int trsm_small(bool upper, bool trans, bool unit, int M, int N, T alpha, const T *F, long ldf,
+               T *B, long ldb, hipStream_t s);
+
+// A += alpha * x * y^T; y = alpha * op(A) * x + beta * y (CBLAS increments)
+int dger(int M, int N, double alpha, const double *x, long incx, const double *y, long incy,
+         double *A, long lda, hipStream_t s);
+int dgemv(bool trans, int M, int N, double alpha, const double *A, long lda, const double *x,
+          long incx, double beta, double *y, long incy, hipStream_t s);
+
+// y[0..M) -= A[0..M, 0..K) * x  (trans: y[0..M) -= A[0..K, 0..M)^T * x)
+template <typename T>
+int gemv_sub(bool trans, int M, int K, const T *A, long lda, const T *x, T *y,
+             hipStream_t s);
+
+// ---- generation, residual, conversion -------------------------------------
+int gen_uniform(uint64_t seed, long row0, long col0, long m, long n, double *A,
+                long lda, hipStream_t s);
+int gen_diagdom(uint64_t seed, long n, double *A, long lda, hipStream_t s);
+int gen_rhs(int seed, long skip, long n, double *b, hipStream_t s);
+// out[0..5] = {max|a_ij|, max|b|, max|x|, max|b-Ax|, max_i sum_j |a_ij|}
+// r (optional) = b - A x.
+int residual(int n, const double *A, long lda, const double *x, const double *b,
+             double *r, double *out, hipStream_t s);
+template <typename TD, typename TS>
+int convert(long m, long n, const TS *src, long lds, TD *dst, long ldd, hipStream_t s);
+int fill_int(int *p, int v, long n, hipStream_t s);
+int fill_u32(unsigned *p, unsigned v, long n, hipStream_t s);
+
+}  // namespace clu

@@ -1,0 +1,709 @@
+// lu_driver.hip -- host side of libcharmlu_amd.so: the C ABI of
+// include/charmlu_amd.h, per-device workspaces, and the step loop of the
+// blocked LU with a one-step look-ahead on two HIP streams.
+//
+// The reference runs one SDAG control script per b x b block (lu.ci:31-370)
+// and a per-PE BlockScheduler that decides when ready trailing updates run
+// (scheduler.C:235-484).  On one GPU the whole matrix is one device-resident
+// row-major array and a step is a handful of launches:
+//
+//   panel stream:  panel(t+1) .................. [waits for the look-ahead
+//                                                  block column of step t]
+//   main stream:   swaps(t) -> U row(t) -> update(block col t+1) -> update(rest)
+//
+// so the latency-bound panel of step t+1 (the critical path, prio_notes:1-27
+// in the reference) overlaps the bulk of step t's trailing update, which is
+// what the reference's priorities (manager.h:54-59) aim at.  The reference's
+// write-after-read hazard -- an L block may not be row-permuted by later
+// pivots before every trailing update of its step has read it
+// (lu.ci:240-242) -- is kept by stream order: the left-side interchanges of
+// step t+1 run on the main stream after step t's update.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/charmlu_amd.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace clu {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  return CLU_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// per-device workspace
+// ---------------------------------------------------------------------------
+constexpr int MAX_GROUPS = 256;  // panel workgroups (rows / 2048)
+constexpr int RUN = 256;         // pivots per move list
+
+struct Workspace {
+  int device = -1;
+  hipStream_t s_panel = nullptr;
+  hipEvent_t ev_panel = nullptr, ev_la = nullptr, ev_start = nullptr;
+  // panel exchange slots (sized for doubles, reused for floats)
+  void *slots = nullptr;
+  void *top = nullptr;
+  unsigned *counter = nullptr;
+  int *info_scratch = nullptr;
+  // move lists: in-panel (one run) and per-step (nruns runs, two parities)
+  int2 *pmoves = nullptr;
+  int *pcount = nullptr;
+  int2 *smoves = nullptr;
+  int *scount = nullptr;
+  int srun_cap = 0;  // runs per parity
+  // host-pointer shim staging
+  void *stage = nullptr;
+  size_t stage_bytes = 0;
+  PanelWork pw{};
+};
+
+static std::mutex g_mu;
+static Workspace g_ws[64];
+
+// ---------------------------------------------------------------------------
+// optional per-launch timing of the factorization's kernel classes
+// ---------------------------------------------------------------------------
+enum ProfKind { PK_GEMM_BIG = 0, PK_GEMM_OTHER = 1, PK_PANEL = 2, PK_SWAP = 3, PK_TRSM = 4,
+                PK_NKINDS = 5 };
+struct Prof {
+  bool on = false;
+  struct Rec { hipEvent_t a, b; int kind; double work; };
+  std::vector<Rec> recs;
+  size_t used = 0;
+};
+static Prof g_prof;
+
+static int prof_begin(int kind, double work, hipStream_t s) {
+  if (!g_prof.on) return -1;
+  if (g_prof.used == g_prof.recs.size()) {
+    Prof::Rec r{};
+    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+    g_prof.recs.push_back(r);
+  }
+  Prof::Rec &r = g_prof.recs[g_prof.used];
+  r.kind = kind;
+  r.work = work;
+  if (hipEventRecord(r.a, s) != hipSuccess) return -1;
+  return (int)g_prof.used++;
+}
+static void prof_end(int id, hipStream_t s) {
+  if (id >= 0) (void)hipEventRecord(g_prof.recs[id].b, s);
+}
+
+static int ensure_runs(Workspace &w, int runs) {
+  if (runs <= w.srun_cap) return CLU_OK;
+  if (w.smoves) { CLU_HIP(hipFree(w.smoves)); w.smoves = nullptr; }
+  if (w.scount) { CLU_HIP(hipFree(w.scount)); w.scount = nullptr; }
+  CLU_HIP(hipMalloc(&w.smoves, sizeof(int2) * 2 * RUN * 2 * (size_t)runs));
+  CLU_HIP(hipMalloc(&w.scount, sizeof(int) * 2 * (size_t)runs));
+  w.srun_cap = runs;
+  return CLU_OK;
+}
+
+static int get_ws(Workspace **out) {
+  int dev = 0;
+  CLU_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) { set_error("device %d out of range", dev); return CLU_ERR_HIP; }
+  std::lock_guard<std::mutex> lk(g_mu);
+  Workspace &w = g_ws[dev];
+  if (w.device < 0) {
+    CLU_HIP(hipStreamCreateWithFlags(&w.s_panel, hipStreamNonBlocking));
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_panel, hipEventDisableTiming));
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_la, hipEventDisableTiming));
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming));
+    const size_t slot_bytes = (size_t)2 * MAX_GROUPS * sizeof(double) +
+                              (size_t)2 * MAX_GROUPS * sizeof(int) +
+                              (size_t)2 * MAX_GROUPS * 16 * sizeof(double) + 256;
+    CLU_HIP(hipMalloc(&w.slots, slot_bytes));
+    CLU_HIP(hipMalloc(&w.top, 2 * 16 * sizeof(double)));
+    CLU_HIP(hipMalloc(&w.counter, 64));
+    CLU_HIP(hipMalloc(&w.info_scratch, 64));
+    CLU_HIP(hipMalloc(&w.pmoves, sizeof(int2) * 2 * RUN));
+    CLU_HIP(hipMalloc(&w.pcount, 64));
+    int rc = ensure_runs(w, 1);
+    if (rc) return rc;
+    w.pw.slots = w.slots;
+    w.pw.top = w.top;
+    w.pw.counter = w.counter;
+    w.pw.epoch = 0;
+    w.pw.max_groups = MAX_GROUPS;
+    w.device = dev;
+  }
+  *out = &w;
+  return CLU_OK;
+}
+
+static int ensure_stage(Workspace &w, size_t bytes) {
+  if (bytes <= w.stage_bytes) return CLU_OK;
+  if (w.stage) CLU_HIP(hipFree(w.stage));
+  w.stage = nullptr;
+  w.stage_bytes = 0;
+  size_t b = bytes < (1u << 20) ? (1u << 20) : bytes;
+  if (hipMalloc(&w.stage, b) != hipSuccess) {
+    set_error("cannot allocate %zu bytes of staging memory", b);
+    return CLU_ERR_NOMEM;
+  }
+  w.stage_bytes = b;
+  return CLU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// building blocks shared by the factorizations and solves
+// ---------------------------------------------------------------------------
+
+// B = L^{-1} B, L unit lower M x M (any M): 256-row diagonal solves plus
+// GEMM updates of the rows below.
+template <typename T>
+static int trsm_llnu_big(int M, int N, const T *L, long ldl, T *B, long ldb, hipStream_t s);
+
+template <>
+int trsm_llnu_big<double>(int M, int N, const double *L, long ldl, double *B, long ldb,
+                          hipStream_t s) {
+  for (int i0 = 0; i0 < M; i0 += 256) {
+    const int mi = M - i0 < 256 ? M - i0 : 256;
+    int rc = trsm_llnu<double>(mi, N, L + (long)i0 * ldl + i0, ldl, B + (long)i0 * ldb, ldb, s);
+    if (rc) return rc;
+    if (i0 + mi < M) {
+      rc = dgemm(false, false, M - i0 - mi, N, mi, -1.0, L + (long)(i0 + mi) * ldl + i0, ldl,
+                 B + (long)i0 * ldb, ldb, 1.0, B + (long)(i0 + mi) * ldb, ldb, s);
+      if (rc) return rc;
+    }
+  }
+  return CLU_OK;
+}
+
+// Factor the strip rows [c0, n) x columns [c0, c0+w) (recursive halving down
+// to 16-column strips factored by panel_base).  The pivots of the left half
+// are applied to the right half before its update, the pivots of the right
+// half to the left half afterwards (rows >= their own diagonal only).
+static int factor_panel_d(double *A, long lda, int n, int c0, int w, int *ipiv, Workspace &ws,
+                          hipStream_t s) {
+  if (w <= 16) return panel_base<double>(A, lda, c0, n, w, ipiv, ws.pw, s);
+  int w1 = (w / 2 + 15) / 16 * 16;
+  if (w1 >= w) w1 = w / 2;
+  const int w2 = w - w1;
+  int rc = factor_panel_d(A, lda, n, c0, w1, ipiv, ws, s);
+  if (rc) return rc;
+  if ((rc = pivots_to_moves(ipiv, c0, c0 + w1, ws.pmoves, ws.pcount, s))) return rc;
+  if ((rc = apply_moves<double>(ws.pmoves, ws.pcount, 2 * w1, A, lda, c0 + w1, c0 + w, c0 + w,
+                                c0 + w, false, s)))
+    return rc;
+  double *A11 = A + (long)c0 * lda + c0;
+  if ((rc = trsm_llnu<double>(w1, w2, A11, lda, A11 + w1, lda, s))) return rc;
+  if (n - c0 - w1 > 0)
+    if ((rc = dgemm(false, false, n - c0 - w1, w2, w1, -1.0, A11 + (long)w1 * lda, lda, A11 + w1,
+                    lda, 1.0, A11 + (long)w1 * lda + w1, lda, s)))
+      return rc;
+  if ((rc = factor_panel_d(A, lda, n, c0 + w1, w2, ipiv, ws, s))) return rc;
+  if ((rc = pivots_to_moves(ipiv, c0 + w1, c0 + w, ws.pmoves, ws.pcount, s))) return rc;
+  return apply_moves<double>(ws.pmoves, ws.pcount, 2 * w2, A, lda, c0, c0 + w1, c0 + w1,
+                             c0 + w1, false, s);
+}
+
+// move lists of ipiv[k1, k2) in runs of RUN pivots
+static int build_moves(const int *ipiv, int k1, int k2, int2 *moves, int *counts, hipStream_t s) {
+  for (int r = 0, k = k1; k < k2; ++r, k += RUN) {
+    const int ke = k2 - k < RUN ? k2 : k + RUN;
+    int rc = pivots_to_moves(ipiv, k, ke, moves + (size_t)r * 2 * RUN, counts + r, s);
+    if (rc) return rc;
+  }
+  return CLU_OK;
+}
+
+template <typename T>
+static int apply_runs(int nruns, const int2 *moves, const int *counts, T *A, long lda, int c0,
+                      int c1, int skip0, int skip1, bool inverse, hipStream_t s) {
+  for (int i = 0; i < nruns; ++i) {
+    const int r = inverse ? nruns - 1 - i : i;
+    int rc = apply_moves<T>(moves + (size_t)r * 2 * RUN, counts + r, 2 * RUN, A, lda, c0, c1,
+                            skip0, skip1, inverse, s);
+    if (rc) return rc;
+  }
+  return CLU_OK;
+}
+
+}  // namespace clu
+
+using namespace clu;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int charmlu_version(void) { return 10000; }
+
+int charmlu_profile(int on) {
+  g_prof.on = on != 0;
+  g_prof.used = 0;
+  return CLU_OK;
+}
+
+int charmlu_profile_read(double *out, int nout) {
+  if (!out || nout < 3 * PK_NKINDS) return -1;
+  for (int i = 0; i < 3 * PK_NKINDS; ++i) out[i] = 0.0;
+  for (size_t i = 0; i < g_prof.used; ++i) {
+    Prof::Rec &r = g_prof.recs[i];
+    CLU_HIP(hipEventSynchronize(r.b));
+    float ms = 0.f;
+    CLU_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    out[3 * r.kind + 0] += 1.0;
+    out[3 * r.kind + 1] += r.work;
+    out[3 * r.kind + 2] += ms;
+  }
+  g_prof.used = 0;
+  return CLU_OK;
+}
+
+const char *charmlu_last_error(void) { return g_err; }
+
+int charmlu_init(int device) {
+  CLU_HIP(hipSetDevice(device));
+  Workspace *ws = nullptr;
+  return get_ws(&ws);
+}
+
+// ---- device-pointer block ops ----------------------------------------------
+int charmlu_dgemm_dev(int transa, int transb, int M, int N, int K, double alpha, const double *dA,
+                      long lda, const double *dB, long ldb, double beta, double *dC, long ldc,
+                      void *stream) {
+  if (transa != CHARMLU_NO_TRANS && transa != CHARMLU_TRANS) return -1;
+  if (transb != CHARMLU_NO_TRANS && transb != CHARMLU_TRANS) return -2;
+  if (M < 0) return -3;
+  if (N < 0) return -4;
+  if (K < 0) return -5;
+  const bool ta = transa == CHARMLU_TRANS, tb = transb == CHARMLU_TRANS;
+  if (lda < ((ta ? M : K) > 1 ? (ta ? M : K) : 1)) return -8;
+  if (ldb < ((tb ? K : N) > 1 ? (tb ? K : N) : 1)) return -10;
+  if (ldc < (N > 1 ? N : 1)) return -13;
+  return dgemm(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, (hipStream_t)stream);
+}
+
+// op(T) is M x M; for M > 256 the solve proceeds in 256-row blocks with
+// GEMM updates between them.
+static int dtrsm_dev_any(bool upper, bool trans, bool unit, int M, int N, double alpha,
+                         const double *T, long ldt, double *B, long ldb, hipStream_t s) {
+  if (M <= 256) return trsm_small<double>(upper, trans, unit, M, N, alpha, T, ldt, B, ldb, s);
+  const bool oplow = (!upper) != trans;
+  const int nblk = (M + 255) / 256;
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int blk = oplow ? bi : nblk - 1 - bi;
+    const int i0 = blk * 256, mi = M - i0 < 256 ? M - i0 : 256;
+    // alpha scales the first block here and every other row through the
+    // beta of the first update, so later blocks solve with alpha = 1
+    const double a = bi == 0 ? alpha : 1.0;
+    int rc = trsm_small<double>(upper, trans, unit, mi, N, a, T + (long)i0 * ldt + i0, ldt,
+                                B + (long)i0 * ldb, ldb, s);
+    if (rc) return rc;
+    // update the rows still to be solved: B_rest -= op(T)[rest, blk] * X_blk
+    if (oplow && i0 + mi < M) {
+      const int r0 = i0 + mi, mr = M - r0;
+      // op(T)[r0.., i0..i0+mi): trans ? T[i0.., r0..]^T : T[r0.., i0..]
+      rc = trans ? dgemm(true, false, mr, N, mi, -1.0, T + (long)i0 * ldt + r0, ldt,
+                         B + (long)i0 * ldb, ldb, alpha, B + (long)r0 * ldb, ldb, s)
+                 : dgemm(false, false, mr, N, mi, -1.0, T + (long)r0 * ldt + i0, ldt,
+                         B + (long)i0 * ldb, ldb, alpha, B + (long)r0 * ldb, ldb, s);
+      if (rc) return rc;
+    } else if (!oplow && i0 > 0) {
+      rc = trans ? dgemm(true, false, i0, N, mi, -1.0, T + (long)i0 * ldt, ldt,
+                         B + (long)i0 * ldb, ldb, alpha, B, ldb, s)
+                 : dgemm(false, false, i0, N, mi, -1.0, T + i0, ldt, B + (long)i0 * ldb, ldb,
+                         alpha, B, ldb, s);
+      if (rc) return rc;
+    }
+    alpha = 1.0;  // the remaining rows were scaled by the update above
+  }
+  return CLU_OK;
+}
+
+int charmlu_dtrsm_dev(int uplo, int transa, int diag, int M, int N, const double *dT, long ldt,
+                      double *dB, long ldb, void *stream) {
+  if (uplo != CHARMLU_UPPER && uplo != CHARMLU_LOWER) return -1;
+  if (transa != CHARMLU_NO_TRANS && transa != CHARMLU_TRANS) return -2;
+  if (diag != CHARMLU_UNIT && diag != CHARMLU_NON_UNIT) return -3;
+  if (M < 0) return -4;
+  if (N < 0) return -5;
+  if (ldt < (M > 1 ? M : 1)) return -7;
+  if (ldb < (N > 1 ? N : 1)) return -9;
+  return dtrsm_dev_any(uplo == CHARMLU_UPPER, transa == CHARMLU_TRANS, diag == CHARMLU_UNIT, M,
+                       N, 1.0, dT, ldt, dB, ldb, (hipStream_t)stream);
+}
+
+int charmlu_dger_dev(int M, int N, double alpha, const double *dx, long incx, const double *dy,
+                     long incy, double *dA, long lda, void *stream) {
+  if (M < 0) return -1;
+  if (N < 0) return -2;
+  if (incx == 0) return -5;
+  if (incy == 0) return -7;
+  if (lda < (N > 1 ? N : 1)) return -9;
+  return dger(M, N, alpha, dx, incx, dy, incy, dA, lda, (hipStream_t)stream);
+}
+
+// ---- host-pointer CBLAS shims -----------------------------------------------
+// Stage operands through device memory, run the device op on the null stream,
+// copy the result back; synchronous like the CBLAS calls they replace.
+static int stage_in(double *dst, const double *src, long rows, long cols, long ld) {
+  if (rows <= 0 || cols <= 0) return CLU_OK;
+  CLU_HIP(hipMemcpy2D(dst, cols * sizeof(double), src, ld * sizeof(double), cols * sizeof(double),
+                      rows, hipMemcpyHostToDevice));
+  return CLU_OK;
+}
+static int stage_out(double *dst, long ld, const double *src, long rows, long cols) {
+  if (rows <= 0 || cols <= 0) return CLU_OK;
+  CLU_HIP(hipMemcpy2D(dst, ld * sizeof(double), src, cols * sizeof(double), cols * sizeof(double),
+                      rows, hipMemcpyDeviceToHost));
+  return CLU_OK;
+}
+
+int charmlu_cblas_dgemm(int order, int transa, int transb, int M, int N, int K, double alpha,
+                        const double *A, int lda, const double *B, int ldb, double beta,
+                        double *C, int ldc) {
+  if (order != CHARMLU_ROW_MAJOR && order != CHARMLU_COL_MAJOR) return -1;
+  if (transa != CHARMLU_NO_TRANS && transa != CHARMLU_TRANS) return -2;
+  if (transb != CHARMLU_NO_TRANS && transb != CHARMLU_TRANS) return -3;
+  if (M < 0) return -4;
+  if (N < 0) return -5;
+  if (K < 0) return -6;
+  if (order == CHARMLU_COL_MAJOR) {
+    // column-major C = op(A) op(B)  <=>  row-major C^T = op(B)^T op(A)^T
+    std::swap(transa, transb);
+    std::swap(M, N);
+    std::swap(A, B);
+    std::swap(lda, ldb);
+  }
+  const bool ta = transa == CHARMLU_TRANS, tb = transb == CHARMLU_TRANS;
+  const long ar = ta ? K : M, ac = ta ? M : K, br = tb ? N : K, bc = tb ? K : N;
+  if (lda < (ac > 1 ? ac : 1)) return order == CHARMLU_COL_MAJOR ? -11 : -9;
+  if (ldb < (bc > 1 ? bc : 1)) return order == CHARMLU_COL_MAJOR ? -9 : -11;
+  if (ldc < (N > 1 ? N : 1)) return -14;
+  if (M == 0 || N == 0) return CLU_OK;
+  Workspace *ws;
+  int rc = get_ws(&ws);
+  if (rc) return rc;
+  const size_t na = (size_t)ar * ac, nbb = (size_t)br * bc, nc = (size_t)M * N;
+  if ((rc = ensure_stage(*ws, (na + nbb + nc) * sizeof(double)))) return rc;
+  double *dA = (double *)ws->stage, *dB = dA + na, *dC = dB + nbb;
+  if ((rc = stage_in(dA, A, ar, ac, lda)) || (rc = stage_in(dB, B, br, bc, ldb))) return rc;
+  if (beta != 0.0 && (rc = stage_in(dC, C, M, N, ldc))) return rc;
+  if ((rc = dgemm(ta, tb, M, N, K, alpha, dA, ac, dB, bc, beta, dC, N, nullptr))) return rc;
+  if ((rc = stage_out(C, ldc, dC, M, N))) return rc;
+  return CLU_OK;
+}
+
+int charmlu_cblas_dtrsm(int order, int side, int uplo, int transa, int diag, int M, int N,
+                        double alpha, const double *A, int lda, double *B, int ldb) {
+  if (order != CHARMLU_ROW_MAJOR && order != CHARMLU_COL_MAJOR) return -1;
+  if (side != CHARMLU_LEFT && side != CHARMLU_RIGHT) return -2;
+  if (uplo != CHARMLU_UPPER && uplo != CHARMLU_LOWER) return -3;
+  if (transa != CHARMLU_NO_TRANS && transa != CHARMLU_TRANS) return -4;
+  if (diag != CHARMLU_UNIT && diag != CHARMLU_NON_UNIT) return -5;
+  if (M < 0) return -6;
+  if (N < 0) return -7;
+  bool upper = uplo == CHARMLU_UPPER;
+  // column-major with the triangle on the right (the ESSL/ACML call of
+  // lu.C:185) is the row-major left solve on the transposed views
+  int rM = M, rN = N;
+  if (order == CHARMLU_COL_MAJOR) {
+    if (side != CHARMLU_RIGHT) { set_error("dtrsm: column-major needs side=Right"); return -2; }
+    upper = !upper;
+    rM = N;
+    rN = M;
+  } else if (side != CHARMLU_LEFT) {
+    set_error("dtrsm: row-major needs side=Left");
+    return -2;
+  }
+  if (lda < (rM > 1 ? rM : 1)) return -10;
+  if (ldb < (rN > 1 ? rN : 1)) return -12;
+  if (rM == 0 || rN == 0) return CLU_OK;
+  Workspace *ws;
+  int rc = get_ws(&ws);
+  if (rc) return rc;
+  const size_t nt = (size_t)rM * rM, nbb = (size_t)rM * rN;
+  if ((rc = ensure_stage(*ws, (nt + nbb) * sizeof(double)))) return rc;
+  double *dT = (double *)ws->stage, *dB = dT + nt;
+  if ((rc = stage_in(dT, A, rM, rM, lda)) || (rc = stage_in(dB, B, rM, rN, ldb))) return rc;
+  if ((rc = dtrsm_dev_any(upper, transa == CHARMLU_TRANS, diag == CHARMLU_UNIT, rM, rN, alpha, dT,
+                          rM, dB, rN, nullptr)))
+    return rc;
+  return stage_out(B, ldb, dB, rM, rN);
+}
+
+int charmlu_cblas_dger(int order, int M, int N, double alpha, const double *x, int incx,
+                       const double *y, int incy, double *A, int lda) {
+  if (order != CHARMLU_ROW_MAJOR && order != CHARMLU_COL_MAJOR) return -1;
+  if (M < 0) return -2;
+  if (N < 0) return -3;
+  if (incx == 0) return -6;
+  if (incy == 0) return -8;
+  if (order == CHARMLU_COL_MAJOR) {
+    // column-major A += x y^T  <=>  row-major A^T += y x^T
+    std::swap(M, N);
+    std::swap(x, y);
+    std::swap(incx, incy);
+  }
+  if (lda < (N > 1 ? N : 1)) return -10;
+  if (M == 0 || N == 0) return CLU_OK;
+  Workspace *ws;
+  int rc = get_ws(&ws);
+  if (rc) return rc;
+  const long ax = labs((long)incx), ay = labs((long)incy);
+  const size_t nx = (size_t)(M - 1) * ax + 1, ny = (size_t)(N - 1) * ay + 1, na = (size_t)M * N;
+  if ((rc = ensure_stage(*ws, (nx + ny + na) * sizeof(double)))) return rc;
+  double *dx = (double *)ws->stage, *dy = dx + nx, *dA = dy + ny;
+  CLU_HIP(hipMemcpy(dx, x, nx * sizeof(double), hipMemcpyHostToDevice));
+  CLU_HIP(hipMemcpy(dy, y, ny * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = stage_in(dA, A, M, N, lda))) return rc;
+  if ((rc = dger(M, N, alpha, dx, incx, dy, incy, dA, N, nullptr))) return rc;
+  return stage_out(A, lda, dA, M, N);
+}
+
+int charmlu_cblas_dgemv(int order, int trans, int M, int N, double alpha, const double *A,
+                        int lda, const double *x, int incx, double beta, double *y, int incy) {
+  if (order != CHARMLU_ROW_MAJOR && order != CHARMLU_COL_MAJOR) return -1;
+  if (trans != CHARMLU_NO_TRANS && trans != CHARMLU_TRANS) return -2;
+  if (M < 0) return -3;
+  if (N < 0) return -4;
+  if (incx == 0) return -9;
+  if (incy == 0) return -12;
+  bool tr = trans == CHARMLU_TRANS;
+  int rM = M, rN = N;
+  if (order == CHARMLU_COL_MAJOR) {
+    // column-major M x N is the row-major N x M transpose
+    tr = !tr;
+    rM = N;
+    rN = M;
+  }
+  if (lda < (rN > 1 ? rN : 1)) return -7;
+  const long lx = tr ? rM : rN, ly = tr ? rN : rM;  // op(A) is ly x lx
+  if (ly == 0) return CLU_OK;
+  Workspace *ws;
+  int rc = get_ws(&ws);
+  if (rc) return rc;
+  const long ax = labs((long)incx), ay = labs((long)incy);
+  const size_t nx = lx > 0 ? (size_t)(lx - 1) * ax + 1 : 1, ny = (size_t)(ly - 1) * ay + 1;
+  const size_t na = (size_t)rM * rN;
+  if ((rc = ensure_stage(*ws, (nx + ny + na + 1) * sizeof(double)))) return rc;
+  double *dx = (double *)ws->stage, *dy = dx + nx, *dA = dy + ny;
+  if (lx > 0) CLU_HIP(hipMemcpy(dx, x, nx * sizeof(double), hipMemcpyHostToDevice));
+  CLU_HIP(hipMemcpy(dy, y, ny * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = stage_in(dA, A, rM, rN, lda))) return rc;
+  if (lx == 0) {
+    // y = beta * y
+    if ((rc = dgemv(false, (int)ly, 1, 0.0, dy, 1, dy, 1, beta, dy, incy, nullptr))) return rc;
+  } else if ((rc = dgemv(tr, rM, rN, alpha, dA, rN, dx, incx, beta, dy, incy, nullptr))) {
+    return rc;
+  }
+  CLU_HIP(hipMemcpy(y, dy, ny * sizeof(double), hipMemcpyDeviceToHost));
+  return CLU_OK;
+}
+
+// ---- factorization ------------------------------------------------------------
+int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, void *stream) {
+  if (n < 0) return -1;
+  if (nb < 1 || (n > 0 && n % nb != 0)) {
+    set_error("block size %d must divide n = %d (benchmark.C:143-147)", nb, n);
+    return -2;
+  }
+  if (!dA && n > 0) return -3;
+  if (lda < (n > 1 ? n : 1)) return -4;
+  if (!dipiv && n > 0) return -5;
+  if (!dinfo) return -6;
+  hipStream_t sm = (hipStream_t)stream;
+  int rc;
+  if ((rc = fill_int(dinfo, 0, 1, sm))) return rc;
+  if (n == 0) return CLU_OK;
+  Workspace *wsp;
+  if ((rc = get_ws(&wsp))) return rc;
+  Workspace &ws = *wsp;
+  const int nruns = (nb + RUN - 1) / RUN;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure_runs(ws, nruns))) return rc;
+  }
+  hipStream_t sp = ws.s_panel;
+  ws.pw.info = dinfo;
+  ws.pw.epoch = 0;
+  if ((rc = fill_u32(ws.counter, 0u, 1, sm))) return rc;
+  CLU_HIP(hipEventRecord(ws.ev_start, sm));
+  CLU_HIP(hipStreamWaitEvent(sp, ws.ev_start, 0));
+
+  const int NB = n / nb;
+  auto moves_of = [&](int t) { return ws.smoves + (size_t)(t & 1) * nruns * 2 * RUN; };
+  auto counts_of = [&](int t) { return ws.scount + (size_t)(t & 1) * nruns; };
+  auto panel = [&](int t) -> int {
+    const int r0 = t * nb;
+    const int pid = prof_begin(PK_PANEL, (double)(n - r0) * nb, sp);
+    int r = factor_panel_d(dA, lda, n, r0, nb, dipiv, ws, sp);
+    if (r) return r;
+    r = build_moves(dipiv, r0, r0 + nb, moves_of(t), counts_of(t), sp);
+    prof_end(pid, sp);
+    return r;
+  };
+  // trailing-update GEMM, timed per launch when profiling is on
+  auto update = [&](int M, int N, const double *L, const double *U, double *C) -> int {
+    const int pid = prof_begin(dgemm_variant(M, N) == 0 ? PK_GEMM_BIG : PK_GEMM_OTHER,
+                               2.0 * M * N * nb, sm);
+    int r = dgemm(false, false, M, N, nb, -1.0, L, lda, U, lda, 1.0, C, lda, sm);
+    prof_end(pid, sm);
+    return r;
+  };
+
+  if ((rc = panel(0))) return rc;
+  CLU_HIP(hipEventRecord(ws.ev_panel, sp));
+  for (int t = 0; t < NB; ++t) {
+    const int r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
+    CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));
+    // interchanges of step t on every column outside the panel
+    int pid = prof_begin(PK_SWAP, (double)(n - nb), sm);
+    if ((rc = apply_runs<double>(nruns, moves_of(t), counts_of(t), dA, lda, 0, n, r0, r1, false,
+                                 sm)))
+      return rc;
+    prof_end(pid, sm);
+    if (Nt <= 0) break;
+    double *A11 = dA + (long)r0 * lda + r0;
+    // U row: A12 <- L11^{-1} A12 (computeU, lu.C:180-189)
+    pid = prof_begin(PK_TRSM, (double)Nt * nb * (nb - 1), sm);
+    if ((rc = trsm_llnu_big<double>(nb, Nt, A11, lda, A11 + nb, lda, sm))) return rc;
+    prof_end(pid, sm);
+    // look-ahead block column, then the panel of step t+1 on the panel stream
+    const int la = Nt < nb ? Nt : nb;
+    if ((rc = update(Nt, la, A11 + (long)nb * lda, A11 + nb, A11 + (long)nb * lda + nb)))
+      return rc;
+    CLU_HIP(hipEventRecord(ws.ev_la, sm));
+    CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
+    if ((rc = panel(t + 1))) return rc;
+    CLU_HIP(hipEventRecord(ws.ev_panel, sp));
+    // the rest of the trailing update (updateMatrix, lu.C:26-43)
+    if (Nt > la)
+      if ((rc = update(Nt, Nt - la, A11 + (long)nb * lda, A11 + nb + la,
+                       A11 + (long)nb * lda + nb + la)))
+        return rc;
+  }
+  return CLU_OK;
+}
+
+// ---- solve ---------------------------------------------------------------------
+int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda, const int *dipiv,
+                   double *dB, long ldb, void *stream) {
+  if (trans != CHARMLU_NO_TRANS && trans != CHARMLU_TRANS) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (!dLU && n > 0) return -4;
+  if (lda < (n > 1 ? n : 1)) return -5;
+  if (!dipiv && n > 0) return -6;
+  if (!dB && n > 0 && nrhs > 0) return -7;
+  if (ldb < (nrhs > 1 ? nrhs : 1)) return -8;
+  if (n == 0 || nrhs == 0) return CLU_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Workspace *wsp;
+  int rc = get_ws(&wsp);
+  if (rc) return rc;
+  Workspace &ws = *wsp;
+  const int nruns = (n + RUN - 1) / RUN;
+  const double *L = dLU;
+  const bool vec = nrhs == 1 && ldb == 1;
+  constexpr int BS = 256;
+  auto interchanges = [&](bool inverse) -> int {
+    // one run at a time through the in-panel move buffer (stream-ordered)
+    for (int i = 0; i < nruns; ++i) {
+      const int r = inverse ? nruns - 1 - i : i;
+      const int k1 = r * RUN, k2 = n - k1 < RUN ? n : k1 + RUN;
+      int e = pivots_to_moves(dipiv, k1, k2, ws.pmoves, ws.pcount, s);
+      if (e) return e;
+      e = apply_moves<double>(ws.pmoves, ws.pcount, 2 * RUN, dB, ldb, 0, nrhs, nrhs, nrhs,
+                              inverse, s);
+      if (e) return e;
+    }
+    return CLU_OK;
+  };
+  // off-diagonal contribution of a solved block (offDiagSolve, lu.ci:439-456)
+  auto update = [&](bool ta, int M, int K, const double *Ablk, const double *X, double *Y) {
+    if (M <= 0) return CLU_OK;
+    if (vec) return gemv_sub<double>(ta, M, K, Ablk, lda, X, Y, s);
+    return dgemm(ta, false, M, nrhs, K, -1.0, Ablk, lda, X, ldb, 1.0, Y, ldb, s);
+  };
+  if (trans == CHARMLU_NO_TRANS) {
+    if ((rc = interchanges(false))) return rc;
+    // forward: unit lower (forwardSolve, lu.ci:387-407)
+    for (int x0 = 0; x0 < n; x0 += BS) {
+      const int m = n - x0 < BS ? n - x0 : BS;
+      if ((rc = trsm_llnu<double>(m, nrhs, L + (long)x0 * lda + x0, lda, dB + (long)x0 * ldb, ldb,
+                                  s)))
+        return rc;
+      if ((rc = update(false, n - x0 - m, m, L + (long)(x0 + m) * lda + x0, dB + (long)x0 * ldb,
+                       dB + (long)(x0 + m) * ldb)))
+        return rc;
+    }
+    // backward: non-unit upper (backwardSolve, lu.ci:408-436)
+    for (int x0 = ((n - 1) / BS) * BS; x0 >= 0; x0 -= BS) {
+      const int m = n - x0 < BS ? n - x0 : BS;
+      if ((rc = trsm_lunn<double>(m, nrhs, L + (long)x0 * lda + x0, lda, dB + (long)x0 * ldb, ldb,
+                                  s)))
+        return rc;
+      if ((rc = update(false, x0, m, L + x0, dB + (long)x0 * ldb, dB))) return rc;
+    }
+    return CLU_OK;
+  }
+  // A^T X = B with P A = L U:  U^T Z = B, L^T W = Z, X = P^T W
+  for (int x0 = 0; x0 < n; x0 += BS) {
+    const int m = n - x0 < BS ? n - x0 : BS;
+    if ((rc = trsm_lutn<double>(m, nrhs, L + (long)x0 * lda + x0, lda, dB + (long)x0 * ldb, ldb,
+                                s)))
+      return rc;
+    if ((rc = update(true, n - x0 - m, m, L + (long)x0 * lda + x0 + m, dB + (long)x0 * ldb,
+                     dB + (long)(x0 + m) * ldb)))
+      return rc;
+  }
+  for (int x0 = ((n - 1) / BS) * BS; x0 >= 0; x0 -= BS) {
+    const int m = n - x0 < BS ? n - x0 : BS;
+    if ((rc = trsm_lltu<double>(m, nrhs, L + (long)x0 * lda + x0, lda, dB + (long)x0 * ldb, ldb,
+                                s)))
+      return rc;
+    if ((rc = update(true, x0, m, L + (long)x0 * lda, dB + (long)x0 * ldb, dB))) return rc;
+  }
+  return interchanges(true);
+}
+
+// ---- generation and validation ---------------------------------------------------
+int charmlu_dgen_matrix(int kind, uint64_t seed, long n, double *dA, long lda, void *stream) {
+  if (kind != 0 && kind != 1) return -1;
+  if (n < 0) return -3;
+  if (!dA && n > 0) return -4;
+  if (lda < (n > 1 ? n : 1)) return -5;
+  if (kind == 0) return gen_uniform(seed, 0, 0, n, n, dA, lda, (hipStream_t)stream);
+  return gen_diagdom(seed, n, dA, lda, (hipStream_t)stream);
+}
+
+int charmlu_dgen_rhs(int seed, long skip, long n, double *db, void *stream) {
+  if (skip < 0) return -2;
+  if (n < 0) return -3;
+  if (!db && n > 0) return -4;
+  return gen_rhs(seed, skip, n, db, (hipStream_t)stream);
+}
+
+int charmlu_dresidual(int n, const double *dA, long lda, const double *dx, const double *db,
+                      double *dr, double *dout, void *stream) {
+  if (n < 0) return -1;
+  if (!dA && n > 0) return -2;
+  if (lda < (n > 1 ? n : 1)) return -3;
+  if (!dout) return -7;
+  return residual(n, dA, lda, dx, db, dr, dout, (hipStream_t)stream);
+}
+
+}  // extern "C"

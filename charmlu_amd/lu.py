@@ -1,0 +1,205 @@
+"""Python host side over the C ABI: device-resident LU, solve, generation and
+the reference driver's validation, on torch CUDA tensors (torch supplies the
+device memory and the stream; all arithmetic runs in libcharmlu_amd.so).
+
+Mirrors the reference driver (benchmark.C): ``LUConfig`` holds its argv
+(benchmark.C:79-149, luConfig.h:8-62); ``run`` generates A and b, factors,
+solves and validates exactly as LUSolver/BenchmarkLUBlk do
+(benchmark.C:231-400, 430-573) and reports the same figures.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+SEED_A = 2998389   # benchmark.C:436
+SEED_B = 9934835   # benchmark.C:441
+EPS = float(np.finfo(np.float64).eps)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream=None):
+    torch = _torch()
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes_void(s.cuda_stream)
+
+
+def ctypes_void(v):
+    import ctypes
+    return ctypes.c_void_p(int(v)) if v else None
+
+
+def _ptr(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else None
+
+
+def _require(t, dtype, name):
+    torch = _torch()
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a CUDA tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if t.dim() == 2 and t.stride(1) != 1:
+        raise ValueError(f"{name} must be row-major (stride(1) == 1)")
+    return t
+
+
+# ---------------------------------------------------------------------------
+# generation (benchmark.C:27-74, 430-460)
+# ---------------------------------------------------------------------------
+def gen_matrix(n: int, seed: int = SEED_A, kind: str = "uniform", device="cuda"):
+    """A (n x n FP64) on the device: 'uniform' U[-0.5,0.5) or 'diagdom'
+    (a_ii = sum_j |a_ij|)."""
+    torch = _torch()
+    A = torch.empty((n, n), dtype=torch.float64, device=device)
+    k = {"uniform": 0, "diagdom": 1}[kind]
+    check(_lib.load().charmlu_dgen_matrix(k, seed, n, _ptr(A), max(n, 1), _stream_ptr()),
+          "charmlu_dgen_matrix")
+    return A
+
+
+def gen_rhs(n: int, seed: int = SEED_B, skip: int = 0, device="cuda"):
+    torch = _torch()
+    b = torch.empty(n, dtype=torch.float64, device=device)
+    check(_lib.load().charmlu_dgen_rhs(seed, skip, n, _ptr(b), _stream_ptr()), "charmlu_dgen_rhs")
+    return b
+
+
+# ---------------------------------------------------------------------------
+# factor / solve
+# ---------------------------------------------------------------------------
+def getrf(A, block: int):
+    """In-place LU with partial pivoting of the row-major FP64 CUDA tensor A.
+    Returns (ipiv int32 tensor, info int32 tensor of one element)."""
+    torch = _torch()
+    _require(A, torch.float64, "A")
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise ValueError("A must be square")
+    ipiv = torch.empty(max(n, 1), dtype=torch.int32, device=A.device)
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    check(_lib.load().charmlu_dgetrf(n, block, _ptr(A), A.stride(0), _ptr(ipiv), _ptr(info),
+                                     _stream_ptr()), "charmlu_dgetrf")
+    return ipiv[:n], info
+
+
+def getrs(LU, ipiv, B, trans: bool = False):
+    """Solve op(A) X = B in place (B: n or n x nrhs FP64 CUDA tensor)."""
+    torch = _torch()
+    _require(LU, torch.float64, "LU")
+    n = LU.shape[0]
+    B2 = B.view(n, 1) if B.dim() == 1 else B
+    _require(B2, torch.float64, "B")
+    nrhs = B2.shape[1]
+    check(_lib.load().charmlu_dgetrs(_lib.TRANS if trans else _lib.NO_TRANS, n, nrhs, _ptr(LU),
+                                     LU.stride(0), _ptr(ipiv), _ptr(B2), B2.stride(0),
+                                     _stream_ptr()), "charmlu_dgetrs")
+    return B
+
+
+def residual(A, x, b):
+    """Reference scaled residual (benchmark.C:385-400) and the normwise form
+    ||Ax-b||_inf / (||A||_inf ||x||_inf n eps).  Returns a dict."""
+    torch = _torch()
+    n = A.shape[0]
+    out = torch.zeros(5, dtype=torch.float64, device=A.device)
+    check(_lib.load().charmlu_dresidual(n, _ptr(A), A.stride(0), _ptr(x), _ptr(b), None,
+                                        _ptr(out), _stream_ptr()), "charmlu_dresidual")
+    amax, bmax, xmax, rmax, arow = out.cpu().tolist()
+    ref = rmax / ((amax * xmax + bmax) * n * EPS) if n else 0.0
+    ns = rmax / (arow * xmax * n * EPS) if n and arow * xmax > 0 else 0.0
+    return dict(residual=ref, residual_ns=ns, a_max=amax, b_max=bmax, x_max=xmax,
+                r_max=rmax, a_inf=arow)
+
+
+# ---------------------------------------------------------------------------
+# the reference driver
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class LUConfig:
+    """argv of the reference benchmark: <n> <b> <memMB> [<pivotBatch>
+    <mapping> [<tileRows> <tileCols> <rotate> <stride>]] (benchmark.C:79-134).
+    Only n and b change the computation here; memThreshold, mapping and the
+    PE tiling steer the reference's multi-process scheduler and are kept for
+    interface compatibility (one device holds the whole matrix).  The pivot
+    batch size groups interchanges in the reference (lu.C:300-302); the row
+    permutation it produces is the same for any batch size."""
+    matrix_size: int
+    block_size: int
+    mem_threshold: int = 500
+    pivot_batch_size: int = 0
+    mapping_scheme: int = 2
+    pe_tile_rows: int = 0
+    pe_tile_cols: int = 0
+    pe_tile_rotate: int = 0
+    pe_tile_stride: int = 0
+
+    def __post_init__(self):
+        if self.pivot_batch_size <= 0:
+            self.pivot_batch_size = max(1, self.block_size // 4)  # benchmark.C:95-98
+
+    @classmethod
+    def from_argv(cls, argv):
+        if len(argv) < 3:
+            raise SystemExit("Usage: <matrix size> <block size> <mem threshold> "
+                             "[<pivot batch size> <mapping scheme> [<peTileRows> <peTileCols> "
+                             "<peRotate> <peStride>] ]")
+        vals = [int(a) for a in argv]
+        names = ["matrix_size", "block_size", "mem_threshold", "pivot_batch_size",
+                 "mapping_scheme", "pe_tile_rows", "pe_tile_cols", "pe_tile_rotate",
+                 "pe_tile_stride"]
+        return cls(**dict(zip(names, vals)))
+
+    def validate(self):
+        if self.block_size <= 0 or self.matrix_size % self.block_size != 0:
+            raise ValueError(f"The matrix size {self.matrix_size} should be a multiple of block "
+                             f"size {self.block_size}!")  # benchmark.C:143-147
+
+
+def hpl_gflops(n: int, seconds: float) -> float:
+    """(2/3 n^3 + 3/2 n^2) / t  (benchmark.C:344-349), in GFlop/s."""
+    return (2.0 / 3.0 * n ** 3 + 1.5 * n ** 2) / seconds / 1e9
+
+
+def run(cfg: LUConfig, seed_a: int = SEED_A, seed_b: int = SEED_B, kind: str = "uniform",
+        verbose: bool = True):
+    """Generate, factor + solve (timed, benchmark.C:242-245 -> 343-349),
+    validate (benchmark.C:462-573).  Returns a dict."""
+    torch = _torch()
+    cfg.validate()
+    n, b = cfg.matrix_size, cfg.block_size
+    A = gen_matrix(n, seed_a, kind)
+    rhs = gen_rhs(n, seed_b)
+    LU = A.clone()
+    x = rhs.clone()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    ipiv, info = getrf(LU, b)
+    getrs(LU, ipiv, x)
+    stop.record()
+    torch.cuda.synchronize()
+    seconds = start.elapsed_time(stop) / 1e3
+    res = residual(A, x, rhs)
+    out = dict(n=n, block=b, seconds=seconds, info=int(info.item()),
+               gflops_hpl=hpl_gflops(n, seconds),
+               tflops_eff=(2.0 / 3.0 * n ** 3) / seconds / 1e12 if seconds > 0 else math.inf,
+               ipiv=ipiv, x=x, **res)
+    if verbose:
+        print(f"RESULT procs: 1 device\tblock size: {b}\tTime(s): {seconds:f}")
+        print(f"HPL flop rate: {out['gflops_hpl']:f} GFlops")
+        print(f"epsilon = {EPS:e}\nresidual = {res['residual']:f}")
+        if res["residual"] > 16:
+            print("=== WARNING: Scaled residual is greater than 16 - OUT OF SPEC ===")
+    return out

@@ -1,0 +1,156 @@
+/*
+ * charmlu_amd.h -- C ABI of the MI355X-native dense LU (libcharmlu_amd.so).
+ *
+ * Drop-in boundary for charmlu's block-update path.  The reference reaches
+ * its arithmetic through the four CBLAS routines that platformBlas.h:4-38
+ * selects; section 1 below exports those routines with the same argument
+ * lists (row-major CBLAS semantics, host pointers, synchronous), so a
+ * maintainer can point the reference's calls at them.  Section 2 gives the
+ * same operations on device pointers, ordered on a HIP stream.  Section 3
+ * replaces the whole chare-array factorization, solve and validation with
+ * device-resident entry points.
+ *
+ * Conventions
+ *   - Every matrix is row-major with a leading dimension in elements, as the
+ *     reference's blocks are (lu.h:151-153, getIndex(i,j) = i*b + j).
+ *   - Pivot indices are 0-based global rows: ipiv[j] = p means row j was
+ *     exchanged with row p when column j was factored (the reference's
+ *     recordPivot(j, p), lu.C:342-352).
+ *   - stream is a hipStream_t passed as void* (NULL = the null stream).
+ *   - Return value: 0 on success, -i if argument i is invalid (LAPACK
+ *     style), CHARMLU_ERR_HIP if a HIP call failed, CHARMLU_ERR_NOMEM if the
+ *     library could not allocate its workspace.  charmlu_last_error() gives
+ *     a message.  Factorization status (singular pivot) is reported through
+ *     an info word, never by aborting (the reference aborts, lu.ci:339-341).
+ *   - Device-resident calls on one device share that device's workspace:
+ *     calls that overlap in time on one device race.
+ */
+#ifndef CHARMLU_AMD_H
+#define CHARMLU_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHARMLU_OK 0
+#define CHARMLU_ERR_HIP (-100)
+#define CHARMLU_ERR_NOMEM (-101)
+
+/* CBLAS enumerations (public CBLAS values) */
+#define CHARMLU_ROW_MAJOR 101
+#define CHARMLU_COL_MAJOR 102
+#define CHARMLU_NO_TRANS 111
+#define CHARMLU_TRANS 112
+#define CHARMLU_UPPER 121
+#define CHARMLU_LOWER 122
+#define CHARMLU_NON_UNIT 131
+#define CHARMLU_UNIT 132
+#define CHARMLU_LEFT 141
+#define CHARMLU_RIGHT 142
+
+/* precision of the trailing update in the mixed-precision factorization */
+#define CHARMLU_PREC_FP64 0
+#define CHARMLU_PREC_FP32 1
+#define CHARMLU_PREC_BF16 2
+#define CHARMLU_PREC_FP16 3
+#define CHARMLU_PREC_FP8 4
+
+int charmlu_version(void);
+const char *charmlu_last_error(void);
+
+/* Per-launch timing of charmlu_dgetrf's kernel classes with HIP events on
+ * the streams they run on (no counterpart in the reference, which times the
+ * whole run only, benchmark.C:242-245, 343-349).  charmlu_profile(1) starts
+ * recording, charmlu_profile(0) stops; charmlu_profile_read fills out[15] =
+ * {count, work, milliseconds} for the classes 0 trailing-update GEMM of the
+ * 128x128-tile kernel (work = flops), 1 other trailing-update GEMMs
+ * (flops), 2 panels (rows x columns), 3 row interchanges (columns moved per
+ * step), 4 U-row solves (flops), then clears the records.  Synchronizes. */
+int charmlu_profile(int on);
+int charmlu_profile_read(double *out, int nout);
+/* Make device `device` current for this thread and set up its workspace. */
+int charmlu_init(int device);
+
+/* ---- 1. platformBlas.h surface: host pointers, synchronous ------------- */
+
+/* Replaces cblas_dgemm(CblasRowMajor, CblasNoTrans, CblasNoTrans, b, b, b,
+ * -1.0, L, b, U, b, 1.0, C, b) in LUBlk::updateMatrix (lu.C:36-41) and the
+ * testdgemm call (benchmark.C:324-329).  order: ROW_MAJOR or COL_MAJOR;
+ * transa/transb: NO_TRANS or TRANS. */
+int charmlu_cblas_dgemm(int order, int transa, int transb, int M, int N, int K,
+                        double alpha, const double *A, int lda, const double *B,
+                        int ldb, double beta, double *C, int ldc);
+
+/* Replaces cblas_dtrsm(CblasRowMajor, CblasLeft, CblasLower, CblasNoTrans,
+ * CblasUnit, b, b, 1.0, L, b, A, b) in LUBlk::computeU (lu.C:187).
+ * Supported: side LEFT; any uplo/trans/diag; M <= 256 per call on device
+ * (larger M is solved in 256-row blocks). */
+int charmlu_cblas_dtrsm(int order, int side, int uplo, int transa, int diag,
+                        int M, int N, double alpha, const double *A, int lda,
+                        double *B, int ldb);
+
+/* Replaces cblas_dger(CblasRowMajor, M, N, -1.0, x, b, y, 1, A, b) in
+ * LUBlk::updateLsubBlock (lu.C:247-252). */
+int charmlu_cblas_dger(int order, int M, int N, double alpha, const double *x,
+                       int incx, const double *y, int incy, double *A, int lda);
+
+/* Replaces cblas_dgemv(CblasRowMajor, CblasNoTrans, b, b, 1.0, A, b, x, 1,
+ * 0.0, y, 1) in BenchmarkLUBlk::recvXvec (benchmark.C:494-496). */
+int charmlu_cblas_dgemv(int order, int trans, int M, int N, double alpha,
+                        const double *A, int lda, const double *x, int incx,
+                        double beta, double *y, int incy);
+
+/* ---- 2. the same block ops on device pointers, stream-ordered ---------- */
+
+/* C = alpha*op(A)*op(B) + beta*C, row-major. (lu.C:26-43) */
+int charmlu_dgemm_dev(int transa, int transb, int M, int N, int K, double alpha,
+                      const double *dA, long lda, const double *dB, long ldb,
+                      double beta, double *dC, long ldc, void *stream);
+/* B = op(T)^{-1} B for a triangular T (M <= 256), left side. (lu.C:180-189) */
+int charmlu_dtrsm_dev(int uplo, int transa, int diag, int M, int N,
+                      const double *dT, long ldt, double *dB, long ldb,
+                      void *stream);
+/* A += alpha * x * y^T. (lu.C:230-254) */
+int charmlu_dger_dev(int M, int N, double alpha, const double *dx, long incx,
+                     const double *dy, long incy, double *dA, long lda,
+                     void *stream);
+
+/* ---- 3. whole factorization, solve, generation, validation ------------- */
+
+/* Blocked right-looking LU with partial pivoting of the n x n matrix dA in
+ * place (the whole chare-array factorization: factor/updateUntilActive/
+ * factorDiagBlock/factorBelowDiagBlock/factorAboveDiagBlock, lu.ci:31-370).
+ * nb: block size (the reference's b; n % nb must be 0, benchmark.C:143-147).
+ * dipiv: n device ints.  dinfo: one device int, 0 or the first column j+1
+ * whose pivot has |u_jj| <= 100*eps (lu.ci:339-341).  Asynchronous. */
+int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo,
+                   void *stream);
+
+/* Solve op(A) X = B with the factors of charmlu_dgetrf: the row
+ * interchanges, then forward and backward block substitution
+ * (forwardSolve/backwardSolve/offDiagSolve, lu.ci:387-458).
+ * trans: NO_TRANS or TRANS.  B is n x nrhs row-major, overwritten by X. */
+int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda,
+                   const int *dipiv, double *dB, long ldb, void *stream);
+
+/* Seeded inputs of the driver (benchmark.C:430-460).
+ * gen_matrix: entry (i,j) of the build's counter generator, kind 0 =
+ * uniform [-0.5,0.5), kind 1 = same with a_ii = sum_j |a_ij|.
+ * gen_rhs: MatGen draws skip+1 .. skip+n of seed (benchmark.C:27-74). */
+int charmlu_dgen_matrix(int kind, uint64_t seed, long n, double *dA, long lda,
+                        void *stream);
+int charmlu_dgen_rhs(int seed, long skip, long n, double *db, void *stream);
+
+/* Scaled-residual inputs (benchmark.C:385-400, 462-573): out[5] (device) =
+ * {max|a_ij|, max|b_i|, max|x_i|, max|b-Ax|_i, max_i sum_j |a_ij|}; dr
+ * (optional, n doubles) receives b - A x.  Reference residual =
+ * out[3] / ((out[0]*out[2] + out[1]) * n * eps). */
+int charmlu_dresidual(int n, const double *dA, long lda, const double *dx,
+                      const double *db, double *dr, double *dout, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHARMLU_AMD_H */
