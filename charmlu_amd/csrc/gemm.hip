@@ -14,7 +14,9 @@
 //
 // MFMA 16x16x4 f64 operand layout (per lane l, 0..63):
 //   A[i][k] from lane i + 16k,  B[k][j] from lane j + 16k,
-//   D[i][j] in lane j + 16*(i/4), element i%4 of the 4-double result.
+//   D[i][j] in lane j + 16*(i%4), element i/4 of the 4-double result
+// (measured on gfx950; the FP32/FP16/BF16 forms use lane j + 16*(i/4),
+// element i%4 instead).
 #include "common.h"
 #include "kernels.h"
 
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
       const int col = n0 + wn0 + j * 16 + lr;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm0 + i * 16 + 4 * lq + v;
+        const int row = m0 + wm0 + i * 16 + lq + 4 * v;
         if (row < M && col < N) {
           double *c = C + (long)row * ldc + col;
           const double prod = alpha * acc[i][j][v];

@@ -57,24 +57,26 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
   __shared__ int s_p;
 
   T a[R][PW];
-#pragma unroll
+#pragma clang loop unroll(full)
   for (int r = 0; r < R; ++r) {
     const int li = g * RPW + r * 256 + t;
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int q = 0; q < PW; ++q)
       a[r][q] = (li < m && q < w) ? A[(long)(j0 + li) * lda + j0 + q] : (T)0;
   }
 
   unsigned target = epoch;
-#pragma unroll
+#pragma clang loop unroll(full)
   for (int c = 0; c < PW; ++c) {
-    if (c >= w) break;
+    // a constant trip count keeps the loop unrollable, so every a[r][c]
+    // below is a register; columns past w do nothing (w is uniform)
+    if (c < w) {
     const int par = c & 1;
     // ---- candidate for column c over local rows >= c (strict '>', rows
     //      visited in increasing order, so the lowest row wins ties) ----
     T bv = (T)0;
     int bi = -1;
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int r = 0; r < R; ++r) {
       const int li = g * RPW + r * 256 + t;
       const T v = fabs(a[r][c]);
@@ -84,22 +86,22 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
     if (lane == 0) { s_val[wave] = bv; s_idx[wave] = bi; }
     __syncthreads();
     bv = s_val[0]; bi = s_idx[0];
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int k = 1; k < 4; ++k)
       if (better(s_val[k], s_idx[k], bv, bi)) { bv = s_val[k]; bi = s_idx[k]; }
 
     const int owner_c = c / RPW;  // workgroup holding the top row
     if (G == 1) {
       const int p = bi < 0 ? c : bi;
-#pragma unroll
+#pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
         const int li = r * 256 + t;
         if (li == p) {
-#pragma unroll
+#pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) s_u[q] = a[r][q];
         }
         if (li == c) {
-#pragma unroll
+#pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) s_top[q] = a[r][q];
         }
       }
@@ -111,15 +113,15 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
         st_agent(&slot_val[par * G + g], bv);
         st_agent(&slot_idx[par * G + g], bi);
       }
-#pragma unroll
+#pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
         const int li = g * RPW + r * 256 + t;
         if (bi >= 0 && li == bi) {
-#pragma unroll
+#pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) st_agent(&slot_row[(par * G + g) * PW + q], a[r][q]);
         }
         if (li == c) {
-#pragma unroll
+#pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) st_agent(&top_row[par * PW + q], a[r][q]);
         }
       }
@@ -139,7 +141,7 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
       if (lane == 0) { s_val[wave] = cv; s_idx[wave] = ci; }
       __syncthreads();
       cv = s_val[0]; ci = s_idx[0];
-#pragma unroll
+#pragma clang loop unroll(full)
       for (int k = 1; k < 4; ++k)
         if (better(s_val[k], s_idx[k], cv, ci)) { cv = s_val[k]; ci = s_idx[k]; }
       const int p = ci < 0 ? c : ci;
@@ -161,14 +163,14 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
     }
     // ---- row exchange inside the strip ----
     if (p != c) {
-#pragma unroll
+#pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
         const int li = g * RPW + r * 256 + t;
         if (li == c) {
-#pragma unroll
+#pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) a[r][q] = s_u[q];
         } else if (li == p) {
-#pragma unroll
+#pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) a[r][q] = s_top[q];
         }
       }
@@ -176,25 +178,26 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
     // ---- multipliers (division) and rank-1 update of columns > c ----
     const T piv = s_u[c];
     if (piv != (T)0) {
-#pragma unroll
+#pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
         const int li = g * RPW + r * 256 + t;
         if (li < m && li > c) {
           const T l = a[r][c] / piv;
           a[r][c] = l;
-#pragma unroll
+#pragma clang loop unroll(full)
           for (int q = c + 1; q < PW; ++q) a[r][q] -= l * s_u[q];
         }
       }
     }
     __syncthreads();  // s_u / s_top / s_val reused by the next column
+    }
   }
 
-#pragma unroll
+#pragma clang loop unroll(full)
   for (int r = 0; r < R; ++r) {
     const int li = g * RPW + r * 256 + t;
     if (li < m) {
-#pragma unroll
+#pragma clang loop unroll(full)
       for (int q = 0; q < PW; ++q)
         if (q < w) A[(long)(j0 + li) * lda + j0 + q] = a[r][q];
     }
