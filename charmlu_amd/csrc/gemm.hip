@@ -177,7 +177,7 @@ static int dispatch(int M, int N, int K, double alpha, const double *A, long lda
   if (N <= 32)
     return launch<256, 32, 4, 1, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   if (N <= 64)
-    return launch<256, 64, 4, 1, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch<64, 64, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   if (M <= 32)
     return launch<32, 256, 1, 4, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   // small problems: more, smaller tiles to fill 256 CUs
@@ -188,7 +188,7 @@ static int dispatch(int M, int N, int K, double alpha, const double *A, long lda
 
 int dgemm_variant(int M, int N) {
   if (N <= 32) return 1;
-  if (N <= 64) return 2;
+  if (N <= 64) return 4;
   if (M <= 32) return 3;
   if ((long)((M + 127) / 128) * ((N + 127) / 128) < 512) return 4;
   return 0;

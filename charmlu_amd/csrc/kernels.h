@@ -16,7 +16,7 @@ int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A,
           hipStream_t s);
 
 // Tile configuration dgemm() picks for an M x N product: 0 = 128x128 (the
-// trailing-update kernel), 1 = 256x32, 2 = 256x64, 3 = 32x256, 4 = 64x64.
+// trailing-update kernel), 1 = 256x32, 3 = 32x256, 4 = 64x64.
 int dgemm_variant(int M, int N);
 
 // ---- pivoting -------------------------------------------------------------
@@ -35,17 +35,17 @@ int apply_moves(const int2 *moves, const int *count, int max_moves, T *A,
 
 // ---- panel factorization (FP64 / FP32) ------------------------------------
 struct PanelWork {
-  void *slots;        // candidate rows, 2 parities
-  void *top;          // the current top row, 2 parities
-  unsigned *counter;  // grid barrier counter
-  unsigned epoch;     // host-side running count of barrier arrivals
+  void *slots;        // per-workgroup candidate records, 2 parities x 256
+  unsigned epoch;     // sequence number of the next column (host-side)
   int max_groups;
   int *info;          // device int, first near-singular column + 1
 };
 // Factor rows [j0, m_end) x columns [j0, j0+w) (w <= 16) of A in place with
 // partial pivoting, one column at a time; ipiv[j0..j0+w) get global rows.
+// The strip's row interchanges are also applied to the other columns
+// [pc0, j0) and [j0+w, pc1) of the panel.
 template <typename T>
-int panel_base(T *A, long lda, int j0, int m_end, int w, int *ipiv,
+int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *ipiv,
                PanelWork &pw, hipStream_t s);
 
 // ---- triangular solves ----------------------------------------------------

@@ -22,6 +22,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -54,11 +55,12 @@ constexpr int RUN = 256;         // pivots per move list
 
 struct Workspace {
   int device = -1;
-  hipStream_t s_panel = nullptr;
-  hipEvent_t ev_panel = nullptr, ev_la = nullptr, ev_start = nullptr;
+  hipStream_t s_panel = nullptr;  // panels: the reserved CUs only
+  hipStream_t s_main = nullptr;   // everything else: the other CUs
+  int panel_cus = 0;
+  hipEvent_t ev_panel = nullptr, ev_la = nullptr, ev_start = nullptr, ev_end = nullptr;
   // panel exchange slots (sized for doubles, reused for floats)
   void *slots = nullptr;
-  void *top = nullptr;
   unsigned *counter = nullptr;
   int *info_scratch = nullptr;
   // move lists: in-panel (one run) and per-step (nruns runs, two parities)
@@ -123,15 +125,45 @@ static int get_ws(Workspace **out) {
   std::lock_guard<std::mutex> lk(g_mu);
   Workspace &w = g_ws[dev];
   if (w.device < 0) {
-    CLU_HIP(hipStreamCreateWithFlags(&w.s_panel, hipStreamNonBlocking));
+    // The update stream and the panel stream get disjoint CU masks, so the
+    // latency-bound panel of the next step runs on compute units of its own
+    // instead of queueing behind (and sharing SIMDs with) tens of thousands
+    // of GEMM workgroups.
+    hipDeviceProp_t prop;
+    CLU_HIP(hipGetDeviceProperties(&prop, dev));
+    const int ncu = prop.multiProcessorCount;
+    int reserve = 32;
+    if (const char *e = getenv("CHARMLU_PANEL_CUS")) reserve = atoi(e);
+    if (reserve < 0 || reserve >= ncu) reserve = 0;
+    w.s_main = w.s_panel = nullptr;
+    if (reserve > 0) {
+      std::vector<uint32_t> mm((ncu + 31) / 32, 0u), pm((ncu + 31) / 32, 0u);
+      for (int c = 0; c < ncu; ++c) {
+        if (c < ncu - reserve) mm[c / 32] |= 1u << (c % 32);
+        else pm[c / 32] |= 1u << (c % 32);
+      }
+      if (hipExtStreamCreateWithCUMask(&w.s_main, (uint32_t)mm.size(), mm.data()) != hipSuccess ||
+          hipExtStreamCreateWithCUMask(&w.s_panel, (uint32_t)pm.size(), pm.data()) != hipSuccess) {
+        (void)hipGetLastError();
+        if (w.s_main) (void)hipStreamDestroy(w.s_main);
+        if (w.s_panel) (void)hipStreamDestroy(w.s_panel);
+        w.s_main = w.s_panel = nullptr;
+        reserve = 0;
+      }
+    }
+    if (!w.s_main) CLU_HIP(hipStreamCreateWithFlags(&w.s_main, hipStreamNonBlocking));
+    if (!w.s_panel) CLU_HIP(hipStreamCreateWithFlags(&w.s_panel, hipStreamNonBlocking));
+    w.panel_cus = reserve;
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_end, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_panel, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_la, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming));
-    const size_t slot_bytes = (size_t)2 * MAX_GROUPS * sizeof(double) +
-                              (size_t)2 * MAX_GROUPS * sizeof(int) +
-                              (size_t)2 * MAX_GROUPS * 16 * sizeof(double) + 256;
+    // panel candidate records: 2 parities x 256 workgroups x 288 bytes,
+    // zeroed once; their sequence numbers only ever grow (pw.epoch), so a
+    // record left by an earlier call can never look current
+    const size_t slot_bytes = (size_t)2 * 256 * 288;
     CLU_HIP(hipMalloc(&w.slots, slot_bytes));
-    CLU_HIP(hipMalloc(&w.top, 2 * 16 * sizeof(double)));
+    CLU_HIP(hipMemset(w.slots, 0, slot_bytes));
     CLU_HIP(hipMalloc(&w.counter, 64));
     CLU_HIP(hipMalloc(&w.info_scratch, 64));
     CLU_HIP(hipMalloc(&w.pmoves, sizeof(int2) * 2 * RUN));
@@ -139,9 +171,7 @@ static int get_ws(Workspace **out) {
     int rc = ensure_runs(w, 1);
     if (rc) return rc;
     w.pw.slots = w.slots;
-    w.pw.top = w.top;
-    w.pw.counter = w.counter;
-    w.pw.epoch = 0;
+    w.pw.epoch = 1;
     w.pw.max_groups = MAX_GROUPS;
     w.device = dev;
   }
@@ -188,32 +218,25 @@ int trsm_llnu_big<double>(int M, int N, const double *L, long ldl, double *B, lo
   return CLU_OK;
 }
 
-// Factor the strip rows [c0, n) x columns [c0, c0+w) (recursive halving down
-// to 16-column strips factored by panel_base).  The pivots of the left half
-// are applied to the right half before its update, the pivots of the right
-// half to the left half afterwards (rows >= their own diagonal only).
-static int factor_panel_d(double *A, long lda, int n, int c0, int w, int *ipiv, Workspace &ws,
-                          hipStream_t s) {
-  if (w <= 16) return panel_base<double>(A, lda, c0, n, w, ipiv, ws.pw, s);
+// Factor rows [c0, n) x columns [c0, c0+w) of the panel [pc0, pc0+pw)
+// (recursive halving down to 16-column strips factored by panel_base, which
+// also applies each strip's interchanges to the rest of the panel, so both
+// halves of every split are already permuted when they are used).
+static int factor_panel_d(double *A, long lda, int n, int c0, int w, int pc0, int pcw,
+                          int *ipiv, Workspace &ws, hipStream_t s) {
+  if (w <= 16) return panel_base<double>(A, lda, c0, n, w, pc0, pc0 + pcw, ipiv, ws.pw, s);
   int w1 = (w / 2 + 15) / 16 * 16;
   if (w1 >= w) w1 = w / 2;
   const int w2 = w - w1;
-  int rc = factor_panel_d(A, lda, n, c0, w1, ipiv, ws, s);
+  int rc = factor_panel_d(A, lda, n, c0, w1, pc0, pcw, ipiv, ws, s);
   if (rc) return rc;
-  if ((rc = pivots_to_moves(ipiv, c0, c0 + w1, ws.pmoves, ws.pcount, s))) return rc;
-  if ((rc = apply_moves<double>(ws.pmoves, ws.pcount, 2 * w1, A, lda, c0 + w1, c0 + w, c0 + w,
-                                c0 + w, false, s)))
-    return rc;
   double *A11 = A + (long)c0 * lda + c0;
   if ((rc = trsm_llnu<double>(w1, w2, A11, lda, A11 + w1, lda, s))) return rc;
   if (n - c0 - w1 > 0)
     if ((rc = dgemm(false, false, n - c0 - w1, w2, w1, -1.0, A11 + (long)w1 * lda, lda, A11 + w1,
                     lda, 1.0, A11 + (long)w1 * lda + w1, lda, s)))
       return rc;
-  if ((rc = factor_panel_d(A, lda, n, c0 + w1, w2, ipiv, ws, s))) return rc;
-  if ((rc = pivots_to_moves(ipiv, c0 + w1, c0 + w, ws.pmoves, ws.pcount, s))) return rc;
-  return apply_moves<double>(ws.pmoves, ws.pcount, 2 * w2, A, lda, c0, c0 + w1, c0 + w1,
-                             c0 + w1, false, s);
+  return factor_panel_d(A, lda, n, c0 + w1, w2, pc0, pcw, ipiv, ws, s);
 }
 
 // move lists of ipiv[k1, k2) in runs of RUN pivots
@@ -524,13 +547,15 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   if (lda < (n > 1 ? n : 1)) return -4;
   if (!dipiv && n > 0) return -5;
   if (!dinfo) return -6;
-  hipStream_t sm = (hipStream_t)stream;
+  hipStream_t su = (hipStream_t)stream;
   int rc;
-  if ((rc = fill_int(dinfo, 0, 1, sm))) return rc;
+  if ((rc = fill_int(dinfo, 0, 1, su))) return rc;
   if (n == 0) return CLU_OK;
   Workspace *wsp;
   if ((rc = get_ws(&wsp))) return rc;
   Workspace &ws = *wsp;
+  // all work runs on the library's two streams, fenced to the caller's
+  hipStream_t sm = ws.s_main;
   const int nruns = (nb + RUN - 1) / RUN;
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -538,10 +563,9 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   }
   hipStream_t sp = ws.s_panel;
   ws.pw.info = dinfo;
-  ws.pw.epoch = 0;
-  if ((rc = fill_u32(ws.counter, 0u, 1, sm))) return rc;
-  CLU_HIP(hipEventRecord(ws.ev_start, sm));
+  CLU_HIP(hipEventRecord(ws.ev_start, su));
   CLU_HIP(hipStreamWaitEvent(sp, ws.ev_start, 0));
+  CLU_HIP(hipStreamWaitEvent(sm, ws.ev_start, 0));
 
   const int NB = n / nb;
   auto moves_of = [&](int t) { return ws.smoves + (size_t)(t & 1) * nruns * 2 * RUN; };
@@ -549,7 +573,7 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   auto panel = [&](int t) -> int {
     const int r0 = t * nb;
     const int pid = prof_begin(PK_PANEL, (double)(n - r0) * nb, sp);
-    int r = factor_panel_d(dA, lda, n, r0, nb, dipiv, ws, sp);
+    int r = factor_panel_d(dA, lda, n, r0, nb, r0, nb, dipiv, ws, sp);
     if (r) return r;
     r = build_moves(dipiv, r0, r0 + nb, moves_of(t), counts_of(t), sp);
     prof_end(pid, sp);
@@ -595,6 +619,8 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
                        A11 + (long)nb * lda + nb + la)))
         return rc;
   }
+  CLU_HIP(hipEventRecord(ws.ev_end, sm));
+  CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
   return CLU_OK;
 }
 

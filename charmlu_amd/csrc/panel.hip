@@ -32,19 +32,21 @@ namespace clu {
 
 constexpr int PW = 16;  // max strip width
 
-template <typename T>
-struct SlotView {
-  T *val;   // [2][G]
-  int *idx; // [2][G]
-  T *row;   // [2][G][PW]
-  T *top;   // [2][PW]
+// One workgroup's published candidate for one column (double storage for
+// both precisions; float values convert exactly).  seq is written last, after
+// the rest is visible device-wide.
+struct alignas(16) PanelRec {
+  double val;
+  int idx;
+  unsigned seq;
+  double cand[PW];
+  double top[PW];
 };
 
 template <typename T, int R>
 __global__ __launch_bounds__(256) void panel_base_kernel(
-    T *__restrict__ A, long lda, int j0, int m, int w, int *__restrict__ ipiv,
-    T *slot_val, int *slot_idx, T *slot_row, T *top_row, unsigned *counter,
-    unsigned epoch, int *info) {
+    T *__restrict__ A, long lda, int j0, int m, int w, int pc0, int pc1,
+    int *__restrict__ ipiv, PanelRec *__restrict__ recs, unsigned seq0, int *info) {
   const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   constexpr int RPW = 256 * R;
@@ -55,6 +57,10 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
   __shared__ T s_u[PW];
   __shared__ T s_top[PW];
   __shared__ int s_p;
+  __shared__ int s_piv[PW];
+  __shared__ double s_cand[256][PW];  // candidate rows of every workgroup (G <= 256)
+  __shared__ double s_cval[256];
+  __shared__ int s_cidx[256];
 
   T a[R][PW];
 #pragma clang loop unroll(full)
@@ -65,12 +71,24 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
       a[r][q] = (li < m && q < w) ? A[(long)(j0 + li) * lda + j0 + q] : (T)0;
   }
 
-  unsigned target = epoch;
+  // The column loop stays rolled (a fully unrolled strip kernel is tens of
+  // thousands of instructions and runs from instruction-cache misses).  To
+  // keep every register index static, each row is rotated left by one
+  // position after every column, so the active column is always a[r][0],
+  // the columns still to come are a[r][1 .. w-c-1], and after 16 rotations
+  // in total the row is back in its original order.
+  auto rotate = [&]() {
 #pragma clang loop unroll(full)
-  for (int c = 0; c < PW; ++c) {
-    // a constant trip count keeps the loop unrollable, so every a[r][c]
-    // below is a register; columns past w do nothing (w is uniform)
-    if (c < w) {
+    for (int r = 0; r < R; ++r) {
+      const T first = a[r][0];
+#pragma clang loop unroll(full)
+      for (int q = 0; q < PW - 1; ++q) a[r][q] = a[r][q + 1];
+      a[r][PW - 1] = first;
+    }
+  };
+#pragma clang loop unroll(disable)
+  for (int c = 0; c < w; ++c) {
+    {
     const int par = c & 1;
     // ---- candidate for column c over local rows >= c (strict '>', rows
     //      visited in increasing order, so the lowest row wins ties) ----
@@ -79,7 +97,7 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
 #pragma clang loop unroll(full)
     for (int r = 0; r < R; ++r) {
       const int li = g * RPW + r * 256 + t;
-      const T v = fabs(a[r][c]);
+      const T v = fabs(a[r][0]);
       if (li < m && li >= c && v > bv) { bv = v; bi = li; }
     }
     wave_argmax(bv, bi);
@@ -90,7 +108,6 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
     for (int k = 1; k < 4; ++k)
       if (better(s_val[k], s_idx[k], bv, bi)) { bv = s_val[k]; bi = s_idx[k]; }
 
-    const int owner_c = c / RPW;  // workgroup holding the top row
     if (G == 1) {
       const int p = bi < 0 ? c : bi;
 #pragma clang loop unroll(full)
@@ -108,35 +125,55 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
       if (t == 0) s_p = p;
       __syncthreads();
     } else {
-      // publish this workgroup's candidate and, if held here, the top row
+      // publish: candidate value/row, and the top row if held here
+      const unsigned seq = seq0 + (unsigned)c + 1u;
+      PanelRec *mine = recs + par * 256 + g;
       if (t == 0) {
-        st_agent(&slot_val[par * G + g], bv);
-        st_agent(&slot_idx[par * G + g], bi);
+        st_agent(&mine->val, (double)bv);
+        st_agent(&mine->idx, bi);
       }
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
         const int li = g * RPW + r * 256 + t;
         if (bi >= 0 && li == bi) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) st_agent(&slot_row[(par * G + g) * PW + q], a[r][q]);
+          for (int q = 0; q < PW; ++q) st_agent(&mine->cand[q], (double)a[r][q]);
         }
         if (li == c) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) st_agent(&top_row[par * PW + q], a[r][q]);
+          for (int q = 0; q < PW; ++q) st_agent(&mine->top[q], (double)a[r][q]);
         }
       }
       wait_all_mem();
       __syncthreads();
-      target += G;
-      if (t == 0) {
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        while (ld_agent(counter) - target > 0x80000000u) __builtin_amdgcn_s_sleep(1);
+      if (t == 0) st_agent(&mine->seq, seq);
+      // wait for every workgroup's record of this column, then read it
+      const int owner_c = c / RPW;
+      if (t < G) {
+        const PanelRec *rt = recs + par * 256 + t;
+        while (ld_agent(&rt->seq) != seq) __builtin_amdgcn_s_sleep(1);
+        // issue every load of the record before using any of them
+        double rv[PW + 1];
+        const int ridx = ld_agent(&rt->idx);
+        rv[PW] = ld_agent(&rt->val);
+#pragma clang loop unroll(full)
+        for (int q = 0; q < PW; ++q) rv[q] = ld_agent(&rt->cand[q]);
+        s_cval[t] = rv[PW];
+        s_cidx[t] = ridx;
+#pragma clang loop unroll(full)
+        for (int q = 0; q < PW; ++q) s_cand[t][q] = rv[q];
+        if (t == owner_c) {
+#pragma clang loop unroll(full)
+          for (int q = 0; q < PW; ++q) rv[q] = ld_agent(&rt->top[q]);
+#pragma clang loop unroll(full)
+          for (int q = 0; q < PW; ++q) s_top[q] = (T)rv[q];
+        }
       }
       __syncthreads();
       // every workgroup reduces the same G candidates the same way
       T cv = (T)0;
       int ci = -1;
-      if (t < G) { cv = ld_agent(&slot_val[par * G + t]); ci = ld_agent(&slot_idx[par * G + t]); }
+      if (t < G) { cv = (T)s_cval[t]; ci = s_cidx[t]; }
       wave_argmax(cv, ci);
       if (lane == 0) { s_val[wave] = cv; s_idx[wave] = ci; }
       __syncthreads();
@@ -145,21 +182,16 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
       for (int k = 1; k < 4; ++k)
         if (better(s_val[k], s_idx[k], cv, ci)) { cv = s_val[k]; ci = s_idx[k]; }
       const int p = ci < 0 ? c : ci;
-      if (t < w) {
-        const int gw = p / RPW;
-        s_u[t] = (p == c) ? ld_agent(&top_row[par * PW + t])
-                          : ld_agent(&slot_row[(par * G + gw) * PW + t]);
-        s_top[t] = ld_agent(&top_row[par * PW + t]);
-      }
+      if (t < PW) s_u[t] = (p == c) ? s_top[t] : (T)s_cand[p / RPW][t];
       if (t == 0) s_p = p;
-      (void)owner_c;
       __syncthreads();
     }
 
     const int p = s_p;
+    if (t == 0) s_piv[c] = p;
     if (g == 0 && t == 0) {
       ipiv[j0 + c] = j0 + p;
-      if (fabs(s_u[c]) <= tiny && *info == 0) *info = j0 + c + 1;
+      if (fabs(s_u[0]) <= tiny && *info == 0) *info = j0 + c + 1;
     }
     // ---- row exchange inside the strip ----
     if (p != c) {
@@ -176,22 +208,27 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
       }
     }
     // ---- multipliers (division) and rank-1 update of columns > c ----
-    const T piv = s_u[c];
+    const T piv = s_u[0];
+    const int rest = w - c;  // active column plus the ones still to come
     if (piv != (T)0) {
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
         const int li = g * RPW + r * 256 + t;
         if (li < m && li > c) {
-          const T l = a[r][c] / piv;
-          a[r][c] = l;
+          const T l = a[r][0] / piv;
+          a[r][0] = l;
 #pragma clang loop unroll(full)
-          for (int q = c + 1; q < PW; ++q) a[r][q] -= l * s_u[q];
+          for (int q = 1; q < PW; ++q)
+            if (q < rest) a[r][q] -= l * s_u[q];
         }
       }
     }
+    rotate();
     __syncthreads();  // s_u / s_top / s_val reused by the next column
     }
   }
+#pragma clang loop unroll(disable)
+  for (int c = w; c < PW; ++c) rotate();
 
 #pragma clang loop unroll(full)
   for (int r = 0; r < R; ++r) {
@@ -202,108 +239,132 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
         if (q < w) A[(long)(j0 + li) * lda + j0 + q] = a[r][q];
     }
   }
+
+  // ---- the strip's interchanges on the other columns of the panel
+  //      (the reference exchanges whole b-wide block rows, lu.C:166-177) ----
+  // Each candidate position (the strip's rows and their pivot rows) finds
+  // the row that ends up there by walking the swaps backwards.
+  __shared__ int s_pos[2 * PW], s_src[2 * PW];
+  __shared__ int s_nmv;
+  if (t < 2 * PW) {
+    int x = -1;
+    if (t < w) {
+      x = t;
+    } else if (t >= PW && t - PW < w) {
+      x = s_piv[t - PW];
+      bool dup = x < w;
+      for (int j = 0; j < t - PW; ++j) dup = dup || s_piv[j] == x;
+      if (dup) x = -1;
+    }
+    int y = x;
+    for (int j = w - 1; j >= 0 && x >= 0; --j) {
+      const int pj = s_piv[j];
+      y = (y == j) ? pj : (y == pj ? j : y);
+    }
+    const bool mv = x >= 0 && y != x;
+    const unsigned long long mask = __ballot(mv);  // the 32 candidates are in wave 0
+    if (mv) {
+      const int off = __popcll(mask & ((1ull << t) - 1ull));
+      s_pos[off] = x;
+      s_src[off] = y;
+    }
+    if (t == 0) s_nmv = __popcll(mask);
+  }
+  __syncthreads();
+  const int nmv = s_nmv;
+  const int left = j0 - pc0, right = pc1 - (j0 + w);
+  // columns of the rest of the panel, spread over all threads of the grid
+  for (int k = g * 256 + t; nmv > 0 && k < left + right; k += G * 256) {
+    const int col = k < left ? pc0 + k : j0 + w + (k - left);
+    T v[2 * PW];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 2 * PW; ++i)
+      if (i < nmv) v[i] = A[(long)(j0 + s_src[i]) * lda + col];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 2 * PW; ++i)
+      if (i < nmv) A[(long)(j0 + s_pos[i]) * lda + col] = v[i];
+  }
 }
 
 template <typename T>
-int panel_base(T *A, long lda, int j0, int m_end, int w, int *ipiv, PanelWork &pw,
-               hipStream_t s) {
+int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *ipiv,
+               PanelWork &pw, hipStream_t s) {
   const int m = m_end - j0;
   if (m <= 0 || w <= 0) return CLU_OK;
   if (w > PW) return -1;
   int R, G;
   if (m <= 256) { R = 1; G = 1; }
   else if (m <= 512) { R = 2; G = 1; }
-  else if (m <= 1024) { R = 4; G = 1; }
-  else { R = 8; G = (m + 2047) / 2048; }
-  if (G > pw.max_groups) return -2;
-  T *sv = reinterpret_cast<T *>(pw.slots);
-  int *si = reinterpret_cast<int *>(sv + 2 * pw.max_groups);
-  T *sr = reinterpret_cast<T *>(si + 2 * pw.max_groups);
-  T *tr = reinterpret_cast<T *>(pw.top);
-  // the slot arrays are laid out for max_groups; index them with G
+  else { R = 4; G = (m + 1023) / 1024; }
+  if (G > pw.max_groups || G > 256) return -2;
+  PanelRec *recs = reinterpret_cast<PanelRec *>(pw.slots);
+  const unsigned seq0 = pw.epoch;
   switch (R) {
-    case 1: hipLaunchKernelGGL((panel_base_kernel<T, 1>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, ipiv, sv, si, sr, tr, pw.counter, pw.epoch, pw.info); break;
-    case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, ipiv, sv, si, sr, tr, pw.counter, pw.epoch, pw.info); break;
-    case 4: hipLaunchKernelGGL((panel_base_kernel<T, 4>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, ipiv, sv, si, sr, tr, pw.counter, pw.epoch, pw.info); break;
-    default: hipLaunchKernelGGL((panel_base_kernel<T, 8>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, ipiv, sv, si, sr, tr, pw.counter, pw.epoch, pw.info); break;
+    case 1: hipLaunchKernelGGL((panel_base_kernel<T, 1>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+    case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+    default: hipLaunchKernelGGL((panel_base_kernel<T, 4>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
   }
   CLU_LAUNCHED("panel_base_kernel");
-  if (G > 1) pw.epoch += (unsigned)G * (unsigned)w;
+  pw.epoch += (unsigned)w;
   return CLU_OK;
 }
 
-template int panel_base<double>(double *, long, int, int, int, int *, PanelWork &, hipStream_t);
-template int panel_base<float>(float *, long, int, int, int, int *, PanelWork &, hipStream_t);
+template int panel_base<double>(double *, long, int, int, int, int, int, int *, PanelWork &,
+                                hipStream_t);
+template int panel_base<float>(float *, long, int, int, int, int, int, int *, PanelWork &,
+                               hipStream_t);
 
 // ---------------------------------------------------------------------------
 // swaps -> moves
 // ---------------------------------------------------------------------------
 constexpr int MAXRUN = 256;
 
-__global__ __launch_bounds__(64) void pivots_to_moves_kernel(
+// 2*MAXRUN threads: thread i < nk takes position k1+i, thread MAXRUN+i the
+// pivot row ipiv[k1+i] if it lies below the run and appears there first.
+// Each walks the swaps backwards to find the row that ends up at its
+// position; the moved positions are compacted in thread order.
+__global__ __launch_bounds__(2 * MAXRUN) void pivots_to_moves_kernel(
     const int *__restrict__ ipiv, int k1, int k2, int2 *__restrict__ moves,
     int *__restrict__ count) {
-  __shared__ int cur[MAXRUN];
-  __shared__ int ext_pos[MAXRUN];
-  __shared__ int ext_src[MAXRUN];
-  const int lane = threadIdx.x;
+  __shared__ int piv[MAXRUN];
+  __shared__ int wave_cnt[2 * MAXRUN / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int nk = k2 - k1;
-  for (int i = lane; i < nk; i += 64) cur[i] = k1 + i;
-  int n_ext = 0;
+  if (t < nk) piv[t] = ipiv[k1 + t];
   __syncthreads();
-  for (int j = k1; j < k2; ++j) {
-    const int p = ipiv[j];
-    if (p == j) continue;
-    if (p < k2) {
-      if (lane == 0) {
-        const int tmp = cur[j - k1];
-        cur[j - k1] = cur[p - k1];
-        cur[p - k1] = tmp;
-      }
-    } else {
-      int found = -1;
-      for (int base = 0; base < n_ext; base += 64) {
-        const int e = base + lane;
-        const bool hit = e < n_ext && ext_pos[e] == p;
-        const unsigned long long mask = __ballot(hit);
-        if (mask) { found = base + __ffsll((long long)mask) - 1; break; }
-      }
-      if (found < 0) {
-        found = n_ext;
-        if (lane == 0) { ext_pos[found] = p; ext_src[found] = p; }
-        ++n_ext;
-      }
-      __syncthreads();
-      if (lane == 0) {
-        const int tmp = cur[j - k1];
-        cur[j - k1] = ext_src[found];
-        ext_src[found] = tmp;
-      }
-    }
-    __syncthreads();
+  int x = -1;
+  if (t < nk) {
+    x = k1 + t;
+  } else if (t >= MAXRUN && t - MAXRUN < nk) {
+    const int i = t - MAXRUN;
+    x = piv[i];
+    bool dup = x < k2;
+    for (int j = 0; j < i && !dup; ++j) dup = piv[j] == x;
+    if (dup) x = -1;
   }
-  // emit moves: positions whose content changed
+  int y = x;
+  for (int j = nk - 1; j >= 0; --j) {
+    const int pj = piv[j], jj = k1 + j;
+    y = (y == jj) ? pj : (y == pj ? jj : y);
+  }
+  const bool mv = x >= 0 && y != x;
+  const unsigned long long mask = __ballot(mv);
+  if (lane == 0) wave_cnt[wave] = __popcll(mask);
+  __syncthreads();
   int base = 0;
-  for (int i0 = 0; i0 < nk + n_ext; i0 += 64) {
-    const int i = i0 + lane;
-    int dst = -1, src = -1;
-    if (i < nk) { dst = k1 + i; src = cur[i]; }
-    else if (i < nk + n_ext) { dst = ext_pos[i - nk]; src = ext_src[i - nk]; }
-    const bool mv = dst >= 0 && dst != src;
-    const unsigned long long mask = __ballot(mv);
-    if (mv) {
-      const int off = __popcll(mask & ((1ull << lane) - 1ull));
-      moves[base + off] = make_int2(dst, src);
-    }
-    base += __popcll(mask);
+  for (int k = 0; k < wave; ++k) base += wave_cnt[k];
+  if (mv) moves[base + __popcll(mask & ((1ull << lane) - 1ull))] = make_int2(x, y);
+  if (t == 0) {
+    int tot = 0;
+    for (int k = 0; k < 2 * MAXRUN / 64; ++k) tot += wave_cnt[k];
+    *count = tot;
   }
-  if (lane == 0) *count = base;
 }
 
 int pivots_to_moves(const int *ipiv, int k1, int k2, int2 *moves, int *count,
                     hipStream_t s) {
   if (k2 - k1 > MAXRUN || k2 < k1) return -1;
-  hipLaunchKernelGGL(pivots_to_moves_kernel, dim3(1), dim3(64), 0, s, ipiv, k1, k2,
+  hipLaunchKernelGGL(pivots_to_moves_kernel, dim3(1), dim3(2 * MAXRUN), 0, s, ipiv, k1, k2,
                      moves, count);
   CLU_LAUNCHED("pivots_to_moves_kernel");
   return CLU_OK;

@@ -7,11 +7,8 @@
 // (backwardSolve, lu.ci:408-436); llnu also serves the forward solve
 // (forwardSolve, lu.ci:387-407).  The transposed forms serve A^T x = b.
 //
-// One wave64 owns 64 right-hand-side columns and keeps them in LDS
-// (M x 64 values, M <= 256); each lane runs the substitution for its own
-// column, reading the triangular factor with wave-uniform (scalar) loads.
-// Each dot product is split over four independent partial sums so the FMA
-// latency overlaps; they are combined in a fixed order.
+// The right-hand sides are solved in 32-column slabs held in LDS, 16 rows
+// of the triangle at a time (see trsm_kernel).
 //
 // gemv_sub is the off-diagonal part of the block solve (offDiagSolve,
 // lu.ci:439-456): y -= A x for a tall block column, 16 lanes per row.
@@ -20,38 +17,93 @@
 
 namespace clu {
 
-// op(T)[i][k] = TR ? T[k][i] : T[i][k]; OPLOW: op(T) is lower triangular
+// op(F)[i][k] = TR ? F[k][i] : F[i][k]; OPLOW: op(F) is lower triangular
 // (forward substitution), else upper (backward).  UNIT: unit diagonal.
+//
+// One 256-thread workgroup owns NC = 32 right-hand-side columns, kept in
+// LDS for the whole solve (M x 32 values, M <= 256).  The rows are taken in
+// blocks of 16 in solve order; for each block the 16 rows of op(F) are
+// staged in LDS, the contributions of every row solved so far are
+// subtracted (8 row groups x 32 columns of threads, 2 rows each), and the
+// 16 x 16 diagonal block is solved by one thread per column.
 template <typename T, bool OPLOW, bool TR, bool UNIT>
-__global__ __launch_bounds__(64) void trsm_kernel(int M, int N, T alpha, const T *__restrict__ F,
-                                                  long ldf, T *__restrict__ B, long ldb) {
+__global__ __launch_bounds__(256) void trsm_kernel(int M, int N, T alpha, const T *__restrict__ F,
+                                                   long ldf, T *__restrict__ B, long ldb) {
+  constexpr int NC = 32, BS = 16;
   extern __shared__ unsigned char smem_raw[];
-  T *X = reinterpret_cast<T *>(smem_raw);  // [M][64]
-  const int lane = threadIdx.x;
-  const int col = blockIdx.x * 64 + lane;
-  const bool ok = col < N;
-  for (int i = 0; i < M; ++i) X[i * 64 + lane] = ok ? alpha * B[(long)i * ldb + col] : (T)0;
-  __syncthreads();
-  for (int ii = 0; ii < M; ++ii) {
-    const int i = OPLOW ? ii : M - 1 - ii;
-    T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    const int k_lo = OPLOW ? 0 : i + 1;
-    const int k_hi = OPLOW ? i : M;
-    auto f = [&](int kk) -> T { return TR ? F[(long)kk * ldf + i] : F[(long)i * ldf + kk]; };
-    int k = k_lo;
-    for (; k + 3 < k_hi; k += 4) {
-      s0 += f(k) * X[k * 64 + lane];
-      s1 += f(k + 1) * X[(k + 1) * 64 + lane];
-      s2 += f(k + 2) * X[(k + 2) * 64 + lane];
-      s3 += f(k + 3) * X[(k + 3) * 64 + lane];
-    }
-    for (; k < k_hi; ++k) s0 += f(k) * X[k * 64 + lane];
-    T x = X[i * 64 + lane] - ((s0 + s1) + (s2 + s3));
-    if (!UNIT) x = x / F[(long)i * ldf + i];
-    X[i * 64 + lane] = x;
+  T *X = reinterpret_cast<T *>(smem_raw);  // [M][NC]
+  T *Fb = X + (size_t)M * NC;              // [BS][M]
+  const int t = threadIdx.x;
+  const int c = t % NC, rg = t / NC;  // rg in 0..7: rows 2*rg, 2*rg+1 of a block
+  const int col0 = blockIdx.x * NC;
+  for (int e = t; e < M * NC; e += 256) {
+    const int i = e / NC, cc = e % NC, gc = col0 + cc;
+    X[e] = gc < N ? alpha * B[(long)i * ldb + gc] : (T)0;
   }
-  if (ok)
-    for (int i = 0; i < M; ++i) B[(long)i * ldb + col] = X[i * 64 + lane];
+  const int nblk = (M + BS - 1) / BS;
+  for (int sidx = 0; sidx < nblk; ++sidx) {
+    const int blk = OPLOW ? sidx : nblk - 1 - sidx;
+    const int r0 = blk * BS;
+    const int rows = M - r0 < BS ? M - r0 : BS;
+    // columns of op(F) this block needs: everything solved before plus
+    // the diagonal block itself
+    const int k_lo = OPLOW ? 0 : r0;
+    const int k_hi = OPLOW ? r0 + rows : M;
+    const int kn = k_hi - k_lo;
+    __syncthreads();  // X loads / previous block's solve done before Fb is rewritten
+    if (!TR) {
+      for (int e = t; e < BS * kn; e += 256) {
+        const int rr = e / kn, k = k_lo + e % kn;
+        Fb[rr * M + k] = rr < rows ? F[(long)(r0 + rr) * ldf + k] : (T)0;
+      }
+    } else {
+      for (int e = t; e < BS * kn; e += 256) {
+        const int k = k_lo + e / BS, rr = e % BS;
+        Fb[rr * M + k] = rr < rows ? F[(long)k * ldf + r0 + rr] : (T)0;
+      }
+    }
+    __syncthreads();
+    // contributions of the rows solved so far
+    const int u_lo = OPLOW ? 0 : r0 + rows;
+    const int u_hi = OPLOW ? r0 : M;
+    const int ra = 2 * rg, rb = 2 * rg + 1;
+    T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int k = u_lo;
+    for (; k + 1 < u_hi; k += 2) {
+      const T x0 = X[k * NC + c], x1 = X[(k + 1) * NC + c];
+      s0 += Fb[ra * M + k] * x0;
+      s1 += Fb[rb * M + k] * x0;
+      s2 += Fb[ra * M + k + 1] * x1;
+      s3 += Fb[rb * M + k + 1] * x1;
+    }
+    if (k < u_hi) {
+      const T x0 = X[k * NC + c];
+      s0 += Fb[ra * M + k] * x0;
+      s1 += Fb[rb * M + k] * x0;
+    }
+    if (ra < rows) X[(r0 + ra) * NC + c] -= s0 + s2;
+    if (rb < rows) X[(r0 + rb) * NC + c] -= s1 + s3;
+    __syncthreads();
+    // the diagonal block, one thread per column
+    if (t < NC) {
+      for (int ii = 0; ii < rows; ++ii) {
+        const int i = OPLOW ? ii : rows - 1 - ii;
+        T x = X[(r0 + i) * NC + t];
+        if (OPLOW) {
+          for (int kk = 0; kk < i; ++kk) x -= Fb[i * M + r0 + kk] * X[(r0 + kk) * NC + t];
+        } else {
+          for (int kk = i + 1; kk < rows; ++kk) x -= Fb[i * M + r0 + kk] * X[(r0 + kk) * NC + t];
+        }
+        if (!UNIT) x = x / Fb[i * M + r0 + i];
+        X[(r0 + i) * NC + t] = x;
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < M * NC; e += 256) {
+    const int i = e / NC, cc = e % NC, gc = col0 + cc;
+    if (gc < N) B[(long)i * ldb + gc] = X[e];
+  }
 }
 
 template <typename T, bool OPLOW, bool TR, bool UNIT>
@@ -59,8 +111,8 @@ static int trsm_launch(int M, int N, T alpha, const T *F, long ldf, T *B, long l
                        hipStream_t s) {
   if (M <= 0 || N <= 0) return CLU_OK;
   if (M > 256) return -1;
-  const size_t shm = (size_t)M * 64 * sizeof(T);
-  hipLaunchKernelGGL((trsm_kernel<T, OPLOW, TR, UNIT>), dim3((N + 63) / 64), dim3(64), shm, s,
+  const size_t shm = (size_t)M * 32 * sizeof(T) + (size_t)16 * M * sizeof(T);
+  hipLaunchKernelGGL((trsm_kernel<T, OPLOW, TR, UNIT>), dim3((N + 31) / 32), dim3(256), shm, s,
                      M, N, alpha, F, ldf, B, ldb);
   CLU_LAUNCHED("trsm_kernel");
   return CLU_OK;
