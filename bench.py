@@ -11,7 +11,7 @@ Validation (the reference's scaled residual, benchmark.C:385-400) runs after
 the timed region, as in the reference.
 
 The JSON line also carries:
-  roofline      the trailing-update GEMM (dgemm_mfma_kernel, 128x128 tiles):
+  roofline      the trailing-update GEMM (dgemm_nn_lds_kernel, 128x128 tiles):
                 algorithmic flops of its launches / their summed duration,
                 both from HIP events the library records around each launch
                 on the stream it runs on (charmlu_profile), against the FP64
@@ -61,6 +61,40 @@ print(json.dumps(dict(seconds=r['seconds'], residual=r['residual'], mkl=mkl)))
                        f"{rec['seconds']:.2f} s, residual {rec['residual']:.3f}")}
 
 
+def gemm_only(lib, _lib, torch, n, nb, steps, warmup):
+    """The step-0 trailing update C -= L U (M = N = n - nb, K = nb) on the
+    factorization's own row-major layout, timed with events on the stream."""
+    import ctypes
+    m = n - nb
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    A.uniform_(-0.5, 0.5)
+    base = A.data_ptr()
+    L = ctypes.c_void_p(base + (nb * n) * 8)
+    U = ctypes.c_void_p(base + nb * 8)
+    C = ctypes.c_void_p(base + (nb * n + nb) * 8)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def run():
+        _lib.check(lib.charmlu_dgemm_dev(_lib.NO_TRANS, _lib.NO_TRANS, m, m, nb, -1.0, L, n, U, n,
+                                         1.0, C, n, stream), "charmlu_dgemm_dev")
+    for _ in range(warmup):
+        run()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    flops = 2.0 * m * m * nb
+    print(json.dumps({"gemm": f"{m}x{m}x{nb}", "ms": round(ms, 4),
+                      "tflops": round(flops / ms / 1e9, 3),
+                      "frac_fp64_peak": round(flops / ms / 1e9 / FP64_PEAK, 4),
+                      "algorithmic_bytes": 8 * (2 * m * m + 2 * m * nb)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +105,10 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=12288)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="time only the trailing-update GEMM of step 0 (M=N=n-nb, K=nb)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run a plain device copy of MIB MiB (calibrates PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the LU runs "
@@ -87,6 +125,16 @@ def main():
     lib = _lib.load()
     _lib.check(lib.charmlu_init(0), "charmlu_init")
     n, nb = args.n, args.nb
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8}))
+        return
+    if args.gemm_only:
+        gemm_only(lib, _lib, torch, n, nb, args.steps, args.warmup)
+        return
     A0 = clu.gen_matrix(n)
     b0 = clu.gen_rhs(n)
     A = torch.empty_like(A0)
@@ -163,13 +211,13 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK, 4),
             "traffic": None,
-            "kernel": "dgemm_mfma_kernel<128,128,2,2,false,false> (trailing update)",
+            "kernel": "dgemm_nn_lds_kernel<4> (trailing update, 128x128 tiles)",
             "launches": int(g_cnt),
             "avg_launch_ms": round(g_ms / g_cnt, 4) if g_cnt else None,
             "flops_per_launch": g_work / g_cnt if g_cnt else None,
         },
         "phases_ms_per_step": {
-            "gemm_128x128": round(g_ms / args.steps, 2),
+            "gemm_trailing_lds": round(g_ms / args.steps, 2),
             "gemm_other": round(p[5] / args.steps, 2),
             "panel_stream": round(pan_ms / args.steps, 2),
             "row_interchanges": round(sw_ms / args.steps, 2),

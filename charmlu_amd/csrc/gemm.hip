@@ -43,6 +43,57 @@ CLU_DEV void tile_coords(int bid, int nblk, int tiles_m, int tiles_n, int &tm,
   tn = in_group / gm;
 }
 
+// C = alpha * acc + beta * C for one wave's TM x TN blocks of 16x16.  The
+// beta test is uniform; with beta != 0 the C values of MFMA row block i+1
+// are requested before block i is combined and stored, so the epilogue costs
+// about one memory round trip instead of one per element.
+template <int TM, int TN>
+CLU_DEV void store_tile(const double4_t (&acc)[TM][TN], double *__restrict__ C, long ldc,
+                        int r0, int c0, int lr, int lq, double alpha, double beta, int M, int N,
+                        bool full) {
+  auto inb = [&](int row, int col) { return full || (row < M && col < N); };
+  if (beta == 0.0) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + i * 16 + lq + 4 * v, col = c0 + j * 16 + lr;
+          if (inb(row, col)) C[(long)row * ldc + col] = alpha * acc[i][j][v];
+        }
+    return;
+  }
+  double cur[TN][4], nxt[TN][4];
+  auto fetch = [&](int i, double (&dst)[TN][4]) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + i * 16 + lq + 4 * v, col = c0 + j * 16 + lr;
+        dst[j][v] = inb(row, col) ? C[(long)row * ldc + col] : 0.0;
+      }
+  };
+  fetch(0, cur);
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    if (i + 1 < TM) fetch(i + 1, nxt);
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + i * 16 + lq + 4 * v, col = c0 + j * 16 + lr;
+        if (inb(row, col)) C[(long)row * ldc + col] = alpha * acc[i][j][v] + beta * cur[j][v];
+      }
+    if (i + 1 < TM) {
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cur[j][v] = nxt[j][v];
+    }
+  }
+}
+
 template <int BM, int BN, int WM, int WN, bool TA, bool TB>
 __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
     int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
@@ -137,22 +188,125 @@ __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
     __syncthreads();
   }
 
-  // epilogue: C = alpha * acc + beta * C
+  const bool full = m0 + BM <= M && n0 + BN <= N;
+  store_tile<TM, TN>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, M, N, full);
+}
+
+// ---------------------------------------------------------------------------
+// Aligned full-tile NN kernel: 128x128 tiles, K in stages of 8 loaded with
+// global_load_lds_dwordx4 straight into LDS (no register staging), NST
+// stages in flight, one barrier per stage.  Used when M and N are multiples
+// of 128, K of 8, and every row start is 16-byte aligned -- the case of the
+// LU trailing update.
+//
+// LDS layout per stage (16 KB): A as 128 rows x 4 chunks of 16 bytes, the
+// chunk index XORed with (row >> 2) & 3; B as 8 rows x 64 chunks, the chunk
+// index XORed with (k & 1) << 3.  A direct-to-LDS load writes lane i's 16
+// bytes at base + 16 i, so the swizzle is applied by choosing which global
+// chunk each lane fetches.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void *lds_ptr_t;
+typedef __attribute__((address_space(1))) void *gbl_ptr_t;
+
+CLU_DEV void load16_to_lds(const double *g, void *lds_base) {
+  __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, (lds_ptr_t)lds_base, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(N) (N < 16) leaving the other counters unconstrained
+template <int N>
+CLU_DEV void wait_vm() {
+  static_assert(N >= 0 && N < 16, "vmcnt");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
+}
+
+template <int NST>
+__global__ __launch_bounds__(256, 2) void dgemm_nn_lds_kernel(
+    int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
+    const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
+    int tiles_m, int tiles_n) {
+  constexpr int BM = 128, BN = 128, BK = 8;
+  constexpr int STAGE = (BM * BK + BK * BN) * 8;  // bytes
+  __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
+
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
+  const int lr = lane & 15, lq = lane >> 4;
+
+  // global source of this thread's chunks (two of A, two of B per stage)
+  const double *ga[2], *gb[2];
+  int la_off[2], lb_off[2];
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int r = 0; r < 2; ++r) {
+    const int c = wave * 64 + 256 * r + lane;  // LDS chunk slot
+    const int am = c >> 2, akq = (c & 3) ^ ((am >> 2) & 3);
+    ga[r] = A + (long)(m0 + am) * lda + 2 * akq;
+    la_off[r] = (wave * 64 + 256 * r) * 16;  // wave-uniform slot base
+    const int bk = c >> 6, bnq = (c & 63) ^ ((bk & 1) << 3);
+    gb[r] = B + (long)bk * ldb + n0 + 2 * bnq;
+    lb_off[r] = BM * BK * 8 + (wave * 64 + 256 * r) * 16;
+  }
+  const int nk = K / BK;
+  auto issue = [&](int kc) {
+    const int kk = (kc < nk ? kc : nk - 1) * BK;  // past the end: reload the last stage
+    char *st = smem + (kc % NST) * STAGE;
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int col = n0 + wn0 + j * 16 + lr;
+    for (int r = 0; r < 2; ++r) load16_to_lds(ga[r] + kk, st + la_off[r]);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm0 + i * 16 + lq + 4 * v;
-        if (row < M && col < N) {
-          double *c = C + (long)row * ldc + col;
-          const double prod = alpha * acc[i][j][v];
-          *c = beta == 0.0 ? prod : prod + beta * *c;
-        }
-      }
+    for (int r = 0; r < 2; ++r) load16_to_lds(gb[r] + (long)kk * ldb, st + lb_off[r]);
+  };
+
+  // per-lane fragment offsets inside a stage
+  int aoff[2][4], boff[2][4];
+#pragma unroll
+  for (int s2 = 0; s2 < 2; ++s2) {
+    const int k = 4 * s2 + lq;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = wm0 + 16 * i + lr;
+      aoff[s2][i] = m * 64 + (((k >> 1) ^ ((m >> 2) & 3)) << 4) + ((k & 1) << 3);
+      const int n = wn0 + 16 * i + lr;
+      boff[s2][i] = BM * BK * 8 + k * 1024 + (((n >> 1) ^ ((k & 1) << 3)) << 4) + ((n & 1) << 3);
     }
+  }
+
+  double4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+#pragma unroll
+  for (int st = 0; st < NST - 1; ++st) issue(st);
+  for (int kc = 0; kc < nk; ++kc) {
+    // This thread's loads of stage kc have landed; after the barrier every
+    // thread's have, and every wave is done reading stage kc-1, whose buffer
+    // the next issue overwrites.  A bare s_barrier: __syncthreads() would
+    // also drain the direct-to-LDS loads of the later stages.
+    wait_vm<4 * (NST - 2)>();
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    issue(kc + NST - 1);
+    const char *st = smem + (kc % NST) * STAGE;
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const double *>(st + aoff[s2][i]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const double *>(st + boff[s2][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  wait_vm<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
+  store_tile<4, 4>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, M, N, true);
 }
 
 template <int BM, int BN, int WM, int WN, bool TA, bool TB>
@@ -169,10 +323,24 @@ static int launch(int M, int N, int K, double alpha, const double *A, long lda,
   return CLU_OK;
 }
 
+static bool lds_path_ok(int M, int N, int K, const double *A, long lda, const double *B,
+                        long ldb) {
+  return M % 128 == 0 && N % 128 == 0 && K % 8 == 0 && K > 0 && lda % 2 == 0 && ldb % 2 == 0 &&
+         ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0 &&
+         (long)(M / 128) * (N / 128) >= 256;
+}
+
 template <bool TA, bool TB>
 static int dispatch(int M, int N, int K, double alpha, const double *A, long lda,
                     const double *B, long ldb, double beta, double *C, long ldc,
                     hipStream_t s) {
+  if (!TA && !TB && lds_path_ok(M, N, K, A, lda, B, ldb)) {
+    const int tiles_m = M / 128, tiles_n = N / 128;
+    hipLaunchKernelGGL((dgemm_nn_lds_kernel<4>), dim3(tiles_m * tiles_n), dim3(256), 0, s, M, N,
+                       K, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+    CLU_LAUNCHED("dgemm_nn_lds_kernel");
+    return CLU_OK;
+  }
   // narrow right-hand sides (panel-internal updates, multi-RHS solves)
   if (N <= 32)
     return launch<256, 32, 4, 1, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
@@ -186,12 +354,14 @@ static int dispatch(int M, int N, int K, double alpha, const double *A, long lda
   return launch<128, 128, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
-int dgemm_variant(int M, int N) {
+int dgemm_variant(bool ta, bool tb, int M, int N, int K, const double *A, long lda,
+                  const double *B, long ldb) {
+  if (!ta && !tb && lds_path_ok(M, N, K, A, lda, B, ldb)) return 0;
   if (N <= 32) return 1;
   if (N <= 64) return 4;
   if (M <= 32) return 3;
   if ((long)((M + 127) / 128) * ((N + 127) / 128) < 512) return 4;
-  return 0;
+  return 2;
 }
 
 int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A,

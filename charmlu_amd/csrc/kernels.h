@@ -15,9 +15,11 @@ int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A,
           long lda, const double *B, long ldb, double beta, double *C, long ldc,
           hipStream_t s);
 
-// Tile configuration dgemm() picks for an M x N product: 0 = 128x128 (the
-// trailing-update kernel), 1 = 256x32, 3 = 32x256, 4 = 64x64.
-int dgemm_variant(int M, int N);
+// Kernel dgemm() launches for these operands: 0 = the direct-to-LDS 128x128
+// NN kernel (the trailing update), 1 = 256x32, 2 = 128x128, 3 = 32x256,
+// 4 = 64x64 register-staged kernels.
+int dgemm_variant(bool ta, bool tb, int M, int N, int K, const double *A, long lda,
+                  const double *B, long ldb);
 
 // ---- pivoting -------------------------------------------------------------
 // Row move list of a run of sequential swaps ipiv[k1..k2) (global rows):
@@ -90,6 +92,7 @@ int residual(int n, const double *A, long lda, const double *x, const double *b,
 template <typename TD, typename TS>
 int convert(long m, long n, const TS *src, long lds, TD *dst, long ldd, hipStream_t s);
 int fill_int(int *p, int v, long n, hipStream_t s);
+int set_identity(double *A, long n, long lda, hipStream_t s);
 int fill_u32(unsigned *p, unsigned v, long n, hipStream_t s);
 
 }  // namespace clu

@@ -72,6 +72,10 @@ struct Workspace {
   // host-pointer shim staging
   void *stage = nullptr;
   size_t stage_bytes = 0;
+  // U-row solve by inverse: inv(L11) (nb x nb) and the U row block (nb x n)
+  double *winv = nullptr;
+  double *ubuf = nullptr;
+  size_t winv_elems = 0, ubuf_elems = 0;
   PanelWork pw{};
 };
 
@@ -190,6 +194,19 @@ static int ensure_stage(Workspace &w, size_t bytes) {
     return CLU_ERR_NOMEM;
   }
   w.stage_bytes = b;
+  return CLU_OK;
+}
+
+static int ensure_dbuf(double **p, size_t *have, size_t need) {
+  if (need <= *have) return CLU_OK;
+  if (*p) CLU_HIP(hipFree(*p));
+  *p = nullptr;
+  *have = 0;
+  if (hipMalloc(p, need * sizeof(double)) != hipSuccess) {
+    set_error("cannot allocate %zu bytes of workspace", need * sizeof(double));
+    return CLU_ERR_NOMEM;
+  }
+  *have = need;
   return CLU_OK;
 }
 
@@ -557,9 +574,16 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   // all work runs on the library's two streams, fenced to the caller's
   hipStream_t sm = ws.s_main;
   const int nruns = (nb + RUN - 1) / RUN;
+  // inv(L11) * A12 on the matrix cores replaces the substitution when the
+  // diagonal block fits one triangular-solve launch
+  const bool use_inv = nb <= 256;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_runs(ws, nruns))) return rc;
+    if (use_inv) {
+      if ((rc = ensure_dbuf(&ws.winv, &ws.winv_elems, (size_t)nb * nb))) return rc;
+      if ((rc = ensure_dbuf(&ws.ubuf, &ws.ubuf_elems, (size_t)nb * n))) return rc;
+    }
   }
   hipStream_t sp = ws.s_panel;
   ws.pw.info = dinfo;
@@ -576,14 +600,22 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     int r = factor_panel_d(dA, lda, n, r0, nb, r0, nb, dipiv, ws, sp);
     if (r) return r;
     r = build_moves(dipiv, r0, r0 + nb, moves_of(t), counts_of(t), sp);
+    if (r) return r;
+    if (use_inv && r0 + nb < n) {
+      // inv(L11): solve L11 X = I (unit lower), off the update stream's path
+      if ((r = set_identity(ws.winv, nb, nb, sp))) return r;
+      r = trsm_llnu<double>(nb, nb, dA + (long)r0 * lda + r0, lda, ws.winv, nb, sp);
+    }
     prof_end(pid, sp);
     return r;
   };
   // trailing-update GEMM, timed per launch when profiling is on
-  auto update = [&](int M, int N, const double *L, const double *U, double *C) -> int {
-    const int pid = prof_begin(dgemm_variant(M, N) == 0 ? PK_GEMM_BIG : PK_GEMM_OTHER,
-                               2.0 * M * N * nb, sm);
-    int r = dgemm(false, false, M, N, nb, -1.0, L, lda, U, lda, 1.0, C, lda, sm);
+  auto update = [&](int M, int N, const double *L, const double *U, long ldu,
+                    double *C) -> int {
+    const int pid = prof_begin(
+        dgemm_variant(false, false, M, N, nb, L, lda, U, ldu) == 0 ? PK_GEMM_BIG : PK_GEMM_OTHER,
+        2.0 * M * N * nb, sm);
+    int r = dgemm(false, false, M, N, nb, -1.0, L, lda, U, ldu, 1.0, C, lda, sm);
     prof_end(pid, sm);
     return r;
   };
@@ -603,11 +635,21 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     double *A11 = dA + (long)r0 * lda + r0;
     // U row: A12 <- L11^{-1} A12 (computeU, lu.C:180-189)
     pid = prof_begin(PK_TRSM, (double)Nt * nb * (nb - 1), sm);
-    if ((rc = trsm_llnu_big<double>(nb, Nt, A11, lda, A11 + nb, lda, sm))) return rc;
+    const double *U = A11 + nb;  // the U row block the trailing update reads
+    long ldu = lda;
+    if (use_inv) {
+      if ((rc = dgemm(false, false, nb, Nt, nb, 1.0, ws.winv, nb, A11 + nb, lda, 0.0, ws.ubuf, Nt,
+                      sm)))
+        return rc;
+      U = ws.ubuf;
+      ldu = Nt;
+    } else if ((rc = trsm_llnu_big<double>(nb, Nt, A11, lda, A11 + nb, lda, sm))) {
+      return rc;
+    }
     prof_end(pid, sm);
     // look-ahead block column, then the panel of step t+1 on the panel stream
     const int la = Nt < nb ? Nt : nb;
-    if ((rc = update(Nt, la, A11 + (long)nb * lda, A11 + nb, A11 + (long)nb * lda + nb)))
+    if ((rc = update(Nt, la, A11 + (long)nb * lda, U, ldu, A11 + (long)nb * lda + nb)))
       return rc;
     CLU_HIP(hipEventRecord(ws.ev_la, sm));
     CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
@@ -615,9 +657,12 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     CLU_HIP(hipEventRecord(ws.ev_panel, sp));
     // the rest of the trailing update (updateMatrix, lu.C:26-43)
     if (Nt > la)
-      if ((rc = update(Nt, Nt - la, A11 + (long)nb * lda, A11 + nb + la,
+      if ((rc = update(Nt, Nt - la, A11 + (long)nb * lda, U + la, ldu,
                        A11 + (long)nb * lda + nb + la)))
         return rc;
+    // the U row block into the factor storage (nothing reads A12 before)
+    if (use_inv && (rc = convert<double, double>(nb, Nt, ws.ubuf, Nt, A11 + nb, lda, sm)))
+      return rc;
   }
   CLU_HIP(hipEventRecord(ws.ev_end, sm));
   CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));

@@ -175,6 +175,19 @@ template int convert<double, float>(long, long, const float *, long, double *, l
 template int convert<double, double>(long, long, const double *, long, double *, long, hipStream_t);
 template int convert<float, float>(long, long, const float *, long, float *, long, hipStream_t);
 
+__global__ void set_identity_kernel(double *A, long n, long lda) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n * n) A[(e / n) * lda + e % n] = (e / n == e % n) ? 1.0 : 0.0;
+}
+
+int set_identity(double *A, long n, long lda, hipStream_t s) {
+  if (n <= 0) return CLU_OK;
+  hipLaunchKernelGGL(set_identity_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, s,
+                     A, n, lda);
+  CLU_LAUNCHED("set_identity_kernel");
+  return CLU_OK;
+}
+
 __global__ void fill_int_kernel(int *p, int v, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

@@ -106,11 +106,76 @@ __global__ __launch_bounds__(256) void trsm_kernel(int M, int N, T alpha, const 
   }
 }
 
+// One right-hand side (the reference's solve has exactly one, lu.ci:387-458):
+// 256 threads as 16 rows x 16 lanes.  Per 16-row block, each row's dot
+// product with the solved part of x is split over its 16 lanes and reduced
+// by shuffles; the 16 x 16 diagonal block is then solved by wave 0 with one
+// shuffle broadcast per row.
+template <typename T, bool OPLOW, bool TR, bool UNIT>
+__global__ __launch_bounds__(256) void trsv_kernel(int M, T alpha, const T *__restrict__ F,
+                                                   long ldf, T *__restrict__ B, long ldb) {
+  __shared__ T x[256];
+  __shared__ T rs[16];  // row sums of the current block
+  const int t = threadIdx.x, r = t >> 4, kl = t & 15;
+  auto f = [&](int i, int k) -> T { return TR ? F[(long)k * ldf + i] : F[(long)i * ldf + k]; };
+  for (int i = t; i < M; i += 256) x[i] = alpha * B[(long)i * ldb];
+  __syncthreads();
+  const int nblk = (M + 15) / 16;
+  for (int sidx = 0; sidx < nblk; ++sidx) {
+    const int blk = OPLOW ? sidx : nblk - 1 - sidx;
+    const int r0 = blk * 16;
+    const int rows = M - r0 < 16 ? M - r0 : 16;
+    const int u_lo = OPLOW ? 0 : r0 + rows;
+    const int u_hi = OPLOW ? r0 : M;
+    T acc0 = 0, acc1 = 0;
+    if (r < rows) {
+      int k = u_lo + kl;
+      for (; k + 16 < u_hi; k += 32) {
+        acc0 += f(r0 + r, k) * x[k];
+        acc1 += f(r0 + r, k + 16) * x[k + 16];
+      }
+      if (k < u_hi) acc0 += f(r0 + r, k) * x[k];
+    }
+    T acc = acc0 + acc1;
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    if (kl == 0 && r < rows) rs[r] = acc;
+    __syncthreads();
+    if (t < 64) {
+      // wave 0: lane i < rows owns row r0 + i of the diagonal block
+      const int i = t;
+      T xi = (i < rows) ? x[r0 + i] - rs[i] : (T)0;
+      T fi[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) fi[j] = (i < rows && j < rows) ? f(r0 + i, r0 + j) : (T)0;
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
+        const int j = OPLOW ? jj : 15 - jj;
+        if (j < rows) {
+          if (!UNIT && i == j) xi = xi / fi[j];
+          const T xj = __shfl(xi, j, 64);
+          const bool later = OPLOW ? (i > j) : (i < j);
+          if (later && i < rows) xi -= fi[j] * xj;
+        }
+      }
+      if (i < rows) x[r0 + i] = xi;
+    }
+    __syncthreads();
+  }
+  for (int i = t; i < M; i += 256) B[(long)i * ldb] = x[i];
+}
+
 template <typename T, bool OPLOW, bool TR, bool UNIT>
 static int trsm_launch(int M, int N, T alpha, const T *F, long ldf, T *B, long ldb,
                        hipStream_t s) {
   if (M <= 0 || N <= 0) return CLU_OK;
   if (M > 256) return -1;
+  if (N == 1) {
+    hipLaunchKernelGGL((trsv_kernel<T, OPLOW, TR, UNIT>), dim3(1), dim3(256), 0, s, M, alpha, F,
+                       ldf, B, ldb);
+    CLU_LAUNCHED("trsv_kernel");
+    return CLU_OK;
+  }
   const size_t shm = (size_t)M * 32 * sizeof(T) + (size_t)16 * M * sizeof(T);
   hipLaunchKernelGGL((trsm_kernel<T, OPLOW, TR, UNIT>), dim3((N + 31) / 32), dim3(256), shm, s,
                      M, N, alpha, F, ldf, B, ldb);

@@ -65,7 +65,7 @@ const char *charmlu_last_error(void);
  * whole run only, benchmark.C:242-245, 343-349).  charmlu_profile(1) starts
  * recording, charmlu_profile(0) stops; charmlu_profile_read fills out[15] =
  * {count, work, milliseconds} for the classes 0 trailing-update GEMM of the
- * 128x128-tile kernel (work = flops), 1 other trailing-update GEMMs
+ * direct-to-LDS 128x128 kernel (work = flops), 1 other trailing-update GEMMs
  * (flops), 2 panels (rows x columns), 3 row interchanges (columns moved per
  * step), 4 U-row solves (flops), then clears the records.  Synchronizes. */
 int charmlu_profile(int on);

@@ -70,7 +70,9 @@ def _gemm_tol(A, B, C, alpha, beta):
 
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (128, 128, 256), (300, 200, 256),
                                    (1000, 64, 256), (64, 1000, 32), (257, 31, 100),
-                                   (2048, 2048, 256), (513, 1031, 77)])
+                                   (2048, 2048, 256), (513, 1031, 77),
+                                   # >= 512 full 128x128 tiles: the direct-to-LDS path
+                                   (4096, 2048, 256), (2816, 3072, 264)])
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
 def test_dgemm_dev(dev, M, N, K, ta, tb):
     from charmlu_amd import _lib
