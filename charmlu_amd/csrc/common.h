@@ -57,20 +57,63 @@ CLU_DEV void wait_all_mem() { __builtin_amdgcn_s_waitcnt(0); }
 // "nothing found" and compares below every real candidate.
 template <typename T>
 CLU_DEV bool better(T av, int ai, T bv, int bi) {
-  // is (av, ai) strictly better than (bv, bi)?
-  if (av > bv) return true;
-  if (av < bv) return false;
-  return ai >= 0 && (bi < 0 || ai < bi);
+  // is (av, ai) strictly better than (bv, bi)?  (branch-free)
+  const bool tie_wins = ai >= 0 && (bi < 0 || ai < bi);
+  return (av > bv) | ((av == bv) & tie_wins);
 }
 
+// One DPP lane move of a 64-bit value (two 32-bit halves); lanes whose
+// source is out of range or whose row is masked off read 0.
+template <int CTRL, int ROW_MASK>
+CLU_DEV double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL, int ROW_MASK>
+CLU_DEV float dpp_move(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, true));
+}
+
+// Wave-wide maximum of non-negative values with DPP lane moves (row_shr 1,
+// 2, 4, 8 leave each row's maximum in its lane 15; row_bcast 15 and 31 carry
+// it across rows to lane 63), then read from lane 63.
+template <typename T>
+CLU_DEV T wave_max_nonneg(T v) {
+  v = fmax(v, dpp_move<0x111, 0xF>(v));  // row_shr:1
+  v = fmax(v, dpp_move<0x112, 0xF>(v));  // row_shr:2
+  v = fmax(v, dpp_move<0x114, 0xF>(v));  // row_shr:4
+  v = fmax(v, dpp_move<0x118, 0xF>(v));  // row_shr:8
+  v = fmax(v, dpp_move<0x142, 0xA>(v));  // row_bcast:15 into rows 1, 3
+  v = fmax(v, dpp_move<0x143, 0xC>(v));  // row_bcast:31 into rows 2, 3
+  if (sizeof(T) == 8) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint((double)v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint((double)v), 63);
+    return (T)__hiloint2double(hi, lo);
+  }
+  return (T)__int_as_float(__builtin_amdgcn_readlane(__float_as_int((float)v), 63));
+}
+
+// Wave-wide argmax of non-negative magnitudes: the maximum by DPP, the
+// lanes holding it by ballot, and only on an exact tie a reduction for the
+// lowest row.  (v, i) = (0, -1) means no candidate.
 template <typename T>
 CLU_DEV void wave_argmax(T &v, int &i) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    T ov = __shfl_xor(v, off, 64);
-    int oi = __shfl_xor(i, off, 64);
-    if (better(ov, oi, v, i)) { v = ov; i = oi; }
+  const T m = wave_max_nonneg(v);
+  const bool mine = i >= 0 && v == m;
+  const unsigned long long b = __ballot(mine);
+  if (b == 0ull) { v = (T)0; i = -1; return; }
+  if (__popcll(b) == 1) {
+    i = __builtin_amdgcn_readlane(i, __ffsll((long long)b) - 1);
+    v = m;
+    return;
   }
+  int ci = mine ? i : 0x7fffffff;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) ci = min(ci, __shfl_xor(ci, off, 64));
+  i = ci;
+  v = m;
 }
 
 CLU_DEV double wave_sum(double v) {

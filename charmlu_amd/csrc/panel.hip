@@ -25,6 +25,8 @@
 // becomes a list of (destination row, source row) moves, applied to whole
 // column ranges as a gather through LDS (one read and one write per moved
 // row segment, coalesced along the row).
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -43,52 +45,64 @@ struct alignas(16) PanelRec {
   double top[PW];
 };
 
-template <typename T, int R>
-__global__ __launch_bounds__(256) void panel_base_kernel(
+template <typename T, int R, int NT>
+__global__ __launch_bounds__(NT) void panel_base_kernel(
     T *__restrict__ A, long lda, int j0, int m, int w, int pc0, int pc1,
     int *__restrict__ ipiv, PanelRec *__restrict__ recs, unsigned seq0, int *info) {
   const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  constexpr int RPW = 256 * R;
+  constexpr int RPW = NT * R;
+  constexpr int NW = NT / 64;  // waves
+  constexpr int GMAX = 64;     // workgroups in one exchange
   const T tiny = (T)100 * (sizeof(T) == 8 ? (T)2.220446049250313e-16 : (T)1.1920929e-07);
 
-  __shared__ T s_val[4];
-  __shared__ int s_idx[4];
-  __shared__ T s_u[PW];
-  __shared__ T s_top[PW];
-  __shared__ int s_p;
+  // every per-column slot is double-buffered by column parity, so a column
+  // needs two barriers (three with several workgroups) and none at its end
+  __shared__ T s_val[2][NW];
+  __shared__ int s_idx[2][NW];
+  __shared__ T s_u[2][PW];
+  __shared__ T s_top[2][PW];
   __shared__ int s_piv[PW];
-  __shared__ double s_cand[256][PW];  // candidate rows of every workgroup (G <= 256)
-  __shared__ double s_cval[256];
-  __shared__ int s_cidx[256];
+  __shared__ T s_pmag[PW];
+  __shared__ T s_cand[2][GMAX][PW];  // candidate rows of every workgroup
+  __shared__ T s_cval[2][GMAX];
+  __shared__ int s_cidx[2][GMAX];
 
   T a[R][PW];
 #pragma clang loop unroll(full)
   for (int r = 0; r < R; ++r) {
-    const int li = g * RPW + r * 256 + t;
+    const int li = g * RPW + r * NT + t;
 #pragma clang loop unroll(full)
     for (int q = 0; q < PW; ++q)
       a[r][q] = (li < m && q < w) ? A[(long)(j0 + li) * lda + j0 + q] : (T)0;
   }
+  // Drain the strip loads here: otherwise the compiler waits for them at the
+  // top of the column loop, where the wait then also covers the previous
+  // column's ipiv store on every iteration.
+  wait_all_mem();
 
   // The column loop stays rolled (a fully unrolled strip kernel is tens of
-  // thousands of instructions and runs from instruction-cache misses).  To
-  // keep every register index static, each row is rotated left by one
-  // position after every column, so the active column is always a[r][0],
-  // the columns still to come are a[r][1 .. w-c-1], and after 16 rotations
-  // in total the row is back in its original order.
-  auto rotate = [&]() {
+  // thousands of instructions and runs from instruction-cache misses).  Four
+  // columns are unrolled per iteration at static register offsets K = 0..3;
+  // after each iteration every row is rotated left by four positions, so
+  // the next four columns are again at offsets 0..3.  In the rotated frame
+  // the columns still to come sit at offsets K+1 .. lim-1 (lim = w - base)
+  // and the finished ones elsewhere; after 16 rotated positions in total a
+  // row is back in its original order.
+  auto rotate4 = [&]() {
 #pragma clang loop unroll(full)
     for (int r = 0; r < R; ++r) {
-      const T first = a[r][0];
+      T keep[4];
 #pragma clang loop unroll(full)
-      for (int q = 0; q < PW - 1; ++q) a[r][q] = a[r][q + 1];
-      a[r][PW - 1] = first;
+      for (int q = 0; q < 4; ++q) keep[q] = a[r][q];
+#pragma clang loop unroll(full)
+      for (int q = 0; q < PW - 4; ++q) a[r][q] = a[r][q + 4];
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 4; ++q) a[r][PW - 4 + q] = keep[q];
     }
   };
-#pragma clang loop unroll(disable)
-  for (int c = 0; c < w; ++c) {
-    {
+  auto column = [&](const int c, const int lim, auto kc) {
+    constexpr int K = decltype(kc)::value;
     const int par = c & 1;
     // ---- candidate for column c over local rows >= c (strict '>', rows
     //      visited in increasing order, so the lowest row wins ties) ----
@@ -96,34 +110,38 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
     int bi = -1;
 #pragma clang loop unroll(full)
     for (int r = 0; r < R; ++r) {
-      const int li = g * RPW + r * 256 + t;
-      const T v = fabs(a[r][0]);
+      const int li = g * RPW + r * NT + t;
+      const T v = fabs(a[r][K]);
       if (li < m && li >= c && v > bv) { bv = v; bi = li; }
     }
     wave_argmax(bv, bi);
-    if (lane == 0) { s_val[wave] = bv; s_idx[wave] = bi; }
+    if (lane == 0) { s_val[par][wave] = bv; s_idx[par][wave] = bi; }
     __syncthreads();
-    bv = s_val[0]; bi = s_idx[0];
+    bv = s_val[par][0]; bi = s_idx[par][0];
 #pragma clang loop unroll(full)
-    for (int k = 1; k < 4; ++k)
-      if (better(s_val[k], s_idx[k], bv, bi)) { bv = s_val[k]; bi = s_idx[k]; }
+    for (int k = 1; k < NW; ++k)
+      if (better(s_val[par][k], s_idx[par][k], bv, bi)) { bv = s_val[par][k]; bi = s_idx[par][k]; }
 
+    int p;
+    const T *u;    // the pivot row (LDS)
+    const T *top;  // the old row c (LDS)
     if (G == 1) {
-      const int p = bi < 0 ? c : bi;
+      p = bi < 0 ? c : bi;
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
-        const int li = r * 256 + t;
+        const int li = r * NT + t;
         if (li == p) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) s_u[q] = a[r][q];
+          for (int q = 0; q < PW; ++q) s_u[par][q] = a[r][q];
         }
         if (li == c) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) s_top[q] = a[r][q];
+          for (int q = 0; q < PW; ++q) s_top[par][q] = a[r][q];
         }
       }
-      if (t == 0) s_p = p;
       __syncthreads();
+      u = s_u[par];
+      top = s_top[par];
     } else {
       // publish: candidate value/row, and the top row if held here
       const unsigned seq = seq0 + (unsigned)c + 1u;
@@ -134,7 +152,7 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
       }
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
-        const int li = g * RPW + r * 256 + t;
+        const int li = g * RPW + r * NT + t;
         if (bi >= 0 && li == bi) {
 #pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) st_agent(&mine->cand[q], (double)a[r][q]);
@@ -147,7 +165,7 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
       wait_all_mem();
       __syncthreads();
       if (t == 0) st_agent(&mine->seq, seq);
-      // wait for every workgroup's record of this column, then read it
+      // thread t < G waits for workgroup t's record of this column, reads it
       const int owner_c = c / RPW;
       if (t < G) {
         const PanelRec *rt = recs + par * 256 + t;
@@ -158,86 +176,94 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
         rv[PW] = ld_agent(&rt->val);
 #pragma clang loop unroll(full)
         for (int q = 0; q < PW; ++q) rv[q] = ld_agent(&rt->cand[q]);
-        s_cval[t] = rv[PW];
-        s_cidx[t] = ridx;
+        s_cval[par][t] = (T)rv[PW];
+        s_cidx[par][t] = ridx;
 #pragma clang loop unroll(full)
-        for (int q = 0; q < PW; ++q) s_cand[t][q] = rv[q];
+        for (int q = 0; q < PW; ++q) s_cand[par][t][q] = (T)rv[q];
         if (t == owner_c) {
 #pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) rv[q] = ld_agent(&rt->top[q]);
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) s_top[q] = (T)rv[q];
+          for (int q = 0; q < PW; ++q) s_top[par][q] = (T)rv[q];
         }
       }
       __syncthreads();
-      // every workgroup reduces the same G candidates the same way
-      T cv = (T)0;
-      int ci = -1;
-      if (t < G) { cv = (T)s_cval[t]; ci = s_cidx[t]; }
-      wave_argmax(cv, ci);
-      if (lane == 0) { s_val[wave] = cv; s_idx[wave] = ci; }
-      __syncthreads();
-      cv = s_val[0]; ci = s_idx[0];
-#pragma clang loop unroll(full)
-      for (int k = 1; k < 4; ++k)
-        if (better(s_val[k], s_idx[k], cv, ci)) { cv = s_val[k]; ci = s_idx[k]; }
-      const int p = ci < 0 ? c : ci;
-      if (t < PW) s_u[t] = (p == c) ? s_top[t] : (T)s_cand[p / RPW][t];
-      if (t == 0) s_p = p;
-      __syncthreads();
+      // every thread scans the same G candidates in the same order
+      T cv = s_cval[par][0];
+      int ci = s_cidx[par][0];
+      for (int k = 1; k < G; ++k)
+        if (better(s_cval[par][k], s_cidx[par][k], cv, ci)) { cv = s_cval[par][k]; ci = s_cidx[par][k]; }
+      p = ci < 0 ? c : ci;
+      top = s_top[par];
+      u = (p == c) ? top : s_cand[par][p / RPW];
     }
 
-    const int p = s_p;
-    if (t == 0) s_piv[c] = p;
-    if (g == 0 && t == 0) {
-      ipiv[j0 + c] = j0 + p;
-      if (fabs(s_u[0]) <= tiny && *info == 0) *info = j0 + c + 1;
-    }
+    // pivots go to global memory after the loop: a store here would make
+    // the next barrier wait for its acknowledgement from memory
+    if (t == 0) { s_piv[c] = p; s_pmag[c] = fabs(u[K]); }
     // ---- row exchange inside the strip ----
     if (p != c) {
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
-        const int li = g * RPW + r * 256 + t;
+        const int li = g * RPW + r * NT + t;
         if (li == c) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) a[r][q] = s_u[q];
+          for (int q = 0; q < PW; ++q) a[r][q] = u[q];
         } else if (li == p) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) a[r][q] = s_top[q];
+          for (int q = 0; q < PW; ++q) a[r][q] = top[q];
         }
       }
     }
     // ---- multipliers (division) and rank-1 update of columns > c ----
-    const T piv = s_u[0];
-    const int rest = w - c;  // active column plus the ones still to come
+    const T piv = u[K];
     if (piv != (T)0) {
+      // the pivot row with every offset that is not a later column zeroed,
+      // so the row updates below need no per-element predicate
+      T uq[PW];
+#pragma clang loop unroll(full)
+      for (int q = K + 1; q < PW; ++q) uq[q] = (q < lim) ? u[q] : (T)0;
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
-        const int li = g * RPW + r * 256 + t;
+        const int li = g * RPW + r * NT + t;
         if (li < m && li > c) {
-          const T l = a[r][0] / piv;
-          a[r][0] = l;
+          const T l = a[r][K] / piv;
+          a[r][K] = l;
 #pragma clang loop unroll(full)
-          for (int q = 1; q < PW; ++q)
-            if (q < rest) a[r][q] -= l * s_u[q];
+          for (int q = K + 1; q < PW; ++q) a[r][q] -= l * uq[q];
         }
       }
     }
-    rotate();
-    __syncthreads();  // s_u / s_top / s_val reused by the next column
-    }
+  };
+#pragma clang loop unroll(disable)
+  for (int base = 0; base < w; base += 4) {
+    const int lim = w - base;
+    column(base + 0, lim, std::integral_constant<int, 0>{});
+    if (base + 1 < w) column(base + 1, lim, std::integral_constant<int, 1>{});
+    if (base + 2 < w) column(base + 2, lim, std::integral_constant<int, 2>{});
+    if (base + 3 < w) column(base + 3, lim, std::integral_constant<int, 3>{});
+    rotate4();
   }
 #pragma clang loop unroll(disable)
-  for (int c = w; c < PW; ++c) rotate();
+  for (int base = (w + 3) / 4 * 4; base < PW; base += 4) rotate4();
 
 #pragma clang loop unroll(full)
   for (int r = 0; r < R; ++r) {
-    const int li = g * RPW + r * 256 + t;
+    const int li = g * RPW + r * NT + t;
     if (li < m) {
 #pragma clang loop unroll(full)
       for (int q = 0; q < PW; ++q)
         if (q < w) A[(long)(j0 + li) * lda + j0 + q] = a[r][q];
     }
+  }
+  __syncthreads();  // s_piv complete
+  if (g == 0 && t < w) ipiv[j0 + t] = j0 + s_piv[t];
+  if (g == 0 && t == 0) {
+    for (int c = 0; c < w; ++c)
+      if (s_pmag[c] <= tiny) {
+        if (*info == 0) *info = j0 + c + 1;
+        break;
+      }
   }
 
   // ---- the strip's interchanges on the other columns of the panel
@@ -274,7 +300,7 @@ __global__ __launch_bounds__(256) void panel_base_kernel(
   const int nmv = s_nmv;
   const int left = j0 - pc0, right = pc1 - (j0 + w);
   // columns of the rest of the panel, spread over all threads of the grid
-  for (int k = g * 256 + t; nmv > 0 && k < left + right; k += G * 256) {
+  for (int k = g * NT + t; nmv > 0 && k < left + right; k += G * NT) {
     const int col = k < left ? pc0 + k : j0 + w + (k - left);
     T v[2 * PW];
 #pragma clang loop unroll(full)
@@ -292,17 +318,24 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   const int m = m_end - j0;
   if (m <= 0 || w <= 0) return CLU_OK;
   if (w > PW) return -1;
-  int R, G;
-  if (m <= 256) { R = 1; G = 1; }
-  else if (m <= 512) { R = 2; G = 1; }
-  else { R = 4; G = (m + 1023) / 1024; }
-  if (G > pw.max_groups || G > 256) return -2;
+  // one 256-thread workgroup up to 1024 rows, then 512-thread workgroups of
+  // 2048 rows (4 per thread) -- fewer participants in every column's exchange
+  int R, G, NT;
+  if (m <= 256) { R = 1; G = 1; NT = 256; }
+  else if (m <= 512) { R = 2; G = 1; NT = 256; }
+  else if (m <= 1024) { R = 4; G = 1; NT = 256; }
+  else { R = 4; NT = 512; G = (m + 2047) / 2048; }
+  if (G > pw.max_groups || G > 64) return -2;
   PanelRec *recs = reinterpret_cast<PanelRec *>(pw.slots);
   const unsigned seq0 = pw.epoch;
-  switch (R) {
-    case 1: hipLaunchKernelGGL((panel_base_kernel<T, 1>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
-    case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
-    default: hipLaunchKernelGGL((panel_base_kernel<T, 4>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+  if (NT == 512) {
+    hipLaunchKernelGGL((panel_base_kernel<T, 4, 512>), dim3(G), dim3(512), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info);
+  } else {
+    switch (R) {
+      case 1: hipLaunchKernelGGL((panel_base_kernel<T, 1, 256>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+      case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2, 256>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+      default: hipLaunchKernelGGL((panel_base_kernel<T, 4, 256>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+    }
   }
   CLU_LAUNCHED("panel_base_kernel");
   pw.epoch += (unsigned)w;
