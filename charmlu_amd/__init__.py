@@ -10,8 +10,8 @@ from ._lib import CharmluError, load
 
 load()
 
-from .lu import (LUConfig, gen_matrix, gen_rhs, getrf, getrs, hpl_gflops,  # noqa: E402
-                 residual, run)
+from .lu import (LUConfig, gen_matrix, gen_rhs, gesv_ir, getrf, getrf_mixed,  # noqa: E402
+                 getrs, getrs32, hpl_gflops, residual, run)
 
-__all__ = ["CharmluError", "LUConfig", "gen_matrix", "gen_rhs", "getrf", "getrs",
-           "hpl_gflops", "residual", "run", "load"]
+__all__ = ["CharmluError", "LUConfig", "gen_matrix", "gen_rhs", "gesv_ir", "getrf",
+           "getrf_mixed", "getrs", "getrs32", "hpl_gflops", "residual", "run", "load"]

@@ -37,6 +37,10 @@ SIGNATURES = {
                                   _c_long, _vp]),
     "charmlu_dgetrf": (_c_int, [_c_int, _c_int, _vp, _c_long, _vp, _vp, _vp]),
     "charmlu_dgetrs": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_long, _vp, _vp, _c_long, _vp]),
+    "charmlu_sgetrf_mixed": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_long, _vp, _vp, _vp]),
+    "charmlu_sgetrs": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_long, _vp, _vp, _c_long, _vp]),
+    "charmlu_dgesv_ir": (_c_int, [_c_int, _vp, _c_long, _vp, _c_long, _vp, _vp, _vp,
+                                  ctypes.c_double, _c_int, _c_int, _vp, _vp]),
     "charmlu_dgen_matrix": (_c_int, [_c_int, _c_u64, _c_long, _vp, _c_long, _vp]),
     "charmlu_dgen_rhs": (_c_int, [_c_int, _c_long, _c_long, _vp, _vp]),
     "charmlu_dresidual": (_c_int, [_c_int, _vp, _c_long, _vp, _vp, _vp, _vp, _vp]),

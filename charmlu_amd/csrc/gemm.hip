@@ -24,6 +24,26 @@ namespace clu {
 
 constexpr int GBK = 16;
 
+// The 16x16x4 matrix-core step for each element type and where element v of
+// a lane's 4-wide result sits: FP64 puts row i of the 16x16 block in lane
+// group i%4, element i/4; FP32 in lane group i/4, element i%4 (measured on
+// gfx950).
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef double4_t acc_t;
+  static CLU_DEV acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static CLU_DEV int row(int lq, int v) { return lq + 4 * v; }
+};
+template <> struct Mfma<float> {
+  typedef float4_t acc_t;
+  static CLU_DEV acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static CLU_DEV int row(int lq, int v) { return 4 * lq + v; }
+};
+
 // Map a workgroup id to a C tile.  Workgroups are handed to the 8 XCDs
 // round-robin (id % 8); renumber so that each XCD receives a contiguous
 // range of tiles, then walk that range in groups of GM tile rows so the
@@ -47,31 +67,31 @@ CLU_DEV void tile_coords(int bid, int nblk, int tiles_m, int tiles_n, int &tm,
 // beta test is uniform; with beta != 0 the C values of MFMA row block i+1
 // are requested before block i is combined and stored, so the epilogue costs
 // about one memory round trip instead of one per element.
-template <int TM, int TN>
-CLU_DEV void store_tile(const double4_t (&acc)[TM][TN], double *__restrict__ C, long ldc,
-                        int r0, int c0, int lr, int lq, double alpha, double beta, int M, int N,
+template <typename T, int TM, int TN>
+CLU_DEV void store_tile(const typename Mfma<T>::acc_t (&acc)[TM][TN], T *__restrict__ C, long ldc,
+                        int r0, int c0, int lr, int lq, T alpha, T beta, int M, int N,
                         bool full) {
   auto inb = [&](int row, int col) { return full || (row < M && col < N); };
-  if (beta == 0.0) {
+  if (beta == (T)0) {
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int row = r0 + i * 16 + lq + 4 * v, col = c0 + j * 16 + lr;
+          const int row = r0 + i * 16 + Mfma<T>::row(lq, v), col = c0 + j * 16 + lr;
           if (inb(row, col)) C[(long)row * ldc + col] = alpha * acc[i][j][v];
         }
     return;
   }
-  double cur[TN][4], nxt[TN][4];
-  auto fetch = [&](int i, double (&dst)[TN][4]) {
+  T cur[TN][4], nxt[TN][4];
+  auto fetch = [&](int i, T (&dst)[TN][4]) {
 #pragma unroll
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = r0 + i * 16 + lq + 4 * v, col = c0 + j * 16 + lr;
-        dst[j][v] = inb(row, col) ? C[(long)row * ldc + col] : 0.0;
+        const int row = r0 + i * 16 + Mfma<T>::row(lq, v), col = c0 + j * 16 + lr;
+        dst[j][v] = inb(row, col) ? C[(long)row * ldc + col] : (T)0;
       }
   };
   fetch(0, cur);
@@ -82,7 +102,7 @@ CLU_DEV void store_tile(const double4_t (&acc)[TM][TN], double *__restrict__ C, 
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = r0 + i * 16 + lq + 4 * v, col = c0 + j * 16 + lr;
+        const int row = r0 + i * 16 + Mfma<T>::row(lq, v), col = c0 + j * 16 + lr;
         if (inb(row, col)) C[(long)row * ldc + col] = alpha * acc[i][j][v] + beta * cur[j][v];
       }
     if (i + 1 < TM) {
@@ -94,10 +114,10 @@ CLU_DEV void store_tile(const double4_t (&acc)[TM][TN], double *__restrict__ C, 
   }
 }
 
-template <int BM, int BN, int WM, int WN, bool TA, bool TB>
-__global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
-    int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
-    const double *__restrict__ B, long ldb, double beta, double *__restrict__ C,
+template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB>
+__global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(
+    int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
+    const T *__restrict__ B, long ldb, T beta, T *__restrict__ C,
     long ldc, int tiles_m, int tiles_n) {
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
@@ -107,8 +127,8 @@ __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
   static_assert(TM >= 1 && TN >= 1, "wave tile");
   static_assert(A_PER >= 1 && B_PER >= 1, "load split");
 
-  __shared__ double As[2][GBK][ALD];
-  __shared__ double Bs[2][GBK][BLD];
+  __shared__ T As[2][GBK][ALD];
+  __shared__ T Bs[2][GBK][BLD];
 
   int tm, tn;
   tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
@@ -118,7 +138,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
   const int wm0 = (wave / WN) * WTM, wn0 = (wave % WN) * WTN;
   const int lr = lane & 15, lq = lane >> 4;
 
-  double ra[A_PER], rb[B_PER];
+  T ra[A_PER], rb[B_PER];
   auto load = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < A_PER; ++r) {
@@ -128,7 +148,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
       const int gm = m0 + m, gk = k0 + k;
       ra[r] = (gm < M && gk < K)
                   ? (!TA ? A[(long)gm * lda + gk] : A[(long)gk * lda + gm])
-                  : 0.0;
+                  : (T)0;
     }
 #pragma unroll
     for (int r = 0; r < B_PER; ++r) {
@@ -138,7 +158,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
       const int gn = n0 + n, gk = k0 + k;
       rb[r] = (gn < N && gk < K)
                   ? (!TB ? B[(long)gk * ldb + gn] : B[(long)gn * ldb + gk])
-                  : 0.0;
+                  : (T)0;
     }
   };
   auto stash = [&](int buf) {
@@ -158,11 +178,11 @@ __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
     }
   };
 
-  double4_t acc[TM][TN];
+  typename Mfma<T>::acc_t acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < TN; ++j) acc[i][j] = (typename Mfma<T>::acc_t){0, 0, 0, 0};
 
   const int nk = (K + GBK - 1) / GBK;
   load(0);
@@ -173,7 +193,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
     if (kc + 1 < nk) load((kc + 1) * GBK);
 #pragma unroll
     for (int s = 0; s < GBK / 4; ++s) {
-      double a[TM], b[TN];
+      T a[TM], b[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) a[i] = As[buf][4 * s + lq][wm0 + i * 16 + lr];
 #pragma unroll
@@ -182,14 +202,14 @@ __global__ __launch_bounds__(256, 2) void dgemm_mfma_kernel(
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = Mfma<T>::step(a[i], b[j], acc[i][j]);
     }
     if (kc + 1 < nk) stash(buf ^ 1);
     __syncthreads();
   }
 
   const bool full = m0 + BM <= M && n0 + BN <= N;
-  store_tile<TM, TN>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, M, N, full);
+  store_tile<T, TM, TN>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, M, N, full);
 }
 
 // ---------------------------------------------------------------------------
@@ -306,20 +326,19 @@ __global__ __launch_bounds__(256, 2) void dgemm_nn_lds_kernel(
     }
   }
   wait_vm<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
-  store_tile<4, 4>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, M, N, true);
+  store_tile<double, 4, 4>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, M, N, true);
 }
 
-template <int BM, int BN, int WM, int WN, bool TA, bool TB>
-static int launch(int M, int N, int K, double alpha, const double *A, long lda,
-                  const double *B, long ldb, double beta, double *C, long ldc,
-                  hipStream_t s) {
+template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB>
+static int launch(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
+                  T beta, T *C, long ldc, hipStream_t s) {
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   const long nblk = (long)tiles_m * tiles_n;
   if (nblk > 0x7fffffffL) return -1;
-  hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WM, WN, TA, TB>), dim3((unsigned)nblk),
-                     dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                     tiles_m, tiles_n);
-  CLU_LAUNCHED("dgemm_mfma_kernel");
+  hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WM, WN, TA, TB>), dim3((unsigned)nblk),
+                     dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m,
+                     tiles_n);
+  CLU_LAUNCHED("gemm_mfma_kernel");
   return CLU_OK;
 }
 
@@ -330,28 +349,29 @@ static bool lds_path_ok(int M, int N, int K, const double *A, long lda, const do
          (long)(M / 128) * (N / 128) >= 256;
 }
 
-template <bool TA, bool TB>
-static int dispatch(int M, int N, int K, double alpha, const double *A, long lda,
-                    const double *B, long ldb, double beta, double *C, long ldc,
-                    hipStream_t s) {
-  if (!TA && !TB && lds_path_ok(M, N, K, A, lda, B, ldb)) {
+template <typename T, bool TA, bool TB>
+static int dispatch(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
+                    T beta, T *C, long ldc, hipStream_t s) {
+  if (sizeof(T) == 8 && !TA && !TB &&
+      lds_path_ok(M, N, K, (const double *)A, lda, (const double *)B, ldb)) {
     const int tiles_m = M / 128, tiles_n = N / 128;
     hipLaunchKernelGGL((dgemm_nn_lds_kernel<4>), dim3(tiles_m * tiles_n), dim3(256), 0, s, M, N,
-                       K, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+                       K, (double)alpha, (const double *)A, lda, (const double *)B, ldb,
+                       (double)beta, (double *)C, ldc, tiles_m, tiles_n);
     CLU_LAUNCHED("dgemm_nn_lds_kernel");
     return CLU_OK;
   }
   // narrow right-hand sides (panel-internal updates, multi-RHS solves)
   if (N <= 32)
-    return launch<256, 32, 4, 1, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch<T, 256, 32, 4, 1, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   if (N <= 64)
-    return launch<64, 64, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch<T, 64, 64, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   if (M <= 32)
-    return launch<32, 256, 1, 4, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch<T, 32, 256, 1, 4, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   // small problems: more, smaller tiles to fill 256 CUs
   if ((long)((M + 127) / 128) * ((N + 127) / 128) < 512)
-    return launch<64, 64, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  return launch<128, 128, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch<T, 64, 64, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  return launch<T, 128, 128, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
 int dgemm_variant(bool ta, bool tb, int M, int N, int K, const double *A, long lda,
@@ -364,18 +384,25 @@ int dgemm_variant(bool ta, bool tb, int M, int N, int K, const double *A, long l
   return 2;
 }
 
-int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A,
-          long lda, const double *B, long ldb, double beta, double *C, long ldc,
-          hipStream_t s) {
+template <typename T>
+static int gemm_any(bool ta, bool tb, int M, int N, int K, T alpha, const T *A, long lda,
+                    const T *B, long ldb, T beta, T *C, long ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return CLU_OK;
-  if (K <= 0 || alpha == 0.0) {
-    // C = beta * C: run the kernel with K = 0 (acc stays zero)
-    K = 0;
-  }
-  if (!ta && !tb) return dispatch<false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  if (ta && !tb) return dispatch<true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  if (!ta && tb) return dispatch<false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  return dispatch<true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  if (K <= 0 || alpha == (T)0) K = 0;  // C = beta * C: the K loop runs zero chunks
+  if (!ta && !tb) return dispatch<T, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  if (ta && !tb) return dispatch<T, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  if (!ta && tb) return dispatch<T, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  return dispatch<T, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+
+int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A, long lda,
+          const double *B, long ldb, double beta, double *C, long ldc, hipStream_t s) {
+  return gemm_any<double>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+
+int sgemm(bool ta, bool tb, int M, int N, int K, float alpha, const float *A, long lda,
+          const float *B, long ldb, float beta, float *C, long ldc, hipStream_t s) {
+  return gemm_any<float>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
 }  // namespace clu

@@ -1,0 +1,254 @@
+// gemm_lp.hip -- the low-precision trailing update of the mixed-precision LU.
+//
+// The FP32 working matrix is updated as C -= L21 * U12 with both operands
+// rounded to BF16 or FP16 and products accumulated in FP32 on the matrix
+// cores (v_mfma_f32_16x16x32_bf16 / _f16): the reference's
+// updateMatrix dgemm (lu.C:26-43) at reduced input precision.  Accuracy is
+// recovered afterwards by FP64 iterative refinement.
+//
+// Operands are packed once per step into 16-bit buffers whose rows hold K
+// contiguous values (L21 as is, U12 transposed), so each lane's MFMA operand
+// -- 8 consecutive k of one row -- is a single 16-byte LDS read.  K is
+// zero-padded to a multiple of 32.
+//
+// hgemm_nt_kernel: 128x128 C tiles, 4 waves of 64x64, K in stages of 32
+// loaded with global_load_lds_dwordx4 straight into LDS, 4 stages in flight.
+// LDS rows are 64 bytes (4 chunks of 16 bytes); the chunk index is XORed with
+// (row >> 2) & 3 so the 16 lanes reading one operand column hit distinct
+// banks.
+//
+// MFMA 16x16x32 (BF16/FP16) layout, per lane l: A[i][k] in lane i + 16*(k/8),
+// element k%8; B[k][j] in lane j + 16*(k/8), element k%8; D[i][j] in lane
+// j + 16*(i/4), element i%4 (measured on gfx950).
+#include <hip/hip_bf16.h>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace clu {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) void *lds_ptr_t;
+typedef __attribute__((address_space(1))) void *gbl_ptr_t;
+
+template <typename T16> struct Lp;
+template <> struct Lp<__bf16> {
+  typedef bf16x8_t vec_t;
+  static CLU_DEV float4_t mfma(vec_t a, vec_t b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+  static CLU_DEV __bf16 from(float x) { return (__bf16)x; }
+};
+template <> struct Lp<_Float16> {
+  typedef f16x8_t vec_t;
+  static CLU_DEV float4_t mfma(vec_t a, vec_t b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+  static CLU_DEV _Float16 from(float x) { return (_Float16)x; }
+};
+
+template <int N>
+CLU_DEV void wait_vm_lp() {
+  static_assert(N >= 0 && N < 16, "vmcnt");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
+}
+
+template <typename T16, int NST>
+__global__ __launch_bounds__(256, 2) void hgemm_nt_kernel(int M, int N, int K, float alpha,
+                                                          const T16 *__restrict__ A, long lda,
+                                                          const T16 *__restrict__ B, long ldb,
+                                                          float beta, float *__restrict__ C,
+                                                          long ldc, int tiles_m, int tiles_n) {
+  constexpr int BM = 128, BN = 128, BK = 32;
+  constexpr int HALF = BM * BK * 2;  // bytes of one operand's stage
+  constexpr int STAGE = 2 * HALF;
+  __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
+  typedef typename Lp<T16>::vec_t vec_t;
+
+  // XCD-aware tile order (same scheme as the FP64 kernels)
+  int tm, tn;
+  {
+    const int bid = blockIdx.x, nblk = gridDim.x, per = nblk / 8;
+    int lid = bid;
+    if (bid < per * 8) lid = (bid % 8) * per + bid / 8;
+    constexpr int GM = 8;
+    const int group_size = GM * tiles_n, group = lid / group_size, first_m = group * GM;
+    const int gm = min(tiles_m - first_m, GM), in_group = lid - group * group_size;
+    tm = first_m + in_group % gm;
+    tn = in_group / gm;
+  }
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
+  const int lr = lane & 15, lq = lane >> 4;
+
+  // this thread's two chunks of A and of B per stage (rows past M or N are
+  // clamped to the last one; their products are never stored)
+  const T16 *ga[2], *gb[2];
+  int loff[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int c = wave * 64 + 256 * r + lane;  // chunk slot in the stage
+    const int row = c >> 2, kq = (c & 3) ^ ((row >> 2) & 3);
+    const int am = min(m0 + row, M - 1), bn = min(n0 + row, N - 1);
+    ga[r] = A + (long)am * lda + 8 * kq;
+    gb[r] = B + (long)bn * ldb + 8 * kq;
+    loff[r] = (wave * 64 + 256 * r) * 16;
+  }
+  const int nk = K / BK;
+  auto issue = [&](int kc) {
+    const int kk = (kc < nk ? kc : nk - 1) * BK;  // past the end: reload the last stage
+    char *st = smem + (kc % NST) * STAGE;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(ga[r] + kk), (lds_ptr_t)(st + loff[r]), 16, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(gb[r] + kk), (lds_ptr_t)(st + HALF + loff[r]),
+                                       16, 0, 0);
+  };
+  int aoff[4], boff[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = wm0 + 16 * i + lr, n = wn0 + 16 * i + lr;
+    aoff[i] = m * 64 + ((lq ^ ((m >> 2) & 3)) << 4);
+    boff[i] = HALF + n * 64 + ((lq ^ ((n >> 2) & 3)) << 4);
+  }
+
+  float4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+  for (int st = 0; st < NST - 1; ++st) issue(st);
+  for (int kc = 0; kc < nk; ++kc) {
+    wait_vm_lp<4 * (NST - 2)>();
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    issue(kc + NST - 1);
+    const char *st = smem + (kc % NST) * STAGE;
+    vec_t a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const vec_t *>(st + aoff[i]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const vec_t *>(st + boff[j]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = Lp<T16>::mfma(a[i], b[j], acc[i][j]);
+  }
+  wait_vm_lp<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
+
+  // epilogue: C = alpha * acc + beta * C; the C values of row block i+1 are
+  // requested before block i is combined and stored
+  const bool full = m0 + BM <= M && n0 + BN <= N;
+  auto inb = [&](int row, int col) { return full || (row < M && col < N); };
+  float cur[4][4], nxt[4][4];
+  auto fetch = [&](int i, float (&dst)[4][4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + wm0 + 16 * i + 4 * lq + v, col = n0 + wn0 + 16 * j + lr;
+        dst[j][v] = (beta != 0.f && inb(row, col)) ? C[(long)row * ldc + col] : 0.f;
+      }
+  };
+  fetch(0, cur);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i + 1 < 4) fetch(i + 1, nxt);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + wm0 + 16 * i + 4 * lq + v, col = n0 + wn0 + 16 * j + lr;
+        if (inb(row, col)) C[(long)row * ldc + col] = alpha * acc[i][j][v] + beta * cur[j][v];
+      }
+    if (i + 1 < 4) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cur[j][v] = nxt[j][v];
+    }
+  }
+}
+
+template <typename T16>
+int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B, long ldb,
+             float beta, float *C, long ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0) return CLU_OK;
+  if (K % 32 != 0 || K <= 0 || lda % 8 != 0 || ldb % 8 != 0 || ((uintptr_t)A & 15) ||
+      ((uintptr_t)B & 15))
+    return -1;
+  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128;
+  hipLaunchKernelGGL((hgemm_nt_kernel<T16, 4>), dim3(tiles_m * tiles_n), dim3(256), 0, s, M, N, K,
+                     alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+  CLU_LAUNCHED("hgemm_nt_kernel");
+  return CLU_OK;
+}
+template int hgemm_nt<__bf16>(int, int, int, float, const __bf16 *, long, const __bf16 *, long,
+                              float, float *, long, hipStream_t);
+template int hgemm_nt<_Float16>(int, int, int, float, const _Float16 *, long, const _Float16 *,
+                                long, float, float *, long, hipStream_t);
+
+// dst (M x Kp, row-major) = src (M x K) rounded to 16 bits, zero for k >= K
+template <typename T16>
+__global__ void pack_rows_kernel(int M, int K, int Kp, const float *__restrict__ src, long lds,
+                                 T16 *__restrict__ dst) {
+  const long total = (long)M * Kp;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const long i = e / Kp;
+    const int k = (int)(e % Kp);
+    dst[e] = Lp<T16>::from(k < K ? src[i * lds + k] : 0.f);
+  }
+}
+
+// dst (N x Kp, row-major) = src (K x N)^T rounded to 16 bits, zero for k >= K;
+// 32 x 32 tiles through LDS so both sides are read and written along rows
+template <typename T16>
+__global__ __launch_bounds__(256) void pack_cols_kernel(int K, int N, int Kp,
+                                                        const float *__restrict__ src, long lds,
+                                                        T16 *__restrict__ dst) {
+  __shared__ float tile[32][33];
+  const int k0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int k = k0 + r, n = n0 + tx;
+    tile[r][tx] = (k < K && n < N) ? src[(long)k * lds + n] : 0.f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int n = n0 + r, k = k0 + tx;
+    if (n < N && k < Kp) dst[(long)n * Kp + k] = Lp<T16>::from(tile[tx][r]);
+  }
+}
+
+template <typename T16>
+int pack_rows(int M, int K, int Kp, const float *src, long lds, T16 *dst, hipStream_t s) {
+  if (M <= 0 || Kp <= 0) return CLU_OK;
+  long blocks = ((long)M * Kp + 255) / 256;
+  if (blocks > 65536L * 16) blocks = 65536L * 16;
+  hipLaunchKernelGGL((pack_rows_kernel<T16>), dim3((unsigned)blocks), dim3(256), 0, s, M, K, Kp,
+                     src, lds, dst);
+  CLU_LAUNCHED("pack_rows_kernel");
+  return CLU_OK;
+}
+template <typename T16>
+int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s) {
+  if (N <= 0 || Kp <= 0) return CLU_OK;
+  dim3 grid((N + 31) / 32, (Kp + 31) / 32);
+  hipLaunchKernelGGL((pack_cols_kernel<T16>), grid, dim3(256), 0, s, K, N, Kp, src, lds, dst);
+  CLU_LAUNCHED("pack_cols_kernel");
+  return CLU_OK;
+}
+template int pack_rows<__bf16>(int, int, int, const float *, long, __bf16 *, hipStream_t);
+template int pack_rows<_Float16>(int, int, int, const float *, long, _Float16 *, hipStream_t);
+template int pack_cols<__bf16>(int, int, int, const float *, long, __bf16 *, hipStream_t);
+template int pack_cols<_Float16>(int, int, int, const float *, long, _Float16 *, hipStream_t);
+
+}  // namespace clu

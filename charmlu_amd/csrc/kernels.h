@@ -15,6 +15,23 @@ int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A,
           long lda, const double *B, long ldb, double beta, double *C, long ldc,
           hipStream_t s);
 
+// The same on FP32 operands (FP32 matrix cores).
+int sgemm(bool ta, bool tb, int M, int N, int K, float alpha, const float *A, long lda,
+          const float *B, long ldb, float beta, float *C, long ldc, hipStream_t s);
+
+// C (FP32) = alpha * A * B^T + beta * C with A (M x K) and B (N x K) in a
+// 16-bit format (T16 = __bf16 or _Float16), rows K-contiguous, K % 32 == 0,
+// lda and ldb multiples of 8, 16-byte aligned bases.
+template <typename T16>
+int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B, long ldb,
+             float beta, float *C, long ldc, hipStream_t s);
+// dst (M x Kp) = 16-bit src (M x K) padded with zeros; dst (N x Kp) = 16-bit
+// transpose of src (K x N) padded with zeros.
+template <typename T16>
+int pack_rows(int M, int K, int Kp, const float *src, long lds, T16 *dst, hipStream_t s);
+template <typename T16>
+int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s);
+
 // Kernel dgemm() launches for these operands: 0 = the direct-to-LDS 128x128
 // NN kernel (the trailing update), 1 = 256x32, 2 = 128x128, 3 = 32x256,
 // 4 = 64x64 register-staged kernels.
@@ -94,5 +111,19 @@ int convert(long m, long n, const TS *src, long lds, TD *dst, long ldd, hipStrea
 int fill_int(int *p, int v, long n, hipStream_t s);
 int set_identity(double *A, long n, long lda, hipStream_t s);
 int fill_u32(unsigned *p, unsigned v, long n, hipStream_t s);
+
+// ---- FP64 vector kernels of the iterative refinement ----------------------
+// h[0..k) = V w (V: k rows of n); w += alpha * V^T h; out = {||x||_2^2,
+// ||x||_inf}; y = a x; y32 = (float)(a x); y = a (double)x32.
+int basis_dot(int k, long n, const double *V, long ldv, const double *w, double *h,
+              hipStream_t s);
+int basis_axpy(int k, long n, double alpha, const double *V, long ldv, const double *h,
+               double *w, hipStream_t s);
+int vec_norms(long n, const double *x, double *out, hipStream_t s);
+int vec_scale(long n, double a, const double *x, double *y, hipStream_t s);
+int vec_d2s(long n, double a, const double *x, float *y, hipStream_t s);
+int vec_s2d(long n, double a, const float *x, double *y, hipStream_t s);
+template <typename T>
+int set_identity_t(T *A, long n, long lda, hipStream_t s);
 
 }  // namespace clu

@@ -188,6 +188,22 @@ int set_identity(double *A, long n, long lda, hipStream_t s) {
   return CLU_OK;
 }
 
+template <typename T>
+__global__ void set_identity_t_kernel(T *A, long n, long lda) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n * n) A[(e / n) * lda + e % n] = (e / n == e % n) ? (T)1 : (T)0;
+}
+template <typename T>
+int set_identity_t(T *A, long n, long lda, hipStream_t s) {
+  if (n <= 0) return CLU_OK;
+  hipLaunchKernelGGL((set_identity_t_kernel<T>), dim3((unsigned)((n * n + 255) / 256)), dim3(256),
+                     0, s, A, n, lda);
+  CLU_LAUNCHED("set_identity_kernel");
+  return CLU_OK;
+}
+template int set_identity_t<float>(float *, long, long, hipStream_t);
+template int set_identity_t<double>(double *, long, long, hipStream_t);
+
 __global__ void fill_int_kernel(int *p, int v, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

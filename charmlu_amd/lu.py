@@ -107,6 +107,59 @@ def getrs(LU, ipiv, B, trans: bool = False):
     return B
 
 
+_PREC = {"fp32": _lib.PREC_FP32, "bf16": _lib.PREC_BF16, "fp16": _lib.PREC_FP16}
+
+
+def getrf_mixed(A32, block: int, prec: str = "bf16"):
+    """In-place LU of the row-major FP32 CUDA tensor A32 with an FP32 panel
+    and the trailing update in ``prec`` ('fp32', 'bf16', 'fp16'; FP32
+    accumulation).  Returns (ipiv, info) like ``getrf``."""
+    torch = _torch()
+    _require(A32, torch.float32, "A32")
+    n = A32.shape[0]
+    if A32.shape[1] != n:
+        raise ValueError("A32 must be square")
+    if prec not in _PREC:
+        raise ValueError(f"prec must be one of {sorted(_PREC)}")
+    ipiv = torch.empty(max(n, 1), dtype=torch.int32, device=A32.device)
+    info = torch.zeros(1, dtype=torch.int32, device=A32.device)
+    check(_lib.load().charmlu_sgetrf_mixed(n, block, _PREC[prec], _ptr(A32), A32.stride(0),
+                                           _ptr(ipiv), _ptr(info), _stream_ptr()),
+          "charmlu_sgetrf_mixed")
+    return ipiv[:n], info
+
+
+def getrs32(LU32, ipiv, B, trans: bool = False):
+    """``getrs`` on FP32 factors and an FP32 right-hand side."""
+    torch = _torch()
+    _require(LU32, torch.float32, "LU32")
+    n = LU32.shape[0]
+    B2 = B.view(n, 1) if B.dim() == 1 else B
+    _require(B2, torch.float32, "B")
+    check(_lib.load().charmlu_sgetrs(_lib.TRANS if trans else _lib.NO_TRANS, n, B2.shape[1],
+                                     _ptr(LU32), LU32.stride(0), _ptr(ipiv), _ptr(B2),
+                                     B2.stride(0), _stream_ptr()), "charmlu_sgetrs")
+    return B
+
+
+def gesv_ir(A, LU32, ipiv, b, x=None, tol: float = 1.0, max_iter: int = 30,
+            restart: int = 50):
+    """FP64 GMRES-IR: x with A x = b (FP64 A, b) preconditioned by the FP32
+    factors of ``getrf_mixed``.  Returns (x, stats dict)."""
+    torch = _torch()
+    _require(A, torch.float64, "A")
+    n = A.shape[0]
+    if x is None:
+        x = torch.empty(n, dtype=torch.float64, device=A.device)
+    st = np.zeros(4, dtype=np.float64)
+    check(_lib.load().charmlu_dgesv_ir(n, _ptr(A), A.stride(0), _ptr(LU32), LU32.stride(0),
+                                       _ptr(ipiv), _ptr(b), _ptr(x), float(tol), int(max_iter),
+                                       int(restart), ctypes_void(st.ctypes.data), _stream_ptr()),
+          "charmlu_dgesv_ir")
+    return x, dict(cycles=int(st[0]), iterations=int(st[1]), residual=float(st[2]),
+                   residual_ns=float(st[3]))
+
+
 def residual(A, x, b):
     """Reference scaled residual (benchmark.C:385-400) and the normwise form
     ||Ax-b||_inf / (||A||_inf ||x||_inf n eps).  Returns a dict."""

@@ -135,6 +135,33 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo,
 int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda,
                    const int *dipiv, double *dB, long ldb, void *stream);
 
+/* ---- 4. mixed precision ------------------------------------------------ */
+
+/* The factorization of charmlu_dgetrf on an FP32 working matrix (the
+ * reference is FP64 only: same algorithm, lu.ci:31-370).  The panel is
+ * factored in FP32; the trailing update (updateMatrix, lu.C:26-43) runs with
+ * its operands rounded to prec (PREC_FP32, PREC_BF16 or PREC_FP16; FP32
+ * accumulation).  dA: n x n FP32 row-major, overwritten by L and U.  The
+ * pivots and info follow charmlu_dgetrf.  Asynchronous. */
+int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipiv,
+                         int *dinfo, void *stream);
+
+/* charmlu_dgetrs on FP32 factors and an FP32 right-hand side. */
+int charmlu_sgetrs(int trans, int n, int nrhs, const float *dLU, long lda,
+                   const int *dipiv, float *dB, long ldb, void *stream);
+
+/* Solve A x = b in FP64 by GMRES-based iterative refinement preconditioned
+ * with the FP32 factors (dLU, dipiv) of charmlu_sgetrf_mixed.  dA is the
+ * original FP64 matrix; x, residuals, the Krylov basis and inner products
+ * are FP64.  Refinement stops once the reference's scaled residual
+ * (benchmark.C:385-400) is <= tol, or after max_iter GMRES cycles of at most
+ * restart (1..200) iterations.  stats (optional, host, 4 doubles) =
+ * {cycles, inner iterations, final reference residual, final normwise
+ * residual ||b-Ax||_inf / (||A||_inf ||x||_inf n eps)}.  Synchronous. */
+int charmlu_dgesv_ir(int n, const double *dA, long lda, const float *dLU, long ldlu,
+                     const int *dipiv, const double *db, double *dx, double tol,
+                     int max_iter, int restart, double *stats, void *stream);
+
 /* Seeded inputs of the driver (benchmark.C:430-460).
  * gen_matrix: entry (i,j) of the build's counter generator, kind 0 =
  * uniform [-0.5,0.5), kind 1 = same with a_ii = sum_j |a_ij|.

@@ -61,6 +61,16 @@ def test_argument_errors_before_any_device_call():
     assert lib.charmlu_cblas_dger(_lib.ROW_MAJOR, 2, 2, 1.0, None, 0, None, 1, None, 2) == -6
     assert lib.charmlu_dgen_matrix(5, 1, 4, None, 4, None) == -1
     assert lib.charmlu_dresidual(-1, None, 1, None, None, None, None, None) == -1
+    # mixed precision: FP8 and FP64 are not trailing-update precisions here
+    assert lib.charmlu_sgetrf_mixed(32, 8, _lib.PREC_FP64, None, 32, None, None, None) == -3
+    assert lib.charmlu_sgetrf_mixed(32, 7, _lib.PREC_BF16, None, 32, None, None, None) == -2
+    assert lib.charmlu_sgetrs(_lib.NO_TRANS, 4, -1, None, 4, None, None, 1, None) == -3
+    assert lib.charmlu_dgesv_ir(4, ctypes.c_void_p(16), 4, ctypes.c_void_p(16), 4,
+                                ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.c_void_p(16),
+                                0.0, 5, 10, None, None) == -9
+    assert lib.charmlu_dgesv_ir(4, ctypes.c_void_p(16), 4, ctypes.c_void_p(16), 4,
+                                ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.c_void_p(16),
+                                1.0, 5, 0, None, None) == -11
 
 
 def test_empty_problems_are_no_ops():

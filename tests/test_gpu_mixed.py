@@ -1,0 +1,158 @@
+"""Mixed-precision factorization and FP64 GMRES-based refinement on the GPU.
+
+The reference factors in FP64 only, so the mixed path is checked against the
+FP64 oracle's solution of the same seeded system:
+  * the refined x within 1e-10 relative (normwise) of the oracle's FP64 x,
+    and the reference's scaled residual < 16 (test_lu.pl:75-76);
+  * the low-precision factors themselves through their backward error
+    ||P A - L U||_max / (max|a_ij| * growth) against the unit roundoff of
+    the update precision (a property of any correct LU, so a wrong
+    operand, sign or index in the update fails it by orders of magnitude);
+  * diagonally dominant matrices need no interchanges: pivots are the
+    identity, as in the oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+UNIT = {"fp32": 2.0 ** -24, "bf16": 2.0 ** -8, "fp16": 2.0 ** -11}
+
+
+def _unpack(LU):
+    L = np.tril(LU, -1) + np.eye(LU.shape[0])
+    U = np.triu(LU)
+    return L, U
+
+
+def _perm_rows(A, ipiv):
+    P = A.copy()
+    for k, p in enumerate(ipiv):
+        if p != k:
+            P[[k, p]] = P[[p, k]]
+    return P
+
+
+def _backward_error(A, LU, ipiv):
+    L, U = _unpack(LU.astype(np.float64))
+    PA = _perm_rows(A, ipiv)
+    return np.abs(PA - L @ U).max() / np.abs(A).max()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("n,nb", [(512, 128), (1024, 256), (768, 256)])
+def test_mixed_factor_diagdom(dev, orc, prec, n, nb):
+    import torch
+    A = orc.gen_diagdom(orc.SEED_A, n)
+    A32 = torch.from_numpy(A.astype(np.float32)).cuda()
+    ipiv, info = dev.getrf_mixed(A32, nb, prec)
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    assert np.array_equal(ipiv.cpu().numpy(), np.arange(n))
+    be = _backward_error(A, A32.cpu().numpy(), ipiv.cpu().numpy())
+    print(f"{prec} n={n} nb={nb} backward error {be:.3e}")
+    # each of the n/nb update stages rounds its operands once
+    assert be < 8 * UNIT[prec] * (n // nb + 1)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_mixed_factor_uniform(dev, orc, prec):
+    """General matrices pivot; the factors are still a backward-stable LU of
+    the FP32 input at the update precision."""
+    import torch
+    n, nb = 1024, 128
+    A = orc.gen_uniform(orc.SEED_A, n)
+    A32 = torch.from_numpy(A.astype(np.float32)).cuda()
+    ipiv, info = dev.getrf_mixed(A32, nb, prec)
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    ip = ipiv.cpu().numpy()
+    assert np.all(ip >= np.arange(n)) and np.all(ip < n)
+    LU = A32.cpu().numpy()
+    # partial pivoting: |l_ij| <= 1
+    assert np.abs(np.tril(LU, -1)).max() <= 1.0 + 1e-6
+    be = _backward_error(A.astype(np.float32).astype(np.float64), LU, ip)
+    print(f"{prec} uniform backward error {be:.3e}")
+    assert be < 64 * UNIT[prec] * np.sqrt(n)
+
+
+def test_sgetrs_both_transposes(dev, orc):
+    import torch
+    n, nb = 768, 128
+    A = orc.gen_uniform(7, n)
+    A32 = torch.from_numpy(A.astype(np.float32)).cuda()
+    ipiv, _ = dev.getrf_mixed(A32, nb, "fp32")
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((n, 5))
+    for trans in (False, True):
+        X = torch.from_numpy(B.astype(np.float32)).cuda()
+        dev.getrs32(A32, ipiv, X, trans=trans)
+        x = X.cpu().numpy().astype(np.float64)
+        ref = np.linalg.solve(A.T if trans else A, B)
+        err = np.abs(x - ref).max() / np.abs(ref).max()
+        assert err < 1e-3, (trans, err)
+    # one right-hand side as a vector
+    xv = torch.from_numpy(B[:, 0].astype(np.float32)).cuda()
+    dev.getrs32(A32, ipiv, xv)
+    ref = np.linalg.solve(A, B[:, 0])
+    assert np.abs(xv.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-3
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32"])
+@pytest.mark.parametrize("n,nb", [(1024, 256), (2048, 256)])
+def test_gmres_ir_matches_fp64_oracle(dev, orc, prec, n, nb):
+    import torch
+    A = orc.gen_diagdom(orc.SEED_A, n)
+    b = orc.gen_rhs(n)
+    x_ref = orc.lu_solve_problem(n, nb, A=A.copy())["x"]
+    dA = torch.from_numpy(A).cuda()
+    A32 = dA.float()
+    ipiv, info = dev.getrf_mixed(A32, nb, prec)
+    assert int(info.item()) == 0
+    db = torch.from_numpy(b).cuda()
+    x, st = dev.gesv_ir(dA, A32, ipiv, db, tol=1.0)
+    xh = x.cpu().numpy()
+    err = np.abs(xh - x_ref).max() / np.abs(x_ref).max()
+    print(f"{prec} n={n}: {st}, error vs FP64 oracle {err:.2e}")
+    assert st["residual"] <= 1.0
+    res = dev.residual(dA, x, db)
+    assert res["residual"] < 16
+    assert err < 1e-10
+    assert st["cycles"] >= 1
+
+
+def test_gmres_ir_uniform_fp32_factors(dev, orc):
+    """A general (pivoting) matrix refined from FP32 factors."""
+    import torch
+    n, nb = 1024, 128
+    A = orc.gen_uniform(orc.SEED_A, n)
+    b = orc.gen_rhs(n)
+    x_ref = orc.lu_solve_problem(n, nb, A=A.copy())["x"]
+    dA = torch.from_numpy(A).cuda()
+    A32 = dA.float()
+    ipiv, _ = dev.getrf_mixed(A32, nb, "fp32")
+    x, st = dev.gesv_ir(dA, A32, ipiv, torch.from_numpy(b).cuda(), tol=1.0)
+    err = np.abs(x.cpu().numpy() - x_ref).max() / np.abs(x_ref).max()
+    print(f"uniform fp32: {st}, error {err:.2e}")
+    assert st["residual"] <= 1.0
+    assert err < 1e-10
+
+
+def test_gmres_ir_already_converged_and_limits(dev, orc):
+    import torch
+    n, nb = 512, 128
+    A = orc.gen_diagdom(3, n)
+    b = orc.gen_rhs(n)
+    dA = torch.from_numpy(A).cuda()
+    A32 = dA.float()
+    ipiv, _ = dev.getrf_mixed(A32, nb, "bf16")
+    db = torch.from_numpy(b).cuda()
+    # max_iter = 0: x is the low-precision solve only
+    x0, st0 = dev.gesv_ir(dA, A32, ipiv, db, tol=1.0, max_iter=0)
+    assert st0["cycles"] == 0 and st0["iterations"] == 0
+    assert st0["residual"] > 1.0
+    x1, st1 = dev.gesv_ir(dA, A32, ipiv, db, tol=1.0)
+    assert st1["residual"] <= 1.0 and st1["cycles"] >= 1
+    # a loose tolerance needs no more work than a tight one
+    _, st2 = dev.gesv_ir(dA, A32, ipiv, db, tol=1e3)
+    assert st2["iterations"] <= st1["iterations"]
