@@ -95,6 +95,88 @@ def gemm_only(lib, _lib, torch, n, nb, steps, warmup):
                       "algorithmic_bytes": 8 * (2 * m * m + 2 * m * nb)}), flush=True)
 
 
+def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps):
+    """Mixed-precision LU + FP64 GMRES-IR against the plain FP64 LU on the
+    same diagonally dominant system (A U[-0.5,0.5) with a_ii = sum_j |a_ij|,
+    b from MatGen).  A mixed step converts A to FP32, factors it with the
+    low-precision trailing update and refines x in FP64 to the reference's
+    scaled residual <= 1; a FP64 step copies A, factors and solves.  Returns
+    the "mixed" object of the JSON line."""
+    import ctypes
+    flops = 2.0 / 3.0 * n ** 3
+    A = clu.gen_matrix(n, kind="diagdom")
+    b = clu.gen_rhs(n)
+    out = {"n": n, "block": nb,
+           "workload": f"A U[-0.5,0.5) with a_ii = sum_j |a_ij| (diagonally dominant), n={n}"}
+    # plain FP64 LU on the same system: the speed reference and the solution
+    # the mixed results are compared with
+    x64 = None
+    if fp64_steps > 0:
+        W = torch.empty_like(A)
+        x64 = torch.empty_like(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(fp64_steps):
+            W.copy_(A)
+            x64.copy_(b)
+            ipiv, info = clu.getrf(W, nb)
+            clu.getrs(W, ipiv, x64)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / fp64_steps
+        res = clu.residual(A, x64, b)
+        out["fp64"] = {"tflops": round(flops / sec / 1e12, 3), "ms": round(sec * 1e3, 1),
+                       "residual": res["residual"], "info": int(info.item()),
+                       "steps": fp64_steps}
+        del W
+        torch.cuda.empty_cache()
+    A32 = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    x = torch.empty_like(b)
+    for prec in precs:
+        split = [0.0, 0.0]
+
+        def step():
+            t_a = time.perf_counter()
+            clu.lag2s(A, A32)
+            ipiv, info = clu.getrf_mixed(A32, nb, prec)
+            torch.cuda.synchronize()
+            t_b = time.perf_counter()
+            _, st = clu.gesv_ir(A, A32, ipiv, b, x, tol=1.0)
+            split[0] += t_b - t_a
+            split[1] += time.perf_counter() - t_b
+            return info, st
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        lib.charmlu_profile(1)
+        split[0] = split[1] = 0.0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            info, st = step()
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        prof = (ctypes.c_double * 15)()
+        _lib.check(lib.charmlu_profile_read(prof, 15), "charmlu_profile_read")
+        lib.charmlu_profile(0)
+        p = list(prof)
+        res = clu.residual(A, x, b)
+        rec = {"tflops": round(flops / sec / 1e12, 3), "ms": round(sec * 1e3, 1),
+               "residual": res["residual"], "residual_ns": res["residual_ns"],
+               "info": int(info.item()), "passed": bool(res["residual"] < 16),
+               "ir_cycles": st["cycles"], "ir_iterations": st["iterations"],
+               "factor_ms": round(split[0] / steps * 1e3, 1),
+               "refine_ms": round(split[1] / steps * 1e3, 1),
+               "update_ms": round(p[2] / steps, 1),
+               "update_tflops": round(p[1] / (p[2] * 1e-3) / 1e12, 1) if p[2] > 0 else None,
+               "panel_stream_ms": round(p[8] / steps, 1),
+               "interchange_ms": round(p[11] / steps, 1),
+               "u_row_ms": round(p[14] / steps, 1)}
+        if x64 is not None:
+            rec["error_vs_fp64"] = float(((x - x64).abs().max() / x64.abs().max()).item())
+            rec["speedup_vs_fp64"] = round(rec["tflops"] / out["fp64"]["tflops"], 3)
+        out[prec] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +189,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gemm-only", action="store_true",
                     help="time only the trailing-update GEMM of step 0 (M=N=n-nb, K=nb)")
+    ap.add_argument("--mixed-n", type=int, default=65536)
+    ap.add_argument("--mixed-nb", type=int, default=256)
+    ap.add_argument("--mixed-precs", default="bf16,fp16")
+    ap.add_argument("--mixed-steps", type=int, default=2)
+    ap.add_argument("--mixed-fp64-steps", type=int, default=1)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -134,6 +223,11 @@ def main():
         return
     if args.gemm_only:
         gemm_only(lib, _lib, torch, n, nb, args.steps, args.warmup)
+        return
+    if args.mixed_only:
+        print(json.dumps(mixed_bench(clu, lib, _lib, torch, args.mixed_n, args.mixed_nb,
+                                     args.mixed_precs.split(","), args.mixed_steps, args.warmup,
+                                     args.mixed_fp64_steps)), flush=True)
         return
     A0 = clu.gen_matrix(n)
     b0 = clu.gen_rhs(n)
@@ -225,6 +319,12 @@ def main():
         },
         "row_interchange_gbs": round(swap_bytes / (sw_ms * 1e-3) / 1e9, 1) if sw_ms > 0 else None,
     }
+    if not args.no_mixed:
+        del A, A0
+        torch.cuda.empty_cache()
+        out["mixed"] = mixed_bench(clu, lib, _lib, torch, args.mixed_n, args.mixed_nb,
+                                   args.mixed_precs.split(","), args.mixed_steps, args.warmup,
+                                   args.mixed_fp64_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, nb, args.cpu_threads)
     print(json.dumps(out), flush=True)

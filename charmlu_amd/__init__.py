@@ -11,7 +11,7 @@ from ._lib import CharmluError, load
 load()
 
 from .lu import (LUConfig, gen_matrix, gen_rhs, gesv_ir, getrf, getrf_mixed,  # noqa: E402
-                 getrs, getrs32, hpl_gflops, residual, run)
+                 getrs, getrs32, hpl_gflops, lag2s, residual, run)
 
 __all__ = ["CharmluError", "LUConfig", "gen_matrix", "gen_rhs", "gesv_ir", "getrf",
-           "getrf_mixed", "getrs", "getrs32", "hpl_gflops", "residual", "run", "load"]
+           "getrf_mixed", "getrs", "getrs32", "hpl_gflops", "lag2s", "residual", "run", "load"]

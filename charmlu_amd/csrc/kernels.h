@@ -93,6 +93,13 @@ int dgemv(bool trans, int M, int N, double alpha, const double *A, long lda, con
           long incx, double beta, double *y, long incy, hipStream_t s);
 
 // y[0..M) -= A[0..M, 0..K) * x  (trans: y[0..M) -= A[0..K, 0..M)^T * x)
+// Inverses of the S x S diagonal blocks of a unit lower / non-unit upper
+// triangle, block k at Inv + k*S*S (ld S), all solved in one launch
+template <typename T>
+int tri_inv_blocks(bool upper, int n, int S, const T *A, long lda, T *Inv, hipStream_t s);
+// y = op(G) x, G one m x m block inverse (m <= 256)
+template <typename T>
+int blk_gemv(bool trans, int m, const T *G, long ldg, const T *x, T *y, hipStream_t s);
 template <typename T>
 int gemv_sub(bool trans, int M, int K, const T *A, long lda, const T *x, T *y,
              hipStream_t s);

@@ -84,6 +84,9 @@ struct Workspace {
   size_t winv32_b = 0, ubuf32_b = 0, pa16_b = 0, pu16_b = 0;
   void *irbuf = nullptr;
   size_t irbuf_b = 0;
+  // solve: inverses of the diagonal blocks of L and U, the intermediate y
+  void *sinv = nullptr, *sy = nullptr;
+  size_t sinv_b = 0, sy_b = 0;
   PanelWork pw{};
 };
 
@@ -316,11 +319,14 @@ static int apply_runs(int nruns, const int2 *moves, const int *counts, T *A, lon
 }
 
 
-// Solve op(A) X = B from P A = L U: the row interchanges, then block forward
-// and backward substitution in 256-row blocks (forwardSolve / backwardSolve
-// / offDiagSolve, lu.ci:387-458): each diagonal block is solved, then its
-// contribution is subtracted from every row still to be solved (a GEMV for
-// one right-hand side, a GEMM for several).
+// Solve op(A) X = B from P A = L U (forwardSolve / backwardSolve /
+// offDiagSolve, lu.ci:387-458): the row interchanges, then block forward and
+// backward substitution over 256-row blocks.  The diagonal blocks of L and U
+// are inverted up front, all in one batched launch each, so the sequential
+// part of each block step is one small matrix-vector (or matrix) product
+// with the block's inverse instead of a 256-step substitution; the block's
+// contribution is then subtracted from every row still to be solved (a GEMV
+// for one right-hand side, a GEMM for several).
 template <typename T>
 static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const int *dipiv, T *dB,
                       long ldb, hipStream_t s) {
@@ -337,10 +343,20 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
   int rc = get_ws(&wsp);
   if (rc) return rc;
   Workspace &ws = *wsp;
+  constexpr int S = 256;
+  const int nblk = (n + S - 1) / S;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure_bytes(&ws.sinv, &ws.sinv_b, sizeof(T) * 2 * (size_t)nblk * S * S))) return rc;
+    if ((rc = ensure_bytes(&ws.sy, &ws.sy_b, sizeof(T) * (size_t)n * nrhs))) return rc;
+  }
+  T *Linv = (T *)ws.sinv, *Uinv = Linv + (size_t)nblk * S * S;
+  T *Y = (T *)ws.sy;
+  const long ldy = nrhs;
+  if ((rc = tri_inv_blocks<T>(false, n, S, dLU, lda, Linv, s))) return rc;
+  if ((rc = tri_inv_blocks<T>(true, n, S, dLU, lda, Uinv, s))) return rc;
   const int nruns = (n + RUN - 1) / RUN;
-  const T *L = dLU;
   const bool vec = nrhs == 1 && ldb == 1;
-  constexpr int BS = 256;
   auto interchanges = [&](bool inverse) -> int {
     // one run at a time through the in-panel move buffer (stream-ordered)
     for (int i = 0; i < nruns; ++i) {
@@ -353,44 +369,59 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
     }
     return CLU_OK;
   };
-  // off-diagonal contribution of a solved block (offDiagSolve, lu.ci:439-456)
-  auto update = [&](bool ta, int M, int K, const T *Ablk, const T *X, T *Y) {
-    if (M <= 0) return CLU_OK;
-    if (vec) return gemv_sub<T>(ta, M, K, Ablk, lda, X, Y, s);
-    return gemm_t<T>(ta, false, M, nrhs, K, (T)-1, Ablk, lda, X, ldb, (T)1, Y, ldb, s);
+  // dst = op(G) src for the inverse G of one diagonal block
+  auto diag = [&](bool tr, int m, const T *G, const T *src, long lds, T *dst, long ldd) -> int {
+    if (vec) return blk_gemv<T>(tr, m, G, S, src, dst, s);
+    return gemm_t<T>(tr, false, m, nrhs, m, (T)1, G, S, src, lds, (T)0, dst, ldd, s);
   };
+  // off-diagonal contribution of a solved block (offDiagSolve, lu.ci:439-456)
+  auto update = [&](bool ta, int M, int K, const T *Ablk, const T *X, long ldx, T *D,
+                    long ldd) -> int {
+    if (M <= 0) return CLU_OK;
+    if (vec) return gemv_sub<T>(ta, M, K, Ablk, lda, X, D, s);
+    return gemm_t<T>(ta, false, M, nrhs, K, (T)-1, Ablk, lda, X, ldx, (T)1, D, ldd, s);
+  };
+  const T *F = dLU;
   if (trans == CHARMLU_NO_TRANS) {
     if ((rc = interchanges(false))) return rc;
-    for (int x0 = 0; x0 < n; x0 += BS) {
-      const int m = n - x0 < BS ? n - x0 : BS;
-      if ((rc = trsm_llnu<T>(m, nrhs, L + (long)x0 * lda + x0, lda, dB + (long)x0 * ldb, ldb, s)))
+    // L y = P b: y into Y, B is the running right-hand side
+    for (int k = 0; k < nblk; ++k) {
+      const int x0 = k * S, m = n - x0 < S ? n - x0 : S;
+      if ((rc = diag(false, m, Linv + (size_t)k * S * S, dB + (long)x0 * ldb, ldb,
+                     Y + (long)x0 * ldy, ldy)))
         return rc;
-      if ((rc = update(false, n - x0 - m, m, L + (long)(x0 + m) * lda + x0, dB + (long)x0 * ldb,
-                       dB + (long)(x0 + m) * ldb)))
+      if ((rc = update(false, n - x0 - m, m, F + (long)(x0 + m) * lda + x0, Y + (long)x0 * ldy,
+                       ldy, dB + (long)(x0 + m) * ldb, ldb)))
         return rc;
     }
-    for (int x0 = ((n - 1) / BS) * BS; x0 >= 0; x0 -= BS) {
-      const int m = n - x0 < BS ? n - x0 : BS;
-      if ((rc = trsm_lunn<T>(m, nrhs, L + (long)x0 * lda + x0, lda, dB + (long)x0 * ldb, ldb, s)))
+    // U x = y: x into B, Y is the running right-hand side
+    for (int k = nblk - 1; k >= 0; --k) {
+      const int x0 = k * S, m = n - x0 < S ? n - x0 : S;
+      if ((rc = diag(false, m, Uinv + (size_t)k * S * S, Y + (long)x0 * ldy, ldy,
+                     dB + (long)x0 * ldb, ldb)))
         return rc;
-      if ((rc = update(false, x0, m, L + x0, dB + (long)x0 * ldb, dB))) return rc;
+      if ((rc = update(false, x0, m, F + x0, dB + (long)x0 * ldb, ldb, Y, ldy))) return rc;
     }
     return CLU_OK;
   }
-  // A^T X = B with P A = L U:  U^T Z = B, L^T W = Z, X = P^T W
-  for (int x0 = 0; x0 < n; x0 += BS) {
-    const int m = n - x0 < BS ? n - x0 : BS;
-    if ((rc = trsm_lutn<T>(m, nrhs, L + (long)x0 * lda + x0, lda, dB + (long)x0 * ldb, ldb, s)))
+  // A^T X = B with P A = L U:  U^T Z = B (Z into Y), L^T W = Z (W into B),
+  // X = P^T W
+  for (int k = 0; k < nblk; ++k) {
+    const int x0 = k * S, m = n - x0 < S ? n - x0 : S;
+    if ((rc = diag(true, m, Uinv + (size_t)k * S * S, dB + (long)x0 * ldb, ldb,
+                   Y + (long)x0 * ldy, ldy)))
       return rc;
-    if ((rc = update(true, n - x0 - m, m, L + (long)x0 * lda + x0 + m, dB + (long)x0 * ldb,
-                     dB + (long)(x0 + m) * ldb)))
+    if ((rc = update(true, n - x0 - m, m, F + (long)x0 * lda + x0 + m, Y + (long)x0 * ldy, ldy,
+                     dB + (long)(x0 + m) * ldb, ldb)))
       return rc;
   }
-  for (int x0 = ((n - 1) / BS) * BS; x0 >= 0; x0 -= BS) {
-    const int m = n - x0 < BS ? n - x0 : BS;
-    if ((rc = trsm_lltu<T>(m, nrhs, L + (long)x0 * lda + x0, lda, dB + (long)x0 * ldb, ldb, s)))
+  for (int k = nblk - 1; k >= 0; --k) {
+    const int x0 = k * S, m = n - x0 < S ? n - x0 : S;
+    if ((rc = diag(true, m, Linv + (size_t)k * S * S, Y + (long)x0 * ldy, ldy,
+                   dB + (long)x0 * ldb, ldb)))
       return rc;
-    if ((rc = update(true, x0, m, L + (long)x0 * lda, dB + (long)x0 * ldb, dB))) return rc;
+    if ((rc = update(true, x0, m, F + (long)x0 * lda, dB + (long)x0 * ldb, ldb, Y, ldy)))
+      return rc;
   }
   return interchanges(true);
 }
@@ -923,6 +954,18 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   CLU_HIP(hipEventRecord(ws.ev_end, sm));
   CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
   return CLU_OK;
+}
+
+// FP64 -> FP32 copy of a matrix (the working matrix of the mixed path)
+int charmlu_dlag2s(long m, long n, const double *dA, long lda, float *dS, long lds,
+                   void *stream) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (!dA && m > 0 && n > 0) return -3;
+  if (lda < (n > 1 ? n : 1)) return -4;
+  if (!dS && m > 0 && n > 0) return -5;
+  if (lds < (n > 1 ? n : 1)) return -6;
+  return convert<float, double>(m, n, dA, lda, dS, lds, (hipStream_t)stream);
 }
 
 // ---- FP64 GMRES-based iterative refinement ------------------------------------

@@ -28,8 +28,12 @@ namespace clu {
 // 16 x 16 diagonal block is solved by one thread per column.
 template <typename T, bool OPLOW, bool TR, bool UNIT>
 __global__ __launch_bounds__(256) void trsm_kernel(int M, int N, T alpha, const T *__restrict__ F,
-                                                   long ldf, T *__restrict__ B, long ldb) {
+                                                   long ldf, T *__restrict__ B, long ldb,
+                                                   long fstride, long bstride) {
   constexpr int NC = 32, BS = 16;
+  // blockIdx.y selects one of a batch of independent solves
+  F += blockIdx.y * fstride;
+  B += blockIdx.y * bstride;
   extern __shared__ unsigned char smem_raw[];
   T *X = reinterpret_cast<T *>(smem_raw);  // [M][NC]
   T *Fb = X + (size_t)M * NC;              // [BS][M]
@@ -178,10 +182,98 @@ static int trsm_launch(int M, int N, T alpha, const T *F, long ldf, T *B, long l
   }
   const size_t shm = (size_t)M * 32 * sizeof(T) + (size_t)16 * M * sizeof(T);
   hipLaunchKernelGGL((trsm_kernel<T, OPLOW, TR, UNIT>), dim3((N + 31) / 32), dim3(256), shm, s,
-                     M, N, alpha, F, ldf, B, ldb);
+                     M, N, alpha, F, ldf, B, ldb, 0L, 0L);
   CLU_LAUNCHED("trsm_kernel");
   return CLU_OK;
 }
+
+__global__ void identity_batch_kernel(void *P, int esize, int m, long count) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count * m * m) return;
+  const long r = (e / m) % m, c = e % m;
+  if (esize == 8) reinterpret_cast<double *>(P)[e] = r == c ? 1.0 : 0.0;
+  else reinterpret_cast<float *>(P)[e] = r == c ? 1.0f : 0.0f;
+}
+
+// Inverses of the diagonal blocks of a unit lower (upper = false) or a
+// non-unit upper (upper = true) triangle: block k (rows and columns
+// [k*S, min(n, (k+1)*S))) goes to Inv + k*S*S, row-major with ld S.  All
+// blocks are solved at once (one grid row per block: X_k = T_kk^{-1} I).
+template <typename T>
+int tri_inv_blocks(bool upper, int n, int S, const T *A, long lda, T *Inv, hipStream_t s) {
+  if (n <= 0) return CLU_OK;
+  if (S > 256) return -1;
+  const int nfull = n / S, last = n - nfull * S;
+  const long cnt = nfull + (last ? 1 : 0);
+  hipLaunchKernelGGL(identity_batch_kernel, dim3((unsigned)((cnt * S * S + 255) / 256)),
+                     dim3(256), 0, s, (void *)Inv, (int)sizeof(T), S, cnt);
+  CLU_LAUNCHED("identity_batch_kernel");
+  auto launch = [&](int M, int nbatch, const T *F, T *B) -> int {
+    const size_t shm = (size_t)M * 32 * sizeof(T) + (size_t)16 * M * sizeof(T);
+    const dim3 grid((M + 31) / 32, nbatch);
+    if (upper)
+      hipLaunchKernelGGL((trsm_kernel<T, false, false, false>), grid, dim3(256), shm, s, M, M,
+                         (T)1, F, lda, B, (long)S, (long)S * (lda + 1), (long)S * S);
+    else
+      hipLaunchKernelGGL((trsm_kernel<T, true, false, true>), grid, dim3(256), shm, s, M, M,
+                         (T)1, F, lda, B, (long)S, (long)S * (lda + 1), (long)S * S);
+    CLU_LAUNCHED("trsm_kernel (batched)");
+    return CLU_OK;
+  };
+  int rc;
+  if (nfull > 0 && (rc = launch(S, nfull, A, Inv))) return rc;
+  if (last > 0 && (rc = launch(last, 1, A + (long)nfull * S * (lda + 1), Inv + (long)nfull * S * S)))
+    return rc;
+  return CLU_OK;
+}
+template int tri_inv_blocks<double>(bool, int, int, const double *, long, double *, hipStream_t);
+template int tri_inv_blocks<float>(bool, int, int, const float *, long, float *, hipStream_t);
+
+// y = op(G) x for one m x m diagonal-block inverse G (m <= 256, ld ldg).
+// No transpose: one wave per row, lanes across the row, shuffle reduction;
+// transpose: one lane per output, rows split over the 4 waves of a group.
+template <typename T, bool TR>
+__global__ __launch_bounds__(256) void blk_gemv_kernel(int m, const T *__restrict__ G, long ldg,
+                                                       const T *__restrict__ x,
+                                                       T *__restrict__ y) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (!TR) {
+    // 8 workgroups x 4 waves; each wave takes rows wave_id, wave_id + 32, ...
+    const int wid = blockIdx.x * 4 + wave;
+    for (int i = wid; i < m; i += gridDim.x * 4) {
+      double acc = 0.0;
+      for (int k = lane; k < m; k += 64) acc += (double)G[(long)i * ldg + k] * (double)x[k];
+      acc = wave_sum(acc);
+      if (lane == 0) y[i] = (T)acc;
+    }
+  } else {
+    __shared__ double part[4][64];
+    const int j = blockIdx.x * 64 + lane;
+    double acc = 0.0;
+    if (j < m)
+      for (int k = wave; k < m; k += 4) acc += (double)G[(long)k * ldg + j] * (double)x[k];
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && j < m)
+      y[j] = (T)((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+  }
+}
+
+template <typename T>
+int blk_gemv(bool trans, int m, const T *G, long ldg, const T *x, T *y, hipStream_t s) {
+  if (m <= 0) return CLU_OK;
+  if (!trans)
+    hipLaunchKernelGGL((blk_gemv_kernel<T, false>), dim3(8), dim3(256), 0, s, m, G, ldg, x, y);
+  else
+    hipLaunchKernelGGL((blk_gemv_kernel<T, true>), dim3((m + 63) / 64), dim3(256), 0, s, m, G,
+                       ldg, x, y);
+  CLU_LAUNCHED("blk_gemv_kernel");
+  return CLU_OK;
+}
+template int blk_gemv<double>(bool, int, const double *, long, const double *, double *,
+                              hipStream_t);
+template int blk_gemv<float>(bool, int, const float *, long, const float *, float *,
+                             hipStream_t);
 
 template <typename T>
 int trsm_small(bool upper, bool trans, bool unit, int M, int N, T alpha, const T *F, long ldf,
@@ -246,6 +338,31 @@ __global__ __launch_bounds__(256) void gemv_sub_n_kernel(int M, int K, const T *
   if (row < M && sub == 0) y[row] = (T)((double)y[row] - s);
 }
 
+// The same with 16-byte loads (rows 16-byte aligned, K a multiple of the
+// vector width): each 16-lane row group reads 256 contiguous bytes per step.
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_sub_nv_kernel(int M, int K, const T *__restrict__ A,
+                                                          long lda, const T *__restrict__ x,
+                                                          T *__restrict__ y) {
+  constexpr int V = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  const int row = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int sub = threadIdx.x & 15;
+  double s = 0.0;
+  if (row < M) {
+    const vec_t *a = reinterpret_cast<const vec_t *>(A + (long)row * lda);
+    const vec_t *xv = reinterpret_cast<const vec_t *>(x);
+    for (int k = sub; k < K / V; k += 16) {
+      const vec_t av = a[k], xx = xv[k];
+#pragma unroll
+      for (int v = 0; v < V; ++v) s += (double)av[v] * (double)xx[v];
+    }
+  }
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
+  if (row < M && sub == 0) y[row] = (T)((double)y[row] - s);
+}
+
 // y[j] -= sum_k A[k][j] x[k]; one lane per column j, K split over 4 waves
 template <typename T>
 __global__ __launch_bounds__(256) void gemv_sub_t_kernel(int M, int K, const T *__restrict__ A,
@@ -266,7 +383,13 @@ __global__ __launch_bounds__(256) void gemv_sub_t_kernel(int M, int K, const T *
 template <typename T>
 int gemv_sub(bool trans, int M, int K, const T *A, long lda, const T *x, T *y, hipStream_t s) {
   if (M <= 0 || K <= 0) return CLU_OK;
-  if (!trans) {
+  constexpr int V = 16 / sizeof(T);
+  const bool aligned = lda % V == 0 && K % V == 0 && ((uintptr_t)A & 15) == 0 &&
+                       ((uintptr_t)x & 15) == 0;
+  if (!trans && aligned) {
+    hipLaunchKernelGGL((gemv_sub_nv_kernel<T>), dim3((M + 15) / 16), dim3(256), 0, s, M, K, A,
+                       lda, x, y);
+  } else if (!trans) {
     hipLaunchKernelGGL((gemv_sub_n_kernel<T>), dim3((M + 15) / 16), dim3(256), 0, s, M, K, A,
                        lda, x, y);
   } else {

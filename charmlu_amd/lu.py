@@ -129,6 +129,19 @@ def getrf_mixed(A32, block: int, prec: str = "bf16"):
     return ipiv[:n], info
 
 
+def lag2s(A, out=None):
+    """FP32 copy of the FP64 CUDA matrix A (the mixed path's working matrix)."""
+    torch = _torch()
+    _require(A, torch.float64, "A")
+    m, n = A.shape
+    if out is None:
+        out = torch.empty((m, n), dtype=torch.float32, device=A.device)
+    _require(out, torch.float32, "out")
+    check(_lib.load().charmlu_dlag2s(m, n, _ptr(A), max(A.stride(0), 1), _ptr(out),
+                                     max(out.stride(0), 1), _stream_ptr()), "charmlu_dlag2s")
+    return out
+
+
 def getrs32(LU32, ipiv, B, trans: bool = False):
     """``getrs`` on FP32 factors and an FP32 right-hand side."""
     torch = _torch()

@@ -146,6 +146,12 @@ int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda,
 int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipiv,
                          int *dinfo, void *stream);
 
+/* dS (m x n FP32, row-major) = dA (FP64) rounded to nearest: the working
+ * copy factored by charmlu_sgetrf_mixed (LAPACK's dlag2s, without its
+ * overflow test).  Asynchronous. */
+int charmlu_dlag2s(long m, long n, const double *dA, long lda, float *dS, long lds,
+                   void *stream);
+
 /* charmlu_dgetrs on FP32 factors and an FP32 right-hand side. */
 int charmlu_sgetrs(int trans, int n, int nrhs, const float *dLU, long lda,
                    const int *dipiv, float *dB, long ldb, void *stream);
