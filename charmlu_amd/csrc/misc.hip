@@ -5,7 +5,7 @@
 // entry of b is computed independently (genVec, benchmark.C:454-460: global
 // row i holds draw i+1).
 //
-// gen_uniform is the build-defined counter generator for A (the reference
+// gen_uniform is the library's own counter generator for A (the reference
 // uses Charm++'s CrnDouble, benchmark.C:436-452): entry (i, j) is the
 // splitmix64 output function of seed*phi ^ (i<<32 | j), top 53 bits scaled
 // to [0,1), minus 0.5.  The CPU oracle computes the same bits.

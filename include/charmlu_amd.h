@@ -169,7 +169,7 @@ int charmlu_dgesv_ir(int n, const double *dA, long lda, const float *dLU, long l
                      int max_iter, int restart, double *stats, void *stream);
 
 /* Seeded inputs of the driver (benchmark.C:430-460).
- * gen_matrix: entry (i,j) of the build's counter generator, kind 0 =
+ * gen_matrix: entry (i,j) of the library's counter generator, kind 0 =
  * uniform [-0.5,0.5), kind 1 = same with a_ii = sum_j |a_ij|.
  * gen_rhs: MatGen draws skip+1 .. skip+n of seed (benchmark.C:27-74). */
 int charmlu_dgen_matrix(int kind, uint64_t seed, long n, double *dA, long lda,

@@ -1,7 +1,7 @@
 """ctypes view of the CPU oracle (``liboracle.so``).
 
 TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
-and the ``cpu_baseline`` leg of ``bench.py``, always as the checker or the
+and the ``cpu_baseline`` leg of ``bench.py``, always as the test oracle or the
 timed CPU baseline, never as the product path.  The oracle restates
 charmlu's block LU (lu.C, lu.ci, benchmark.C of the reference); see
 ``lu_oracle.h`` for what pins each function.
@@ -208,7 +208,8 @@ def apply_ipiv(ipiv, v):
 
 
 def residual(A, x, b):
-    """Returns (reference scaled residual, north-star normwise residual, maxima)."""
+    """Returns (reference scaled residual, normwise residual
+    ||b-Ax||_inf / (||A||_inf ||x||_inf n eps), maxima)."""
     out4 = np.zeros(4)
     ns = np.zeros(1)
     n = A.shape[0]

@@ -1,14 +1,14 @@
 /*
  * lu_oracle.c -- CPU restatement of charmlu's block LU with partial pivoting.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline); the product
+ * TEST INFRASTRUCTURE ONLY (the test oracle and the CPU baseline); the product
  * never links or calls it.  See lu_oracle.h for what pins each function.
  *
  * The reference (Charm++/SDAG, lu.ci + lu.C) distributes b x b blocks over
  * processes; this file runs the same per-block operations in the order the
- * reference's dependencies force (SURVEY Appendix A restated in global
- * indices), with a fixed order wherever the reference's order is decided at
- * run time (argmax ties: lowest row; reduction sums: ascending block index).
+ * reference's dependencies force (restated in global indices), with a fixed
+ * order wherever the reference's order is decided at run time (argmax ties:
+ * lowest row; reduction sums: ascending block index).
  */
 #define _GNU_SOURCE
 #include "lu_oracle.h"
@@ -54,7 +54,7 @@ void orc_matgen_doubles(int seed, long skip, long count, double *out) {
     out[i] = (double)matgen_next(&cur) / 2147483647 - 0.5;
 }
 
-/* Build-defined A generator (the reference draws A from Charm++'s CrnDouble,
+/* Library-defined A generator (the reference draws A from Charm++'s CrnDouble,
  * benchmark.C:436-437,446-452, whose source is not available).  splitmix64's
  * output function (Steele, Lea, Flood 2014; constants as published by
  * Vigna) applied to a per-entry counter; 53 high bits -> [0,1) -> -0.5. */

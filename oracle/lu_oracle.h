@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (charmlu_amd/) links,
  * loads or calls this code; only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py use it, and only as the checker / the timed
+ * cpu_baseline leg of bench.py use it, and only as the test oracle / the timed
  * CPU baseline.
  *
  * Every function restates the reference algorithm and cites the reference
@@ -13,7 +13,7 @@
  * Parity pins (tests/test_oracle.py):
  *   matgen      -- C++ std::minstd_rand known answer (10000th value from
  *                  seed 1 is 399268537) and the first outputs of seed 9934835
- *   gen_uniform -- build-defined counter generator (splitmix64 finalizer);
+ *   gen_uniform -- library-defined counter generator (splitmix64 finalizer);
  *                  pinned by a published splitmix64 test vector and by the
  *                  GPU generator producing the same bits
  *   blas ops    -- MKL cblas_* (when present), brute-force loops, closed forms
@@ -38,7 +38,7 @@ extern "C" {
 void orc_matgen_doubles(int seed, long skip, long count, double *out);
 int orc_matgen_nth_int(int seed, long k); /* k-th draw, k >= 1 */
 
-/* A generator: build-defined counter-based uniform [-0.5, 0.5).
+/* A generator: library-defined counter-based uniform [-0.5, 0.5).
  * value(i,j) = (splitmix64(seed*phi ^ (i<<32 | j)) >> 11) * 2^-53 - 0.5 */
 double orc_gen_value(uint64_t seed, long i, long j);
 void orc_gen_uniform(uint64_t seed, long row0, long col0, long m, long n,
