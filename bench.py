@@ -95,6 +95,54 @@ def gemm_only(lib, _lib, torch, n, nb, steps, warmup):
                       "algorithmic_bytes": 8 * (2 * m * m + 2 * m * nb)}), flush=True)
 
 
+def lp_gemm_only(lib, _lib, torch, n, nb, steps, warmup, prec="bf16"):
+    """The step-0 low-precision trailing update C += A B^T (M = N = n - nb,
+    K = nb; FP32 C, BF16/FP16 operands), timed with events on the stream."""
+    import ctypes
+    m = n - nb
+    dt = torch.bfloat16 if prec == "bf16" else torch.float16
+    A = (torch.rand((m, nb), device="cuda") - 0.5).to(dt)
+    B = (torch.rand((m, nb), device="cuda") - 0.5).to(dt)
+    C = torch.rand((m, m), device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = _lib.PREC_BF16 if prec == "bf16" else _lib.PREC_FP16
+    res = {}
+    for alpha, tag in ((1.0, "accumulate"), (-1.0, "alpha_beta")):
+        def run():
+            _lib.check(lib.charmlu_gemm_lp_dev(p, m, m, nb, alpha, ctypes.c_void_p(A.data_ptr()),
+                                               nb, ctypes.c_void_p(B.data_ptr()), nb, 1.0,
+                                               ctypes.c_void_p(C.data_ptr()), m, stream),
+                       "charmlu_gemm_lp_dev")
+        for _ in range(warmup):
+            run()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        flops = 2.0 * m * m * nb
+        res[tag] = {"ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 1),
+                    "c_gbs": round(8.0 * m * m / ms / 1e6, 1)}
+    # reference point for the C stream: an elementwise read-modify-write of
+    # the same FP32 matrix (8 bytes per element)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    C.add_(1.0)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        C.add_(1.0)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    res["rmw_reference_gbs"] = round(8.0 * m * m / ms / 1e6, 1)
+    print(json.dumps({"lp_gemm": f"{m}x{m}x{nb} {prec}", **res}), flush=True)
+
+
 def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps):
     """Mixed-precision LU + FP64 GMRES-IR against the plain FP64 LU on the
     same diagonally dominant system (A U[-0.5,0.5) with a_ii = sum_j |a_ij|,
@@ -189,8 +237,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gemm-only", action="store_true",
                     help="time only the trailing-update GEMM of step 0 (M=N=n-nb, K=nb)")
+    ap.add_argument("--lp-gemm-only", action="store_true",
+                    help="time only the low-precision trailing update of step 0 at --mixed-n")
     ap.add_argument("--mixed-n", type=int, default=65536)
-    ap.add_argument("--mixed-nb", type=int, default=256)
+    ap.add_argument("--mixed-nb", type=int, default=512)
     ap.add_argument("--mixed-precs", default="bf16,fp16")
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--mixed-fp64-steps", type=int, default=1)
@@ -223,6 +273,9 @@ def main():
         return
     if args.gemm_only:
         gemm_only(lib, _lib, torch, n, nb, args.steps, args.warmup)
+        return
+    if args.lp_gemm_only:
+        lp_gemm_only(lib, _lib, torch, args.mixed_n, args.mixed_nb, args.steps, args.warmup)
         return
     if args.mixed_only:
         print(json.dumps(mixed_bench(clu, lib, _lib, torch, args.mixed_n, args.mixed_nb,

@@ -13,5 +13,7 @@ load()
 from .lu import (LUConfig, gen_matrix, gen_rhs, gesv_ir, getrf, getrf_mixed,  # noqa: E402
                  getrs, getrs32, hpl_gflops, lag2s, residual, run)
 
-__all__ = ["CharmluError", "LUConfig", "gen_matrix", "gen_rhs", "gesv_ir", "getrf",
+from .autograd import LUSolve, solve  # noqa: E402
+
+__all__ = ["LUSolve", "solve", "CharmluError", "LUConfig", "gen_matrix", "gen_rhs", "gesv_ir", "getrf",
            "getrf_mixed", "getrs", "getrs32", "hpl_gflops", "lag2s", "residual", "run", "load"]

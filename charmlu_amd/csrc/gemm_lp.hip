@@ -22,6 +22,8 @@
 // j + 16*(i/4), element i%4 (measured on gfx950).
 #include <hip/hip_bf16.h>
 
+#include <stdlib.h>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -54,15 +56,20 @@ CLU_DEV void wait_vm_lp() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
 }
 
-template <typename T16, int NST>
-__global__ __launch_bounds__(256, 2) void hgemm_nt_kernel(int M, int N, int K, float alpha,
+template <typename T16, int BM, int NST, int OCC, bool CINIT, bool VEC>
+__global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K, float alpha,
                                                           const T16 *__restrict__ A, long lda,
                                                           const T16 *__restrict__ B, long ldb,
                                                           float beta, float *__restrict__ C,
                                                           long ldc, int tiles_m, int tiles_n) {
-  constexpr int BM = 128, BN = 128, BK = 32;
-  constexpr int HALF = BM * BK * 2;  // bytes of one operand's stage
-  constexpr int STAGE = 2 * HALF;
+  // BM x BN C tile, 4 waves of 64 x 64 (BM = 128: 2 x 2 waves; BM = 64:
+  // 1 x 4 waves, so every C row segment is 1 KB of contiguous FP32)
+  constexpr int BN = 16384 / BM, BK = 32;
+  constexpr int WGN = BN / 64;
+  constexpr int HALF = BM * BK * 2;     // bytes of the A operand's stage
+  constexpr int STAGE = (BM + BN) * BK * 2;
+  constexpr int CA = BM * 4 / 256, CB = BN * 4 / 256;  // 16-byte chunks per thread
+  static_assert(BM == 64 || BM == 128, "tile");
   __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
   typedef typename Lp<T16>::vec_t vec_t;
 
@@ -80,33 +87,70 @@ __global__ __launch_bounds__(256, 2) void hgemm_nt_kernel(int M, int N, int K, f
   }
   const int m0 = tm * BM, n0 = tn * BN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
+  const int wm0 = (wave / WGN) * 64, wn0 = (wave % WGN) * 64;
   const int lr = lane & 15, lq = lane >> 4;
+  const bool full = m0 + BM <= M && n0 + BN <= N;
 
-  // this thread's two chunks of A and of B per stage (rows past M or N are
-  // clamped to the last one; their products are never stored)
-  const T16 *ga[2], *gb[2];
-  int loff[2];
+  // The MFMA's first operand is the B tile (16 C columns), its second the A
+  // tile (16 C rows), so D[r][c] of block (i, j) is C[row][col] with
+  // row = m0 + wm0 + 16 j + lr and col = n0 + wn0 + 16 i + 4 lq + r: each
+  // lane owns 4 consecutive columns of one row (one 16-byte access).
+  float4_t acc[4][4];
+  auto crow = [&](int j) { return m0 + wm0 + 16 * j + lr; };
+  auto ccol = [&](int i) { return n0 + wn0 + 16 * i + 4 * lq; };
+  if (CINIT) {
+    // C += A B^T: C itself is the initial accumulator, requested before the
+    // first operand stage so its latency overlaps the K loop
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int c = wave * 64 + 256 * r + lane;  // chunk slot in the stage
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = crow(j), col = ccol(i);
+        if (VEC && full) {
+          acc[i][j] = *reinterpret_cast<const float4_t *>(C + (long)row * ldc + col);
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            acc[i][j][r] = (row < M && col + r < N) ? C[(long)row * ldc + col + r] : 0.f;
+        }
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  }
+
+  // this thread's chunks of A and of B per stage (rows past M or N are
+  // clamped to the last one; their products are never stored)
+  const T16 *ga[CA], *gb[CB];
+  int loffa[CA], loffb[CB];
+#pragma unroll
+  for (int r = 0; r < CA; ++r) {
+    const int c = wave * 64 + 256 * r + lane;  // chunk slot in the A stage
     const int row = c >> 2, kq = (c & 3) ^ ((row >> 2) & 3);
-    const int am = min(m0 + row, M - 1), bn = min(n0 + row, N - 1);
-    ga[r] = A + (long)am * lda + 8 * kq;
-    gb[r] = B + (long)bn * ldb + 8 * kq;
-    loff[r] = (wave * 64 + 256 * r) * 16;
+    ga[r] = A + (long)min(m0 + row, M - 1) * lda + 8 * kq;
+    loffa[r] = (wave * 64 + 256 * r) * 16;
+  }
+#pragma unroll
+  for (int r = 0; r < CB; ++r) {
+    const int c = wave * 64 + 256 * r + lane;  // chunk slot in the B stage
+    const int row = c >> 2, kq = (c & 3) ^ ((row >> 2) & 3);
+    gb[r] = B + (long)min(n0 + row, N - 1) * ldb + 8 * kq;
+    loffb[r] = HALF + (wave * 64 + 256 * r) * 16;
   }
   const int nk = K / BK;
   auto issue = [&](int kc) {
     const int kk = (kc < nk ? kc : nk - 1) * BK;  // past the end: reload the last stage
     char *st = smem + (kc % NST) * STAGE;
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
-      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(ga[r] + kk), (lds_ptr_t)(st + loff[r]), 16, 0, 0);
+    for (int r = 0; r < CA; ++r)
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(ga[r] + kk), (lds_ptr_t)(st + loffa[r]), 16, 0,
+                                       0);
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
-      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(gb[r] + kk), (lds_ptr_t)(st + HALF + loff[r]),
-                                       16, 0, 0);
+    for (int r = 0; r < CB; ++r)
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(gb[r] + kk), (lds_ptr_t)(st + loffb[r]), 16, 0,
+                                       0);
   };
   int aoff[4], boff[4];
 #pragma unroll
@@ -116,16 +160,12 @@ __global__ __launch_bounds__(256, 2) void hgemm_nt_kernel(int M, int N, int K, f
     boff[i] = HALF + n * 64 + ((lq ^ ((n >> 2) & 3)) << 4);
   }
 
-  float4_t acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
-
 #pragma unroll
   for (int st = 0; st < NST - 1; ++st) issue(st);
   for (int kc = 0; kc < nk; ++kc) {
-    wait_vm_lp<4 * (NST - 2)>();
+    // the C prefetch (CINIT) was issued before the stages, so waiting for
+    // stage kc also waits for it on the first iteration
+    wait_vm_lp<(CA + CB) * (NST - 2)>();
     __asm__ volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     __asm__ volatile("" ::: "memory");
@@ -139,42 +179,35 @@ __global__ __launch_bounds__(256, 2) void hgemm_nt_kernel(int M, int N, int K, f
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = Lp<T16>::mfma(a[i], b[j], acc[i][j]);
+      for (int j = 0; j < 4; ++j) acc[i][j] = Lp<T16>::mfma(b[i], a[j], acc[i][j]);
   }
   wait_vm_lp<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
 
-  // epilogue: C = alpha * acc + beta * C; the C values of row block i+1 are
-  // requested before block i is combined and stored
-  const bool full = m0 + BM <= M && n0 + BN <= N;
-  auto inb = [&](int row, int col) { return full || (row < M && col < N); };
-  float cur[4][4], nxt[4][4];
-  auto fetch = [&](int i, float (&dst)[4][4]) {
+  // epilogue: CINIT stores the accumulators; otherwise C = alpha acc + beta C
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm0 + 16 * i + 4 * lq + v, col = n0 + wn0 + 16 * j + lr;
-        dst[j][v] = (beta != 0.f && inb(row, col)) ? C[(long)row * ldc + col] : 0.f;
+    for (int j = 0; j < 4; ++j) {
+      const int row = crow(j), col = ccol(i);
+      float *cp = C + (long)row * ldc + col;
+      if (VEC && full) {
+        float4_t v = acc[i][j];
+        if (!CINIT) {
+          const float4_t old = beta != 0.f ? *reinterpret_cast<const float4_t *>(cp)
+                                           : (float4_t){0.f, 0.f, 0.f, 0.f};
+          v = alpha * v + beta * old;
+        }
+        *reinterpret_cast<float4_t *>(cp) = v;
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (row < M && col + r < N) {
+            float v = acc[i][j][r];
+            if (!CINIT) v = alpha * v + (beta != 0.f ? beta * cp[r] : 0.f);
+            cp[r] = v;
+          }
       }
-  };
-  fetch(0, cur);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    if (i + 1 < 4) fetch(i + 1, nxt);
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm0 + 16 * i + 4 * lq + v, col = n0 + wn0 + 16 * j + lr;
-        if (inb(row, col)) C[(long)row * ldc + col] = alpha * acc[i][j][v] + beta * cur[j][v];
-      }
-    if (i + 1 < 4) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) cur[j][v] = nxt[j][v];
     }
-  }
 }
 
 template <typename T16>
@@ -185,8 +218,32 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
       ((uintptr_t)B & 15))
     return -1;
   const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128;
-  hipLaunchKernelGGL((hgemm_nt_kernel<T16, 4>), dim3(tiles_m * tiles_n), dim3(256), 0, s, M, N, K,
-                     alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+  const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;
+  const bool cinit = alpha == 1.f && beta == 1.f;
+  static int cfg = -1;
+  if (cfg < 0) {
+    const char *e = getenv("CHARMLU_HGEMM_CFG");
+    cfg = e ? atoi(e) : 0;
+  }
+#define CLU_HGEMM(BMv, NS, OC, CI, V)                                                      \
+  hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, NS, OC, CI, V>),                          \
+                     dim3(((M + BMv - 1) / BMv) * ((N + 16384 / BMv - 1) / (16384 / BMv))),  \
+                     dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,          \
+                     (M + BMv - 1) / BMv, (N + 16384 / BMv - 1) / (16384 / BMv))
+#define CLU_HGEMM_CFG(BMv, NS, OC)                      \
+  if (cinit && vec) CLU_HGEMM(BMv, NS, OC, true, true);  \
+  else if (cinit) CLU_HGEMM(BMv, NS, OC, true, false);   \
+  else if (vec) CLU_HGEMM(BMv, NS, OC, false, true);     \
+  else CLU_HGEMM(BMv, NS, OC, false, false);
+  (void)tiles_m;
+  (void)tiles_n;
+  // 0: 128 x 128 tiles, 2 stages, 4 workgroups per CU (fastest measured at
+  // 65280 x 65280 x 256: 212 TFLOP/s vs 188 with 4 stages / 2 per CU)
+  if (cfg == 1) { CLU_HGEMM_CFG(128, 4, 2) }
+  else if (cfg == 2) { CLU_HGEMM_CFG(64, 2, 4) }
+  else { CLU_HGEMM_CFG(128, 2, 4) }
+#undef CLU_HGEMM_CFG
+#undef CLU_HGEMM
   CLU_LAUNCHED("hgemm_nt_kernel");
   return CLU_OK;
 }
@@ -195,16 +252,17 @@ template int hgemm_nt<__bf16>(int, int, int, float, const __bf16 *, long, const 
 template int hgemm_nt<_Float16>(int, int, int, float, const _Float16 *, long, const _Float16 *,
                                 long, float, float *, long, hipStream_t);
 
-// dst (M x Kp, row-major) = src (M x K) rounded to 16 bits, zero for k >= K
+// dst (M x Kp, row-major) = scale * src (M x K) rounded to 16 bits, zero for
+// k >= K (scale = -1 folds the update's minus sign into the operand)
 template <typename T16>
-__global__ void pack_rows_kernel(int M, int K, int Kp, const float *__restrict__ src, long lds,
-                                 T16 *__restrict__ dst) {
+__global__ void pack_rows_kernel(int M, int K, int Kp, float scale, const float *__restrict__ src,
+                                 long lds, T16 *__restrict__ dst) {
   const long total = (long)M * Kp;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x) {
     const long i = e / Kp;
     const int k = (int)(e % Kp);
-    dst[e] = Lp<T16>::from(k < K ? src[i * lds + k] : 0.f);
+    dst[e] = Lp<T16>::from(k < K ? scale * src[i * lds + k] : 0.f);
   }
 }
 
@@ -229,12 +287,13 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(int K, int N, int Kp,
 }
 
 template <typename T16>
-int pack_rows(int M, int K, int Kp, const float *src, long lds, T16 *dst, hipStream_t s) {
+int pack_rows(int M, int K, int Kp, float scale, const float *src, long lds, T16 *dst,
+              hipStream_t s) {
   if (M <= 0 || Kp <= 0) return CLU_OK;
   long blocks = ((long)M * Kp + 255) / 256;
   if (blocks > 65536L * 16) blocks = 65536L * 16;
   hipLaunchKernelGGL((pack_rows_kernel<T16>), dim3((unsigned)blocks), dim3(256), 0, s, M, K, Kp,
-                     src, lds, dst);
+                     scale, src, lds, dst);
   CLU_LAUNCHED("pack_rows_kernel");
   return CLU_OK;
 }
@@ -246,8 +305,9 @@ int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStr
   CLU_LAUNCHED("pack_cols_kernel");
   return CLU_OK;
 }
-template int pack_rows<__bf16>(int, int, int, const float *, long, __bf16 *, hipStream_t);
-template int pack_rows<_Float16>(int, int, int, const float *, long, _Float16 *, hipStream_t);
+template int pack_rows<__bf16>(int, int, int, float, const float *, long, __bf16 *, hipStream_t);
+template int pack_rows<_Float16>(int, int, int, float, const float *, long, _Float16 *,
+                                 hipStream_t);
 template int pack_cols<__bf16>(int, int, int, const float *, long, __bf16 *, hipStream_t);
 template int pack_cols<_Float16>(int, int, int, const float *, long, _Float16 *, hipStream_t);
 

@@ -28,7 +28,8 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
 // dst (M x Kp) = 16-bit src (M x K) padded with zeros; dst (N x Kp) = 16-bit
 // transpose of src (K x N) padded with zeros.
 template <typename T16>
-int pack_rows(int M, int K, int Kp, const float *src, long lds, T16 *dst, hipStream_t s);
+int pack_rows(int M, int K, int Kp, float scale, const float *src, long lds, T16 *dst,
+              hipStream_t s);
 template <typename T16>
 int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s);
 

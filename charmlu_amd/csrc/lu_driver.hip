@@ -253,6 +253,23 @@ int gemm_t<float>(bool ta, bool tb, int M, int N, int K, float alpha, const floa
   return sgemm(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
+// B = L^{-1} B, L unit lower M x M (any M): 256-row diagonal solves plus
+// GEMM updates of the rows below.
+template <typename T>
+static int trsm_llnu_big(int M, int N, const T *L, long ldl, T *B, long ldb, hipStream_t s) {
+  for (int i0 = 0; i0 < M; i0 += 256) {
+    const int mi = M - i0 < 256 ? M - i0 : 256;
+    int rc = trsm_llnu<T>(mi, N, L + (long)i0 * ldl + i0, ldl, B + (long)i0 * ldb, ldb, s);
+    if (rc) return rc;
+    if (i0 + mi < M) {
+      rc = gemm_t<T>(false, false, M - i0 - mi, N, mi, (T)-1, L + (long)(i0 + mi) * ldl + i0, ldl,
+                     B + (long)i0 * ldb, ldb, (T)1, B + (long)(i0 + mi) * ldb, ldb, s);
+      if (rc) return rc;
+    }
+  }
+  return CLU_OK;
+}
+
 // Factor rows [c0, n) x columns [c0, c0+w) of the panel [pc0, pc0+pw)
 // (recursive halving down to 16-column strips factored by panel_base, which
 // also applies each strip's interchanges to the rest of the panel, so both
@@ -267,33 +284,12 @@ static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, 
   int rc = factor_panel<T>(A, lda, n, c0, w1, pc0, pcw, ipiv, ws, s);
   if (rc) return rc;
   T *A11 = A + (long)c0 * lda + c0;
-  if ((rc = trsm_llnu<T>(w1, w2, A11, lda, A11 + w1, lda, s))) return rc;
+  if ((rc = trsm_llnu_big<T>(w1, w2, A11, lda, A11 + w1, lda, s))) return rc;
   if (n - c0 - w1 > 0)
     if ((rc = gemm_t<T>(false, false, n - c0 - w1, w2, w1, (T)-1, A11 + (long)w1 * lda, lda,
                         A11 + w1, lda, (T)1, A11 + (long)w1 * lda + w1, lda, s)))
       return rc;
   return factor_panel<T>(A, lda, n, c0 + w1, w2, pc0, pcw, ipiv, ws, s);
-}
-
-// B = L^{-1} B, L unit lower M x M (any M): 256-row diagonal solves plus
-// GEMM updates of the rows below.
-template <typename T>
-static int trsm_llnu_big(int M, int N, const T *L, long ldl, T *B, long ldb, hipStream_t s);
-
-template <>
-int trsm_llnu_big<double>(int M, int N, const double *L, long ldl, double *B, long ldb,
-                          hipStream_t s) {
-  for (int i0 = 0; i0 < M; i0 += 256) {
-    const int mi = M - i0 < 256 ? M - i0 : 256;
-    int rc = trsm_llnu<double>(mi, N, L + (long)i0 * ldl + i0, ldl, B + (long)i0 * ldb, ldb, s);
-    if (rc) return rc;
-    if (i0 + mi < M) {
-      rc = dgemm(false, false, M - i0 - mi, N, mi, -1.0, L + (long)(i0 + mi) * ldl + i0, ldl,
-                 B + (long)i0 * ldb, ldb, 1.0, B + (long)(i0 + mi) * ldb, ldb, s);
-      if (rc) return rc;
-    }
-  }
-  return CLU_OK;
 }
 
 // move lists of ipiv[k1, k2) in runs of RUN pivots
@@ -481,6 +477,40 @@ int charmlu_dgemm_dev(int transa, int transb, int M, int N, int K, double alpha,
   if (ldb < ((tb ? K : N) > 1 ? (tb ? K : N) : 1)) return -10;
   if (ldc < (N > 1 ? N : 1)) return -13;
   return dgemm(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, (hipStream_t)stream);
+}
+
+int charmlu_sgemm_dev(int transa, int transb, int M, int N, int K, float alpha, const float *dA,
+                      long lda, const float *dB, long ldb, float beta, float *dC, long ldc,
+                      void *stream) {
+  if (transa != CHARMLU_NO_TRANS && transa != CHARMLU_TRANS) return -1;
+  if (transb != CHARMLU_NO_TRANS && transb != CHARMLU_TRANS) return -2;
+  if (M < 0) return -3;
+  if (N < 0) return -4;
+  if (K < 0) return -5;
+  const bool ta = transa == CHARMLU_TRANS, tb = transb == CHARMLU_TRANS;
+  if (lda < ((ta ? M : K) > 1 ? (ta ? M : K) : 1)) return -8;
+  if (ldb < ((tb ? K : N) > 1 ? (tb ? K : N) : 1)) return -10;
+  if (ldc < (N > 1 ? N : 1)) return -13;
+  return sgemm(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, (hipStream_t)stream);
+}
+
+int charmlu_gemm_lp_dev(int prec, int M, int N, int K, float alpha, const void *dA, long lda,
+                        const void *dB, long ldb, float beta, float *dC, long ldc,
+                        void *stream) {
+  if (prec != CHARMLU_PREC_BF16 && prec != CHARMLU_PREC_FP16) return -1;
+  if (M < 0) return -2;
+  if (N < 0) return -3;
+  if (K < 0 || K % 32 != 0) return -4;
+  if (lda < (K > 1 ? K : 1) || lda % 8 != 0) return -7;
+  if (ldb < (K > 1 ? K : 1) || ldb % 8 != 0) return -9;
+  if (ldc < (N > 1 ? N : 1)) return -12;
+  if (M == 0 || N == 0 || K == 0) return CLU_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == CHARMLU_PREC_BF16)
+    return hgemm_nt<__bf16>(M, N, K, alpha, (const __bf16 *)dA, lda, (const __bf16 *)dB, ldb,
+                            beta, dC, ldc, s);
+  return hgemm_nt<_Float16>(M, N, K, alpha, (const _Float16 *)dA, lda, (const _Float16 *)dB, ldb,
+                            beta, dC, ldc, s);
 }
 
 // op(T) is M x M; for M > 256 the solve proceeds in 256-row blocks with
@@ -889,23 +919,24 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     if (r) return r;
     if (r0 + nb < n) {
       if ((r = set_identity_t<float>(winv, nb, nb, sp))) return r;
-      r = trsm_llnu<float>(nb, nb, dA + (long)r0 * lda + r0, lda, winv, nb, sp);
+      r = trsm_llnu_big<float>(nb, nb, dA + (long)r0 * lda + r0, lda, winv, nb, sp);
     }
     prof_end(pid, sp);
     return r;
   };
   // C (M x N) -= L21 (M x nb) * U (nb x N); U column c0 is column c0 of the
-  // packed transposed operand
+  // packed transposed operand; the 16-bit L21 is packed negated, so the
+  // low-precision kernel accumulates onto C (C += (-L21) U)
   auto update = [&](int M, int N, const float *L, int c0, float *C, int Nt) -> int {
     const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb, sm);
     int r;
     if (!lp)
       r = sgemm(false, false, M, N, nb, -1.0f, L, lda, ubuf + c0, Nt, 1.0f, C, lda, sm);
     else if (prec == CHARMLU_PREC_BF16)
-      r = hgemm_nt<__bf16>(M, N, Kp, -1.0f, (const __bf16 *)ws.pa16, Kp,
+      r = hgemm_nt<__bf16>(M, N, Kp, 1.0f, (const __bf16 *)ws.pa16, Kp,
                            (const __bf16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, sm);
     else
-      r = hgemm_nt<_Float16>(M, N, Kp, -1.0f, (const _Float16 *)ws.pa16, Kp,
+      r = hgemm_nt<_Float16>(M, N, Kp, 1.0f, (const _Float16 *)ws.pa16, Kp,
                              (const _Float16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, sm);
     prof_end(pid, sm);
     return r;
@@ -929,11 +960,12 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     if (lp) {
       // operands of the trailing update, rounded to 16 bits once per step
       if (prec == CHARMLU_PREC_BF16) {
-        if ((rc = pack_rows<__bf16>(Nt, nb, Kp, A11 + (long)nb * lda, lda, (__bf16 *)ws.pa16, sm)))
+        if ((rc = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, A11 + (long)nb * lda, lda,
+                                    (__bf16 *)ws.pa16, sm)))
           return rc;
         if ((rc = pack_cols<__bf16>(nb, Nt, Kp, ubuf, Nt, (__bf16 *)ws.pu16, sm))) return rc;
       } else {
-        if ((rc = pack_rows<_Float16>(Nt, nb, Kp, A11 + (long)nb * lda, lda,
+        if ((rc = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, A11 + (long)nb * lda, lda,
                                       (_Float16 *)ws.pa16, sm)))
           return rc;
         if ((rc = pack_cols<_Float16>(nb, Nt, Kp, ubuf, Nt, (_Float16 *)ws.pu16, sm))) return rc;

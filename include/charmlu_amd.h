@@ -152,6 +152,22 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
 int charmlu_dlag2s(long m, long n, const double *dA, long lda, float *dS, long lds,
                    void *stream);
 
+/* charmlu_dgemm_dev in FP32 (the FP32 form of the differentiable solve
+ * forms its matrix gradient with it; no reference counterpart). */
+int charmlu_sgemm_dev(int transa, int transb, int M, int N, int K, float alpha,
+                      const float *dA, long lda, const float *dB, long ldb,
+                      float beta, float *dC, long ldc, void *stream);
+
+/* The low-precision trailing update of charmlu_sgetrf_mixed as a block op
+ * (updateMatrix, lu.C:26-43, at reduced input precision):
+ * C = alpha * A * B^T + beta * C with A (M x K) and B (N x K) row-major BF16
+ * (prec = PREC_BF16) or FP16 (PREC_FP16), C FP32, products accumulated in
+ * FP32.  K a multiple of 32, lda and ldb multiples of 8, A and B 16-byte
+ * aligned. */
+int charmlu_gemm_lp_dev(int prec, int M, int N, int K, float alpha, const void *dA,
+                        long lda, const void *dB, long ldb, float beta, float *dC,
+                        long ldc, void *stream);
+
 /* charmlu_dgetrs on FP32 factors and an FP32 right-hand side. */
 int charmlu_sgetrs(int trans, int n, int nrhs, const float *dLU, long lda,
                    const int *dipiv, float *dB, long ldb, void *stream);

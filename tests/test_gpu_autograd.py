@@ -1,0 +1,150 @@
+"""The differentiable solve (charmlu_amd.autograd.LUSolve) on the GPU.
+
+The backward pass must equal
+  * the analytic adjoint built from the FP64 oracle's factors:
+    A^T Lambda = W (LAPACK getrs with trans = T on the oracle's LU and
+    pivots), dA = -Lambda X^T, dB = Lambda;
+  * FP64 central finite differences of loss = sum(W * X).
+The test matrix is A = Q1 diag(s) Q2^T, each Q a product of 8 Householder
+reflectors from a seeded normal stream, s log-spaced 1..1e4 (condition
+number 1e4), 64 right-hand sides.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+pytestmark = pytest.mark.gpu
+
+
+def householder_matrix(n, seed, nref=8, smax=1e4):
+    """A = Q1 diag(s) Q2^T with Q = H_1 ... H_nref, H = I - 2 v v^T / v^T v."""
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    s = torch.logspace(0, float(np.log10(smax)), n, dtype=torch.float64)
+    A = torch.diag(s).cuda()
+    vs = [torch.randn(n, dtype=torch.float64, generator=g).cuda() for _ in range(2 * nref)]
+    for v in vs[nref:]:          # right factor: A <- A H (H symmetric)
+        A.addr_(A @ v, v, alpha=-2.0 / float(v @ v))
+    for v in vs[:nref]:          # left factor: A <- H A
+        A.addr_(v, v @ A, alpha=-2.0 / float(v @ v))
+    return A
+
+
+def _loss_and_grads(dev, A, B, W):
+    import torch
+    Ar = A.clone().requires_grad_(True)
+    Br = B.clone().requires_grad_(True)
+    X = dev.solve(Ar, Br)
+    loss = (W * X).sum()
+    loss.backward()
+    return X.detach(), Ar.grad, Br.grad
+
+
+@pytest.mark.parametrize("n,k,block", [(256, 4, 64), (520, 3, 8), (384, 1, 128)])
+def test_backward_matches_adjoint_small(dev, n, k, block):
+    import torch
+    torch.manual_seed(n)
+    A = householder_matrix(n, seed=n, nref=4, smax=1e3)
+    B = torch.randn(n, k, dtype=torch.float64, device="cuda")
+    W = torch.randn(n, k, dtype=torch.float64, device="cuda")
+    Ar = A.clone().requires_grad_(True)
+    Br = B.clone().requires_grad_(True)
+    X = dev.solve(Ar, Br, block=block)
+    (W * X).sum().backward()
+    An, Bn, Wn = A.cpu().numpy(), B.cpu().numpy(), W.cpu().numpy()
+    Xr = np.linalg.solve(An, Bn)
+    Lam = np.linalg.solve(An.T, Wn)
+    assert np.abs(X.detach().cpu().numpy() - Xr).max() / np.abs(Xr).max() < 1e-11
+    dA = Ar.grad.cpu().numpy()
+    assert np.abs(dA + Lam @ Xr.T).max() / np.abs(Lam @ Xr.T).max() < 1e-11
+    assert np.abs(Br.grad.cpu().numpy() - Lam).max() / np.abs(Lam).max() < 1e-11
+
+
+def test_backward_vector_rhs(dev):
+    import torch
+    n = 300
+    A = householder_matrix(n, seed=5, nref=2, smax=1e2)
+    b = torch.randn(n, dtype=torch.float64, device="cuda")
+    w = torch.randn(n, dtype=torch.float64, device="cuda")
+    Ar = A.clone().requires_grad_(True)
+    br = b.clone().requires_grad_(True)
+    x = dev.solve(Ar, br)
+    (w * x).sum().backward()
+    An = A.cpu().numpy()
+    lam = np.linalg.solve(An.T, w.cpu().numpy())
+    xr = np.linalg.solve(An, b.cpu().numpy())
+    assert np.abs(Ar.grad.cpu().numpy() + np.outer(lam, xr)).max() < 1e-11 * np.abs(
+        np.outer(lam, xr)).max()
+    assert np.abs(br.grad.cpu().numpy() - lam).max() < 1e-11 * np.abs(lam).max()
+
+
+def test_singular_matrix_raises(dev):
+    import torch
+    A = torch.zeros(64, 64, dtype=torch.float64, device="cuda")
+    B = torch.ones(64, 2, dtype=torch.float64, device="cuda")
+    with pytest.raises(torch.linalg.LinAlgError):
+        dev.solve(A, B)
+
+
+def test_c4_fp64_vs_oracle_adjoint_and_finite_differences(dev, orc):
+    """n = 8192, 64 right-hand sides, condition number 1e4."""
+    import torch
+    n, k, block = 8192, 64, 256
+    A = householder_matrix(n, seed=8192)
+    g = torch.Generator(device="cpu").manual_seed(64)
+    B = torch.randn(n, k, dtype=torch.float64, generator=g).cuda()
+    W = torch.randn(n, k, dtype=torch.float64, generator=g).cuda()
+    X, dA, dB = _loss_and_grads(dev, A, B, W)
+    # analytic adjoint from the oracle's FP64 factors
+    An = A.cpu().numpy()
+    info, ipiv = orc.getrf(LU := An.copy(), block, nthreads=8)
+    assert info == 0
+    Xr = sl.lu_solve((LU, ipiv), B.cpu().numpy())
+    Lam = sl.lu_solve((LU, ipiv), W.cpu().numpy(), trans=1)
+    rel = lambda a, b: np.abs(a - b).max() / np.abs(b).max()
+    e_x = rel(X.cpu().numpy(), Xr)
+    dA_ref = -(Lam @ Xr.T)
+    e_a = rel(dA.cpu().numpy(), dA_ref)
+    e_b = rel(dB.cpu().numpy(), Lam)
+    print(f"C4 fp64: x {e_x:.2e} dA {e_a:.2e} dB {e_b:.2e}")
+    assert e_x < 1e-10 and e_a < 1e-10 and e_b < 1e-10
+    # central finite differences of the loss at the largest-gradient entries
+    rng = np.random.default_rng(1)
+    cand = rng.integers(0, n, size=(256, 2))
+    vals = np.abs(dA_ref[cand[:, 0], cand[:, 1]])
+    pick = cand[np.argsort(vals)[-4:]]
+
+    def loss(Ap, Bp):
+        return float((W * dev.solve(Ap, Bp)).sum().item())
+    # ||A^-1|| = 1 here, so a step of 1e-3 keeps the central difference's
+    # truncation error (~h^2 (A^-1)_ji^2 relative) below 1e-6 while the
+    # rounding noise of the loss, ~1e-13 |loss|, stays far below h
+    for i, j in pick:
+        h = 1e-3
+        Ap = A.clone(); Ap[i, j] += h
+        Am = A.clone(); Am[i, j] -= h
+        fd = (loss(Ap, B) - loss(Am, B)) / (2 * h)
+        assert abs(fd - dA_ref[i, j]) <= 1e-5 * abs(dA_ref[i, j]), (i, j, fd, dA_ref[i, j])
+    for i, j in [(17, 3), (5000, 63)]:
+        h = 1e-3
+        Bp = B.clone(); Bp[i, j] += h
+        Bm = B.clone(); Bm[i, j] -= h
+        fd = (loss(A, Bp) - loss(A, Bm)) / (2 * h)
+        assert abs(fd - Lam[i, j]) <= 1e-5 * abs(Lam[i, j]) + 1e-9 * np.abs(Lam).max()
+
+
+def test_c4_fp32_vs_fp64_adjoint(dev):
+    """The FP32 form: same problem, FP32 factors and FP32 gradient; error
+    bounded by condition number x FP32 unit roundoff."""
+    import torch
+    n, k = 8192, 64
+    A = householder_matrix(n, seed=8192)
+    g = torch.Generator(device="cpu").manual_seed(64)
+    B = torch.randn(n, k, dtype=torch.float64, generator=g).cuda()
+    W = torch.randn(n, k, dtype=torch.float64, generator=g).cuda()
+    X64, dA64, dB64 = _loss_and_grads(dev, A, B, W)
+    X32, dA32, dB32 = _loss_and_grads(dev, A.float(), B.float(), W.float())
+    rel = lambda a, b: float(((a.double() - b).abs().max() / b.abs().max()).item())
+    e = (rel(X32, X64), rel(dA32, dA64), rel(dB32, dB64))
+    print(f"C4 fp32 vs fp64: x {e[0]:.2e} dA {e[1]:.2e} dB {e[2]:.2e}")
+    assert max(e) < 1e4 * 2.0 ** -24 * 64

@@ -39,8 +39,43 @@ def _backward_error(A, LU, ipiv):
     return np.abs(PA - L @ U).max() / np.abs(A).max()
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("M,N,K,alpha,beta,ldc_pad", [
+    (512, 384, 256, 1.0, 1.0, 0), (300, 200, 64, 1.0, 1.0, 0), (256, 256, 32, -1.0, 1.0, 0),
+    (129, 257, 96, 0.5, -2.0, 3), (1000, 130, 160, 1.0, 0.0, 1), (128, 128, 256, 1.0, 1.0, 4)])
+def test_gemm_lp_dev_vs_fp32(dev, prec, M, N, K, alpha, beta, ldc_pad):
+    """charmlu_gemm_lp_dev against an FP32 product of the same 16-bit
+    operands: the matrix cores accumulate in FP32, so the only difference is
+    the summation order (|err| <= K * 2^-23 * sum |a||b| + |C| terms)."""
+    import ctypes
+    import torch
+    from charmlu_amd import _lib
+    torch.manual_seed(M + N + K)
+    dt = torch.bfloat16 if prec == "bf16" else torch.float16
+    A = (torch.rand(M, K, device="cuda") - 0.5).to(dt)
+    B = (torch.rand(N, K, device="cuda") - 0.5).to(dt)
+    ldc = N + ldc_pad
+    Cfull = torch.rand(M, ldc, device="cuda")
+    C0 = Cfull.clone()
+    if beta == 0.0:
+        Cfull[:, :N] = float("nan")  # beta = 0 must not read C
+    rc = _lib.load().charmlu_gemm_lp_dev(
+        _lib.PREC_BF16 if prec == "bf16" else _lib.PREC_FP16, M, N, K, alpha,
+        ctypes.c_void_p(A.data_ptr()), K, ctypes.c_void_p(B.data_ptr()), K, beta,
+        ctypes.c_void_p(Cfull.data_ptr()), ldc, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    ref = alpha * (A.double() @ B.double().T) + (beta * C0[:, :N].double() if beta != 0 else 0)
+    bound = K * 2.0 ** -23 * (abs(alpha) * (A.double().abs() @ B.double().abs().T)) + \
+        2.0 ** -23 * (abs(beta) * C0[:, :N].double().abs() + ref.abs()) * 4
+    err = (Cfull[:, :N].double() - ref).abs()
+    assert bool((err <= bound + 1e-30).all()), float(err.max())
+    # columns past N untouched
+    assert torch.equal(Cfull[:, N:], C0[:, N:])
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
-@pytest.mark.parametrize("n,nb", [(512, 128), (1024, 256), (768, 256)])
+@pytest.mark.parametrize("n,nb", [(512, 128), (1024, 256), (768, 256), (960, 192), (1002, 167)])
 def test_mixed_factor_diagdom(dev, orc, prec, n, nb):
     import torch
     A = orc.gen_diagdom(orc.SEED_A, n)
@@ -56,11 +91,12 @@ def test_mixed_factor_diagdom(dev, orc, prec, n, nb):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
-def test_mixed_factor_uniform(dev, orc, prec):
+@pytest.mark.parametrize("n,nb", [(1024, 128), (1002, 167)])
+def test_mixed_factor_uniform(dev, orc, prec, n, nb):
     """General matrices pivot; the factors are still a backward-stable LU of
-    the FP32 input at the update precision."""
+    the FP32 input at the update precision (edge tiles and an unaligned
+    leading dimension with n = 1002)."""
     import torch
-    n, nb = 1024, 128
     A = orc.gen_uniform(orc.SEED_A, n)
     A32 = torch.from_numpy(A.astype(np.float32)).cuda()
     ipiv, info = dev.getrf_mixed(A32, nb, prec)
@@ -99,7 +135,7 @@ def test_sgetrs_both_transposes(dev, orc):
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32"])
-@pytest.mark.parametrize("n,nb", [(1024, 256), (2048, 256)])
+@pytest.mark.parametrize("n,nb", [(1024, 256), (2048, 256), (2048, 512)])
 def test_gmres_ir_matches_fp64_oracle(dev, orc, prec, n, nb):
     import torch
     A = orc.gen_diagdom(orc.SEED_A, n)
