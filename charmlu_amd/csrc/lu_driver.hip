@@ -187,7 +187,10 @@ static int get_ws(Workspace **out) {
     if (rc) return rc;
     w.pw.slots = w.slots;
     w.pw.epoch = 1;
-    w.pw.max_groups = MAX_GROUPS;
+    // a multi-workgroup panel needs all its workgroups resident at once:
+    // at most one 512-thread workgroup per CU of the panel stream
+    w.pw.max_groups = reserve > 0 ? (reserve < MAX_GROUPS ? reserve : MAX_GROUPS)
+                                  : (ncu < MAX_GROUPS ? ncu : MAX_GROUPS);
     w.device = dev;
   }
   *out = &w;
