@@ -179,13 +179,17 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps):
         torch.cuda.empty_cache()
     A32 = torch.empty((n, n), dtype=torch.float32, device="cuda")
     x = torch.empty_like(b)
-    for prec in precs:
+    for key in precs:
+        # "<prec>" factors with partial pivoting, "<prec>-nopiv" without
+        # interchanges (the matrix is diagonally dominant)
+        prec, _, mode = key.partition("-")
+        pivot = mode != "nopiv"
         split = [0.0, 0.0]
 
         def step():
             t_a = time.perf_counter()
             clu.lag2s(A, A32)
-            ipiv, info = clu.getrf_mixed(A32, nb, prec)
+            ipiv, info = clu.getrf_mixed(A32, nb, prec, pivot=pivot)
             torch.cuda.synchronize()
             t_b = time.perf_counter()
             _, st = clu.gesv_ir(A, A32, ipiv, b, x, tol=1.0)
@@ -218,10 +222,18 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps):
                "panel_stream_ms": round(p[8] / steps, 1),
                "interchange_ms": round(p[11] / steps, 1),
                "u_row_ms": round(p[14] / steps, 1)}
+        if not pivot:
+            # partial pivoting picks the same (diagonal) pivots iff every
+            # computed multiplier is below 1 in magnitude
+            rec["max_abs_l"] = float(A32.tril(-1).abs().max().item())
+            rec["pivoting"] = "none (max |l_ij| < 1: partial pivoting would agree)" \
+                if rec["max_abs_l"] < 1.0 else "none"
+        else:
+            rec["pivoting"] = "partial"
         if x64 is not None:
             rec["error_vs_fp64"] = float(((x - x64).abs().max() / x64.abs().max()).item())
             rec["speedup_vs_fp64"] = round(rec["tflops"] / out["fp64"]["tflops"], 3)
-        out[prec] = rec
+        out[key] = rec
     return out
 
 
@@ -241,7 +253,7 @@ def main():
                     help="time only the low-precision trailing update of step 0 at --mixed-n")
     ap.add_argument("--mixed-n", type=int, default=65536)
     ap.add_argument("--mixed-nb", type=int, default=512)
-    ap.add_argument("--mixed-precs", default="bf16,fp16")
+    ap.add_argument("--mixed-precs", default="bf16,fp16,bf16-nopiv")
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--mixed-fp64-steps", type=int, default=1)
     ap.add_argument("--no-mixed", action="store_true")

@@ -58,6 +58,7 @@ UPPER, LOWER = 121, 122
 NON_UNIT, UNIT = 131, 132
 LEFT, RIGHT = 141, 142
 PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16, PREC_FP8 = 0, 1, 2, 3, 4
+MIXED_NOPIV = 0x100
 
 _lib = None
 

@@ -56,7 +56,7 @@ CLU_DEV void wait_vm_lp() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
 }
 
-template <typename T16, int BM, int NST, int OCC, bool CINIT, bool VEC>
+template <typename T16, int BM, int NST, int OCC, bool CINIT, bool VEC, int NTC>
 __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K, float alpha,
                                                           const T16 *__restrict__ A, long lda,
                                                           const T16 *__restrict__ B, long ldb,
@@ -107,7 +107,8 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
       for (int j = 0; j < 4; ++j) {
         const int row = crow(j), col = ccol(i);
         if (VEC && full) {
-          acc[i][j] = *reinterpret_cast<const float4_t *>(C + (long)row * ldc + col);
+          const float4_t *cp4 = reinterpret_cast<const float4_t *>(C + (long)row * ldc + col);
+          acc[i][j] = (NTC & 1) ? __builtin_nontemporal_load(cp4) : *cp4;
         } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r)
@@ -197,7 +198,8 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
                                            : (float4_t){0.f, 0.f, 0.f, 0.f};
           v = alpha * v + beta * old;
         }
-        *reinterpret_cast<float4_t *>(cp) = v;
+        if (NTC & 2) __builtin_nontemporal_store(v, reinterpret_cast<float4_t *>(cp));
+        else *reinterpret_cast<float4_t *>(cp) = v;
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -210,6 +212,42 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
     }
 }
 
+template <typename T16, int BMv, int NS, int OC, bool CI, bool V, int NT>
+static void hgemm_launch(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B,
+                         long ldb, float beta, float *C, long ldc, hipStream_t s) {
+  constexpr int BNv = 16384 / BMv;
+  const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
+  hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, NS, OC, CI, V, NT>), dim3(tm * tn), dim3(256), 0,
+                     s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
+}
+template <typename T16, int BMv, int NS, int OC, int NT>
+static void hgemm_launch2(bool cinit, bool vec, int M, int N, int K, float alpha, const T16 *A,
+                          long lda, const T16 *B, long ldb, float beta, float *C, long ldc,
+                          hipStream_t s) {
+  if (cinit && vec)
+    hgemm_launch<T16, BMv, NS, OC, true, true, NT>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  else if (cinit)
+    hgemm_launch<T16, BMv, NS, OC, true, false, NT>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                    s);
+  else if (vec)
+    hgemm_launch<T16, BMv, NS, OC, false, true, NT>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                    s);
+  else
+    hgemm_launch<T16, BMv, NS, OC, false, false, NT>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                     s);
+}
+template <typename T16, int BMv, int NS, int OC>
+static void hgemm_launch3(int ntc, bool cinit, bool vec, int M, int N, int K, float alpha,
+                          const T16 *A, long lda, const T16 *B, long ldb, float beta, float *C,
+                          long ldc, hipStream_t s) {
+  switch (ntc) {
+    case 1: hgemm_launch2<T16, BMv, NS, OC, 1>(cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); break;
+    case 2: hgemm_launch2<T16, BMv, NS, OC, 2>(cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); break;
+    case 3: hgemm_launch2<T16, BMv, NS, OC, 3>(cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); break;
+    default: hgemm_launch2<T16, BMv, NS, OC, 0>(cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); break;
+  }
+}
+
 template <typename T16>
 int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B, long ldb,
              float beta, float *C, long ldc, hipStream_t s) {
@@ -217,33 +255,23 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
   if (K % 32 != 0 || K <= 0 || lda % 8 != 0 || ldb % 8 != 0 || ((uintptr_t)A & 15) ||
       ((uintptr_t)B & 15))
     return -1;
-  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128;
   const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;
   const bool cinit = alpha == 1.f && beta == 1.f;
-  static int cfg = -1;
+  static int cfg = -1, ntc = 0;
   if (cfg < 0) {
     const char *e = getenv("CHARMLU_HGEMM_CFG");
     cfg = e ? atoi(e) : 0;
+    const char *f = getenv("CHARMLU_HGEMM_NT");
+    ntc = f ? atoi(f) & 3 : 0;
   }
-#define CLU_HGEMM(BMv, NS, OC, CI, V)                                                      \
-  hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, NS, OC, CI, V>),                          \
-                     dim3(((M + BMv - 1) / BMv) * ((N + 16384 / BMv - 1) / (16384 / BMv))),  \
-                     dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,          \
-                     (M + BMv - 1) / BMv, (N + 16384 / BMv - 1) / (16384 / BMv))
-#define CLU_HGEMM_CFG(BMv, NS, OC)                      \
-  if (cinit && vec) CLU_HGEMM(BMv, NS, OC, true, true);  \
-  else if (cinit) CLU_HGEMM(BMv, NS, OC, true, false);   \
-  else if (vec) CLU_HGEMM(BMv, NS, OC, false, true);     \
-  else CLU_HGEMM(BMv, NS, OC, false, false);
-  (void)tiles_m;
-  (void)tiles_n;
   // 0: 128 x 128 tiles, 2 stages, 4 workgroups per CU (fastest measured at
   // 65280 x 65280 x 256: 212 TFLOP/s vs 188 with 4 stages / 2 per CU)
-  if (cfg == 1) { CLU_HGEMM_CFG(128, 4, 2) }
-  else if (cfg == 2) { CLU_HGEMM_CFG(64, 2, 4) }
-  else { CLU_HGEMM_CFG(128, 2, 4) }
-#undef CLU_HGEMM_CFG
-#undef CLU_HGEMM
+  if (cfg == 1)
+    hgemm_launch3<T16, 128, 4, 2>(ntc, cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  else if (cfg == 2)
+    hgemm_launch3<T16, 64, 2, 4>(ntc, cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  else
+    hgemm_launch3<T16, 128, 2, 4>(ntc, cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   CLU_LAUNCHED("hgemm_nt_kernel");
   return CLU_OK;
 }

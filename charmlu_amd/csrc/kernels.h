@@ -63,10 +63,11 @@ struct PanelWork {
 // Factor rows [j0, m_end) x columns [j0, j0+w) (w <= 16) of A in place with
 // partial pivoting, one column at a time; ipiv[j0..j0+w) get global rows.
 // The strip's row interchanges are also applied to the other columns
-// [pc0, j0) and [j0+w, pc1) of the panel.
+// [pc0, j0) and [j0+w, pc1) of the panel.  pivot = false: no interchanges
+// (the diagonal is the pivot; one workgroup, m_end - j0 <= 1024).
 template <typename T>
 int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *ipiv,
-               PanelWork &pw, hipStream_t s);
+               PanelWork &pw, hipStream_t s, bool pivot = true);
 
 // ---- triangular solves ----------------------------------------------------
 // B = L^{-1} B with L unit lower (M x M), B is M x N (M <= 256).

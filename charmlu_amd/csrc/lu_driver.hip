@@ -82,6 +82,9 @@ struct Workspace {
   // operands of the trailing update, and the refinement's FP64 vectors
   void *winv32 = nullptr, *ubuf32 = nullptr, *pa16 = nullptr, *pu16 = nullptr;
   size_t winv32_b = 0, ubuf32_b = 0, pa16_b = 0, pu16_b = 0;
+  // unpivoted mixed form: inv(U11) and L21 = A21 inv(U11) before it is copied back
+  void *uinv32 = nullptr, *l21buf = nullptr;
+  size_t uinv32_b = 0, l21buf_b = 0;
   void *irbuf = nullptr;
   size_t irbuf_b = 0;
   // solve: inverses of the diagonal blocks of L and U, the intermediate y
@@ -273,18 +276,35 @@ static int trsm_llnu_big(int M, int N, const T *L, long ldl, T *B, long ldb, hip
   return CLU_OK;
 }
 
+// B = U^{-1} B, U non-unit upper M x M (any M): 256-row diagonal solves
+// from the bottom up plus GEMM updates of the rows above.
+template <typename T>
+static int trsm_lunn_big(int M, int N, const T *U, long ldu, T *B, long ldb, hipStream_t s) {
+  for (int i0 = ((M - 1) / 256) * 256; i0 >= 0; i0 -= 256) {
+    const int mi = M - i0 < 256 ? M - i0 : 256;
+    int rc = trsm_lunn<T>(mi, N, U + (long)i0 * ldu + i0, ldu, B + (long)i0 * ldb, ldb, s);
+    if (rc) return rc;
+    if (i0 > 0) {
+      rc = gemm_t<T>(false, false, i0, N, mi, (T)-1, U + i0, ldu, B + (long)i0 * ldb, ldb, (T)1, B,
+                     ldb, s);
+      if (rc) return rc;
+    }
+  }
+  return CLU_OK;
+}
+
 // Factor rows [c0, n) x columns [c0, c0+w) of the panel [pc0, pc0+pw)
 // (recursive halving down to 16-column strips factored by panel_base, which
 // also applies each strip's interchanges to the rest of the panel, so both
 // halves of every split are already permuted when they are used).
 template <typename T>
 static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, int *ipiv,
-                        Workspace &ws, hipStream_t s) {
-  if (w <= 16) return panel_base<T>(A, lda, c0, n, w, pc0, pc0 + pcw, ipiv, ws.pw, s);
+                        Workspace &ws, hipStream_t s, bool pivot = true) {
+  if (w <= 16) return panel_base<T>(A, lda, c0, n, w, pc0, pc0 + pcw, ipiv, ws.pw, s, pivot);
   int w1 = (w / 2 + 15) / 16 * 16;
   if (w1 >= w) w1 = w / 2;
   const int w2 = w - w1;
-  int rc = factor_panel<T>(A, lda, n, c0, w1, pc0, pcw, ipiv, ws, s);
+  int rc = factor_panel<T>(A, lda, n, c0, w1, pc0, pcw, ipiv, ws, s, pivot);
   if (rc) return rc;
   T *A11 = A + (long)c0 * lda + c0;
   if ((rc = trsm_llnu_big<T>(w1, w2, A11, lda, A11 + w1, lda, s))) return rc;
@@ -292,7 +312,7 @@ static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, 
     if ((rc = gemm_t<T>(false, false, n - c0 - w1, w2, w1, (T)-1, A11 + (long)w1 * lda, lda,
                         A11 + w1, lda, (T)1, A11 + (long)w1 * lda + w1, lda, s)))
       return rc;
-  return factor_panel<T>(A, lda, n, c0 + w1, w2, pc0, pcw, ipiv, ws, s);
+  return factor_panel<T>(A, lda, n, c0 + w1, w2, pc0, pcw, ipiv, ws, s, pivot);
 }
 
 // move lists of ipiv[k1, k2) in runs of RUN pivots
@@ -875,9 +895,15 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     set_error("block size %d must divide n = %d (benchmark.C:143-147)", nb, n);
     return -2;
   }
+  const bool nopiv = (prec & CHARMLU_MIXED_NOPIV) != 0;
+  prec &= ~CHARMLU_MIXED_NOPIV;
   if (prec != CHARMLU_PREC_FP32 && prec != CHARMLU_PREC_BF16 && prec != CHARMLU_PREC_FP16) {
     set_error("trailing-update precision %d is not supported", prec);
     return -3;
+  }
+  if (nopiv && nb > 1024) {
+    set_error("the unpivoted form needs block size <= 1024 (got %d)", nb);
+    return -2;
   }
   if (!dA && n > 0) return -4;
   if (lda < (n > 1 ? n : 1)) return -5;
@@ -903,6 +929,10 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if ((rc = ensure_bytes(&ws.pa16, &ws.pa16_b, (size_t)2 * n * Kp))) return rc;
       if ((rc = ensure_bytes(&ws.pu16, &ws.pu16_b, (size_t)2 * n * Kp))) return rc;
     }
+    if (nopiv) {
+      if ((rc = ensure_bytes(&ws.uinv32, &ws.uinv32_b, sizeof(float) * nb * nb))) return rc;
+      if ((rc = ensure_bytes(&ws.l21buf, &ws.l21buf_b, sizeof(float) * nb * n))) return rc;
+    }
   }
   float *winv = (float *)ws.winv32, *ubuf = (float *)ws.ubuf32;
   ws.pw.info = dinfo;
@@ -916,10 +946,23 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   auto panel = [&](int t) -> int {
     const int r0 = t * nb;
     const int pid = prof_begin(PK_PANEL, (double)(n - r0) * nb, sp);
-    int r = factor_panel<float>(dA, lda, n, r0, nb, r0, nb, dipiv, ws, sp);
-    if (r) return r;
-    r = build_moves(dipiv, r0, r0 + nb, moves_of(t), counts_of(t), sp);
-    if (r) return r;
+    int r;
+    if (nopiv) {
+      // without interchanges the panel splits into the LU of the diagonal
+      // block (here) and L21 = A21 inv(U11), one GEMM over all rows below
+      // that the update stream runs with all its CUs at the start of step t
+      float *A11 = dA + (long)r0 * lda + r0;
+      if ((r = factor_panel<float>(dA, lda, r0 + nb, r0, nb, r0, nb, dipiv, ws, sp, false)))
+        return r;
+      if (r0 + nb < n) {
+        float *uinv = (float *)ws.uinv32;
+        if ((r = set_identity_t<float>(uinv, nb, nb, sp))) return r;
+        if ((r = trsm_lunn_big<float>(nb, nb, A11, lda, uinv, nb, sp))) return r;
+      }
+    } else {
+      if ((r = factor_panel<float>(dA, lda, n, r0, nb, r0, nb, dipiv, ws, sp))) return r;
+      if ((r = build_moves(dipiv, r0, r0 + nb, moves_of(t), counts_of(t), sp))) return r;
+    }
     if (r0 + nb < n) {
       if ((r = set_identity_t<float>(winv, nb, nb, sp))) return r;
       r = trsm_llnu_big<float>(nb, nb, dA + (long)r0 * lda + r0, lda, winv, nb, sp);
@@ -930,11 +973,11 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   // C (M x N) -= L21 (M x nb) * U (nb x N); U column c0 is column c0 of the
   // packed transposed operand; the 16-bit L21 is packed negated, so the
   // low-precision kernel accumulates onto C (C += (-L21) U)
-  auto update = [&](int M, int N, const float *L, int c0, float *C, int Nt) -> int {
+  auto update = [&](int M, int N, const float *L, long ldl, int c0, float *C, int Nt) -> int {
     const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb, sm);
     int r;
     if (!lp)
-      r = sgemm(false, false, M, N, nb, -1.0f, L, lda, ubuf + c0, Nt, 1.0f, C, lda, sm);
+      r = sgemm(false, false, M, N, nb, -1.0f, L, ldl, ubuf + c0, Nt, 1.0f, C, lda, sm);
     else if (prec == CHARMLU_PREC_BF16)
       r = hgemm_nt<__bf16>(M, N, Kp, 1.0f, (const __bf16 *)ws.pa16, Kp,
                            (const __bf16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, sm);
@@ -951,25 +994,36 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     const int r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
     CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));
     int pid = prof_begin(PK_SWAP, (double)(n - nb), sm);
-    if ((rc = apply_runs<float>(nruns, moves_of(t), counts_of(t), dA, lda, 0, n, r0, r1, false,
-                                sm)))
+    if (!nopiv && (rc = apply_runs<float>(nruns, moves_of(t), counts_of(t), dA, lda, 0, n, r0, r1,
+                                          false, sm)))
       return rc;
     prof_end(pid, sm);
     if (Nt <= 0) break;
     float *A11 = dA + (long)r0 * lda + r0;
+    // L21: in the matrix, or (unpivoted form) A21 inv(U11) in l21buf
+    const float *L21 = A11 + (long)nb * lda;
+    long ldl21 = lda;
+    if (nopiv) {
+      float *l21 = (float *)ws.l21buf;
+      pid = prof_begin(PK_GEMM_OTHER, 2.0 * Nt * nb * nb, sm);
+      if ((rc = sgemm(false, false, Nt, nb, nb, 1.0f, A11 + (long)nb * lda, lda,
+                      (const float *)ws.uinv32, nb, 0.0f, l21, nb, sm)))
+        return rc;
+      prof_end(pid, sm);
+      L21 = l21;
+      ldl21 = nb;
+    }
     pid = prof_begin(PK_TRSM, (double)Nt * nb * (nb - 1), sm);
     if ((rc = sgemm(false, false, nb, Nt, nb, 1.0f, winv, nb, A11 + nb, lda, 0.0f, ubuf, Nt, sm)))
       return rc;
     if (lp) {
       // operands of the trailing update, rounded to 16 bits once per step
       if (prec == CHARMLU_PREC_BF16) {
-        if ((rc = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, A11 + (long)nb * lda, lda,
-                                    (__bf16 *)ws.pa16, sm)))
+        if ((rc = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, L21, ldl21, (__bf16 *)ws.pa16, sm)))
           return rc;
         if ((rc = pack_cols<__bf16>(nb, Nt, Kp, ubuf, Nt, (__bf16 *)ws.pu16, sm))) return rc;
       } else {
-        if ((rc = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, A11 + (long)nb * lda, lda,
-                                      (_Float16 *)ws.pa16, sm)))
+        if ((rc = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, L21, ldl21, (_Float16 *)ws.pa16, sm)))
           return rc;
         if ((rc = pack_cols<_Float16>(nb, Nt, Kp, ubuf, Nt, (_Float16 *)ws.pu16, sm))) return rc;
       }
@@ -977,14 +1031,16 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     prof_end(pid, sm);
     const int la = Nt < nb ? Nt : nb;
     float *C = A11 + (long)nb * lda + nb;
-    if ((rc = update(Nt, la, A11 + (long)nb * lda, 0, C, Nt))) return rc;
+    if ((rc = update(Nt, la, L21, ldl21, 0, C, Nt))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_la, sm));
     CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
     if ((rc = panel(t + 1))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_panel, sp));
     if (Nt > la)
-      if ((rc = update(Nt, Nt - la, A11 + (long)nb * lda, la, C + la, Nt))) return rc;
+      if ((rc = update(Nt, Nt - la, L21, ldl21, la, C + la, Nt))) return rc;
     if ((rc = convert<float, float>(nb, Nt, ubuf, Nt, A11 + nb, lda, sm))) return rc;
+    if (nopiv && (rc = convert<float, float>(Nt, nb, L21, ldl21, A11 + (long)nb * lda, lda, sm)))
+      return rc;
   }
   CLU_HIP(hipEventRecord(ws.ev_end, sm));
   CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));

@@ -45,7 +45,7 @@ struct alignas(16) PanelRec {
   double top[PW];
 };
 
-template <typename T, int R, int NT>
+template <typename T, int R, int NT, bool PIV>
 __global__ __launch_bounds__(NT) void panel_base_kernel(
     T *__restrict__ A, long lda, int j0, int m, int w, int pc0, int pc1,
     int *__restrict__ ipiv, PanelRec *__restrict__ recs, unsigned seq0, int *info) {
@@ -105,28 +105,31 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
     constexpr int K = decltype(kc)::value;
     const int par = c & 1;
     // ---- candidate for column c over local rows >= c (strict '>', rows
-    //      visited in increasing order, so the lowest row wins ties) ----
+    //      visited in increasing order, so the lowest row wins ties);
+    //      without pivoting the diagonal is the pivot ----
     T bv = (T)0;
     int bi = -1;
+    if (PIV) {
 #pragma clang loop unroll(full)
-    for (int r = 0; r < R; ++r) {
-      const int li = g * RPW + r * NT + t;
-      const T v = fabs(a[r][K]);
-      if (li < m && li >= c && v > bv) { bv = v; bi = li; }
+      for (int r = 0; r < R; ++r) {
+        const int li = g * RPW + r * NT + t;
+        const T v = fabs(a[r][K]);
+        if (li < m && li >= c && v > bv) { bv = v; bi = li; }
+      }
+      wave_argmax(bv, bi);
+      if (lane == 0) { s_val[par][wave] = bv; s_idx[par][wave] = bi; }
+      __syncthreads();
+      bv = s_val[par][0]; bi = s_idx[par][0];
+#pragma clang loop unroll(full)
+      for (int k = 1; k < NW; ++k)
+        if (better(s_val[par][k], s_idx[par][k], bv, bi)) { bv = s_val[par][k]; bi = s_idx[par][k]; }
     }
-    wave_argmax(bv, bi);
-    if (lane == 0) { s_val[par][wave] = bv; s_idx[par][wave] = bi; }
-    __syncthreads();
-    bv = s_val[par][0]; bi = s_idx[par][0];
-#pragma clang loop unroll(full)
-    for (int k = 1; k < NW; ++k)
-      if (better(s_val[par][k], s_idx[par][k], bv, bi)) { bv = s_val[par][k]; bi = s_idx[par][k]; }
 
     int p;
     const T *u;    // the pivot row (LDS)
     const T *top;  // the old row c (LDS)
     if (G == 1) {
-      p = bi < 0 ? c : bi;
+      p = (!PIV || bi < 0) ? c : bi;
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
         const int li = r * NT + t;
@@ -314,10 +317,11 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
 
 template <typename T>
 int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *ipiv,
-               PanelWork &pw, hipStream_t s) {
+               PanelWork &pw, hipStream_t s, bool pivot) {
   const int m = m_end - j0;
   if (m <= 0 || w <= 0) return CLU_OK;
   if (w > PW) return -1;
+  if (!pivot && m > 1024) return -3;  // the unpivoted strip is one workgroup
   // one 256-thread workgroup up to 1024 rows, then 512-thread workgroups of
   // 2048 rows (4 per thread) -- fewer participants in every column's exchange
   int R, G, NT;
@@ -328,13 +332,19 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   if (G > pw.max_groups || G > 64) return -2;
   PanelRec *recs = reinterpret_cast<PanelRec *>(pw.slots);
   const unsigned seq0 = pw.epoch;
-  if (NT == 512) {
-    hipLaunchKernelGGL((panel_base_kernel<T, 4, 512>), dim3(G), dim3(512), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info);
+  if (!pivot) {
+    switch (R) {
+      case 1: hipLaunchKernelGGL((panel_base_kernel<T, 1, 256, false>), dim3(1), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+      case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2, 256, false>), dim3(1), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+      default: hipLaunchKernelGGL((panel_base_kernel<T, 4, 256, false>), dim3(1), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+    }
+  } else if (NT == 512) {
+    hipLaunchKernelGGL((panel_base_kernel<T, 4, 512, true>), dim3(G), dim3(512), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info);
   } else {
     switch (R) {
-      case 1: hipLaunchKernelGGL((panel_base_kernel<T, 1, 256>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
-      case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2, 256>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
-      default: hipLaunchKernelGGL((panel_base_kernel<T, 4, 256>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+      case 1: hipLaunchKernelGGL((panel_base_kernel<T, 1, 256, true>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+      case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2, 256, true>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+      default: hipLaunchKernelGGL((panel_base_kernel<T, 4, 256, true>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
     }
   }
   CLU_LAUNCHED("panel_base_kernel");
@@ -342,10 +352,8 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   return CLU_OK;
 }
 
-template int panel_base<double>(double *, long, int, int, int, int, int, int *, PanelWork &,
-                                hipStream_t);
-template int panel_base<float>(float *, long, int, int, int, int, int, int *, PanelWork &,
-                               hipStream_t);
+template int panel_base<double>(double *, long, int, int, int, int, int, int *, PanelWork &, hipStream_t, bool);
+template int panel_base<float>(float *, long, int, int, int, int, int, int *, PanelWork &, hipStream_t, bool);
 
 // ---------------------------------------------------------------------------
 // swaps -> moves

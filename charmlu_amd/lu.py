@@ -110,10 +110,12 @@ def getrs(LU, ipiv, B, trans: bool = False):
 _PREC = {"fp32": _lib.PREC_FP32, "bf16": _lib.PREC_BF16, "fp16": _lib.PREC_FP16}
 
 
-def getrf_mixed(A32, block: int, prec: str = "bf16"):
+def getrf_mixed(A32, block: int, prec: str = "bf16", pivot: bool = True):
     """In-place LU of the row-major FP32 CUDA tensor A32 with an FP32 panel
     and the trailing update in ``prec`` ('fp32', 'bf16', 'fp16'; FP32
-    accumulation).  Returns (ipiv, info) like ``getrf``."""
+    accumulation).  pivot=False factors without interchanges (for matrices
+    that need none, e.g. diagonally dominant ones).  Returns (ipiv, info)
+    like ``getrf``."""
     torch = _torch()
     _require(A32, torch.float32, "A32")
     n = A32.shape[0]
@@ -123,7 +125,8 @@ def getrf_mixed(A32, block: int, prec: str = "bf16"):
         raise ValueError(f"prec must be one of {sorted(_PREC)}")
     ipiv = torch.empty(max(n, 1), dtype=torch.int32, device=A32.device)
     info = torch.zeros(1, dtype=torch.int32, device=A32.device)
-    check(_lib.load().charmlu_sgetrf_mixed(n, block, _PREC[prec], _ptr(A32), A32.stride(0),
+    code = _PREC[prec] | (0 if pivot else _lib.MIXED_NOPIV)
+    check(_lib.load().charmlu_sgetrf_mixed(n, block, code, _ptr(A32), A32.stride(0),
                                            _ptr(ipiv), _ptr(info), _stream_ptr()),
           "charmlu_sgetrf_mixed")
     return ipiv[:n], info

@@ -56,6 +56,8 @@ extern "C" {
 #define CHARMLU_PREC_BF16 2
 #define CHARMLU_PREC_FP16 3
 #define CHARMLU_PREC_FP8 4
+/* or-ed into prec of charmlu_sgetrf_mixed: factor without interchanges */
+#define CHARMLU_MIXED_NOPIV 0x100
 
 int charmlu_version(void);
 const char *charmlu_last_error(void);
@@ -142,7 +144,12 @@ int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda,
  * factored in FP32; the trailing update (updateMatrix, lu.C:26-43) runs with
  * its operands rounded to prec (PREC_FP32, PREC_BF16 or PREC_FP16; FP32
  * accumulation).  dA: n x n FP32 row-major, overwritten by L and U.  The
- * pivots and info follow charmlu_dgetrf.  Asynchronous. */
+ * pivots and info follow charmlu_dgetrf.  prec | CHARMLU_MIXED_NOPIV
+ * factors without row interchanges (ipiv[j] = j), for matrices that need
+ * none, such as diagonally dominant ones; nb <= 1024 then.  The panel is
+ * then the LU of the diagonal block plus L21 = A21 inv(U11).  If every
+ * computed |l_ij| < 1, partial pivoting would have chosen the same
+ * (diagonal) pivots.  Asynchronous. */
 int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipiv,
                          int *dinfo, void *stream);
 

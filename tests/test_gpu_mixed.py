@@ -192,3 +192,44 @@ def test_gmres_ir_already_converged_and_limits(dev, orc):
     # a loose tolerance needs no more work than a tight one
     _, st2 = dev.gesv_ir(dA, A32, ipiv, db, tol=1e3)
     assert st2["iterations"] <= st1["iterations"]
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32"])
+@pytest.mark.parametrize("n,nb", [(1024, 256), (2048, 512), (1536, 384), (1000, 200)])
+def test_unpivoted_mixed_on_diagdom(dev, orc, prec, n, nb):
+    """Without interchanges on a diagonally dominant matrix: identity
+    pivots, every |l_ij| < 1 (so partial pivoting would choose the same
+    pivots), factors close to the pivoted ones, and the refined x within
+    1e-10 of the FP64 oracle."""
+    import torch
+    A = orc.gen_diagdom(orc.SEED_A, n)
+    b = orc.gen_rhs(n)
+    dA = torch.from_numpy(A).cuda()
+    P32 = dA.float()
+    ipiv_p, _ = dev.getrf_mixed(P32, nb, prec)
+    U32 = dA.float()
+    ipiv, info = dev.getrf_mixed(U32, nb, prec, pivot=False)
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    assert np.array_equal(ipiv.cpu().numpy(), np.arange(n))
+    assert np.array_equal(ipiv_p.cpu().numpy(), np.arange(n))
+    LU = U32.cpu().numpy()
+    assert np.abs(np.tril(LU, -1)).max() < 1.0
+    be = _backward_error(A, LU, np.arange(n))
+    assert be < 8 * UNIT[prec] * (n // nb + 1)
+    diff = np.abs(LU - P32.cpu().numpy()).max() / np.abs(LU).max()
+    assert diff < 64 * UNIT[prec]
+    x_ref = orc.lu_solve_problem(n, nb, A=A.copy())["x"]
+    x, st = dev.gesv_ir(dA, U32, ipiv, torch.from_numpy(b).cuda(), tol=1.0)
+    err = np.abs(x.cpu().numpy() - x_ref).max() / np.abs(x_ref).max()
+    print(f"nopiv {prec} n={n} nb={nb}: be {be:.2e} vs pivoted {diff:.2e} {st} err {err:.2e}")
+    assert st["residual"] <= 1.0 and err < 1e-10
+
+
+def test_unpivoted_singular_block_reports_info(dev):
+    import torch
+    n, nb = 512, 128
+    A = torch.eye(n, dtype=torch.float32, device="cuda")
+    A[200, 200] = 0.0
+    ipiv, info = dev.getrf_mixed(A, nb, "bf16", pivot=False)
+    assert int(info.item()) == 201
