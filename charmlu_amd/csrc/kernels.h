@@ -52,6 +52,9 @@ template <typename T>
 int apply_moves(const int2 *moves, const int *count, int max_moves, T *A,
                 long lda, int c0, int c1, int skip0, int skip1, bool inverse,
                 hipStream_t s);
+// the interchanges ipiv[0..n) (or their inverse) applied to one vector
+template <typename T>
+int permute_vec(const int *ipiv, int n, T *v, bool inverse, hipStream_t s);
 
 // ---- panel factorization (FP64 / FP32) ------------------------------------
 struct PanelWork {

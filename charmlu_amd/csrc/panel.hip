@@ -460,6 +460,59 @@ int apply_moves(const int2 *moves, const int *count, int max_moves, T *A, long l
   return CLU_OK;
 }
 
+// The interchanges ipiv[0..n) applied to one vector in a single launch: the
+// runs of MAXRUN pivots are taken in order (reverse order and inverted moves
+// for the inverse permutation); within a run every position finds its source
+// by the same backward walk as pivots_to_moves, then all moves of the run are
+// read before any is written.
+template <typename T>
+__global__ __launch_bounds__(2 * MAXRUN) void permute_vec_kernel(const int *__restrict__ ipiv,
+                                                                 int n, T *__restrict__ v,
+                                                                 int inverse) {
+  __shared__ int piv[MAXRUN];
+  const int t = threadIdx.x;
+  const int nruns = (n + MAXRUN - 1) / MAXRUN;
+  for (int i = 0; i < nruns; ++i) {
+    const int r = inverse ? nruns - 1 - i : i;
+    const int k1 = r * MAXRUN, k2 = min(n, k1 + MAXRUN), nk = k2 - k1;
+    __syncthreads();  // the previous run's moves are written
+    if (t < nk) piv[t] = ipiv[k1 + t];
+    __syncthreads();
+    int x = -1;
+    if (t < nk) {
+      x = k1 + t;
+    } else if (t >= MAXRUN && t - MAXRUN < nk) {
+      const int q = t - MAXRUN;
+      x = piv[q];
+      bool dup = x < k2;
+      for (int j = 0; j < q && !dup; ++j) dup = piv[j] == x;
+      if (dup) x = -1;
+    }
+    int y = x;
+    for (int j = nk - 1; j >= 0 && x >= 0; --j) {
+      const int pj = piv[j], jj = k1 + j;
+      y = (y == jj) ? pj : (y == pj ? jj : y);
+    }
+    const bool mv = x >= 0 && y != x;
+    // forward: new[x] = old[y]; inverse: new[y] = old[x]
+    T val = (T)0;
+    if (mv) val = v[inverse ? x : y];
+    __syncthreads();
+    if (mv) v[inverse ? y : x] = val;
+  }
+}
+
+template <typename T>
+int permute_vec(const int *ipiv, int n, T *v, bool inverse, hipStream_t s) {
+  if (n <= 0) return CLU_OK;
+  hipLaunchKernelGGL((permute_vec_kernel<T>), dim3(1), dim3(2 * MAXRUN), 0, s, ipiv, n, v,
+                     inverse ? 1 : 0);
+  CLU_LAUNCHED("permute_vec_kernel");
+  return CLU_OK;
+}
+template int permute_vec<double>(const int *, int, double *, bool, hipStream_t);
+template int permute_vec<float>(const int *, int, float *, bool, hipStream_t);
+
 template int apply_moves<double>(const int2 *, const int *, int, double *, long, int, int,
                                  int, int, bool, hipStream_t);
 template int apply_moves<float>(const int2 *, const int *, int, float *, long, int, int,

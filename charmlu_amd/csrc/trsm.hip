@@ -230,22 +230,29 @@ template int tri_inv_blocks<double>(bool, int, int, const double *, long, double
 template int tri_inv_blocks<float>(bool, int, int, const float *, long, float *, hipStream_t);
 
 // y = op(G) x for one m x m diagonal-block inverse G (m <= 256, ld ldg).
-// No transpose: one wave per row, lanes across the row, shuffle reduction;
-// transpose: one lane per output, rows split over the 4 waves of a group.
+// No transpose: one wave per row (all of a lane's loads issued before the
+// shuffle reduction); transpose: one lane per output, rows split over the 4
+// waves of a workgroup.
 template <typename T, bool TR>
 __global__ __launch_bounds__(256) void blk_gemv_kernel(int m, const T *__restrict__ G, long ldg,
                                                        const T *__restrict__ x,
                                                        T *__restrict__ y) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (!TR) {
-    // 8 workgroups x 4 waves; each wave takes rows wave_id, wave_id + 32, ...
-    const int wid = blockIdx.x * 4 + wave;
-    for (int i = wid; i < m; i += gridDim.x * 4) {
-      double acc = 0.0;
-      for (int k = lane; k < m; k += 64) acc += (double)G[(long)i * ldg + k] * (double)x[k];
-      acc = wave_sum(acc);
-      if (lane == 0) y[i] = (T)acc;
+    const int i = blockIdx.x * 4 + wave;
+    if (i >= m) return;
+    T g[4], xv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int c = lane + 64 * k;
+      g[k] = c < m ? G[(long)i * ldg + c] : (T)0;
+      xv[k] = c < m ? x[c] : (T)0;
     }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc += (double)g[k] * (double)xv[k];
+    acc = wave_sum(acc);
+    if (lane == 0) y[i] = (T)acc;
   } else {
     __shared__ double part[4][64];
     const int j = blockIdx.x * 64 + lane;
@@ -262,8 +269,10 @@ __global__ __launch_bounds__(256) void blk_gemv_kernel(int m, const T *__restric
 template <typename T>
 int blk_gemv(bool trans, int m, const T *G, long ldg, const T *x, T *y, hipStream_t s) {
   if (m <= 0) return CLU_OK;
+  if (m > 256) return -1;
   if (!trans)
-    hipLaunchKernelGGL((blk_gemv_kernel<T, false>), dim3(8), dim3(256), 0, s, m, G, ldg, x, y);
+    hipLaunchKernelGGL((blk_gemv_kernel<T, false>), dim3((m + 3) / 4), dim3(256), 0, s, m, G, ldg,
+                       x, y);
   else
     hipLaunchKernelGGL((blk_gemv_kernel<T, true>), dim3((m + 63) / 64), dim3(256), 0, s, m, G,
                        ldg, x, y);
