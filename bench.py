@@ -244,7 +244,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--nb", type=int, default=256)
-    ap.add_argument("--cpu-n", type=int, default=12288)
+    ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gemm-only", action="store_true",
@@ -343,6 +343,20 @@ def main():
                 pos[j], pos[pj] = b_, a
         moved += sum(1 for k, v in pos.items() if k != v)
     swap_bytes = 16.0 * moved * (n - nb) * args.steps
+    # HBM traffic of one trailing-update launch, from the committed PMC
+    # summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the step-0
+    # launch of this configuration
+    traffic = None
+    traffic_note = None
+    tpath = os.path.join(ROOT, "profiles", "dgemm_nn_lds_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            trec = json.load(f)
+        if trec.get("launch", "").startswith(f"{n - nb}x{n - nb}x{nb} "):
+            traffic = trec["traffic_bytes"]
+            traffic_note = (f"step-0 launch {n - nb}x{n - nb}x{nb}: {trec['traffic_bytes'] / 1e9:.2f} GB "
+                            f"HBM vs {trec['algorithmic_bytes'] / 1e9:.2f} GB algorithmic "
+                            "(profiles/dgemm_nn_lds_traffic.md)")
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -369,8 +383,9 @@ def main():
             "peak": FP64_PEAK,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK, 4),
-            "traffic": None,
-            "kernel": "dgemm_nn_lds_kernel<4> (trailing update, 128x128 tiles)",
+            "traffic": traffic,
+            "traffic_note": traffic_note,
+            "kernel": "dgemm_nn_lds_kernel<4, 1> (trailing update, 128x128 tiles)",
             "launches": int(g_cnt),
             "avg_launch_ms": round(g_ms / g_cnt, 4) if g_cnt else None,
             "flops_per_launch": g_work / g_cnt if g_cnt else None,

@@ -239,7 +239,9 @@ CLU_DEV void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
 }
 
-template <int NST>
+// ROLE only names the launch site in profiles: 1 = the factorization's
+// trailing update (dgemm_trailing), 0 = every other caller.
+template <int NST, int ROLE>
 __global__ __launch_bounds__(256, 2) void dgemm_nn_lds_kernel(
     int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
     const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
@@ -351,13 +353,18 @@ static bool lds_path_ok(int M, int N, int K, const double *A, long lda, const do
 
 template <typename T, bool TA, bool TB>
 static int dispatch(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
-                    T beta, T *C, long ldc, hipStream_t s) {
+                    T beta, T *C, long ldc, hipStream_t s, int role) {
   if (sizeof(T) == 8 && !TA && !TB &&
       lds_path_ok(M, N, K, (const double *)A, lda, (const double *)B, ldb)) {
     const int tiles_m = M / 128, tiles_n = N / 128;
-    hipLaunchKernelGGL((dgemm_nn_lds_kernel<4>), dim3(tiles_m * tiles_n), dim3(256), 0, s, M, N,
-                       K, (double)alpha, (const double *)A, lda, (const double *)B, ldb,
-                       (double)beta, (double *)C, ldc, tiles_m, tiles_n);
+    if (role == 1)
+      hipLaunchKernelGGL((dgemm_nn_lds_kernel<4, 1>), dim3(tiles_m * tiles_n), dim3(256), 0, s,
+                         M, N, K, (double)alpha, (const double *)A, lda, (const double *)B, ldb,
+                         (double)beta, (double *)C, ldc, tiles_m, tiles_n);
+    else
+      hipLaunchKernelGGL((dgemm_nn_lds_kernel<4, 0>), dim3(tiles_m * tiles_n), dim3(256), 0, s,
+                         M, N, K, (double)alpha, (const double *)A, lda, (const double *)B, ldb,
+                         (double)beta, (double *)C, ldc, tiles_m, tiles_n);
     CLU_LAUNCHED("dgemm_nn_lds_kernel");
     return CLU_OK;
   }
@@ -386,13 +393,21 @@ int dgemm_variant(bool ta, bool tb, int M, int N, int K, const double *A, long l
 
 template <typename T>
 static int gemm_any(bool ta, bool tb, int M, int N, int K, T alpha, const T *A, long lda,
-                    const T *B, long ldb, T beta, T *C, long ldc, hipStream_t s) {
+                    const T *B, long ldb, T beta, T *C, long ldc, hipStream_t s, int role = 0) {
   if (M <= 0 || N <= 0) return CLU_OK;
   if (K <= 0 || alpha == (T)0) K = 0;  // C = beta * C: the K loop runs zero chunks
-  if (!ta && !tb) return dispatch<T, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  if (ta && !tb) return dispatch<T, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  if (!ta && tb) return dispatch<T, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  return dispatch<T, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  if (!ta && !tb)
+    return dispatch<T, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, role);
+  if (ta && !tb)
+    return dispatch<T, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, role);
+  if (!ta && tb)
+    return dispatch<T, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, role);
+  return dispatch<T, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, role);
+}
+
+int dgemm_trailing(int M, int N, int K, double alpha, const double *A, long lda, const double *B,
+                   long ldb, double beta, double *C, long ldc, hipStream_t s) {
+  return gemm_any<double>(false, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, 1);
 }
 
 int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A, long lda,

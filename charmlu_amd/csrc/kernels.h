@@ -14,6 +14,10 @@ namespace clu {
 int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A,
           long lda, const double *B, long ldb, double beta, double *C, long ldc,
           hipStream_t s);
+// dgemm NN for the factorization's trailing update: the same kernels, with
+// the direct-to-LDS one instantiated under its own name for profiles
+int dgemm_trailing(int M, int N, int K, double alpha, const double *A, long lda,
+                   const double *B, long ldb, double beta, double *C, long ldc, hipStream_t s);
 
 // The same on FP32 operands (FP32 matrix cores).
 int sgemm(bool ta, bool tb, int M, int N, int K, float alpha, const float *A, long lda,

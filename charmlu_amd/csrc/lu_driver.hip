@@ -817,7 +817,7 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     const int pid = prof_begin(
         dgemm_variant(false, false, M, N, nb, L, lda, U, ldu) == 0 ? PK_GEMM_BIG : PK_GEMM_OTHER,
         2.0 * M * N * nb, sm);
-    int r = dgemm(false, false, M, N, nb, -1.0, L, lda, U, ldu, 1.0, C, lda, sm);
+    int r = dgemm_trailing(M, N, nb, -1.0, L, lda, U, ldu, 1.0, C, lda, sm);
     prof_end(pid, sm);
     return r;
   };

@@ -61,6 +61,7 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   __shared__ T s_val[2][NW];
   __shared__ int s_idx[2][NW];
   __shared__ T s_u[2][PW];
+  __shared__ T s_wcand[2][NW][PW];  // one workgroup: each wave's candidate row
   __shared__ T s_top[2][PW];
   __shared__ int s_piv[PW];
   __shared__ T s_pmag[PW];
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
     //      without pivoting the diagonal is the pivot ----
     T bv = (T)0;
     int bi = -1;
+    int bw = 0;  // the wave whose candidate won (one workgroup)
     if (PIV) {
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
@@ -118,33 +120,51 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
       }
       wave_argmax(bv, bi);
       if (lane == 0) { s_val[par][wave] = bv; s_idx[par][wave] = bi; }
+      if (G == 1) {
+        // one workgroup: the wave's candidate row and the top row go to LDS
+        // before the only barrier of the column
+#pragma clang loop unroll(full)
+        for (int r = 0; r < R; ++r) {
+          const int li = r * NT + t;
+          if (bi >= 0 && li == bi) {
+#pragma clang loop unroll(full)
+            for (int q = 0; q < PW; ++q) s_wcand[par][wave][q] = a[r][q];
+          }
+          if (li == c) {
+#pragma clang loop unroll(full)
+            for (int q = 0; q < PW; ++q) s_top[par][q] = a[r][q];
+          }
+        }
+      }
       __syncthreads();
       bv = s_val[par][0]; bi = s_idx[par][0];
 #pragma clang loop unroll(full)
       for (int k = 1; k < NW; ++k)
-        if (better(s_val[par][k], s_idx[par][k], bv, bi)) { bv = s_val[par][k]; bi = s_idx[par][k]; }
+        if (better(s_val[par][k], s_idx[par][k], bv, bi)) {
+          bv = s_val[par][k]; bi = s_idx[par][k]; bw = k;
+        }
     }
 
     int p;
     const T *u;    // the pivot row (LDS)
     const T *top;  // the old row c (LDS)
-    if (G == 1) {
-      p = (!PIV || bi < 0) ? c : bi;
+    if (G == 1 && PIV) {
+      p = bi < 0 ? c : bi;
+      top = s_top[par];
+      u = (p == c) ? top : s_wcand[par][bw];
+    } else if (G == 1) {
+      // no pivoting: the pivot row is row c
+      p = c;
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
         const int li = r * NT + t;
-        if (li == p) {
-#pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) s_u[par][q] = a[r][q];
-        }
         if (li == c) {
 #pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) s_top[par][q] = a[r][q];
         }
       }
       __syncthreads();
-      u = s_u[par];
-      top = s_top[par];
+      u = top = s_top[par];
     } else {
       // publish: candidate value/row, and the top row if held here
       const unsigned seq = seq0 + (unsigned)c + 1u;
