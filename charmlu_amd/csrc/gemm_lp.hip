@@ -49,6 +49,17 @@ template <> struct Lp<_Float16> {
   }
   static CLU_DEV _Float16 from(float x) { return (_Float16)x; }
 };
+// FP8 e4m3 (OCP, largest finite 448): a lane's 16-byte LDS read holds 16 k
+// values, consumed by two v_mfma_f32_16x16x32_fp8_fp8 (8 bytes each; lane
+// layout as for the 16-bit types, measured on gfx950)
+typedef long long2_t __attribute__((ext_vector_type(2)));
+template <> struct Lp<uint8_t> {
+  typedef long2_t vec_t;
+  static CLU_DEV float4_t mfma(vec_t a, vec_t b, float4_t c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a.x, b.x, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a.y, b.y, c, 0, 0, 0);
+  }
+};
 
 template <int N>
 CLU_DEV void wait_vm_lp() {
@@ -61,10 +72,17 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
                                                           const T16 *__restrict__ A, long lda,
                                                           const T16 *__restrict__ B, long ldb,
                                                           float beta, float *__restrict__ C,
-                                                          long ldc, int tiles_m, int tiles_n) {
+                                                          long ldc, int tiles_m, int tiles_n,
+                                                          const int *__restrict__ escA,
+                                                          const int *__restrict__ escB) {
   // BM x BN C tile, 4 waves of 64 x 64 (BM = 128: 2 x 2 waves; BM = 64:
-  // 1 x 4 waves, so every C row segment is 1 KB of contiguous FP32)
-  constexpr int BN = 16384 / BM, BK = 32;
+  // 1 x 4 waves, so every C row segment is 1 KB of contiguous FP32).
+  // A stage holds 64 bytes of K per row: 32 16-bit or 64 FP8 values.
+  // FP8 operands carry power-of-two scales per 128-row tile (escA: A * 2^e
+  // was quantized; escB likewise per 128 rows of B, i.e. columns of C), so
+  // the tile's products are 2^(escA[tm] + escB[tn]) too large.
+  constexpr int EPC = 16 / sizeof(T16);  // elements per 16-byte chunk
+  constexpr int BN = 16384 / BM, BK = 4 * EPC;
   constexpr int WGN = BN / 64;
   constexpr int HALF = BM * BK * 2;     // bytes of the A operand's stage
   constexpr int STAGE = (BM + BN) * BK * 2;
@@ -90,6 +108,8 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
   const int wm0 = (wave / WGN) * 64, wn0 = (wave % WGN) * 64;
   const int lr = lane & 15, lq = lane >> 4;
   const bool full = m0 + BM <= M && n0 + BN <= N;
+  // products of this tile carry 2^escale (FP8 only; BM = BN = 128 there)
+  const int escale = escA ? escA[tm] + escB[tn] : 0;
 
   // The MFMA's first operand is the B tile (16 C columns), its second the A
   // tile (16 C rows), so D[r][c] of block (i, j) is C[row][col] with
@@ -114,6 +134,10 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
           for (int r = 0; r < 4; ++r)
             acc[i][j][r] = (row < M && col + r < N) ? C[(long)row * ldc + col + r] : 0.f;
         }
+        if (escA) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][j][r] = ldexpf(acc[i][j][r], escale);
+        }
       }
   } else {
 #pragma unroll
@@ -130,14 +154,14 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
   for (int r = 0; r < CA; ++r) {
     const int c = wave * 64 + 256 * r + lane;  // chunk slot in the A stage
     const int row = c >> 2, kq = (c & 3) ^ ((row >> 2) & 3);
-    ga[r] = A + (long)min(m0 + row, M - 1) * lda + 8 * kq;
+    ga[r] = A + (long)min(m0 + row, M - 1) * lda + EPC * kq;
     loffa[r] = (wave * 64 + 256 * r) * 16;
   }
 #pragma unroll
   for (int r = 0; r < CB; ++r) {
     const int c = wave * 64 + 256 * r + lane;  // chunk slot in the B stage
     const int row = c >> 2, kq = (c & 3) ^ ((row >> 2) & 3);
-    gb[r] = B + (long)min(n0 + row, N - 1) * ldb + 8 * kq;
+    gb[r] = B + (long)min(n0 + row, N - 1) * ldb + EPC * kq;
     loffb[r] = HALF + (wave * 64 + 256 * r) * 16;
   }
   const int nk = K / BK;
@@ -185,6 +209,14 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
   wait_vm_lp<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
 
   // epilogue: CINIT stores the accumulators; otherwise C = alpha acc + beta C
+  if (escA) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[i][j][r] = ldexpf(acc[i][j][r], -escale);
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -214,11 +246,12 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
 
 template <typename T16, int BMv, int NS, int OC, bool CI, bool V, int NT>
 static void hgemm_launch(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B,
-                         long ldb, float beta, float *C, long ldc, hipStream_t s) {
+                         long ldb, float beta, float *C, long ldc, hipStream_t s,
+                         const int *escA = nullptr, const int *escB = nullptr) {
   constexpr int BNv = 16384 / BMv;
   const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
   hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, NS, OC, CI, V, NT>), dim3(tm * tn), dim3(256), 0,
-                     s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
+                     s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB);
 }
 template <typename T16, int BMv, int NS, int OC, int NT>
 static void hgemm_launch2(bool cinit, bool vec, int M, int N, int K, float alpha, const T16 *A,
@@ -275,6 +308,150 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
   CLU_LAUNCHED("hgemm_nt_kernel");
   return CLU_OK;
 }
+// FP8 operands with per-tile power-of-two scales (128 x 128 tiles only, so a
+// C tile sees one scale of A and one of B)
+int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
+             const uint8_t *B, long ldb, const int *escB, float beta, float *C, long ldc,
+             hipStream_t s) {
+  if (M <= 0 || N <= 0) return CLU_OK;
+  if (K % 64 != 0 || K <= 0 || lda % 16 != 0 || ldb % 16 != 0 || ((uintptr_t)A & 15) ||
+      ((uintptr_t)B & 15) || !escA || !escB)
+    return -1;
+  const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;
+  const bool cinit = alpha == 1.f && beta == 1.f;
+  if (cinit && vec)
+    hgemm_launch<uint8_t, 128, 2, 4, true, true, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                    s, escA, escB);
+  else if (cinit)
+    hgemm_launch<uint8_t, 128, 2, 4, true, false, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                     s, escA, escB);
+  else if (vec)
+    hgemm_launch<uint8_t, 128, 2, 4, false, true, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                     s, escA, escB);
+  else
+    hgemm_launch<uint8_t, 128, 2, 4, false, false, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C,
+                                                      ldc, s, escA, escB);
+  CLU_LAUNCHED("hgemm_nt_kernel (fp8)");
+  return CLU_OK;
+}
+
+// ---- FP8 quantization with per-tile power-of-two scales --------------------
+// A tile's scale 2^e is the largest power of two with 2^e max|v| <= 448 (the
+// largest finite e4m3 value), so no value saturates; v 2^e is rounded to
+// nearest-even e4m3 by the hardware conversion.
+
+CLU_DEV float block_max_256(float v, float *red) {
+  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  const float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  __syncthreads();
+  return m;
+}
+
+CLU_DEV int tile_exponent(float mx) {
+  if (!(mx > 0.f) || !isfinite(mx)) return 0;
+  int e = (int)floorf(log2f(448.f / mx));
+  e = max(-120, min(120, e));
+  while (e > -120 && ldexpf(mx, e) > 448.f) --e;  // guard log2f rounding
+  return e;
+}
+
+CLU_DEV unsigned quant4(float a, float b, float c, float d) {
+  auto cl = [](float x) { return fminf(448.f, fmaxf(-448.f, x)); };
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(cl(a), cl(b), 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(cl(c), cl(d), w, true);
+  return (unsigned)w;
+}
+
+// dst (M x Kp bytes) = fp8(sign * src * 2^e_g) for the rows of 128-row tile
+// g, zero for k >= K; esc[g] = e_g.  One workgroup per tile.
+__global__ __launch_bounds__(256) void quant_rows_fp8_kernel(int M, int K, int Kp, float sign,
+                                                             const float *__restrict__ src,
+                                                             long lds, uint8_t *__restrict__ dst,
+                                                             int *__restrict__ esc) {
+  __shared__ float red[4];
+  const int g = blockIdx.x, t = threadIdx.x, r0 = g * 128;
+  const int rows = min(128, M - r0);
+  float mx = 0.f;
+  for (int e = t; e < rows * K; e += 256)
+    mx = fmaxf(mx, fabsf(src[(long)(r0 + e / K) * lds + e % K]));
+  mx = block_max_256(mx, red);
+  const int ex = tile_exponent(mx);
+  if (t == 0) esc[g] = ex;
+  const int kq = Kp / 4;
+  for (int e = t; e < rows * kq; e += 256) {
+    const int i = e / kq, k = 4 * (e % kq);
+    const float *sp = src + (long)(r0 + i) * lds;
+    float v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sign * ldexpf(sp[k + q], ex) : 0.f;
+    *reinterpret_cast<unsigned *>(dst + (long)(r0 + i) * Kp + k) = quant4(v[0], v[1], v[2], v[3]);
+  }
+}
+
+// dst (N x Kp bytes) = fp8(src^T * 2^e_g) for the columns of 128-column tile
+// g of src (K x N), zero for k >= K; esc[g] = e_g.  One workgroup per tile,
+// K in slabs of 64 transposed through LDS.
+__global__ __launch_bounds__(256) void quant_cols_fp8_kernel(int K, int N, int Kp,
+                                                             const float *__restrict__ src,
+                                                             long lds, uint8_t *__restrict__ dst,
+                                                             int *__restrict__ esc) {
+  __shared__ float red[4];
+  __shared__ float tile[64][129];
+  const int g = blockIdx.x, t = threadIdx.x, c0 = g * 128;
+  const int cols = min(128, N - c0);
+  float mx = 0.f;
+  for (int e = t; e < K * 128; e += 256) {
+    const int k = e / 128, c = e % 128;
+    if (c < cols) mx = fmaxf(mx, fabsf(src[(long)k * lds + c0 + c]));
+  }
+  mx = block_max_256(mx, red);
+  const int ex = tile_exponent(mx);
+  if (t == 0) esc[g] = ex;
+  const int c = t & 127, h = t >> 7;  // column, half of the 64-row slab
+  for (int k0 = 0; k0 < Kp; k0 += 64) {
+    for (int e = t; e < 64 * 128; e += 256) {
+      const int kk = e / 128, cc = e % 128;
+      tile[kk][cc] = (k0 + kk < K && cc < cols) ? src[(long)(k0 + kk) * lds + c0 + cc] : 0.f;
+    }
+    __syncthreads();
+    if (c < cols) {
+      unsigned w[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int kk = 32 * h + 4 * q;
+        w[q] = quant4(ldexpf(tile[kk][c], ex), ldexpf(tile[kk + 1][c], ex),
+                      ldexpf(tile[kk + 2][c], ex), ldexpf(tile[kk + 3][c], ex));
+      }
+      uint4 *dp = reinterpret_cast<uint4 *>(dst + (long)(c0 + c) * Kp + k0 + 32 * h);
+      dp[0] = make_uint4(w[0], w[1], w[2], w[3]);
+      dp[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
+    __syncthreads();
+  }
+}
+
+int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds, uint8_t *dst,
+                   int *esc, hipStream_t s) {
+  if (M <= 0) return CLU_OK;
+  if (Kp % 64 != 0 || Kp < K) return -1;
+  hipLaunchKernelGGL(quant_rows_fp8_kernel, dim3((M + 127) / 128), dim3(256), 0, s, M, K, Kp,
+                     sign, src, lds, dst, esc);
+  CLU_LAUNCHED("quant_rows_fp8_kernel");
+  return CLU_OK;
+}
+int quant_cols_fp8(int K, int N, int Kp, const float *src, long lds, uint8_t *dst, int *esc,
+                   hipStream_t s) {
+  if (N <= 0) return CLU_OK;
+  if (Kp % 64 != 0 || Kp < K) return -1;
+  hipLaunchKernelGGL(quant_cols_fp8_kernel, dim3((N + 127) / 128), dim3(256), 0, s, K, N, Kp, src,
+                     lds, dst, esc);
+  CLU_LAUNCHED("quant_cols_fp8_kernel");
+  return CLU_OK;
+}
+
 template int hgemm_nt<__bf16>(int, int, int, float, const __bf16 *, long, const __bf16 *, long,
                               float, float *, long, hipStream_t);
 template int hgemm_nt<_Float16>(int, int, int, float, const _Float16 *, long, const _Float16 *,

@@ -29,6 +29,19 @@ int sgemm(bool ta, bool tb, int M, int N, int K, float alpha, const float *A, lo
 template <typename T16>
 int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B, long ldb,
              float beta, float *C, long ldc, hipStream_t s);
+// C = alpha * A * B^T + beta * C with FP8 e4m3 operands (A: M x K, B:
+// N x K, K a multiple of 64) carrying power-of-two scales per 128-row tile
+// (escA[M/128], escB[N/128]: the stored values are v * 2^esc)
+int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
+             const uint8_t *B, long ldb, const int *escB, float beta, float *C, long ldc,
+             hipStream_t s);
+// FP8 quantization with per-tile power-of-two scales: rows of src (M x K)
+// in 128-row tiles (times sign), or columns of src (K x N) in 128-column
+// tiles stored transposed; dst rows hold Kp bytes (Kp % 64 == 0, zero pad)
+int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds, uint8_t *dst,
+                   int *esc, hipStream_t s);
+int quant_cols_fp8(int K, int N, int Kp, const float *src, long lds, uint8_t *dst, int *esc,
+                   hipStream_t s);
 // dst (M x Kp) = 16-bit src (M x K) padded with zeros; dst (N x Kp) = 16-bit
 // transpose of src (K x N) padded with zeros.
 template <typename T16>

@@ -85,6 +85,9 @@ struct Workspace {
   // unpivoted mixed form: inv(U11) and L21 = A21 inv(U11) before it is copied back
   void *uinv32 = nullptr, *l21buf = nullptr;
   size_t uinv32_b = 0, l21buf_b = 0;
+  // FP8 update: power-of-two exponents per 128-row tile of L21 and of U12^T
+  void *esc = nullptr;
+  size_t esc_b = 0;
   void *irbuf = nullptr;
   size_t irbuf_b = 0;
   // solve: inverses of the diagonal blocks of L and U, the intermediate y
@@ -537,6 +540,40 @@ int charmlu_gemm_lp_dev(int prec, int M, int N, int K, float alpha, const void *
                             beta, dC, ldc, s);
 }
 
+int charmlu_gemm_fp8_dev(int M, int N, int K, float alpha, const float *dA, long lda,
+                         const float *dB, long ldb, float beta, float *dC, long ldc,
+                         void *stream) {
+  if (M < 0) return -1;
+  if (N < 0) return -2;
+  if (K < 0) return -3;
+  if (lda < (K > 1 ? K : 1)) return -6;
+  if (ldb < (N > 1 ? N : 1)) return -8;
+  if (ldc < (N > 1 ? N : 1)) return -11;
+  if (M == 0 || N == 0) return CLU_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (K == 0 || alpha == 0.f) {
+    // C = beta C
+    return sgemm(false, false, M, N, 0, 0.0f, dA, lda, dB, ldb, beta, dC, ldc, s);
+  }
+  Workspace *wsp;
+  int rc;
+  if ((rc = get_ws(&wsp))) return rc;
+  Workspace &ws = *wsp;
+  const int Kp = (K + 63) / 64 * 64;
+  const int tm = (M + 127) / 128, tn = (N + 127) / 128;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure_bytes(&ws.pa16, &ws.pa16_b, (size_t)M * Kp))) return rc;
+    if ((rc = ensure_bytes(&ws.pu16, &ws.pu16_b, (size_t)N * Kp))) return rc;
+    if ((rc = ensure_bytes(&ws.esc, &ws.esc_b, sizeof(int) * (size_t)(tm + tn)))) return rc;
+  }
+  uint8_t *qa = (uint8_t *)ws.pa16, *qb = (uint8_t *)ws.pu16;
+  int *ea = (int *)ws.esc, *eb = ea + tm;
+  if ((rc = quant_rows_fp8(M, K, Kp, 1.0f, dA, lda, qa, ea, s))) return rc;
+  if ((rc = quant_cols_fp8(K, N, Kp, dB, ldb, qb, eb, s))) return rc;
+  return fgemm_nt(M, N, Kp, alpha, qa, Kp, ea, qb, Kp, eb, beta, dC, ldc, s);
+}
+
 // op(T) is M x M; for M > 256 the solve proceeds in 256-row blocks with
 // GEMM updates between them.
 static int dtrsm_dev_any(bool upper, bool trans, bool unit, int M, int N, double alpha,
@@ -898,9 +935,14 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   }
   const bool nopiv = (prec & CHARMLU_MIXED_NOPIV) != 0;
   prec &= ~CHARMLU_MIXED_NOPIV;
-  if (prec != CHARMLU_PREC_FP32 && prec != CHARMLU_PREC_BF16 && prec != CHARMLU_PREC_FP16) {
+  if (prec != CHARMLU_PREC_FP32 && prec != CHARMLU_PREC_BF16 && prec != CHARMLU_PREC_FP16 &&
+      prec != CHARMLU_PREC_FP8) {
     set_error("trailing-update precision %d is not supported", prec);
     return -3;
+  }
+  if (prec == CHARMLU_PREC_FP8 && nb % 128 != 0) {
+    set_error("the FP8 update needs a block size that is a multiple of 128 (got %d)", nb);
+    return -2;
   }
   if (nopiv && nb > 1024) {
     set_error("the unpivoted form needs block size <= 1024 (got %d)", nb);
@@ -920,7 +962,9 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   hipStream_t sm = ws.s_main, sp = ws.s_panel;
   const int nruns = (nb + RUN - 1) / RUN;
   const bool lp = prec != CHARMLU_PREC_FP32;
-  const int Kp = (nb + 31) / 32 * 32;  // K of the 16-bit operands, zero-padded
+  const bool fp8 = prec == CHARMLU_PREC_FP8;
+  // K of the low-precision operands, zero-padded (64 bytes per LDS stage row)
+  const int Kp = fp8 ? (nb + 63) / 64 * 64 : (nb + 31) / 32 * 32;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_runs(ws, nruns))) return rc;
@@ -930,6 +974,8 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if ((rc = ensure_bytes(&ws.pa16, &ws.pa16_b, (size_t)2 * n * Kp))) return rc;
       if ((rc = ensure_bytes(&ws.pu16, &ws.pu16_b, (size_t)2 * n * Kp))) return rc;
     }
+    if (fp8 && (rc = ensure_bytes(&ws.esc, &ws.esc_b, sizeof(int) * 2 * (size_t)(n / 128 + 2))))
+      return rc;
     if (nopiv) {
       if ((rc = ensure_bytes(&ws.uinv32, &ws.uinv32_b, sizeof(float) * nb * nb))) return rc;
       if ((rc = ensure_bytes(&ws.l21buf, &ws.l21buf_b, sizeof(float) * nb * n))) return rc;
@@ -979,6 +1025,10 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     int r;
     if (!lp)
       r = sgemm(false, false, M, N, nb, -1.0f, L, ldl, ubuf + c0, Nt, 1.0f, C, lda, sm);
+    else if (fp8)
+      r = fgemm_nt(M, N, Kp, 1.0f, (const uint8_t *)ws.pa16, Kp, (const int *)ws.esc,
+                   (const uint8_t *)ws.pu16 + (size_t)c0 * Kp, Kp,
+                   (const int *)ws.esc + (n / 128 + 2) + c0 / 128, 1.0f, C, lda, sm);
     else if (prec == CHARMLU_PREC_BF16)
       r = hgemm_nt<__bf16>(M, N, Kp, 1.0f, (const __bf16 *)ws.pa16, Kp,
                            (const __bf16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, sm);
@@ -1017,7 +1067,14 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     pid = prof_begin(PK_TRSM, (double)Nt * nb * (nb - 1), sm);
     if ((rc = sgemm(false, false, nb, Nt, nb, 1.0f, winv, nb, A11 + nb, lda, 0.0f, ubuf, Nt, sm)))
       return rc;
-    if (lp) {
+    if (fp8) {
+      // FP8 operands with power-of-two scales per 128-row tile of L21 and
+      // per 128-column tile of U12
+      int *escL = (int *)ws.esc, *escU = escL + (n / 128 + 2);
+      if ((rc = quant_rows_fp8(Nt, nb, Kp, -1.0f, L21, ldl21, (uint8_t *)ws.pa16, escL, sm)))
+        return rc;
+      if ((rc = quant_cols_fp8(nb, Nt, Kp, ubuf, Nt, (uint8_t *)ws.pu16, escU, sm))) return rc;
+    } else if (lp) {
       // operands of the trailing update, rounded to 16 bits once per step
       if (prec == CHARMLU_PREC_BF16) {
         if ((rc = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, L21, ldl21, (__bf16 *)ws.pa16, sm)))

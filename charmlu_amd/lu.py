@@ -107,13 +107,15 @@ def getrs(LU, ipiv, B, trans: bool = False):
     return B
 
 
-_PREC = {"fp32": _lib.PREC_FP32, "bf16": _lib.PREC_BF16, "fp16": _lib.PREC_FP16}
+_PREC = {"fp32": _lib.PREC_FP32, "bf16": _lib.PREC_BF16, "fp16": _lib.PREC_FP16,
+         "fp8": _lib.PREC_FP8}
 
 
 def getrf_mixed(A32, block: int, prec: str = "bf16", pivot: bool = True):
     """In-place LU of the row-major FP32 CUDA tensor A32 with an FP32 panel
-    and the trailing update in ``prec`` ('fp32', 'bf16', 'fp16'; FP32
-    accumulation).  pivot=False factors without interchanges (for matrices
+    and the trailing update in ``prec`` ('fp32', 'bf16', 'fp16', or 'fp8':
+    e4m3 with power-of-two scales per 128-row tile, block a multiple of 128;
+    FP32 accumulation).  pivot=False factors without interchanges (for matrices
     that need none, e.g. diagonally dominant ones).  Returns (ipiv, info)
     like ``getrf``."""
     torch = _torch()

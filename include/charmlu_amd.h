@@ -142,8 +142,9 @@ int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda,
 /* The factorization of charmlu_dgetrf on an FP32 working matrix (the
  * reference is FP64 only: same algorithm, lu.ci:31-370).  The panel is
  * factored in FP32; the trailing update (updateMatrix, lu.C:26-43) runs with
- * its operands rounded to prec (PREC_FP32, PREC_BF16 or PREC_FP16; FP32
- * accumulation).  dA: n x n FP32 row-major, overwritten by L and U.  The
+ * its operands rounded to prec (PREC_FP32, PREC_BF16, PREC_FP16, or
+ * PREC_FP8: e4m3 with a power-of-two scale per 128-row tile of L21 and per
+ * 128-column tile of U12, nb a multiple of 128; FP32 accumulation).  dA: n x n FP32 row-major, overwritten by L and U.  The
  * pivots and info follow charmlu_dgetrf.  prec | CHARMLU_MIXED_NOPIV
  * factors without row interchanges (ipiv[j] = j), for matrices that need
  * none, such as diagonally dominant ones; nb <= 1024 then.  The panel is
@@ -174,6 +175,15 @@ int charmlu_sgemm_dev(int transa, int transb, int M, int N, int K, float alpha,
 int charmlu_gemm_lp_dev(int prec, int M, int N, int K, float alpha, const void *dA,
                         long lda, const void *dB, long ldb, float beta, float *dC,
                         long ldc, void *stream);
+
+/* The FP8 form of the update as a block op: C = alpha * q(A) * q(B) + beta *
+ * C for FP32 A (M x K), B (K x N), C, where q() rounds to e4m3 after
+ * scaling each 128-row tile of A and each 128-column tile of B by the
+ * largest power of two that keeps it within +-448; products accumulate in
+ * FP32 and are scaled back exactly.  Uses the device workspace. */
+int charmlu_gemm_fp8_dev(int M, int N, int K, float alpha, const float *dA, long lda,
+                         const float *dB, long ldb, float beta, float *dC, long ldc,
+                         void *stream);
 
 /* charmlu_dgetrs on FP32 factors and an FP32 right-hand side. */
 int charmlu_sgetrs(int trans, int n, int nrhs, const float *dLU, long lda,

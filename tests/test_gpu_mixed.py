@@ -233,3 +233,95 @@ def test_unpivoted_singular_block_reports_info(dev):
     A[200, 200] = 0.0
     ipiv, info = dev.getrf_mixed(A, nb, "bf16", pivot=False)
     assert int(info.item()) == 201
+
+
+# ---------------------------------------------------------------------------
+# FP8 e4m3 update with per-tile power-of-two scales
+# ---------------------------------------------------------------------------
+def _fp8_gemm(dev, A, B, alpha, beta, C):
+    import ctypes
+    import torch
+    from charmlu_amd import _lib
+    M, K = A.shape
+    N = B.shape[1]
+    rc = _lib.load().charmlu_gemm_fp8_dev(
+        M, N, K, alpha, ctypes.c_void_p(A.data_ptr()), K, ctypes.c_void_p(B.data_ptr()), N, beta,
+        ctypes.c_void_p(C.data_ptr()), N, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    return C
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 384, 512), (300, 140, 100), (128, 128, 64)])
+def test_fp8_gemm_exact_on_representable_tiles(dev, M, N, K):
+    """Entries in {+-0.5, +-1, +-1.5, +-2} with a 2 in every 128-row tile of A
+    and every 128-column tile of B: each tile scales by 2^7 onto exact e4m3
+    values, every product and partial sum is exact in FP32, so the result
+    equals the FP64 product bit for bit (both the alpha/beta and the
+    accumulate-into-C forms)."""
+    import torch
+    g = torch.Generator().manual_seed(M + N + K)
+    vals = torch.tensor([-2.0, -1.5, -1.0, -0.5, 0.5, 1.0, 1.5, 2.0])
+    A = vals[torch.randint(0, 8, (M, K), generator=g)]
+    B = vals[torch.randint(0, 8, (K, N), generator=g)]
+    A[::128, 0] = 2.0
+    B[0, ::128] = -2.0
+    ref = (A.double() @ B.double()).float()
+    A, B = A.cuda(), B.cuda()
+    C = torch.full((M, N), float("nan"), device="cuda")
+    _fp8_gemm(dev, A, B, 1.0, 0.0, C)
+    assert torch.equal(C.cpu(), ref)
+    C0 = torch.zeros(M, N, device="cuda")
+    _fp8_gemm(dev, A, B, 1.0, 1.0, C0)
+    assert torch.equal(C0.cpu(), ref)
+
+
+def test_fp8_gemm_random_within_e4m3_rounding(dev):
+    """Random operands: each element within the e4m3 rounding bound
+    (2 * 2^-4 relative per product, summed as |a||b|), and the overall
+    relative error at the statistical size of that rounding."""
+    import torch
+    torch.manual_seed(5)
+    M, N, K = 512, 256, 256
+    A = (torch.rand(M, K, device="cuda") - 0.5) * torch.logspace(-6, 2, M, device="cuda")[:, None]
+    B = torch.rand(K, N, device="cuda") - 0.5
+    C0 = torch.rand(M, N, device="cuda")
+    C = _fp8_gemm(dev, A, B, -1.0, 1.0, C0.clone())
+    ref = C0.double() - A.double() @ B.double()
+    bound = 2 * 2.0 ** -4 * (A.double().abs() @ B.double().abs()) + 2.0 ** -22 * ref.abs() + 1e-30
+    err = (C.double() - ref).abs()
+    assert bool((err <= bound).all())
+    # per-tile scaling keeps small rows accurate: relative error of A@B per row
+    P = C0.double() - C.double()
+    Pr = A.double() @ B.double()
+    rel = (P - Pr).norm(dim=1) / Pr.norm(dim=1)
+    assert float(rel.max()) < 0.08, float(rel.max())
+
+
+@pytest.mark.parametrize("pivot", [True, False])
+@pytest.mark.parametrize("n,nb", [(1024, 128), (2048, 512), (1536, 256)])
+def test_fp8_mixed_factor_and_refinement(dev, orc, n, nb, pivot):
+    import torch
+    A = orc.gen_diagdom(orc.SEED_A, n)
+    b = orc.gen_rhs(n)
+    dA = torch.from_numpy(A).cuda()
+    A32 = dA.float()
+    ipiv, info = dev.getrf_mixed(A32, nb, "fp8", pivot=pivot)
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    assert np.array_equal(ipiv.cpu().numpy(), np.arange(n))
+    be = _backward_error(A, A32.cpu().numpy(), np.arange(n))
+    assert be < 8 * 2.0 ** -4 * (n // nb + 1)
+    x_ref = orc.lu_solve_problem(n, nb, A=A.copy())["x"]
+    x, st = dev.gesv_ir(dA, A32, ipiv, torch.from_numpy(b).cuda(), tol=1.0)
+    err = np.abs(x.cpu().numpy() - x_ref).max() / np.abs(x_ref).max()
+    print(f"fp8 pivot={pivot} n={n} nb={nb}: backward error {be:.2e}, {st}, err {err:.2e}")
+    assert st["residual"] <= 1.0 and err < 1e-10
+
+
+def test_fp8_rejects_block_not_multiple_of_128(dev):
+    import torch
+    from charmlu_amd import CharmluError
+    A = torch.eye(960, dtype=torch.float32, device="cuda")
+    with pytest.raises(CharmluError):
+        dev.getrf_mixed(A, 192, "fp8")
