@@ -349,6 +349,8 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   else if (m <= 512) { R = 2; G = 1; NT = 256; }
   else if (m <= 1024) { R = 4; G = 1; NT = 256; }
   else { R = 4; NT = 512; G = (m + 2047) / 2048; }
+  // FP32 strips taller than the resident workgroups allow: 8 rows per thread
+  if (pivot && sizeof(T) == 4 && G > pw.max_groups) { R = 8; G = (m + 4095) / 4096; }
   if (G > pw.max_groups || G > 64) return -2;
   PanelRec *recs = reinterpret_cast<PanelRec *>(pw.slots);
   const unsigned seq0 = pw.epoch;
@@ -358,6 +360,9 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
       case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2, 256, false>), dim3(1), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
       default: hipLaunchKernelGGL((panel_base_kernel<T, 4, 256, false>), dim3(1), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
     }
+  } else if (NT == 512 && R == 8) {
+    if (sizeof(T) != 4) return -2;
+    hipLaunchKernelGGL((panel_base_kernel<float, 8, 512, true>), dim3(G), dim3(512), 0, s, (float *)A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info);
   } else if (NT == 512) {
     hipLaunchKernelGGL((panel_base_kernel<T, 4, 512, true>), dim3(G), dim3(512), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info);
   } else {
