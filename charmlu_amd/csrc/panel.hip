@@ -441,29 +441,41 @@ __global__ __launch_bounds__(256) void apply_moves_kernel(
     const int2 *__restrict__ moves, const int *__restrict__ count, T *__restrict__ A,
     long lda, int c0, int c1, int skip0, int skip1, int inverse) {
   extern __shared__ unsigned char smem_raw[];
-  T *buf = reinterpret_cast<T *>(smem_raw);
+  int2 *s_mv = reinterpret_cast<int2 *>(smem_raw);  // the move list, then the rows
   const int cnt = *count;
+  T *buf = reinterpret_cast<T *>(smem_raw + (size_t)2 * MAXRUN * sizeof(int2));
   const int t = threadIdx.x;
+  for (int i = t; i < cnt; i += 256) s_mv[i] = moves[i];
+  __syncthreads();
   const int left = skip0 - c0;  // columns before the skipped range
   auto phys = [&](int logical) { return logical < left ? c0 + logical : logical - left + skip1; };
   const int chunk = blockIdx.x * CW;
-  for (int e = t; e < cnt * CW; e += 256) {
-    const int mv = e / CW, q = e % CW;
-    const int col = phys(chunk + q);
-    if (col < c1) {
-      const int2 m2 = moves[mv];
-      const int src = inverse ? m2.x : m2.y;
-      buf[e] = A[(long)src * lda + col];
+  const int q = t % CW;  // this thread's column within the chunk (fixed: 256 % CW == 0)
+  const int col = phys(chunk + q);
+  const bool colok = col < c1;
+  // gather: U independent loads in flight per thread before any LDS store
+  constexpr int U = 8;
+  for (int e0 = t; e0 < cnt * CW; e0 += 256 * U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * 256;
+      if (e < cnt * CW && colok) {
+        const int2 m2 = s_mv[e / CW];
+        v[u] = A[(long)(inverse ? m2.x : m2.y) * lda + col];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * 256;
+      if (e < cnt * CW && colok) buf[e] = v[u];
     }
   }
   __syncthreads();
   for (int e = t; e < cnt * CW; e += 256) {
-    const int mv = e / CW, q = e % CW;
-    const int col = phys(chunk + q);
-    if (col < c1) {
-      const int2 m2 = moves[mv];
-      const int dst = inverse ? m2.y : m2.x;
-      A[(long)dst * lda + col] = buf[e];
+    if (colok) {
+      const int2 m2 = s_mv[e / CW];
+      A[(long)(inverse ? m2.y : m2.x) * lda + col] = buf[e];
     }
   }
 }
@@ -477,8 +489,9 @@ int apply_moves(const int2 *moves, const int *count, int max_moves, T *A, long l
   const int ncols = (c1 - c0) - (skip1 - skip0);
   if (ncols <= 0 || max_moves <= 0) return CLU_OK;
   constexpr int CW = 16;
+  if (max_moves > 2 * MAXRUN) return -1;
   const int nchunks = (ncols + CW - 1) / CW;
-  const size_t shm = (size_t)max_moves * CW * sizeof(T);
+  const size_t shm = (size_t)2 * MAXRUN * sizeof(int2) + (size_t)max_moves * CW * sizeof(T);
   hipLaunchKernelGGL((apply_moves_kernel<T, CW>), dim3(nchunks), dim3(256), shm, s,
                      moves, count, A, lda, c0, c1, skip0, skip1, inverse ? 1 : 0);
   CLU_LAUNCHED("apply_moves_kernel");
