@@ -22,8 +22,6 @@
 // j + 16*(i/4), element i%4 (measured on gfx950).
 #include <hip/hip_bf16.h>
 
-#include <stdlib.h>
-
 #include "common.h"
 #include "kernels.h"
 
@@ -67,7 +65,7 @@ CLU_DEV void wait_vm_lp() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
 }
 
-template <typename T16, int BM, int NST, int OCC, bool CINIT, bool VEC, int NTC>
+template <typename T16, int BM, int NST, int OCC, bool CINIT, bool VEC>
 __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K, float alpha,
                                                           const T16 *__restrict__ A, long lda,
                                                           const T16 *__restrict__ B, long ldb,
@@ -91,13 +89,17 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
   __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
   typedef typename Lp<T16>::vec_t vec_t;
 
-  // XCD-aware tile order (same scheme as the FP64 kernels)
+  // XCD-aware tile order: consecutive workgroups go to the 8 XCDs in turn,
+  // so each XCD gets a contiguous range of tiles, walked in groups of GM
+  // tile rows (GM = 4 measured best: 325 TFLOP/s at 65024^2 x 512 BF16 vs
+  // 315 with 8 and 295 with 16; without the XCD remap 293 and 1.6x the
+  // fetched bytes)
   int tm, tn;
   {
     const int bid = blockIdx.x, nblk = gridDim.x, per = nblk / 8;
     int lid = bid;
     if (bid < per * 8) lid = (bid % 8) * per + bid / 8;
-    constexpr int GM = 8;
+    constexpr int GM = 4;
     const int group_size = GM * tiles_n, group = lid / group_size, first_m = group * GM;
     const int gm = min(tiles_m - first_m, GM), in_group = lid - group * group_size;
     tm = first_m + in_group % gm;
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
         const int row = crow(j), col = ccol(i);
         if (VEC && full) {
           const float4_t *cp4 = reinterpret_cast<const float4_t *>(C + (long)row * ldc + col);
-          acc[i][j] = (NTC & 1) ? __builtin_nontemporal_load(cp4) : *cp4;
+          acc[i][j] = *cp4;
         } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r)
@@ -230,8 +232,7 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
                                            : (float4_t){0.f, 0.f, 0.f, 0.f};
           v = alpha * v + beta * old;
         }
-        if (NTC & 2) __builtin_nontemporal_store(v, reinterpret_cast<float4_t *>(cp));
-        else *reinterpret_cast<float4_t *>(cp) = v;
+        *reinterpret_cast<float4_t *>(cp) = v;
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -244,41 +245,34 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
     }
 }
 
-template <typename T16, int BMv, int NS, int OC, bool CI, bool V, int NT>
+template <typename T16, int BMv, int NS, int OC, bool CI, bool V>
 static void hgemm_launch(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B,
                          long ldb, float beta, float *C, long ldc, hipStream_t s,
                          const int *escA = nullptr, const int *escB = nullptr) {
   constexpr int BNv = 16384 / BMv;
   const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
-  hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, NS, OC, CI, V, NT>), dim3(tm * tn), dim3(256), 0,
-                     s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB);
+  hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, NS, OC, CI, V>), dim3(tm * tn), dim3(256), 0, s,
+                     M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB);
 }
-template <typename T16, int BMv, int NS, int OC, int NT>
+// 128 x 128 tiles, 2 stages, 4 workgroups per CU (measured at 65280^2 x 256
+// BF16: 212 TFLOP/s against 188 with 4 stages and 2 per CU, 198 with 64 x
+// 256 tiles; non-temporal C loads or stores were slower)
+template <typename T16>
 static void hgemm_launch2(bool cinit, bool vec, int M, int N, int K, float alpha, const T16 *A,
                           long lda, const T16 *B, long ldb, float beta, float *C, long ldc,
-                          hipStream_t s) {
+                          hipStream_t s, const int *escA = nullptr, const int *escB = nullptr) {
   if (cinit && vec)
-    hgemm_launch<T16, BMv, NS, OC, true, true, NT>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    hgemm_launch<T16, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+                                             escA, escB);
   else if (cinit)
-    hgemm_launch<T16, BMv, NS, OC, true, false, NT>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                                                    s);
+    hgemm_launch<T16, 128, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+                                              escA, escB);
   else if (vec)
-    hgemm_launch<T16, BMv, NS, OC, false, true, NT>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                                                    s);
+    hgemm_launch<T16, 128, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+                                              escA, escB);
   else
-    hgemm_launch<T16, BMv, NS, OC, false, false, NT>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                                                     s);
-}
-template <typename T16, int BMv, int NS, int OC>
-static void hgemm_launch3(int ntc, bool cinit, bool vec, int M, int N, int K, float alpha,
-                          const T16 *A, long lda, const T16 *B, long ldb, float beta, float *C,
-                          long ldc, hipStream_t s) {
-  switch (ntc) {
-    case 1: hgemm_launch2<T16, BMv, NS, OC, 1>(cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); break;
-    case 2: hgemm_launch2<T16, BMv, NS, OC, 2>(cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); break;
-    case 3: hgemm_launch2<T16, BMv, NS, OC, 3>(cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); break;
-    default: hgemm_launch2<T16, BMv, NS, OC, 0>(cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); break;
-  }
+    hgemm_launch<T16, 128, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+                                               escA, escB);
 }
 
 template <typename T16>
@@ -289,25 +283,12 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
       ((uintptr_t)B & 15))
     return -1;
   const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;
-  const bool cinit = alpha == 1.f && beta == 1.f;
-  static int cfg = -1, ntc = 0;
-  if (cfg < 0) {
-    const char *e = getenv("CHARMLU_HGEMM_CFG");
-    cfg = e ? atoi(e) : 0;
-    const char *f = getenv("CHARMLU_HGEMM_NT");
-    ntc = f ? atoi(f) & 3 : 0;
-  }
-  // 0: 128 x 128 tiles, 2 stages, 4 workgroups per CU (fastest measured at
-  // 65280 x 65280 x 256: 212 TFLOP/s vs 188 with 4 stages / 2 per CU)
-  if (cfg == 1)
-    hgemm_launch3<T16, 128, 4, 2>(ntc, cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  else if (cfg == 2)
-    hgemm_launch3<T16, 64, 2, 4>(ntc, cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  else
-    hgemm_launch3<T16, 128, 2, 4>(ntc, cinit, vec, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  hgemm_launch2<T16>(alpha == 1.f && beta == 1.f, vec, M, N, K, alpha, A, lda, B, ldb, beta, C,
+                     ldc, s);
   CLU_LAUNCHED("hgemm_nt_kernel");
   return CLU_OK;
 }
+
 // FP8 operands with per-tile power-of-two scales (128 x 128 tiles only, so a
 // C tile sees one scale of A and one of B)
 int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
@@ -318,19 +299,8 @@ int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const
       ((uintptr_t)B & 15) || !escA || !escB)
     return -1;
   const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;
-  const bool cinit = alpha == 1.f && beta == 1.f;
-  if (cinit && vec)
-    hgemm_launch<uint8_t, 128, 2, 4, true, true, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                                                    s, escA, escB);
-  else if (cinit)
-    hgemm_launch<uint8_t, 128, 2, 4, true, false, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                                                     s, escA, escB);
-  else if (vec)
-    hgemm_launch<uint8_t, 128, 2, 4, false, true, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                                                     s, escA, escB);
-  else
-    hgemm_launch<uint8_t, 128, 2, 4, false, false, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C,
-                                                      ldc, s, escA, escB);
+  hgemm_launch2<uint8_t>(alpha == 1.f && beta == 1.f, vec, M, N, K, alpha, A, lda, B, ldb, beta,
+                         C, ldc, s, escA, escB);
   CLU_LAUNCHED("hgemm_nt_kernel (fp8)");
   return CLU_OK;
 }
