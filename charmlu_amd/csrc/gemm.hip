@@ -114,11 +114,20 @@ CLU_DEV void store_tile(const typename Mfma<T>::acc_t (&acc)[TM][TN], T *__restr
   }
 }
 
+// Grid rows and layers (blockIdx.y, blockIdx.z) select independent problems
+// of a strided batch: operand X of problem (y, z) starts at X + y*bs.x + z*bs.y.
+struct BatchStrides {
+  long a_y, a_z, b_y, b_z, c_y, c_z;
+};
+
 template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB>
 __global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(
     int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
     const T *__restrict__ B, long ldb, T beta, T *__restrict__ C,
-    long ldc, int tiles_m, int tiles_n) {
+    long ldc, int tiles_m, int tiles_n, BatchStrides bs) {
+  A += blockIdx.y * bs.a_y + blockIdx.z * bs.a_z;
+  B += blockIdx.y * bs.b_y + blockIdx.z * bs.b_z;
+  C += blockIdx.y * bs.c_y + blockIdx.z * bs.c_z;
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
   constexpr int A_PER = BM * GBK / 256, B_PER = BN * GBK / 256;
@@ -239,20 +248,17 @@ CLU_DEV void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
 }
 
-// ROLE only names the launch site in profiles: 1 = the factorization's
-// trailing update (dgemm_trailing), 0 = every other caller.
-template <int NST, int ROLE>
-__global__ __launch_bounds__(256, 2) void dgemm_nn_lds_kernel(
-    int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
-    const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
-    int tiles_m, int tiles_n) {
+// One 128 x 128 tile of C = alpha A B + beta C.  CINIT (alpha = +-1 only):
+// beta C is loaded into the accumulators before the K loop and alpha folded
+// into the sign of the A fragments, so the epilogue is stores only and the
+// C reads overlap the first stages' loads instead of following the last
+// MFMA.  Otherwise the epilogue reads C once the products are done.
+template <int NST, bool CINIT>
+CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const double *__restrict__ A,
+                      long lda, const double *__restrict__ B, long ldb, double beta,
+                      double *__restrict__ C, long ldc) {
   constexpr int BM = 128, BN = 128, BK = 8;
   constexpr int STAGE = (BM * BK + BK * BN) * 8;  // bytes
-  __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
-
-  int tm, tn;
-  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
-  const int m0 = tm * BM, n0 = tn * BN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
   const int lr = lane & 15, lq = lane >> 4;
@@ -294,11 +300,25 @@ __global__ __launch_bounds__(256, 2) void dgemm_nn_lds_kernel(
     }
   }
 
+  // accumulator element (i, j, v) is C row 16 i + lq + 4 v, column 16 j + lr
+  // of this wave's 64 x 64 block
+  double *Cw = C + (long)(m0 + wm0) * ldc + n0 + wn0;
   double4_t acc[4][4];
+  if (CINIT && beta != 0.0) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          acc[i][j][v] = beta * Cw[(long)(i * 16 + lq + 4 * v) * ldc + j * 16 + lr];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  }
+  const bool neg = CINIT && alpha < 0.0;
 
 #pragma unroll
   for (int st = 0; st < NST - 1; ++st) issue(st);
@@ -320,6 +340,10 @@ __global__ __launch_bounds__(256, 2) void dgemm_nn_lds_kernel(
       for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const double *>(st + aoff[s2][i]);
 #pragma unroll
       for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const double *>(st + boff[s2][j]);
+      if (neg) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = -a[i];
+      }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -327,44 +351,78 @@ __global__ __launch_bounds__(256, 2) void dgemm_nn_lds_kernel(
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
-  wait_vm<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
-  store_tile<double, 4, 4>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, M, N, true);
+  wait_vm<0>();  // no direct-to-LDS write may be in flight past this tile
+  if (CINIT) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          Cw[(long)(i * 16 + lq + 4 * v) * ldc + j * 16 + lr] = acc[i][j][v];
+  } else {
+    store_tile<double, 4, 4>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, 0, 0, true);
+  }
+}
+
+// Aligned full-tile NN kernel, one tile per workgroup.  ROLE only names the
+// launch site in profiles: 1 = the factorization's trailing update, 2 = its
+// look-ahead block column, 0 = every other caller.
+template <int NST, int ROLE, bool CINIT>
+__global__ __launch_bounds__(256, 2) void dgemm_nn_lds_kernel(
+    int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
+    const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
+    int tiles_m, int tiles_n) {
+  __shared__ __attribute__((aligned(1024))) char smem[NST * (128 * 8 + 8 * 128) * 8];
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  lds_tile<NST, CINIT>(smem, tm * 128, tn * 128, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB>
 static int launch(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
-                  T beta, T *C, long ldc, hipStream_t s) {
+                  T beta, T *C, long ldc, hipStream_t s, int ny = 1, int nz = 1,
+                  BatchStrides bs = BatchStrides{0, 0, 0, 0, 0, 0}) {
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   const long nblk = (long)tiles_m * tiles_n;
-  if (nblk > 0x7fffffffL) return -1;
-  hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WM, WN, TA, TB>), dim3((unsigned)nblk),
-                     dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m,
-                     tiles_n);
+  if (nblk > 0x7fffffffL || ny < 1 || ny > 65535 || nz < 1 || nz > 65535) return -1;
+  hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WM, WN, TA, TB>),
+                     dim3((unsigned)nblk, (unsigned)ny, (unsigned)nz), dim3(256), 0, s, M, N, K,
+                     alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, bs);
   CLU_LAUNCHED("gemm_mfma_kernel");
   return CLU_OK;
 }
 
-static bool lds_path_ok(int M, int N, int K, const double *A, long lda, const double *B,
+static bool lds_aligned(int M, int N, int K, const double *A, long lda, const double *B,
                         long ldb) {
   return M % 128 == 0 && N % 128 == 0 && K % 8 == 0 && K > 0 && lda % 2 == 0 && ldb % 2 == 0 &&
-         ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0 &&
-         (long)(M / 128) * (N / 128) >= 256;
+         ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
+}
+// the direct-to-LDS kernel for general callers: aligned and enough tiles to
+// fill the chip (smaller grids do better with 64 x 64 tiles)
+static bool lds_path_ok(int M, int N, int K, const double *A, long lda, const double *B,
+                        long ldb) {
+  return lds_aligned(M, N, K, A, lda, B, ldb) && (long)(M / 128) * (N / 128) >= 256;
 }
 
 template <typename T, bool TA, bool TB>
 static int dispatch(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
                     T beta, T *C, long ldc, hipStream_t s, int role) {
+  const bool trailing = role == 1 && (alpha == (T)1 || alpha == (T)-1);
   if (sizeof(T) == 8 && !TA && !TB &&
-      lds_path_ok(M, N, K, (const double *)A, lda, (const double *)B, ldb)) {
+      (trailing ? lds_aligned(M, N, K, (const double *)A, lda, (const double *)B, ldb)
+                : lds_path_ok(M, N, K, (const double *)A, lda, (const double *)B, ldb))) {
     const int tiles_m = M / 128, tiles_n = N / 128;
-    if (role == 1)
-      hipLaunchKernelGGL((dgemm_nn_lds_kernel<4, 1>), dim3(tiles_m * tiles_n), dim3(256), 0, s,
-                         M, N, K, (double)alpha, (const double *)A, lda, (const double *)B, ldb,
-                         (double)beta, (double *)C, ldc, tiles_m, tiles_n);
-    else
-      hipLaunchKernelGGL((dgemm_nn_lds_kernel<4, 0>), dim3(tiles_m * tiles_n), dim3(256), 0, s,
-                         M, N, K, (double)alpha, (const double *)A, lda, (const double *)B, ldb,
-                         (double)beta, (double *)C, ldc, tiles_m, tiles_n);
+    const bool cinit = alpha == (T)1 || alpha == (T)-1;
+#define CLU_LDS_LAUNCH(R, CI)                                                                   \
+  hipLaunchKernelGGL((dgemm_nn_lds_kernel<4, R, CI>), dim3(tiles_m * tiles_n), dim3(256), 0, s, \
+                     M, N, K, (double)alpha, (const double *)A, lda, (const double *)B, ldb,    \
+                     (double)beta, (double *)C, ldc, tiles_m, tiles_n)
+    if (trailing) CLU_LDS_LAUNCH(1, true);
+    else if (role == 2 && cinit) CLU_LDS_LAUNCH(2, true);
+    else if (cinit) CLU_LDS_LAUNCH(0, true);
+    else CLU_LDS_LAUNCH(0, false);
+#undef CLU_LDS_LAUNCH
     CLU_LAUNCHED("dgemm_nn_lds_kernel");
     return CLU_OK;
   }
@@ -405,9 +463,23 @@ static int gemm_any(bool ta, bool tb, int M, int N, int K, T alpha, const T *A, 
   return dispatch<T, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, role);
 }
 
+int dgemm_lookahead(int M, int N, int K, double alpha, const double *A, long lda,
+                    const double *B, long ldb, double beta, double *C, long ldc, hipStream_t s) {
+  if (alpha != 1.0 && alpha != -1.0) return -1;
+  return gemm_any<double>(false, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, 2);
+}
+
+// The trailing update (alpha = +-1): the direct-to-LDS kernel whenever the
+// operands are aligned, whatever the grid size, so both parts of a split
+// update run the same kernel.
 int dgemm_trailing(int M, int N, int K, double alpha, const double *A, long lda, const double *B,
                    long ldb, double beta, double *C, long ldc, hipStream_t s) {
+  if (alpha != 1.0 && alpha != -1.0) return -1;
   return gemm_any<double>(false, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, 1);
+}
+bool dgemm_trailing_lds(int M, int N, int K, const double *A, long lda, const double *B,
+                        long ldb) {
+  return lds_aligned(M, N, K, A, lda, B, ldb);
 }
 
 int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A, long lda,
@@ -419,5 +491,27 @@ int sgemm(bool ta, bool tb, int M, int N, int K, float alpha, const float *A, lo
           const float *B, long ldb, float beta, float *C, long ldc, hipStream_t s) {
   return gemm_any<float>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
+
+// NN products of a strided batch of small problems (the blocks of a
+// triangular inverse): 32 x 256-row tiles for N <= 32, else 64 x 64.
+template <typename T>
+int gemm_batched(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
+                 T beta, T *C, long ldc, int ny, int nz, long a_y, long a_z, long b_y, long b_z,
+                 long c_y, long c_z, hipStream_t s) {
+  if (M <= 0 || N <= 0 || ny <= 0 || nz <= 0) return CLU_OK;
+  if (K <= 0 || alpha == (T)0) K = 0;
+  const BatchStrides bs{a_y, a_z, b_y, b_z, c_y, c_z};
+  if (N <= 32)
+    return launch<T, 256, 32, 4, 1, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                  s, ny, nz, bs);
+  return launch<T, 64, 64, 2, 2, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+                                              ny, nz, bs);
+}
+template int gemm_batched<double>(int, int, int, double, const double *, long, const double *,
+                                  long, double, double *, long, int, int, long, long, long, long,
+                                  long, long, hipStream_t);
+template int gemm_batched<float>(int, int, int, float, const float *, long, const float *, long,
+                                 float, float *, long, int, int, long, long, long, long, long,
+                                 long, hipStream_t);
 
 }  // namespace clu

@@ -14,10 +14,17 @@ namespace clu {
 int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A,
           long lda, const double *B, long ldb, double beta, double *C, long ldc,
           hipStream_t s);
-// dgemm NN for the factorization's trailing update: the same kernels, with
-// the direct-to-LDS one instantiated under its own name for profiles
+// dgemm NN for the factorization's look-ahead block column (alpha = +-1):
+// the same kernels, the direct-to-LDS one under its own name for profiles
+int dgemm_lookahead(int M, int N, int K, double alpha, const double *A, long lda,
+                    const double *B, long ldb, double beta, double *C, long ldc, hipStream_t s);
+// The factorization's trailing update (alpha = +-1): the direct-to-LDS
+// kernel whenever dgemm_trailing_lds holds (aligned operands, M and N
+// multiples of 128, K of 8), whatever the number of tiles.
 int dgemm_trailing(int M, int N, int K, double alpha, const double *A, long lda,
                    const double *B, long ldb, double beta, double *C, long ldc, hipStream_t s);
+bool dgemm_trailing_lds(int M, int N, int K, const double *A, long lda, const double *B,
+                        long ldb);
 
 // The same on FP32 operands (FP32 matrix cores).
 int sgemm(bool ta, bool tb, int M, int N, int K, float alpha, const float *A, long lda,
@@ -119,6 +126,18 @@ int dgemv(bool trans, int M, int N, double alpha, const double *A, long lda, con
 // triangle, block k at Inv + k*S*S (ld S), all solved in one launch
 template <typename T>
 int tri_inv_blocks(bool upper, int n, int S, const T *A, long lda, T *Inv, hipStream_t s);
+// X = inverse of the n x n triangle of A (unit lower, or upper; unit
+// selects an implied unit diagonal), every element of X written (zeros on
+// the other side).  ntri triangles at once: triangle i at A + i*sa, its
+// inverse at X + i*sx.  tbuf: scratch of ntri * n * n elements.
+template <typename T>
+int tri_inv(bool upper, bool unit, int n, int ntri, const T *A, long lda, long sa, T *X, long ldx,
+            long sx, T *tbuf, hipStream_t s);
+// strided-batched NN GEMM (grid rows y and layers z, strides per operand)
+template <typename T>
+int gemm_batched(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
+                 T beta, T *C, long ldc, int ny, int nz, long a_y, long a_z, long b_y, long b_z,
+                 long c_y, long c_z, hipStream_t s);
 // y = op(G) x, G one m x m block inverse (m <= 256)
 template <typename T>
 int blk_gemv(bool trans, int m, const T *G, long ldg, const T *x, T *y, hipStream_t s);

@@ -9,11 +9,14 @@
 //
 //   panel stream:  panel(t+1) .................. [waits for the look-ahead
 //                                                  block column of step t]
-//   main stream:   swaps(t) -> U row(t) -> update(block col t+1) -> update(rest)
+//   main stream:   swaps(t) -> U row(t) -> update(block col t+1) -> update(rest, part 1)
+//   full stream:                          [after panel(t+1)] update(rest, part 2)
 //
 // so the latency-bound panel of step t+1 (the critical path, prio_notes:1-27
 // in the reference) overlaps the bulk of step t's trailing update, which is
-// what the reference's priorities (manager.h:54-59) aim at.  The reference's
+// what the reference's priorities (manager.h:54-59) aim at.  The panel and
+// main streams have disjoint CU masks; the full stream has every CU and
+// takes the part of the FP64 update left when the panel is done.  The reference's
 // write-after-read hazard -- an L block may not be row-permuted by later
 // pivots before every trailing update of its step has read it
 // (lu.ci:240-242) -- is kept by stream order: the left-side interchanges of
@@ -59,8 +62,17 @@ struct Workspace {
   int device = -1;
   hipStream_t s_panel = nullptr;  // panels: the reserved CUs only
   hipStream_t s_main = nullptr;   // everything else: the other CUs
-  int panel_cus = 0;
+  hipStream_t s_full = nullptr;   // the end of each trailing update: every CU
+  int panel_cus = 0, ncu = 0;
   hipEvent_t ev_panel = nullptr, ev_la = nullptr, ev_start = nullptr, ev_end = nullptr;
+  hipEvent_t ev_full = nullptr;
+  // Durations of the last FP64 factorization's panels (begin/end events on
+  // the panel stream, per step), read back at the next call with the same
+  // n and block size to size the part of each trailing update that runs
+  // beside the panel.
+  int hist_n = 0, hist_nb = 0;
+  std::vector<hipEvent_t> hist_ev;  // 2 per step
+  std::vector<float> hist_ms;       // per step, < 0 when unknown
   // panel exchange slots (sized for doubles, reused for floats)
   void *slots = nullptr;
   unsigned *counter = nullptr;
@@ -93,6 +105,9 @@ struct Workspace {
   // solve: inverses of the diagonal blocks of L and U, the intermediate y
   void *sinv = nullptr, *sy = nullptr;
   size_t sinv_b = 0, sy_b = 0;
+  // scratch of the triangular inverses (products of the doubling steps)
+  void *tinv = nullptr;
+  size_t tinv_b = 0;
   PanelWork pw{};
 };
 
@@ -174,7 +189,10 @@ static int get_ws(Workspace **out) {
     }
     if (!w.s_main) CLU_HIP(hipStreamCreateWithFlags(&w.s_main, hipStreamNonBlocking));
     if (!w.s_panel) CLU_HIP(hipStreamCreateWithFlags(&w.s_panel, hipStreamNonBlocking));
+    CLU_HIP(hipStreamCreateWithFlags(&w.s_full, hipStreamNonBlocking));
     w.panel_cus = reserve;
+    w.ncu = ncu;
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_full, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_end, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_panel, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_la, hipEventDisableTiming));
@@ -273,23 +291,6 @@ static int trsm_llnu_big(int M, int N, const T *L, long ldl, T *B, long ldb, hip
     if (i0 + mi < M) {
       rc = gemm_t<T>(false, false, M - i0 - mi, N, mi, (T)-1, L + (long)(i0 + mi) * ldl + i0, ldl,
                      B + (long)i0 * ldb, ldb, (T)1, B + (long)(i0 + mi) * ldb, ldb, s);
-      if (rc) return rc;
-    }
-  }
-  return CLU_OK;
-}
-
-// B = U^{-1} B, U non-unit upper M x M (any M): 256-row diagonal solves
-// from the bottom up plus GEMM updates of the rows above.
-template <typename T>
-static int trsm_lunn_big(int M, int N, const T *U, long ldu, T *B, long ldb, hipStream_t s) {
-  for (int i0 = ((M - 1) / 256) * 256; i0 >= 0; i0 -= 256) {
-    const int mi = M - i0 < 256 ? M - i0 : 256;
-    int rc = trsm_lunn<T>(mi, N, U + (long)i0 * ldu + i0, ldu, B + (long)i0 * ldb, ldb, s);
-    if (rc) return rc;
-    if (i0 > 0) {
-      rc = gemm_t<T>(false, false, i0, N, mi, (T)-1, U + i0, ldu, B + (long)i0 * ldb, ldb, (T)1, B,
-                     ldb, s);
       if (rc) return rc;
     }
   }
@@ -468,15 +469,40 @@ int charmlu_profile(int on) {
 
 int charmlu_profile_read(double *out, int nout) {
   if (!out || nout < 3 * PK_NKINDS) return -1;
-  for (int i = 0; i < 3 * PK_NKINDS; ++i) out[i] = 0.0;
+  const int nfill = nout >= 4 * PK_NKINDS ? 4 * PK_NKINDS : 3 * PK_NKINDS;
+  for (int i = 0; i < nfill; ++i) out[i] = 0.0;
+  // intervals of every record on one time axis (ms after the first start)
+  std::vector<std::vector<std::pair<double, double>>> iv(PK_NKINDS);
   for (size_t i = 0; i < g_prof.used; ++i) {
     Prof::Rec &r = g_prof.recs[i];
     CLU_HIP(hipEventSynchronize(r.b));
-    float ms = 0.f;
+    float ms = 0.f, t0 = 0.f, t1 = 0.f;
     CLU_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    CLU_HIP(hipEventElapsedTime(&t0, g_prof.recs[0].a, r.a));
+    CLU_HIP(hipEventElapsedTime(&t1, g_prof.recs[0].a, r.b));
     out[3 * r.kind + 0] += 1.0;
     out[3 * r.kind + 1] += r.work;
     out[3 * r.kind + 2] += ms;
+    iv[r.kind].push_back({(double)t0, (double)t1});
+  }
+  if (nfill == 4 * PK_NKINDS) {
+    // time during which at least one record of the class was running
+    for (int k = 0; k < PK_NKINDS; ++k) {
+      auto &v = iv[k];
+      std::sort(v.begin(), v.end());
+      double busy = 0.0, cs = 0.0, ce = -1.0;
+      for (auto &x : v) {
+        if (x.first > ce) {
+          if (ce > cs) busy += ce - cs;
+          cs = x.first;
+          ce = x.second;
+        } else if (x.second > ce) {
+          ce = x.second;
+        }
+      }
+      if (ce > cs) busy += ce - cs;
+      out[3 * PK_NKINDS + k] = busy;
+    }
   }
   g_prof.used = 0;
   return CLU_OK;
@@ -813,15 +839,15 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   // all work runs on the library's two streams, fenced to the caller's
   hipStream_t sm = ws.s_main;
   const int nruns = (nb + RUN - 1) / RUN;
-  // inv(L11) * A12 on the matrix cores replaces the substitution when the
-  // diagonal block fits one triangular-solve launch
-  const bool use_inv = nb <= 256;
+  // inv(L11) * A12 on the matrix cores replaces the substitution
+  const bool use_inv = nb <= 1024;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_runs(ws, nruns))) return rc;
     if (use_inv) {
       if ((rc = ensure_dbuf(&ws.winv, &ws.winv_elems, (size_t)nb * nb))) return rc;
       if ((rc = ensure_dbuf(&ws.ubuf, &ws.ubuf_elems, (size_t)nb * n))) return rc;
+      if ((rc = ensure_bytes(&ws.tinv, &ws.tinv_b, sizeof(double) * nb * nb))) return rc;
     }
   }
   hipStream_t sp = ws.s_panel;
@@ -840,30 +866,97 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     if (r) return r;
     r = build_moves(dipiv, r0, r0 + nb, moves_of(t), counts_of(t), sp);
     if (r) return r;
-    if (use_inv && r0 + nb < n) {
-      // inv(L11): solve L11 X = I (unit lower), off the update stream's path
-      if ((r = set_identity(ws.winv, nb, nb, sp))) return r;
-      r = trsm_llnu<double>(nb, nb, dA + (long)r0 * lda + r0, lda, ws.winv, nb, sp);
-    }
+    if (use_inv && r0 + nb < n)  // inv(L11), off the update stream's path
+      r = tri_inv<double>(false, true, nb, 1, dA + (long)r0 * lda + r0, lda, 0, ws.winv, nb, 0,
+                          (double *)ws.tinv, sp);
     prof_end(pid, sp);
     return r;
   };
-  // trailing-update GEMM, timed per launch when profiling is on
-  auto update = [&](int M, int N, const double *L, const double *U, long ldu,
-                    double *C) -> int {
-    const int pid = prof_begin(
-        dgemm_variant(false, false, M, N, nb, L, lda, U, ldu) == 0 ? PK_GEMM_BIG : PK_GEMM_OTHER,
-        2.0 * M * N * nb, sm);
-    int r = dgemm_trailing(M, N, nb, -1.0, L, lda, U, ldu, 1.0, C, lda, sm);
+  // the look-ahead block column, timed when profiling is on
+  auto update_la = [&](int M, int N, const double *L, const double *U, long ldu,
+                       double *C) -> int {
+    const int pid = prof_begin(PK_GEMM_OTHER, 2.0 * M * N * nb, sm);
+    int r = dgemm_lookahead(M, N, nb, -1.0, L, lda, U, ldu, 1.0, C, lda, sm);
     prof_end(pid, sm);
     return r;
   };
+  // The rest of the trailing update (updateMatrix, lu.C:26-43).  With the
+  // panel on CUs of its own, the update is split by rows: the first Ma rows
+  // run on the update stream beside the panel of step t+1, the rest on every
+  // CU (s_full) once that panel is done.  Ma is sized so that the first part
+  // lasts about as long as the panel (its duration in the previous call with
+  // this n and block size, else a model), so neither set of CUs waits for
+  // long.  Each tile is computed the same way whatever the split.
+  const bool split = ws.panel_cus > 0;
+  bool full_pending = false;
+  auto split_rows = [&](int t, int M, int N, const double *L, const double *U, long ldu) -> int {
+    if (!split || !dgemm_trailing_lds(M, N, nb, L, lda, U, ldu)) return M;
+    const int m = n - (t + 1) * nb;  // rows of panel t+1
+    double p_ms = (t + 1 < (int)ws.hist_ms.size()) ? ws.hist_ms[t + 1] : -1.0;
+    if (p_ms <= 0.0) p_ms = nb / 256.0 * (1.3 + 2.0 * (m < 32768 ? m / 32768.0 : 1.0));
+    // FP64 matrix-core rate of the update stream's CUs at 80 % of peak
+    // (optimistic, so the first part errs long), plus 10 % margin
+    const double rate = 0.8 * 78.6e9 * (ws.ncu - ws.panel_cus) / ws.ncu;  // flop per ms
+    const double rows = 1.1 * p_ms * rate / (2.0 * N * nb);
+    if (rows >= M) return M;
+    const int Ma = ((int)rows + 127) / 128 * 128;
+    return Ma > M ? M : Ma;
+  };
+  // part 1: rows [0, Ma) on the update stream, enqueued before the panel's
+  // many small launches so the host does not delay it
+  auto update_part1 = [&](int Ma, int N, const double *L, const double *U, long ldu,
+                          double *C) -> int {
+    if (Ma <= 0) return CLU_OK;
+    const int pid = prof_begin(PK_GEMM_BIG, 2.0 * Ma * N * nb, sm);
+    int r = dgemm_trailing(Ma, N, nb, -1.0, L, lda, U, ldu, 1.0, C, lda, sm);
+    prof_end(pid, sm);
+    return r;
+  };
+  // part 2: rows [Ma, M) on every CU once the panel (ev_panel) is done
+  auto update_part2 = [&](int Ma, int M, int N, const double *L, const double *U, long ldu,
+                          double *C) -> int {
+    if (Ma >= M) return CLU_OK;
+    hipStream_t sf = ws.s_full;
+    CLU_HIP(hipStreamWaitEvent(sf, ws.ev_la, 0));
+    CLU_HIP(hipStreamWaitEvent(sf, ws.ev_panel, 0));
+    const int pid = prof_begin(PK_GEMM_BIG, 2.0 * (M - Ma) * N * nb, sf);
+    int r = dgemm_trailing(M - Ma, N, nb, -1.0, L + (long)Ma * lda, lda, U, ldu, 1.0,
+                           C + (long)Ma * lda, lda, sf);
+    prof_end(pid, sf);
+    if (r) return r;
+    CLU_HIP(hipEventRecord(ws.ev_full, sf));
+    full_pending = true;
+    return CLU_OK;
+  };
+  // panel durations of the previous call with the same shape, then events
+  // for this call's
+  if (ws.hist_n != n || ws.hist_nb != nb) ws.hist_ms.assign(NB, -1.0f);
+  if (ws.hist_n == n && ws.hist_nb == nb && (int)ws.hist_ev.size() == 2 * NB) {
+    for (int t = 0; t < NB; ++t) {
+      float ms = -1.0f;
+      if (hipEventQuery(ws.hist_ev[2 * t + 1]) == hipSuccess &&
+          hipEventElapsedTime(&ms, ws.hist_ev[2 * t], ws.hist_ev[2 * t + 1]) == hipSuccess)
+        ws.hist_ms[t] = ms;
+      (void)hipGetLastError();
+    }
+  }
+  while ((int)ws.hist_ev.size() < 2 * NB) {
+    hipEvent_t e;
+    CLU_HIP(hipEventCreate(&e));
+    ws.hist_ev.push_back(e);
+  }
+  ws.hist_n = n;
+  ws.hist_nb = nb;
 
   if ((rc = panel(0))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_panel, sp));
   for (int t = 0; t < NB; ++t) {
     const int r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
     CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));
+    if (full_pending) {  // the previous update's second launch
+      CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+      full_pending = false;
+    }
     // interchanges of step t on every column outside the panel
     int pid = prof_begin(PK_SWAP, (double)(n - nb), sm);
     if ((rc = apply_runs<double>(nruns, moves_of(t), counts_of(t), dA, lda, 0, n, r0, r1, false,
@@ -888,21 +981,23 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     prof_end(pid, sm);
     // look-ahead block column, then the panel of step t+1 on the panel stream
     const int la = Nt < nb ? Nt : nb;
-    if ((rc = update(Nt, la, A11 + (long)nb * lda, U, ldu, A11 + (long)nb * lda + nb)))
-      return rc;
+    const double *L21 = A11 + (long)nb * lda;
+    if ((rc = update_la(Nt, la, L21, U, ldu, A11 + (long)nb * lda + nb))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_la, sm));
+    double *C2 = A11 + (long)nb * lda + nb + la;
+    const int Ma = Nt > la ? split_rows(t, Nt, Nt - la, L21, U + la, ldu) : 0;
+    if (Nt > la && (rc = update_part1(Ma, Nt - la, L21, U + la, ldu, C2))) return rc;
     CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
+    CLU_HIP(hipEventRecord(ws.hist_ev[2 * (t + 1)], sp));
     if ((rc = panel(t + 1))) return rc;
+    CLU_HIP(hipEventRecord(ws.hist_ev[2 * (t + 1) + 1], sp));
     CLU_HIP(hipEventRecord(ws.ev_panel, sp));
-    // the rest of the trailing update (updateMatrix, lu.C:26-43)
-    if (Nt > la)
-      if ((rc = update(Nt, Nt - la, A11 + (long)nb * lda, U + la, ldu,
-                       A11 + (long)nb * lda + nb + la)))
-        return rc;
+    if (Nt > la && (rc = update_part2(Ma, Nt, Nt - la, L21, U + la, ldu, C2))) return rc;
     // the U row block into the factor storage (nothing reads A12 before)
     if (use_inv && (rc = convert<double, double>(nb, Nt, ws.ubuf, Nt, A11 + nb, lda, sm)))
       return rc;
   }
+  if (full_pending) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
   CLU_HIP(hipEventRecord(ws.ev_end, sm));
   CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
   return CLU_OK;
@@ -970,6 +1065,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     if ((rc = ensure_runs(ws, nruns))) return rc;
     if ((rc = ensure_bytes(&ws.winv32, &ws.winv32_b, sizeof(float) * nb * nb))) return rc;
     if ((rc = ensure_bytes(&ws.ubuf32, &ws.ubuf32_b, sizeof(float) * nb * n))) return rc;
+    if ((rc = ensure_bytes(&ws.tinv, &ws.tinv_b, sizeof(float) * nb * nb))) return rc;
     if (lp) {
       if ((rc = ensure_bytes(&ws.pa16, &ws.pa16_b, (size_t)2 * n * Kp))) return rc;
       if ((rc = ensure_bytes(&ws.pu16, &ws.pu16_b, (size_t)2 * n * Kp))) return rc;
@@ -1001,19 +1097,17 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       float *A11 = dA + (long)r0 * lda + r0;
       if ((r = factor_panel<float>(dA, lda, r0 + nb, r0, nb, r0, nb, dipiv, ws, sp, false)))
         return r;
-      if (r0 + nb < n) {
-        float *uinv = (float *)ws.uinv32;
-        if ((r = set_identity_t<float>(uinv, nb, nb, sp))) return r;
-        if ((r = trsm_lunn_big<float>(nb, nb, A11, lda, uinv, nb, sp))) return r;
-      }
+      if (r0 + nb < n &&
+          (r = tri_inv<float>(true, false, nb, 1, A11, lda, 0, (float *)ws.uinv32, nb, 0,
+                              (float *)ws.tinv, sp)))
+        return r;
     } else {
       if ((r = factor_panel<float>(dA, lda, n, r0, nb, r0, nb, dipiv, ws, sp))) return r;
       if ((r = build_moves(dipiv, r0, r0 + nb, moves_of(t), counts_of(t), sp))) return r;
     }
-    if (r0 + nb < n) {
-      if ((r = set_identity_t<float>(winv, nb, nb, sp))) return r;
-      r = trsm_llnu_big<float>(nb, nb, dA + (long)r0 * lda + r0, lda, winv, nb, sp);
-    }
+    if (r0 + nb < n)
+      r = tri_inv<float>(false, true, nb, 1, dA + (long)r0 * lda + r0, lda, 0, winv, nb, 0,
+                         (float *)ws.tinv, sp);
     prof_end(pid, sp);
     return r;
   };

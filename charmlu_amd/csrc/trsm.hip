@@ -229,6 +229,135 @@ int tri_inv_blocks(bool upper, int n, int S, const T *A, long lda, T *Inv, hipSt
 template int tri_inv_blocks<double>(bool, int, int, const double *, long, double *, hipStream_t);
 template int tri_inv_blocks<float>(bool, int, int, const float *, long, float *, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// Full inverse of a triangle by recursive doubling on the matrix cores.
+// The 32 x 32 diagonal blocks are inverted first, one wave each (lane j
+// forms column j of the block inverse in registers, reading the triangle
+// from LDS as broadcasts).  Then, for s = 32, 64, ..., pairs of adjacent
+// inverted diagonal blocks of size s are joined into one of size 2s:
+//   lower:  X21 = -X22 (L21 X11),   upper:  X12 = -X11 (U12 X22),
+// two strided-batched GEMMs per level over all pairs at once.
+// ---------------------------------------------------------------------------
+constexpr int TIB = 32;  // base block
+
+// Block k of every triangle (grid (nblocks, ntri)): invert the diagonal
+// block, write it into X and zero the part of its block row of X that lies
+// on the zero side of the triangle.
+template <typename T, bool UPPER, bool UNIT>
+__global__ __launch_bounds__(64) void trinv_base_kernel(int n, const T *__restrict__ A, long lda,
+                                                        long sa, T *__restrict__ X, long ldx,
+                                                        long sx) {
+  __shared__ T Ls[TIB][TIB + 1];
+  A += blockIdx.y * sa;
+  X += blockIdx.y * sx;
+  const int k0 = blockIdx.x * TIB;
+  const int bs = n - k0 < TIB ? n - k0 : TIB;
+  const int lane = threadIdx.x;
+  // all 16 loads of a lane in flight before the first LDS store
+  T v[TIB * TIB / 64];
+#pragma unroll
+  for (int r = 0; r < TIB * TIB / 64; ++r) {
+    const int i = r * 2 + (lane >> 5), j = lane & 31;
+    v[r] = (i < bs && j < bs) ? A[(long)(k0 + i) * lda + k0 + j] : (T)0;
+  }
+#pragma unroll
+  for (int r = 0; r < TIB * TIB / 64; ++r) Ls[r * 2 + (lane >> 5)][lane & 31] = v[r];
+  __syncthreads();
+  const int j = lane;  // lanes 0..31: column j of the block inverse
+  T x[TIB];
+  if (!UPPER) {
+#pragma unroll
+    for (int i = 0; i < TIB; ++i) {
+      T acc = (T)0;
+#pragma unroll
+      for (int q = 0; q < i; ++q) acc += Ls[i][q] * x[q];
+      x[i] = (i == j) ? (T)1 : -acc;
+      if (!UNIT) x[i] = x[i] / Ls[i][i];
+    }
+  } else {
+#pragma unroll
+    for (int i = TIB - 1; i >= 0; --i) {
+      T acc = (T)0;
+#pragma unroll
+      for (int q = i + 1; q < TIB; ++q) acc += Ls[i][q] * x[q];
+      // rows and columns past the block's end are zero: x there stays 0
+      T v = (i == j) ? (T)1 : -acc;
+      if (!UNIT) v = (i < bs) ? v / Ls[i][i] : (T)0;
+      x[i] = v;
+    }
+  }
+  if (j < bs) {
+#pragma unroll
+    for (int i = 0; i < TIB; ++i)
+      if (i < bs) X[(long)(k0 + i) * ldx + k0 + j] = x[i];
+  }
+  // zero side of this block row: columns right of the block (lower) or
+  // left of it (upper)
+  const int c_lo = UPPER ? 0 : k0 + bs, c_hi = UPPER ? k0 : n;
+  for (int i = 0; i < bs; ++i)
+    for (int c = c_lo + lane; c < c_hi; c += 64) X[(long)(k0 + i) * ldx + c] = (T)0;
+}
+
+template <typename T>
+int gemm_batched(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
+                 T beta, T *C, long ldc, int ny, int nz, long a_y, long a_z, long b_y, long b_z,
+                 long c_y, long c_z, hipStream_t s);
+
+template <typename T>
+int tri_inv(bool upper, bool unit, int n, int ntri, const T *A, long lda, long sa, T *X, long ldx,
+            long sx, T *tbuf, hipStream_t s) {
+  if (n <= 0 || ntri <= 0) return CLU_OK;
+  if (ntri > 65535) return -1;
+  const dim3 g((n + TIB - 1) / TIB, ntri);
+#define CLU_TRINV(U, UN) \
+  if (upper == U && unit == UN) \
+    hipLaunchKernelGGL((trinv_base_kernel<T, U, UN>), g, dim3(64), 0, s, n, A, lda, sa, X, ldx, sx);
+  CLU_TRINV(false, true)
+  CLU_TRINV(false, false)
+  CLU_TRINV(true, true)
+  CLU_TRINV(true, false)
+#undef CLU_TRINV
+  CLU_LAUNCHED("trinv_base_kernel");
+  // tbuf holds one (at most s x s) product per pair: n x n per triangle
+  const long tstride = (long)n * n;
+  for (int sz = TIB; sz < n; sz *= 2) {
+    const int full = n / (2 * sz), rem = n - full * 2 * sz;
+    // the full pairs (grid rows), then the partial last pair if it has a right part
+    for (int part = 0; part < 2; ++part) {
+      const int np = part == 0 ? full : (rem > sz ? 1 : 0);
+      if (np == 0) continue;
+      const long o = part == 0 ? 0 : (long)full * 2 * sz;  // first pair's diagonal offset
+      const int s2 = part == 0 ? sz : rem - sz;
+      const long pa = 2L * sz * (lda + 1), px = 2L * sz * (ldx + 1), pt = (long)sz * sz;
+      const T *Ad = A + o * (lda + 1);
+      T *Xd = X + o * (ldx + 1);
+      T *Tb = tbuf + (part == 0 ? 0 : (long)full * sz * sz);
+      int rc;
+      if (!upper) {
+        // T = L21 (s2 x sz) X11 (sz x sz);  X21 = -X22 (s2 x s2) T
+        rc = gemm_batched<T>(s2, sz, sz, (T)1, Ad + (long)sz * lda, lda, Xd, ldx, (T)0, Tb, sz,
+                             np, ntri, pa, sa, px, sx, pt, tstride, s);
+        if (rc) return rc;
+        rc = gemm_batched<T>(s2, sz, s2, (T)-1, Xd + (long)sz * ldx + sz, ldx, Tb, sz, (T)0,
+                             Xd + (long)sz * ldx, ldx, np, ntri, px, sx, pt, tstride, px, sx, s);
+      } else {
+        // T = U12 (sz x s2) X22 (s2 x s2);  X12 = -X11 (sz x sz) T
+        rc = gemm_batched<T>(sz, s2, s2, (T)1, Ad + sz, lda, Xd + (long)sz * ldx + sz, ldx, (T)0,
+                             Tb, s2, np, ntri, pa, sa, px, sx, pt, tstride, s);
+        if (rc) return rc;
+        rc = gemm_batched<T>(sz, s2, sz, (T)-1, Xd, ldx, Tb, s2, (T)0, Xd + sz, ldx, np, ntri,
+                             px, sx, pt, tstride, px, sx, s);
+      }
+      if (rc) return rc;
+    }
+  }
+  return CLU_OK;
+}
+template int tri_inv<double>(bool, bool, int, int, const double *, long, long, double *, long,
+                             long, double *, hipStream_t);
+template int tri_inv<float>(bool, bool, int, int, const float *, long, long, float *, long, long,
+                            float *, hipStream_t);
+
 // y = op(G) x for one m x m diagonal-block inverse G (m <= 256, ld ldg).
 // No transpose: one wave per row (all of a lane's loads issued before the
 // shuffle reduction); transpose: one lane per output, rows split over the 4

@@ -69,7 +69,10 @@ const char *charmlu_last_error(void);
  * {count, work, milliseconds} for the classes 0 trailing-update GEMM of the
  * direct-to-LDS 128x128 kernel (work = flops), 1 other trailing-update GEMMs
  * (flops), 2 panels (rows x columns), 3 row interchanges (columns moved per
- * step), 4 U-row solves (flops), then clears the records.  Synchronizes. */
+ * step), 4 U-row solves (flops), then clears the records.  With nout >= 20,
+ * out[15 + k] is the time during which at least one record of class k was
+ * running (the two parts of a split trailing update overlap).
+ * Synchronizes. */
 int charmlu_profile(int on);
 int charmlu_profile_read(double *out, int nout);
 /* Make device `device` current for this thread and set up its workspace. */
