@@ -34,40 +34,56 @@ namespace clu {
 
 constexpr int PW = 16;  // max strip width
 
-// One workgroup's published candidate for one column (double storage for
-// both precisions; float values convert exactly).  seq is written last, after
-// the rest is visible device-wide.
-struct alignas(16) PanelRec {
-  double val;
-  int idx;
-  unsigned seq;
-  double cand[PW];
-  double top[PW];
-};
+// Cross-workgroup records.  Every 8-byte word carries a 32-bit payload in
+// its low half and the column's sequence number in its high half, so a
+// reader can check each word on its own: a column costs one memory round
+// trip (stores are not waited for, nothing is ordered by fences).  Record
+// of workgroup g for column parity par at recs + (par * GMAX + g) * RECW,
+// in 32-bit payload words (VW per value, FP64 low half first):
+//   [0, PW*VW)          the workgroup's best row (its PW values; zeros if none)
+//   [PW*VW, PW*VW+VW)   |value| of that row in the current column
+//   PW*VW+VW            its row (local index, -1: none)
+//   [TOPW, TOPW+PW*VW)  workgroup 0 only: row c (the top row) before the swap
+constexpr int GMAX = 64;  // workgroups in one exchange
+constexpr int RECW = 72;  // words per record
+constexpr int CHW = 4;    // record words per polling lane per round
+constexpr int SPIN_MAX = 1 << 22;  // polling rounds before giving up (seconds)
 
-template <typename T, int R, int NT, bool PIV>
+// Payload word k of a value: FP64 low half first (little-endian), so the
+// payloads restaged in LDS read back as the values themselves.
+CLU_DEV unsigned value_word(double v, int k) {
+  return (unsigned)(k == 0 ? __double2loint(v) : __double2hiint(v));
+}
+CLU_DEV unsigned value_word(float v, int) { return __float_as_uint(v); }
+template <typename T>
+CLU_DEV unsigned row_word(const T *row, int k) {  // word k of an LDS row
+  return reinterpret_cast<const unsigned *>(row)[k];
+}
+
+template <typename T, int R, int NT, bool MULTI, bool PIV>
 __global__ __launch_bounds__(NT) void panel_base_kernel(
     T *__restrict__ A, long lda, int j0, int m, int w, int pc0, int pc1,
-    int *__restrict__ ipiv, PanelRec *__restrict__ recs, unsigned seq0, int *info) {
-  const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  constexpr int RPW = NT * R;
-  constexpr int NW = NT / 64;  // waves
-  constexpr int GMAX = 64;     // workgroups in one exchange
+    int *__restrict__ ipiv, unsigned long long *__restrict__ recs, unsigned seq0, int *info) {
+  static_assert(PIV || !MULTI, "the unpivoted strip is one workgroup");
+  constexpr int VW = sizeof(T) / 4;      // payload words per value
+  constexpr int CW = PW * VW;            // words of a row
+  constexpr int L1 = CW + VW + 1;        // words of a record without the top row
+  constexpr int TOPW = (L1 + 1) / 2 * 2; // the top row, 8-byte aligned
+  constexpr int RPW = NT * R;            // rows per workgroup
+  constexpr int NW = NT / 64;            // waves
+  static_assert(TOPW + CW <= RECW, "record size");
+  const int G = MULTI ? (int)gridDim.x : 1, g = MULTI ? (int)blockIdx.x : 0;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const T tiny = (T)100 * (sizeof(T) == 8 ? (T)2.220446049250313e-16 : (T)1.1920929e-07);
 
-  // every per-column slot is double-buffered by column parity, so a column
-  // needs two barriers (three with several workgroups) and none at its end
+  // per-column LDS slots, double-buffered by column parity
   __shared__ T s_val[2][NW];
   __shared__ int s_idx[2][NW];
-  __shared__ T s_u[2][PW];
-  __shared__ T s_wcand[2][NW][PW];  // one workgroup: each wave's candidate row
-  __shared__ T s_top[2][PW];
+  __shared__ T s_wcand[2][NW][PW];  // each wave's candidate row
+  __shared__ T s_top[2][PW];        // row c before the swap (in the workgroup holding it)
+  __shared__ __attribute__((aligned(16))) unsigned s_rec[2][MULTI ? GMAX : 1][RECW];
   __shared__ int s_piv[PW];
   __shared__ T s_pmag[PW];
-  __shared__ T s_cand[2][GMAX][PW];  // candidate rows of every workgroup
-  __shared__ T s_cval[2][GMAX];
-  __shared__ int s_cidx[2][GMAX];
 
   T a[R][PW];
 #pragma clang loop unroll(full)
@@ -82,14 +98,27 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   // column's ipiv store on every iteration.
   wait_all_mem();
 
+  // MULTI: the record words this lane polls (word offsets inside one
+  // parity's records; -1: none, or this workgroup's own record).  Wave v
+  // takes words [v * 64 * CHW, (v + 1) * 64 * CHW) of the list; the host
+  // keeps G * L1 + CW <= NW * 64 * CHW.
+  int poff[MULTI ? CHW : 1];
+  bool poller = false;
+#pragma clang loop unroll(full)
+  for (int i = 0; i < (MULTI ? CHW : 1); ++i) {
+    const int e = wave * 64 * CHW + lane + 64 * i;
+    const int rg = e < G * L1 ? e / L1 : 0;
+    const int rw = e < G * L1 ? e % L1 : TOPW + (e - G * L1);
+    poff[i] = (MULTI && e < G * L1 + CW && rg != g) ? rg * RECW + rw : -1;
+    poller = poller || poff[i] >= 0;
+  }
+  poller = __any(poller);  // wave-uniform
+
   // The column loop stays rolled (a fully unrolled strip kernel is tens of
   // thousands of instructions and runs from instruction-cache misses).  Four
   // columns are unrolled per iteration at static register offsets K = 0..3;
-  // after each iteration every row is rotated left by four positions, so
-  // the next four columns are again at offsets 0..3.  In the rotated frame
-  // the columns still to come sit at offsets K+1 .. lim-1 (lim = w - base)
-  // and the finished ones elsewhere; after 16 rotated positions in total a
-  // row is back in its original order.
+  // after each iteration every row is rotated left by four positions, so the
+  // next four columns are again at offsets 0..3.
   auto rotate4 = [&]() {
 #pragma clang loop unroll(full)
     for (int r = 0; r < R; ++r) {
@@ -102,169 +131,178 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
       for (int q = 0; q < 4; ++q) a[r][PW - 4 + q] = keep[q];
     }
   };
-  auto column = [&](const int c, const int lim, auto kc) {
+  // this thread's candidate for column c over its rows >= c: strictly
+  // larger magnitude, rows in increasing order (lowest row on ties)
+  auto candidate = [&](int c, auto kc, T &bv, int &bi) {
     constexpr int K = decltype(kc)::value;
-    const int par = c & 1;
-    // ---- candidate for column c over local rows >= c (strict '>', rows
-    //      visited in increasing order, so the lowest row wins ties);
-    //      without pivoting the diagonal is the pivot ----
-    T bv = (T)0;
-    int bi = -1;
-    int bw = 0;  // the wave whose candidate won (one workgroup)
-    if (PIV) {
+#pragma clang loop unroll(full)
+    for (int r = 0; r < R; ++r) {
+      const int li = g * RPW + r * NT + t;
+      const T v = fabs(a[r][K]);
+      if (li >= c && v > bv) { bv = v; bi = li; }
+    }
+  };
+  // multipliers (division) and the rank-1 update of the later columns.  In
+  // the rotated frame the columns still to come sit at offsets K+1 .. lim-1
+  // (lim = w - base); the finished ones (rotated to the end) and the
+  // padding get a zero pivot-row entry, so they are left as they are.
+  // Branch-free: every row computes its multiplier, rows <= c keep their
+  // values by selection and update with a zero multiplier.
+  auto eliminate = [&](int c, int lim, const T *u, auto kc) {
+    constexpr int K = decltype(kc)::value;
+    T uq[PW];
+#pragma clang loop unroll(full)
+    for (int q = 0; q < PW; ++q) uq[q] = u[q];  // all of the row: no per-word branches
+#pragma clang loop unroll(full)
+    for (int q = K + 1; q < PW; ++q) uq[q] = q < lim ? uq[q] : (T)0;
+    const T piv = uq[K];
+    if (piv == (T)0) return;
+#pragma clang loop unroll(full)
+    for (int r = 0; r < R; ++r) {
+      const bool below = g * RPW + r * NT + t > c;
+      const T l = a[r][K] / piv;
+      const T le = below ? l : (T)0;
+      a[r][K] = below ? l : a[r][K];
+#pragma clang loop unroll(full)
+      for (int q = K + 1; q < PW; ++q) a[r][q] -= le * uq[q];
+    }
+  };
+  using K0 = std::integral_constant<int, 0>;
+  using K1 = std::integral_constant<int, 1>;
+  using K2 = std::integral_constant<int, 2>;
+  using K3 = std::integral_constant<int, 3>;
+
+  auto column = [&](const int c, const int lim, auto kc) {
+    constexpr int kk = decltype(kc)::value;
+    {
+      const int par = c & 1;
+      T bv = (T)0;
+      int bi = -1;
+      if (PIV) {
+        candidate(c, kc, bv, bi);
+        wave_argmax(bv, bi);
+        if (lane == 0) { s_val[par][wave] = bv; s_idx[par][wave] = bi; }
+      }
 #pragma clang loop unroll(full)
       for (int r = 0; r < R; ++r) {
         const int li = g * RPW + r * NT + t;
-        const T v = fabs(a[r][K]);
-        if (li < m && li >= c && v > bv) { bv = v; bi = li; }
-      }
-      wave_argmax(bv, bi);
-      if (lane == 0) { s_val[par][wave] = bv; s_idx[par][wave] = bi; }
-      if (G == 1) {
-        // one workgroup: the wave's candidate row and the top row go to LDS
-        // before the only barrier of the column
+        if (PIV && bi >= 0 && li == bi) {
 #pragma clang loop unroll(full)
-        for (int r = 0; r < R; ++r) {
-          const int li = r * NT + t;
-          if (bi >= 0 && li == bi) {
-#pragma clang loop unroll(full)
-            for (int q = 0; q < PW; ++q) s_wcand[par][wave][q] = a[r][q];
-          }
-          if (li == c) {
-#pragma clang loop unroll(full)
-            for (int q = 0; q < PW; ++q) s_top[par][q] = a[r][q];
-          }
+          for (int q = 0; q < PW; ++q) s_wcand[par][wave][q] = a[r][q];
         }
-      }
-      __syncthreads();
-      bv = s_val[par][0]; bi = s_idx[par][0];
-#pragma clang loop unroll(full)
-      for (int k = 1; k < NW; ++k)
-        if (better(s_val[par][k], s_idx[par][k], bv, bi)) {
-          bv = s_val[par][k]; bi = s_idx[par][k]; bw = k;
-        }
-    }
-
-    int p;
-    const T *u;    // the pivot row (LDS)
-    const T *top;  // the old row c (LDS)
-    if (G == 1 && PIV) {
-      p = bi < 0 ? c : bi;
-      top = s_top[par];
-      u = (p == c) ? top : s_wcand[par][bw];
-    } else if (G == 1) {
-      // no pivoting: the pivot row is row c
-      p = c;
-#pragma clang loop unroll(full)
-      for (int r = 0; r < R; ++r) {
-        const int li = r * NT + t;
         if (li == c) {
 #pragma clang loop unroll(full)
           for (int q = 0; q < PW; ++q) s_top[par][q] = a[r][q];
         }
       }
       __syncthreads();
-      u = top = s_top[par];
-    } else {
-      // publish: candidate value/row, and the top row if held here
-      const unsigned seq = seq0 + (unsigned)c + 1u;
-      PanelRec *mine = recs + par * 256 + g;
-      if (t == 0) {
-        st_agent(&mine->val, (double)bv);
-        st_agent(&mine->idx, bi);
+      // ---- this workgroup's best over its waves ----
+      int bw = 0;
+      if (PIV) {
+        bv = s_val[par][0];
+        bi = s_idx[par][0];
+#pragma clang loop unroll(full)
+        for (int k = 1; k < NW; ++k)
+          if (better(s_val[par][k], s_idx[par][k], bv, bi)) {
+            bv = s_val[par][k]; bi = s_idx[par][k]; bw = k;
+          }
       }
-#pragma clang loop unroll(full)
-      for (int r = 0; r < R; ++r) {
-        const int li = g * RPW + r * NT + t;
-        if (bi >= 0 && li == bi) {
-#pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) st_agent(&mine->cand[q], (double)a[r][q]);
+      int p;
+      const T *u;    // the pivot row (LDS)
+      const T *top;  // row c before the swap (LDS)
+      if (!MULTI) {
+        p = (PIV && bi >= 0) ? bi : c;
+        top = s_top[par];
+        u = (p == c) ? top : s_wcand[par][bw];
+      } else {
+        const unsigned long long sq = (unsigned long long)(seq0 + (unsigned)c + 1u) << 32;
+        if (wave == 0) {
+          // publish: lane k writes word k of this workgroup's record (and
+          // of the top row in workgroup 0), straight from LDS
+          unsigned long long *mine = recs + ((long)par * GMAX + g) * RECW;
+          unsigned *own = &s_rec[par][g][0];
+          for (int k = lane; k < (g == 0 ? L1 + CW : L1); k += 64) {
+            unsigned pl;
+            if (k < CW) pl = bi >= 0 ? row_word(s_wcand[par][bw], k) : 0u;
+            else if (k < CW + VW) pl = value_word(bv, k - CW);
+            else if (k == CW + VW) pl = (unsigned)bi;
+            else pl = row_word(s_top[par], k - L1);
+            const int kw = k < L1 ? k : TOPW + (k - L1);
+            st_agent(mine + kw, sq | pl);
+            own[kw] = pl;  // this workgroup's own record, without the round trip
+          }
         }
-        if (li == c) {
+        if (poller) {
+          // poll the other records (and workgroup 0's top row) until every
+          // word carries this column's sequence number; bounded, so a lost
+          // record cannot hang the kernel (info then reports -1)
+          const unsigned long long *prec = recs + (long)par * GMAX * RECW;
+          unsigned long long wv[CHW];
+          bool ok = false;
+          for (int spin = 0; !ok; ++spin) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) st_agent(&mine->top[q], (double)a[r][q]);
+            for (int i = 0; i < CHW; ++i) wv[i] = poff[i] >= 0 ? ld_agent(prec + poff[i]) : sq;
+            ok = true;
+#pragma clang loop unroll(full)
+            for (int i = 0; i < CHW; ++i) ok = ok && (wv[i] & 0xffffffff00000000ull) == sq;
+            ok = __all(ok);
+            if (!ok && spin >= SPIN_MAX) {
+              if (lane == 0) *info = -1;
+              ok = true;
+            }
+            if (!ok) __builtin_amdgcn_s_sleep(1);
+          }
+          unsigned *srec = &s_rec[par][0][0];
+#pragma clang loop unroll(full)
+          for (int i = 0; i < CHW; ++i)
+            if (poff[i] >= 0) srec[poff[i]] = (unsigned)wv[i];
         }
+        __syncthreads();
+        // the pivot: the best of the G records (lane k holds record k; every
+        // wave decides the same way from the same data)
+        T cv = (T)0;
+        int ci = -1;
+        if (lane < G) {
+          ci = (int)s_rec[par][lane][CW + VW];
+          cv = ci >= 0 ? *reinterpret_cast<const T *>(&s_rec[par][lane][CW]) : (T)0;
+        }
+        const int my_row = lane < G ? ci : -1;
+        wave_argmax(cv, ci);
+        // rows are distinct across workgroups: one record holds row ci
+        const unsigned long long hold = __ballot(ci >= 0 && my_row == ci);
+        p = ci < 0 ? c : ci;
+        top = reinterpret_cast<const T *>(&s_rec[par][0][TOPW]);
+        const int holder = hold ? __ffsll((long long)hold) - 1 : 0;
+        u = (p == c) ? top : reinterpret_cast<const T *>(&s_rec[par][holder][0]);
       }
-      wait_all_mem();
-      __syncthreads();
-      if (t == 0) st_agent(&mine->seq, seq);
-      // thread t < G waits for workgroup t's record of this column, reads it
-      const int owner_c = c / RPW;
-      if (t < G) {
-        const PanelRec *rt = recs + par * 256 + t;
-        while (ld_agent(&rt->seq) != seq) __builtin_amdgcn_s_sleep(1);
-        // issue every load of the record before using any of them
-        double rv[PW + 1];
-        const int ridx = ld_agent(&rt->idx);
-        rv[PW] = ld_agent(&rt->val);
-#pragma clang loop unroll(full)
-        for (int q = 0; q < PW; ++q) rv[q] = ld_agent(&rt->cand[q]);
-        s_cval[par][t] = (T)rv[PW];
-        s_cidx[par][t] = ridx;
-#pragma clang loop unroll(full)
-        for (int q = 0; q < PW; ++q) s_cand[par][t][q] = (T)rv[q];
-        if (t == owner_c) {
-#pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) rv[q] = ld_agent(&rt->top[q]);
-#pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) s_top[par][q] = (T)rv[q];
-        }
-      }
-      __syncthreads();
-      // every thread scans the same G candidates in the same order
-      T cv = s_cval[par][0];
-      int ci = s_cidx[par][0];
-      for (int k = 1; k < G; ++k)
-        if (better(s_cval[par][k], s_cidx[par][k], cv, ci)) { cv = s_cval[par][k]; ci = s_cidx[par][k]; }
-      p = ci < 0 ? c : ci;
-      top = s_top[par];
-      u = (p == c) ? top : s_cand[par][p / RPW];
-    }
 
-    // pivots go to global memory after the loop: a store here would make
-    // the next barrier wait for its acknowledgement from memory
-    if (t == 0) { s_piv[c] = p; s_pmag[c] = fabs(u[K]); }
-    // ---- row exchange inside the strip ----
-    if (p != c) {
+      // pivots go to global memory after the loop: a store here would make
+      // the next barrier wait for its acknowledgement from memory
+      if (t == 0) { s_piv[c] = p; s_pmag[c] = fabs(u[kk]); }
+      // ---- row exchange inside the strip ----
+      if (p != c) {
 #pragma clang loop unroll(full)
-      for (int r = 0; r < R; ++r) {
-        const int li = g * RPW + r * NT + t;
-        if (li == c) {
+        for (int r = 0; r < R; ++r) {
+          const int li = g * RPW + r * NT + t;
+          if (li == c) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) a[r][q] = u[q];
-        } else if (li == p) {
+            for (int q = 0; q < PW; ++q) a[r][q] = u[q];
+          } else if (li == p) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) a[r][q] = top[q];
+            for (int q = 0; q < PW; ++q) a[r][q] = top[q];
+          }
         }
       }
-    }
-    // ---- multipliers (division) and rank-1 update of columns > c ----
-    const T piv = u[K];
-    if (piv != (T)0) {
-      // the pivot row with every offset that is not a later column zeroed,
-      // so the row updates below need no per-element predicate
-      T uq[PW];
-#pragma clang loop unroll(full)
-      for (int q = K + 1; q < PW; ++q) uq[q] = (q < lim) ? u[q] : (T)0;
-#pragma clang loop unroll(full)
-      for (int r = 0; r < R; ++r) {
-        const int li = g * RPW + r * NT + t;
-        if (li < m && li > c) {
-          const T l = a[r][K] / piv;
-          a[r][K] = l;
-#pragma clang loop unroll(full)
-          for (int q = K + 1; q < PW; ++q) a[r][q] -= l * uq[q];
-        }
-      }
+      eliminate(c, lim, u, kc);
     }
   };
 #pragma clang loop unroll(disable)
   for (int base = 0; base < w; base += 4) {
     const int lim = w - base;
-    column(base + 0, lim, std::integral_constant<int, 0>{});
-    if (base + 1 < w) column(base + 1, lim, std::integral_constant<int, 1>{});
-    if (base + 2 < w) column(base + 2, lim, std::integral_constant<int, 2>{});
-    if (base + 3 < w) column(base + 3, lim, std::integral_constant<int, 3>{});
+    column(base + 0, lim, K0{});
+    if (base + 1 < w) column(base + 1, lim, K1{});
+    if (base + 2 < w) column(base + 2, lim, K2{});
+    if (base + 3 < w) column(base + 3, lim, K3{});
     rotate4();
   }
 #pragma clang loop unroll(disable)
@@ -341,6 +379,7 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   const int m = m_end - j0;
   if (m <= 0 || w <= 0) return CLU_OK;
   if (w > PW) return -1;
+  if (m < w) return -4;  // row c of every column lies in the strip
   if (!pivot && m > 1024) return -3;  // the unpivoted strip is one workgroup
   // one 256-thread workgroup up to 1024 rows, then 512-thread workgroups of
   // 2048 rows (4 per thread) -- fewer participants in every column's exchange
@@ -351,27 +390,38 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   else { R = 4; NT = 512; G = (m + 2047) / 2048; }
   // FP32 strips taller than the resident workgroups allow: 8 rows per thread
   if (pivot && sizeof(T) == 4 && G > pw.max_groups) { R = 8; G = (m + 4095) / 4096; }
-  if (G > pw.max_groups || G > 64) return -2;
-  PanelRec *recs = reinterpret_cast<PanelRec *>(pw.slots);
+  if (G > pw.max_groups || G > GMAX) return -2;
+  // the polling wave reads G records and a row of words in one round
+  {
+    const int vw = (int)sizeof(T) / 4, l1 = PW * vw + vw + 1;
+    if (G > 1 && G * l1 + PW * vw > (NT / 64) * 64 * CHW) return -2;
+  }
+  unsigned long long *recs = reinterpret_cast<unsigned long long *>(pw.slots);
   const unsigned seq0 = pw.epoch;
+#define CLU_PANEL(TT, RR, NN, MU, PV)                                                         \
+  hipLaunchKernelGGL((panel_base_kernel<TT, RR, NN, MU, PV>), dim3(G), dim3(NN), 0, s,        \
+                     (TT *)A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info)
   if (!pivot) {
     switch (R) {
-      case 1: hipLaunchKernelGGL((panel_base_kernel<T, 1, 256, false>), dim3(1), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
-      case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2, 256, false>), dim3(1), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
-      default: hipLaunchKernelGGL((panel_base_kernel<T, 4, 256, false>), dim3(1), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+      case 1: CLU_PANEL(T, 1, 256, false, false); break;
+      case 2: CLU_PANEL(T, 2, 256, false, false); break;
+      default: CLU_PANEL(T, 4, 256, false, false); break;
     }
-  } else if (NT == 512 && R == 8) {
+  } else if (G > 1 && R == 8) {
     if (sizeof(T) != 4) return -2;
-    hipLaunchKernelGGL((panel_base_kernel<float, 8, 512, true>), dim3(G), dim3(512), 0, s, (float *)A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info);
+    CLU_PANEL(float, 8, 512, true, true);
+  } else if (G > 1) {
+    CLU_PANEL(T, 4, 512, true, true);
   } else if (NT == 512) {
-    hipLaunchKernelGGL((panel_base_kernel<T, 4, 512, true>), dim3(G), dim3(512), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info);
+    CLU_PANEL(T, 4, 512, false, true);
   } else {
     switch (R) {
-      case 1: hipLaunchKernelGGL((panel_base_kernel<T, 1, 256, true>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
-      case 2: hipLaunchKernelGGL((panel_base_kernel<T, 2, 256, true>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
-      default: hipLaunchKernelGGL((panel_base_kernel<T, 4, 256, true>), dim3(G), dim3(256), 0, s, A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info); break;
+      case 1: CLU_PANEL(T, 1, 256, false, true); break;
+      case 2: CLU_PANEL(T, 2, 256, false, true); break;
+      default: CLU_PANEL(T, 4, 256, false, true); break;
     }
   }
+#undef CLU_PANEL
   CLU_LAUNCHED("panel_base_kernel");
   pw.epoch += (unsigned)w;
   return CLU_OK;

@@ -231,14 +231,14 @@ template int tri_inv_blocks<float>(bool, int, int, const float *, long, float *,
 
 // ---------------------------------------------------------------------------
 // Full inverse of a triangle by recursive doubling on the matrix cores.
-// The 32 x 32 diagonal blocks are inverted first, one wave each (lane j
+// The 16 x 16 diagonal blocks are inverted first, one wave each (lane j
 // forms column j of the block inverse in registers, reading the triangle
 // from LDS as broadcasts).  Then, for s = 32, 64, ..., pairs of adjacent
 // inverted diagonal blocks of size s are joined into one of size 2s:
 //   lower:  X21 = -X22 (L21 X11),   upper:  X12 = -X11 (U12 X22),
 // two strided-batched GEMMs per level over all pairs at once.
 // ---------------------------------------------------------------------------
-constexpr int TIB = 32;  // base block
+constexpr int TIB = 16;  // base block
 
 // Block k of every triangle (grid (nblocks, ntri)): invert the diagonal
 // block, write it into X and zero the part of its block row of X that lies
@@ -254,16 +254,17 @@ __global__ __launch_bounds__(64) void trinv_base_kernel(int n, const T *__restri
   const int bs = n - k0 < TIB ? n - k0 : TIB;
   const int lane = threadIdx.x;
   // all 16 loads of a lane in flight before the first LDS store
-  T v[TIB * TIB / 64];
+  constexpr int RPP = 64 / TIB;  // rows per pass of the wave
+  T v[TIB / RPP];
 #pragma unroll
-  for (int r = 0; r < TIB * TIB / 64; ++r) {
-    const int i = r * 2 + (lane >> 5), j = lane & 31;
+  for (int r = 0; r < TIB / RPP; ++r) {
+    const int i = r * RPP + lane / TIB, j = lane % TIB;
     v[r] = (i < bs && j < bs) ? A[(long)(k0 + i) * lda + k0 + j] : (T)0;
   }
 #pragma unroll
-  for (int r = 0; r < TIB * TIB / 64; ++r) Ls[r * 2 + (lane >> 5)][lane & 31] = v[r];
+  for (int r = 0; r < TIB / RPP; ++r) Ls[r * RPP + lane / TIB][lane % TIB] = v[r];
   __syncthreads();
-  const int j = lane;  // lanes 0..31: column j of the block inverse
+  const int j = lane;  // lanes 0..TIB-1: column j of the block inverse
   T x[TIB];
   if (!UPPER) {
 #pragma unroll
