@@ -46,7 +46,6 @@ constexpr int PW = 16;  // max strip width
 //   [TOPW, TOPW+PW*VW)  workgroup 0 only: row c (the top row) before the swap
 constexpr int GMAX = 64;  // workgroups in one exchange
 constexpr int RECW = 72;  // words per record
-constexpr int CHW = 4;    // record words per polling lane per round
 constexpr int SPIN_MAX = 1 << 22;  // polling rounds before giving up (seconds)
 
 // Payload word k of a value: FP64 low half first (little-endian), so the
@@ -71,6 +70,7 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   constexpr int TOPW = (L1 + 1) / 2 * 2; // the top row, 8-byte aligned
   constexpr int RPW = NT * R;            // rows per workgroup
   constexpr int NW = NT / 64;            // waves
+  constexpr int CHW = NW >= 8 ? 4 : 5;   // record words per polling lane
   static_assert(TOPW + CW <= RECW, "record size");
   const int G = MULTI ? (int)gridDim.x : 1, g = MULTI ? (int)blockIdx.x : 0;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -383,19 +383,28 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   if (!pivot && m > 1024) return -3;  // the unpivoted strip is one workgroup
   // one 256-thread workgroup up to 1024 rows, then 512-thread workgroups of
   // 2048 rows (4 per thread) -- fewer participants in every column's exchange
+  // One 256-thread workgroup up to 1024 rows.  Taller strips: 256-thread
+  // workgroups of 1024 rows while one per panel CU suffices, else 512-thread
+  // ones of 2048 rows (4-wave workgroups spend 23 % less per column, but an
+  // exchange among 64 of them costs more than it saves); FP32 strips too
+  // tall for either use 8 rows per thread.  All workgroups of a strip must
+  // be resident at once (one per panel CU), and the polling waves must
+  // cover G records in one round.
+  const int vw = (int)sizeof(T) / 4, l1 = PW * vw + vw + 1;
+  auto poll_ok = [&](int g, int nt) {
+    return g * l1 + PW * vw <= (nt / 64) * 64 * (nt >= 512 ? 4 : 5);
+  };
   int R, G, NT;
   if (m <= 256) { R = 1; G = 1; NT = 256; }
   else if (m <= 512) { R = 2; G = 1; NT = 256; }
   else if (m <= 1024) { R = 4; G = 1; NT = 256; }
-  else { R = 4; NT = 512; G = (m + 2047) / 2048; }
-  // FP32 strips taller than the resident workgroups allow: 8 rows per thread
-  if (pivot && sizeof(T) == 4 && G > pw.max_groups) { R = 8; G = (m + 4095) / 4096; }
-  if (G > pw.max_groups || G > GMAX) return -2;
-  // the polling wave reads G records and a row of words in one round
-  {
-    const int vw = (int)sizeof(T) / 4, l1 = PW * vw + vw + 1;
-    if (G > 1 && G * l1 + PW * vw > (NT / 64) * 64 * CHW) return -2;
+  else {
+    R = 4; NT = 256; G = (m + 1023) / 1024;
+    if (G > pw.max_groups || !poll_ok(G, 256)) { NT = 512; G = (m + 2047) / 2048; }
+    if (pivot && sizeof(T) == 4 && G > pw.max_groups) { R = 8; NT = 512; G = (m + 4095) / 4096; }
   }
+  if (G > pw.max_groups || G > GMAX) return -2;
+  if (G > 1 && !poll_ok(G, NT)) return -2;
   unsigned long long *recs = reinterpret_cast<unsigned long long *>(pw.slots);
   const unsigned seq0 = pw.epoch;
 #define CLU_PANEL(TT, RR, NN, MU, PV)                                                         \
@@ -410,6 +419,8 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   } else if (G > 1 && R == 8) {
     if (sizeof(T) != 4) return -2;
     CLU_PANEL(float, 8, 512, true, true);
+  } else if (G > 1 && NT == 256) {
+    CLU_PANEL(T, 4, 256, true, true);
   } else if (G > 1) {
     CLU_PANEL(T, 4, 512, true, true);
   } else if (NT == 512) {
