@@ -108,6 +108,10 @@ struct Workspace {
   // scratch of the triangular inverses (products of the doubling steps)
   void *tinv = nullptr;
   size_t tinv_b = 0;
+  // recursive panel: inverse of the left half's L11 and the product
+  // inv(L11) A12 before it replaces A12 (panel stream only)
+  void *pinv = nullptr, *ptmp = nullptr;
+  size_t pinv_b = 0, ptmp_b = 0;
   PanelWork pw{};
 };
 
@@ -311,7 +315,21 @@ static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, 
   int rc = factor_panel<T>(A, lda, n, c0, w1, pc0, pcw, ipiv, ws, s, pivot);
   if (rc) return rc;
   T *A11 = A + (long)c0 * lda + c0;
-  if ((rc = trsm_llnu_big<T>(w1, w2, A11, lda, A11 + w1, lda, s))) return rc;
+  // A12 <- inv(L11) A12: for w1 >= 128 through the inverse (about ten short
+  // launches, most of them matrix-core GEMMs) rather than the 16-row blocked
+  // substitution, whose serial steps cost more at that size
+  if (w1 >= 128 && ws.pinv_b >= sizeof(T) * w1 * w1 && ws.ptmp_b >= sizeof(T) * w1 * w2 &&
+      ws.tinv_b >= sizeof(T) * w1 * w1) {
+    T *pinv = (T *)ws.pinv, *ptmp = (T *)ws.ptmp;
+    if ((rc = tri_inv<T>(false, true, w1, 1, A11, lda, 0, pinv, w1, 0, (T *)ws.tinv, s)))
+      return rc;
+    if ((rc = gemm_t<T>(false, false, w1, w2, w1, (T)1, pinv, w1, A11 + w1, lda, (T)0, ptmp, w2,
+                        s)))
+      return rc;
+    if ((rc = convert<T, T>(w1, w2, ptmp, w2, A11 + w1, lda, s))) return rc;
+  } else if ((rc = trsm_llnu_big<T>(w1, w2, A11, lda, A11 + w1, lda, s))) {
+    return rc;
+  }
   if (n - c0 - w1 > 0)
     if ((rc = gemm_t<T>(false, false, n - c0 - w1, w2, w1, (T)-1, A11 + (long)w1 * lda, lda,
                         A11 + w1, lda, (T)1, A11 + (long)w1 * lda + w1, lda, s)))
@@ -849,6 +867,8 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
       if ((rc = ensure_dbuf(&ws.ubuf, &ws.ubuf_elems, (size_t)nb * n))) return rc;
       if ((rc = ensure_bytes(&ws.tinv, &ws.tinv_b, sizeof(double) * nb * nb))) return rc;
     }
+    if ((rc = ensure_bytes(&ws.pinv, &ws.pinv_b, sizeof(double) * nb * nb / 4))) return rc;
+    if ((rc = ensure_bytes(&ws.ptmp, &ws.ptmp_b, sizeof(double) * nb * nb / 4))) return rc;
   }
   hipStream_t sp = ws.s_panel;
   ws.pw.info = dinfo;
@@ -1066,6 +1086,8 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     if ((rc = ensure_bytes(&ws.winv32, &ws.winv32_b, sizeof(float) * nb * nb))) return rc;
     if ((rc = ensure_bytes(&ws.ubuf32, &ws.ubuf32_b, sizeof(float) * nb * n))) return rc;
     if ((rc = ensure_bytes(&ws.tinv, &ws.tinv_b, sizeof(float) * nb * nb))) return rc;
+    if ((rc = ensure_bytes(&ws.pinv, &ws.pinv_b, sizeof(float) * nb * nb / 4))) return rc;
+    if ((rc = ensure_bytes(&ws.ptmp, &ws.ptmp_b, sizeof(float) * nb * nb / 4))) return rc;
     if (lp) {
       if ((rc = ensure_bytes(&ws.pa16, &ws.pa16_b, (size_t)2 * n * Kp))) return rc;
       if ((rc = ensure_bytes(&ws.pu16, &ws.pu16_b, (size_t)2 * n * Kp))) return rc;
