@@ -143,7 +143,7 @@ def lp_gemm_only(lib, _lib, torch, n, nb, steps, warmup, prec="bf16"):
     print(json.dumps({"lp_gemm": f"{m}x{m}x{nb} {prec}", **res}), flush=True)
 
 
-def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps):
+def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, fp64_nb=None):
     """Mixed-precision LU + FP64 GMRES-IR against the plain FP64 LU on the
     same diagonally dominant system (A U[-0.5,0.5) with a_ii = sum_j |a_ij|,
     b from MatGen).  A mixed step converts A to FP32, factors it with the
@@ -160,21 +160,37 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps):
     # the mixed results are compared with
     x64 = None
     if fp64_steps > 0:
+        # the FP64 LU with its own best block size (1024 measured fastest at
+        # n = 65536: 57.0 TFLOP/s against 56.4 with 512 and 52.8 with 256)
+        nb64 = fp64_nb or nb
         W = torch.empty_like(A)
         x64 = torch.empty_like(b)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(fp64_steps):
+
+        def step64():
             W.copy_(A)
             x64.copy_(b)
-            ipiv, info = clu.getrf(W, nb)
+            ipiv, info = clu.getrf(W, nb64)
             clu.getrs(W, ipiv, x64)
+            return info
+        step64()  # warmup (also records the panel durations the update split uses)
+        torch.cuda.synchronize()
+        lib.charmlu_profile(1)
+        t0 = time.perf_counter()
+        for _ in range(fp64_steps):
+            info = step64()
         torch.cuda.synchronize()
         sec = (time.perf_counter() - t0) / fp64_steps
+        prof = (ctypes.c_double * 20)()
+        _lib.check(lib.charmlu_profile_read(prof, 20), "charmlu_profile_read")
+        lib.charmlu_profile(0)
         res = clu.residual(A, x64, b)
+        g_work, g_busy = prof[1], prof[15]
+        upd = g_work / (g_busy * 1e-3) / 1e12 if g_busy > 0 else 0.0
         out["fp64"] = {"tflops": round(flops / sec / 1e12, 3), "ms": round(sec * 1e3, 1),
-                       "residual": res["residual"], "info": int(info.item()),
-                       "steps": fp64_steps}
+                       "block": nb64, "residual": res["residual"], "info": int(info.item()),
+                       "steps": fp64_steps,
+                       "trailing_update_tflops": round(upd, 3),
+                       "trailing_update_frac": round(upd / FP64_PEAK, 4)}
         del W
         torch.cuda.empty_cache()
     A32 = torch.empty((n, n), dtype=torch.float32, device="cuda")
@@ -255,13 +271,14 @@ def main():
     ap.add_argument("--lp-gemm-only", action="store_true",
                     help="time only the low-precision trailing update of step 0 at --mixed-n")
     ap.add_argument("--mixed-n", type=int, default=65536)
-    ap.add_argument("--mixed-nb", type=int, default=512)
+    ap.add_argument("--mixed-nb", type=int, default=1024)
+    ap.add_argument("--mixed-fp64-nb", type=int, default=1024)
     ap.add_argument("--mixed-precs", default="bf16,fp16,bf16-nopiv")
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--mixed-fp64-steps", type=int, default=1)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--c5", action="store_true",
-                    help="only the FP8 form at n=131072 (block 512, diagonally dominant; "
+                    help="only the FP8 form at n=131072 (block --mixed-nb, diagonally dominant; "
                          "FP64 A 137 GB + FP32 factors 69 GB in HBM), one step each")
     ap.add_argument("--mixed-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -296,13 +313,13 @@ def main():
         lp_gemm_only(lib, _lib, torch, args.mixed_n, args.mixed_nb, args.steps, args.warmup)
         return
     if args.c5:
-        print(json.dumps({"c5": mixed_bench(clu, lib, _lib, torch, 131072, 512,
+        print(json.dumps({"c5": mixed_bench(clu, lib, _lib, torch, 131072, args.mixed_nb,
                                              ["fp8-nopiv", "fp8"], 1, 0, 0)}), flush=True)
         return
     if args.mixed_only:
         print(json.dumps(mixed_bench(clu, lib, _lib, torch, args.mixed_n, args.mixed_nb,
                                      args.mixed_precs.split(","), args.mixed_steps, args.warmup,
-                                     args.mixed_fp64_steps)), flush=True)
+                                     args.mixed_fp64_steps, args.mixed_fp64_nb)), flush=True)
         return
     A0 = clu.gen_matrix(n)
     b0 = clu.gen_rhs(n)
@@ -421,7 +438,7 @@ def main():
         torch.cuda.empty_cache()
         out["mixed"] = mixed_bench(clu, lib, _lib, torch, args.mixed_n, args.mixed_nb,
                                    args.mixed_precs.split(","), args.mixed_steps, args.warmup,
-                                   args.mixed_fp64_steps)
+                                   args.mixed_fp64_steps, args.mixed_fp64_nb)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, nb, args.cpu_threads)
     print(json.dumps(out), flush=True)

@@ -325,3 +325,29 @@ def test_fp8_rejects_block_not_multiple_of_128(dev):
     A = torch.eye(960, dtype=torch.float32, device="cuda")
     with pytest.raises(CharmluError):
         dev.getrf_mixed(A, 192, "fp8")
+
+
+def test_tall_panel_mixed_refines_to_fp64(dev):
+    # n = 40960, block 1024: FP32 panels taller than 32768 rows run on 512-
+    # thread workgroups (20 of them).  The refined x must pass the
+    # reference's criterion and agree with the FP64 device LU's x.
+    import torch
+    n, nb = 40960, 1024
+    A = dev.gen_matrix(n, kind="diagdom")
+    b = dev.gen_rhs(n)
+    A32 = dev.lag2s(A)
+    ipiv, info = dev.getrf_mixed(A32, nb, "bf16")
+    assert int(info.item()) == 0
+    x, st = dev.gesv_ir(A, A32, ipiv, b, tol=1.0)
+    del A32
+    W = A.clone()
+    ip64, info64 = dev.getrf(W, nb)
+    x64 = b.clone()
+    dev.getrs(W, ip64, x64)
+    torch.cuda.synchronize()
+    assert int(info64.item()) == 0
+    res = dev.residual(A, x, b)
+    err = float(((x - x64).abs().max() / x64.abs().max()).item())
+    print(f"n={n}: {st}, residual {res['residual']:.3f}, error vs FP64 LU {err:.2e}")
+    assert st["residual"] <= 1.0 and res["residual"] < 16
+    assert err < 1e-10
