@@ -63,6 +63,8 @@ struct Workspace {
   hipStream_t s_panel = nullptr;  // panels: the reserved CUs only
   hipStream_t s_main = nullptr;   // everything else: the other CUs
   hipStream_t s_full = nullptr;   // the end of each trailing update: every CU
+  hipStream_t s_pgemm = nullptr;  // the large GEMMs inside a panel: every CU
+  hipEvent_t ev_pg_in = nullptr, ev_pg_out = nullptr;
   int panel_cus = 0, ncu = 0;
   hipEvent_t ev_panel = nullptr, ev_la = nullptr, ev_start = nullptr, ev_end = nullptr;
   hipEvent_t ev_full = nullptr;
@@ -194,6 +196,9 @@ static int get_ws(Workspace **out) {
     if (!w.s_main) CLU_HIP(hipStreamCreateWithFlags(&w.s_main, hipStreamNonBlocking));
     if (!w.s_panel) CLU_HIP(hipStreamCreateWithFlags(&w.s_panel, hipStreamNonBlocking));
     CLU_HIP(hipStreamCreateWithFlags(&w.s_full, hipStreamNonBlocking));
+    CLU_HIP(hipStreamCreateWithFlags(&w.s_pgemm, hipStreamNonBlocking));
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_pg_in, hipEventDisableTiming));
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_pg_out, hipEventDisableTiming));
     w.panel_cus = reserve;
     w.ncu = ncu;
     CLU_HIP(hipEventCreateWithFlags(&w.ev_full, hipEventDisableTiming));
@@ -330,10 +335,26 @@ static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, 
   } else if ((rc = trsm_llnu_big<T>(w1, w2, A11, lda, A11 + w1, lda, s))) {
     return rc;
   }
-  if (n - c0 - w1 > 0)
-    if ((rc = gemm_t<T>(false, false, n - c0 - w1, w2, w1, (T)-1, A11 + (long)w1 * lda, lda,
-                        A11 + w1, lda, (T)1, A11 + (long)w1 * lda + w1, lda, s)))
+  // A22 -= A21 A12.  With the panel on CUs of its own, a GEMM of 1 GFLOP or
+  // more runs on a stream of every CU instead (ordered by events): it still
+  // has the panel's CUs, which are idle meanwhile, and takes update CUs as
+  // they free up.  The strips stay on the panel stream.
+  const int Mr = n - c0 - w1;
+  if (Mr > 0) {
+    const bool wide = ws.panel_cus > 0 && 2.0 * Mr * w2 * w1 >= 1e9;
+    hipStream_t sg = wide ? ws.s_pgemm : s;
+    if (wide) {
+      CLU_HIP(hipEventRecord(ws.ev_pg_in, s));
+      CLU_HIP(hipStreamWaitEvent(sg, ws.ev_pg_in, 0));
+    }
+    if ((rc = gemm_t<T>(false, false, Mr, w2, w1, (T)-1, A11 + (long)w1 * lda, lda, A11 + w1, lda,
+                        (T)1, A11 + (long)w1 * lda + w1, lda, sg)))
       return rc;
+    if (wide) {
+      CLU_HIP(hipEventRecord(ws.ev_pg_out, sg));
+      CLU_HIP(hipStreamWaitEvent(s, ws.ev_pg_out, 0));
+    }
+  }
   return factor_panel<T>(A, lda, n, c0 + w1, w2, pc0, pcw, ipiv, ws, s, pivot);
 }
 
