@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
   // so each XCD gets a contiguous range of tiles, walked in groups of GM
   // tile rows (GM = 4 measured best: 325 TFLOP/s at 65024^2 x 512 BF16 vs
   // 315 with 8 and 295 with 16; without the XCD remap 293 and 1.6x the
-  // fetched bytes)
+  // fetched bytes; at K = 1024, 548 with 4 or 8, 530 with 2, 534 with 16)
   int tm, tn;
   {
     const int bid = blockIdx.x, nblk = gridDim.x, per = nblk / 8;
@@ -261,6 +261,14 @@ template <typename T16>
 static void hgemm_launch2(bool cinit, bool vec, int M, int N, int K, float alpha, const T16 *A,
                           long lda, const T16 *B, long ldb, float beta, float *C, long ldc,
                           hipStream_t s, const int *escA = nullptr, const int *escB = nullptr) {
+  // K >= 1024 (the mixed LU's block 1024): 3 stages, 3 workgroups per CU
+  // (546 TFLOP/s at 64512^2 x 1024 BF16 against 526 with 2 stages and 4 per
+  // CU, 522 with 4 and 2)
+  if (cinit && vec && K >= 1024) {
+    hgemm_launch<T16, 128, 3, 3, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+                                             escA, escB);
+    return;
+  }
   if (cinit && vec)
     hgemm_launch<T16, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
                                              escA, escB);
