@@ -76,9 +76,14 @@ template <typename T>
 int apply_moves(const int2 *moves, const int *count, int max_moves, T *A,
                 long lda, int c0, int c1, int skip0, int skip1, bool inverse,
                 hipStream_t s);
-// the interchanges ipiv[0..n) (or their inverse) applied to one vector
+// Move lists of every run of 256 pivots of ipiv[0..n) at once (one
+// workgroup per run): run r at moves + r*512, its length at counts[r].
+int pivots_to_moves_all(const int *ipiv, int n, int2 *moves, int *counts, hipStream_t s);
+// the interchanges ipiv[0..n) (or their inverse) applied to one vector;
+// moves / counts: scratch for the move lists of all (n + 255) / 256 runs
 template <typename T>
-int permute_vec(const int *ipiv, int n, T *v, bool inverse, hipStream_t s);
+int permute_vec(const int *ipiv, int n, T *v, bool inverse, int2 *moves, int *counts,
+                hipStream_t s);
 
 // ---- panel factorization (FP64 / FP32) ------------------------------------
 struct PanelWork {
@@ -122,10 +127,6 @@ int dgemv(bool trans, int M, int N, double alpha, const double *A, long lda, con
           long incx, double beta, double *y, long incy, hipStream_t s);
 
 // y[0..M) -= A[0..M, 0..K) * x  (trans: y[0..M) -= A[0..K, 0..M)^T * x)
-// Inverses of the S x S diagonal blocks of a unit lower / non-unit upper
-// triangle, block k at Inv + k*S*S (ld S), all solved in one launch
-template <typename T>
-int tri_inv_blocks(bool upper, int n, int S, const T *A, long lda, T *Inv, hipStream_t s);
 // X = inverse of the n x n triangle of A (unit lower, or upper; unit
 // selects an implied unit diagonal), every element of X written (zeros on
 // the other side).  ntri triangles at once: triangle i at A + i*sa, its

@@ -67,6 +67,7 @@ struct Workspace {
   hipEvent_t ev_pg_in = nullptr, ev_pg_out = nullptr;
   int panel_cus = 0, ncu = 0;
   hipEvent_t ev_panel = nullptr, ev_la = nullptr, ev_start = nullptr, ev_end = nullptr;
+  hipEvent_t ev_urow = nullptr;  // the step's interchanges and U row done (update stream)
   hipEvent_t ev_full = nullptr;
   // Durations of the last FP64 factorization's panels (begin/end events on
   // the panel stream, per step), read back at the next call with the same
@@ -105,8 +106,8 @@ struct Workspace {
   void *irbuf = nullptr;
   size_t irbuf_b = 0;
   // solve: inverses of the diagonal blocks of L and U, the intermediate y
-  void *sinv = nullptr, *sy = nullptr;
-  size_t sinv_b = 0, sy_b = 0;
+  void *sinv = nullptr, *sy = nullptr, *smv = nullptr;  // solve: inverses, Y, move lists
+  size_t sinv_b = 0, sy_b = 0, smv_b = 0;
   // scratch of the triangular inverses (products of the doubling steps)
   void *tinv = nullptr;
   size_t tinv_b = 0;
@@ -205,6 +206,7 @@ static int get_ws(Workspace **out) {
     CLU_HIP(hipEventCreateWithFlags(&w.ev_end, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_panel, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_la, hipEventDisableTiming));
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_urow, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming));
     // panel candidate records: 2 parities x 64 workgroups x 72 words of 8
     // bytes, zeroed once; their sequence numbers only ever grow (pw.epoch),
@@ -406,31 +408,45 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
   if (rc) return rc;
   Workspace &ws = *wsp;
   constexpr int S = 256;
+  static_assert(S == RUN, "one move list per block");
   const int nblk = (n + S - 1) / S;
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    if ((rc = ensure_bytes(&ws.sinv, &ws.sinv_b, sizeof(T) * 2 * (size_t)nblk * S * S))) return rc;
+    if ((rc = ensure_bytes(&ws.sinv, &ws.sinv_b, sizeof(T) * 3 * (size_t)nblk * S * S))) return rc;
     if ((rc = ensure_bytes(&ws.sy, &ws.sy_b, sizeof(T) * (size_t)n * nrhs))) return rc;
+    if ((rc = ensure_bytes(&ws.smv, &ws.smv_b,
+                           (sizeof(int2) * 2 * RUN + sizeof(int)) * (size_t)nblk)))
+      return rc;
   }
-  T *Linv = (T *)ws.sinv, *Uinv = Linv + (size_t)nblk * S * S;
+  T *Linv = (T *)ws.sinv, *Uinv = Linv + (size_t)nblk * S * S, *tbuf = Uinv + (size_t)nblk * S * S;
   T *Y = (T *)ws.sy;
   const long ldy = nrhs;
-  if ((rc = tri_inv_blocks<T>(false, n, S, dLU, lda, Linv, s))) return rc;
-  if ((rc = tri_inv_blocks<T>(true, n, S, dLU, lda, Uinv, s))) return rc;
+  int2 *vmoves = (int2 *)ws.smv;
+  int *vcounts = (int *)(vmoves + (size_t)nblk * 2 * RUN);
+  // inverses of the diagonal blocks, by recursive doubling, all blocks of a
+  // triangle in one batch (the last, shorter block on its own)
+  const int nfull = n / S, last = n - nfull * S;
+  for (int up = 0; up < 2; ++up) {
+    T *Inv = up ? Uinv : Linv;
+    if (nfull > 0 && (rc = tri_inv<T>(up, !up, S, nfull, dLU, lda, (long)S * (lda + 1), Inv, S,
+                                      (long)S * S, tbuf, s)))
+      return rc;
+    if (last > 0 && (rc = tri_inv<T>(up, !up, last, 1, dLU + (long)nfull * S * (lda + 1), lda, 0,
+                                     Inv + (size_t)nfull * S * S, S, 0, tbuf, s)))
+      return rc;
+  }
   const int nruns = (n + RUN - 1) / RUN;
   const bool vec = nrhs == 1 && ldb == 1;
   auto interchanges = [&](bool inverse) -> int {
-    if (vec) return permute_vec<T>(dipiv, n, dB, inverse, s);
-    // one run at a time through the in-panel move buffer (stream-ordered)
-    for (int i = 0; i < nruns; ++i) {
+    if (vec) return permute_vec<T>(dipiv, n, dB, inverse, vmoves, vcounts, s);
+    // every run's move list in one launch, then the runs in order
+    int e = pivots_to_moves_all(dipiv, n, vmoves, vcounts, s);
+    for (int i = 0; !e && i < nruns; ++i) {
       const int r = inverse ? nruns - 1 - i : i;
-      const int k1 = r * RUN, k2 = n - k1 < RUN ? n : k1 + RUN;
-      int e = pivots_to_moves(dipiv, k1, k2, ws.pmoves, ws.pcount, s);
-      if (e) return e;
-      e = apply_moves<T>(ws.pmoves, ws.pcount, 2 * RUN, dB, ldb, 0, nrhs, nrhs, nrhs, inverse, s);
-      if (e) return e;
+      e = apply_moves<T>(vmoves + (size_t)r * 2 * RUN, vcounts + r, 2 * RUN, dB, ldb, 0, nrhs,
+                         nrhs, nrhs, inverse, s);
     }
-    return CLU_OK;
+    return e;
   };
   // dst = op(G) src for the inverse G of one diagonal block
   auto diag = [&](bool tr, int m, const T *G, const T *src, long lds, T *dst, long ldd) -> int {
@@ -884,13 +900,16 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_runs(ws, nruns))) return rc;
     if (use_inv) {
-      if ((rc = ensure_dbuf(&ws.winv, &ws.winv_elems, (size_t)nb * nb))) return rc;
+      if ((rc = ensure_dbuf(&ws.winv, &ws.winv_elems, (size_t)2 * nb * nb))) return rc;
       if ((rc = ensure_dbuf(&ws.ubuf, &ws.ubuf_elems, (size_t)nb * n))) return rc;
       if ((rc = ensure_bytes(&ws.tinv, &ws.tinv_b, sizeof(double) * nb * nb))) return rc;
     }
     if ((rc = ensure_bytes(&ws.pinv, &ws.pinv_b, sizeof(double) * nb * nb / 4))) return rc;
     if ((rc = ensure_bytes(&ws.ptmp, &ws.ptmp_b, sizeof(double) * nb * nb / 4))) return rc;
   }
+  // inv(L11) of step t, by step parity: the panel of step t+1 writes its
+  // own while the U row of step t still reads this one
+  auto winv_of = [&](int t) { return ws.winv + (size_t)(t & 1) * nb * nb; };
   hipStream_t sp = ws.s_panel;
   ws.pw.info = dinfo;
   CLU_HIP(hipEventRecord(ws.ev_start, su));
@@ -908,7 +927,7 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     r = build_moves(dipiv, r0, r0 + nb, moves_of(t), counts_of(t), sp);
     if (r) return r;
     if (use_inv && r0 + nb < n)  // inv(L11), off the update stream's path
-      r = tri_inv<double>(false, true, nb, 1, dA + (long)r0 * lda + r0, lda, 0, ws.winv, nb, 0,
+      r = tri_inv<double>(false, true, nb, 1, dA + (long)r0 * lda + r0, lda, 0, winv_of(t), nb, 0,
                           (double *)ws.tinv, sp);
     prof_end(pid, sp);
     return r;
@@ -958,7 +977,7 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
                           double *C) -> int {
     if (Ma >= M) return CLU_OK;
     hipStream_t sf = ws.s_full;
-    CLU_HIP(hipStreamWaitEvent(sf, ws.ev_la, 0));
+    CLU_HIP(hipStreamWaitEvent(sf, ws.ev_urow, 0));
     CLU_HIP(hipStreamWaitEvent(sf, ws.ev_panel, 0));
     const int pid = prof_begin(PK_GEMM_BIG, 2.0 * (M - Ma) * N * nb, sf);
     int r = dgemm_trailing(M - Ma, N, nb, -1.0, L + (long)Ma * lda, lda, U, ldu, 1.0,
@@ -998,33 +1017,47 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
       CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
       full_pending = false;
     }
-    // interchanges of step t on every column outside the panel
-    int pid = prof_begin(PK_SWAP, (double)(n - nb), sm);
-    if ((rc = apply_runs<double>(nruns, moves_of(t), counts_of(t), dA, lda, 0, n, r0, r1, false,
-                                 sm)))
-      return rc;
-    prof_end(pid, sm);
-    if (Nt <= 0) break;
+    // Interchanges of step t on every column outside the panel and the U
+    // row, A12 <- L11^{-1} A12 (computeU, lu.C:180-189): first for the
+    // look-ahead block column alone, whose update the next panel waits for,
+    // then for the other columns while that panel runs.
+    const int la = Nt < nb ? Nt : nb;
     double *A11 = dA + (long)r0 * lda + r0;
-    // U row: A12 <- L11^{-1} A12 (computeU, lu.C:180-189)
-    pid = prof_begin(PK_TRSM, (double)Nt * nb * (nb - 1), sm);
     const double *U = A11 + nb;  // the U row block the trailing update reads
     long ldu = lda;
     if (use_inv) {
-      if ((rc = dgemm(false, false, nb, Nt, nb, 1.0, ws.winv, nb, A11 + nb, lda, 0.0, ws.ubuf, Nt,
-                      sm)))
-        return rc;
       U = ws.ubuf;
       ldu = Nt;
-    } else if ((rc = trsm_llnu_big<double>(nb, Nt, A11, lda, A11 + nb, lda, sm))) {
-      return rc;
     }
-    prof_end(pid, sm);
+    auto swap_cols = [&](int c0, int c1, int skip0, int skip1) -> int {
+      const int pid = prof_begin(PK_SWAP, (double)(c1 - c0 - (skip1 - skip0)), sm);
+      int e = apply_runs<double>(nruns, moves_of(t), counts_of(t), dA, lda, c0, c1, skip0, skip1,
+                                 false, sm);
+      prof_end(pid, sm);
+      return e;
+    };
+    auto u_row = [&](int j0, int j1) -> int {  // U row columns [j0, j1) of the trailing part
+      if (j1 <= j0) return CLU_OK;
+      const int pid = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
+      int e = use_inv ? dgemm(false, false, nb, j1 - j0, nb, 1.0, winv_of(t), nb, A11 + nb + j0,
+                              lda, 0.0, ws.ubuf + j0, Nt, sm)
+                      : trsm_llnu_big<double>(nb, j1 - j0, A11, lda, A11 + nb + j0, lda, sm);
+      prof_end(pid, sm);
+      return e;
+    };
+    if (Nt <= 0) {
+      if ((rc = swap_cols(0, n, r0, r1))) return rc;
+      break;
+    }
+    if ((rc = swap_cols(r1, r1 + la, r1 + la, r1 + la))) return rc;
+    if ((rc = u_row(0, la))) return rc;
     // look-ahead block column, then the panel of step t+1 on the panel stream
-    const int la = Nt < nb ? Nt : nb;
     const double *L21 = A11 + (long)nb * lda;
     if ((rc = update_la(Nt, la, L21, U, ldu, A11 + (long)nb * lda + nb))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_la, sm));
+    if ((rc = swap_cols(0, n, r0, r1 + la))) return rc;
+    if ((rc = u_row(la, Nt))) return rc;
+    CLU_HIP(hipEventRecord(ws.ev_urow, sm));
     double *C2 = A11 + (long)nb * lda + nb + la;
     const int Ma = Nt > la ? split_rows(t, Nt, Nt - la, L21, U + la, ldu) : 0;
     if (Nt > la && (rc = update_part1(Ma, Nt - la, L21, U + la, ldu, C2))) return rc;

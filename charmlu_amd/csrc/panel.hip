@@ -381,15 +381,15 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   if (w > PW) return -1;
   if (m < w) return -4;  // row c of every column lies in the strip
   if (!pivot && m > 1024) return -3;  // the unpivoted strip is one workgroup
-  // one 256-thread workgroup up to 1024 rows, then 512-thread workgroups of
-  // 2048 rows (4 per thread) -- fewer participants in every column's exchange
-  // One 256-thread workgroup up to 1024 rows.  Taller strips: 256-thread
-  // workgroups of 1024 rows while one per panel CU suffices, else 512-thread
-  // ones of 2048 rows (4-wave workgroups spend 23 % less per column, but an
-  // exchange among 64 of them costs more than it saves); FP32 strips too
-  // tall for either use 8 rows per thread.  All workgroups of a strip must
-  // be resident at once (one per panel CU), and the polling waves must
-  // cover G records in one round.
+  // One 256-thread workgroup up to 1024 rows, one 512-thread workgroup up
+  // to 2048 (3.2-3.9 us per column against 4.0 for two workgroups, whose
+  // exchange goes through memory).  Taller strips: 256-thread workgroups of
+  // 1024 rows while one per panel CU suffices, else 512-thread ones of 2048
+  // rows (4-wave workgroups spend 23 % less per column, but an exchange
+  // among 64 of them costs more than it saves); FP32 strips too tall for
+  // either use 8 rows per thread.  All workgroups of a strip must be
+  // resident at once (one per panel CU), and the polling waves must cover G
+  // records in one round.
   const int vw = (int)sizeof(T) / 4, l1 = PW * vw + vw + 1;
   auto poll_ok = [&](int g, int nt) {
     return g * l1 + PW * vw <= (nt / 64) * 64 * (nt >= 512 ? 4 : 5);
@@ -398,6 +398,7 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   if (m <= 256) { R = 1; G = 1; NT = 256; }
   else if (m <= 512) { R = 2; G = 1; NT = 256; }
   else if (m <= 1024) { R = 4; G = 1; NT = 256; }
+  else if (m <= 2048 && pivot) { R = 4; G = 1; NT = 512; }
   else {
     R = 4; NT = 256; G = (m + 1023) / 1024;
     if (G > pw.max_groups || !poll_ok(G, 256)) { NT = 512; G = (m + 2047) / 2048; }
@@ -446,34 +447,58 @@ template int panel_base<float>(float *, long, int, int, int, int, int, int *, Pa
 // ---------------------------------------------------------------------------
 constexpr int MAXRUN = 256;
 
-// 2*MAXRUN threads: thread i < nk takes position k1+i, thread MAXRUN+i the
-// pivot row ipiv[k1+i] if it lies below the run and appears there first.
-// Each walks the swaps backwards to find the row that ends up at its
-// position; the moved positions are compacted in thread order.
-__global__ __launch_bounds__(2 * MAXRUN) void pivots_to_moves_kernel(
-    const int *__restrict__ ipiv, int k1, int k2, int2 *__restrict__ moves,
-    int *__restrict__ count) {
-  __shared__ int piv[MAXRUN];
-  __shared__ int wave_cnt[2 * MAXRUN / 64];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+// The move of one thread of a run of swaps ipiv[k1, k2) (global rows).
+// piv[0..MAXRUN) holds the run in LDS, padded past its end with k1 + j (a
+// row swapped with itself).  Thread i < nk takes position k1 + i, thread
+// MAXRUN + i the pivot row piv[i] if it lies below the run and appears there
+// first; x is that position (-1: none) and y the old row that ends up at x,
+// found by walking the swaps backwards.  Both scans are branch-free over
+// four pivots per LDS read, so the reads pipeline instead of paying one LDS
+// round trip per swap.
+CLU_DEV void run_move(const int *piv, int k1, int k2, int t, int &x, int &y) {
   const int nk = k2 - k1;
-  if (t < nk) piv[t] = ipiv[k1 + t];
-  __syncthreads();
-  int x = -1;
+  const int4 *p4 = reinterpret_cast<const int4 *>(piv);
+  x = -1;
   if (t < nk) {
     x = k1 + t;
   } else if (t >= MAXRUN && t - MAXRUN < nk) {
     const int i = t - MAXRUN;
     x = piv[i];
     bool dup = x < k2;
-    for (int j = 0; j < i && !dup; ++j) dup = piv[j] == x;
+    for (int j4 = 0; 4 * j4 < i; ++j4) {
+      const int4 v = p4[j4];
+      const int j = 4 * j4;
+      dup = dup || (v.x == x) || (j + 1 < i && v.y == x) || (j + 2 < i && v.z == x) ||
+            (j + 3 < i && v.w == x);
+    }
     if (dup) x = -1;
   }
-  int y = x;
-  for (int j = nk - 1; j >= 0; --j) {
-    const int pj = piv[j], jj = k1 + j;
-    y = (y == jj) ? pj : (y == pj ? jj : y);
+  int z = x;
+#pragma unroll 4
+  for (int j4 = MAXRUN / 4 - 1; j4 >= 0; --j4) {
+    const int4 v = p4[j4];
+    const int j = k1 + 4 * j4;
+    z = (z == j + 3) ? v.w : (z == v.w ? j + 3 : z);
+    z = (z == j + 2) ? v.z : (z == v.z ? j + 2 : z);
+    z = (z == j + 1) ? v.y : (z == v.y ? j + 1 : z);
+    z = (z == j) ? v.x : (z == v.x ? j : z);
   }
+  y = z;
+}
+
+// 2*MAXRUN threads, one run: every moved position becomes a move {x, y}
+// ("row x receives old row y"), compacted in thread order.
+__global__ __launch_bounds__(2 * MAXRUN) void pivots_to_moves_kernel(
+    const int *__restrict__ ipiv, int k1, int k2, int2 *__restrict__ moves,
+    int *__restrict__ count) {
+  __shared__ __attribute__((aligned(16))) int piv[MAXRUN];
+  __shared__ int wave_cnt[2 * MAXRUN / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int nk = k2 - k1;
+  if (t < MAXRUN) piv[t] = t < nk ? ipiv[k1 + t] : k1 + t;
+  __syncthreads();
+  int x, y;
+  run_move(piv, k1, k2, t, x, y);
   const bool mv = x >= 0 && y != x;
   const unsigned long long mask = __ballot(mv);
   if (lane == 0) wave_cnt[wave] = __popcll(mask);
@@ -497,46 +522,45 @@ int pivots_to_moves(const int *ipiv, int k1, int k2, int2 *moves, int *count,
   return CLU_OK;
 }
 
+// One workgroup per chunk of CW columns (128 bytes of every row): all moves'
+// source values of the chunk are gathered into registers (up to 2*MAXRUN
+// rows, NP per thread, every load issued before the first is used), then,
+// once the whole workgroup holds them, written to the destination rows.
+// Moves form cycles, so a row read by one thread is written by another:
+// the barrier between the phases orders them; chunks are independent.
 template <typename T, int CW>
 __global__ __launch_bounds__(256) void apply_moves_kernel(
     const int2 *__restrict__ moves, const int *__restrict__ count, T *__restrict__ A,
     long lda, int c0, int c1, int skip0, int skip1, int inverse) {
-  extern __shared__ unsigned char smem_raw[];
-  int2 *s_mv = reinterpret_cast<int2 *>(smem_raw);  // the move list, then the rows
+  constexpr int RPP = 256 / CW;          // rows per pass
+  constexpr int NP = 2 * MAXRUN / RPP;   // passes for the longest list
+  __shared__ int2 s_mv[2 * MAXRUN];
   const int cnt = *count;
-  T *buf = reinterpret_cast<T *>(smem_raw + (size_t)2 * MAXRUN * sizeof(int2));
   const int t = threadIdx.x;
   for (int i = t; i < cnt; i += 256) s_mv[i] = moves[i];
   __syncthreads();
   const int left = skip0 - c0;  // columns before the skipped range
-  auto phys = [&](int logical) { return logical < left ? c0 + logical : logical - left + skip1; };
-  const int chunk = blockIdx.x * CW;
-  const int q = t % CW;  // this thread's column within the chunk (fixed: 256 % CW == 0)
-  const int col = phys(chunk + q);
+  const int logical = blockIdx.x * CW + t % CW;
+  const int col = logical < left ? c0 + logical : logical - left + skip1;
   const bool colok = col < c1;
-  // gather: U independent loads in flight per thread before any LDS store
-  constexpr int U = 8;
-  for (int e0 = t; e0 < cnt * CW; e0 += 256 * U) {
-    T v[U];
+  const int r0 = t / CW;
+  T v[NP];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = e0 + u * 256;
-      if (e < cnt * CW && colok) {
-        const int2 m2 = s_mv[e / CW];
-        v[u] = A[(long)(inverse ? m2.x : m2.y) * lda + col];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = e0 + u * 256;
-      if (e < cnt * CW && colok) buf[e] = v[u];
+  for (int p = 0; p < NP; ++p) {
+    const int e = p * RPP + r0;
+    if (colok && e < cnt) {
+      const int2 m2 = s_mv[e];
+      v[p] = A[(long)(inverse ? m2.x : m2.y) * lda + col];
     }
   }
+  wait_all_mem();
   __syncthreads();
-  for (int e = t; e < cnt * CW; e += 256) {
-    if (colok) {
-      const int2 m2 = s_mv[e / CW];
-      A[(long)(inverse ? m2.y : m2.x) * lda + col] = buf[e];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int e = p * RPP + r0;
+    if (colok && e < cnt) {
+      const int2 m2 = s_mv[e];
+      A[(long)(inverse ? m2.y : m2.x) * lda + col] = v[p];
     }
   }
 }
@@ -549,68 +573,100 @@ int apply_moves(const int2 *moves, const int *count, int max_moves, T *A, long l
   skip1 = max(skip0, min(skip1, c1));
   const int ncols = (c1 - c0) - (skip1 - skip0);
   if (ncols <= 0 || max_moves <= 0) return CLU_OK;
-  constexpr int CW = 16;
+  constexpr int CW = 128 / (int)sizeof(T);
   if (max_moves > 2 * MAXRUN) return -1;
   const int nchunks = (ncols + CW - 1) / CW;
-  const size_t shm = (size_t)2 * MAXRUN * sizeof(int2) + (size_t)max_moves * CW * sizeof(T);
-  hipLaunchKernelGGL((apply_moves_kernel<T, CW>), dim3(nchunks), dim3(256), shm, s,
-                     moves, count, A, lda, c0, c1, skip0, skip1, inverse ? 1 : 0);
+  hipLaunchKernelGGL((apply_moves_kernel<T, CW>), dim3(nchunks), dim3(256), 0, s, moves, count,
+                     A, lda, c0, c1, skip0, skip1, inverse ? 1 : 0);
   CLU_LAUNCHED("apply_moves_kernel");
   return CLU_OK;
 }
 
-// The interchanges ipiv[0..n) applied to one vector in a single launch: the
-// runs of MAXRUN pivots are taken in order (reverse order and inverted moves
-// for the inverse permutation); within a run every position finds its source
-// by the same backward walk as pivots_to_moves, then all moves of the run are
-// read before any is written.
-template <typename T>
-__global__ __launch_bounds__(2 * MAXRUN) void permute_vec_kernel(const int *__restrict__ ipiv,
-                                                                 int n, T *__restrict__ v,
-                                                                 int inverse) {
-  __shared__ int piv[MAXRUN];
-  const int t = threadIdx.x;
+// One workgroup per run of MAXRUN pivots: the run's move list, as
+// pivots_to_moves_kernel computes it, at moves + r * 2 * MAXRUN.
+__global__ __launch_bounds__(2 * MAXRUN) void pivots_to_moves_all_kernel(
+    const int *__restrict__ ipiv, int n, int2 *__restrict__ moves, int *__restrict__ counts) {
+  __shared__ __attribute__((aligned(16))) int piv[MAXRUN];
+  __shared__ int wave_cnt[2 * MAXRUN / 64];
+  const int r = blockIdx.x, k1 = r * MAXRUN, k2 = min(n, k1 + MAXRUN), nk = k2 - k1;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (t < MAXRUN) piv[t] = t < nk ? ipiv[k1 + t] : k1 + t;
+  __syncthreads();
+  int x, y;
+  run_move(piv, k1, k2, t, x, y);
+  const bool mv = x >= 0 && y != x;
+  const unsigned long long mask = __ballot(mv);
+  if (lane == 0) wave_cnt[wave] = __popcll(mask);
+  __syncthreads();
+  int base = 0;
+  for (int k = 0; k < wave; ++k) base += wave_cnt[k];
+  int2 *out = moves + (long)r * 2 * MAXRUN;
+  if (mv) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = make_int2(x, y);
+  if (t == 0) {
+    int tot = 0;
+    for (int k = 0; k < 2 * MAXRUN / 64; ++k) tot += wave_cnt[k];
+    counts[r] = tot;
+  }
+}
+
+int pivots_to_moves_all(const int *ipiv, int n, int2 *moves, int *counts, hipStream_t s) {
+  if (n <= 0) return CLU_OK;
   const int nruns = (n + MAXRUN - 1) / MAXRUN;
-  for (int i = 0; i < nruns; ++i) {
+  hipLaunchKernelGGL(pivots_to_moves_all_kernel, dim3(nruns), dim3(2 * MAXRUN), 0, s, ipiv, n,
+                     moves, counts);
+  CLU_LAUNCHED("pivots_to_moves_all_kernel");
+  return CLU_OK;
+}
+
+// The runs' move lists applied to one vector, in order (reverse order and
+// inverted moves for the inverse permutation), by one workgroup: all of a
+// run's sources are read before any destination is written; the next run's
+// moves are fetched while the current one is applied.
+template <typename T>
+__global__ __launch_bounds__(2 * MAXRUN) void permute_vec_kernel(const int2 *__restrict__ moves,
+                                                                 const int *__restrict__ counts,
+                                                                 int nruns, T *__restrict__ v,
+                                                                 int inverse) {
+  const int t = threadIdx.x;
+  auto fetch = [&](int i, int2 &m, int &c) {
     const int r = inverse ? nruns - 1 - i : i;
-    const int k1 = r * MAXRUN, k2 = min(n, k1 + MAXRUN), nk = k2 - k1;
-    __syncthreads();  // the previous run's moves are written
-    if (t < nk) piv[t] = ipiv[k1 + t];
-    __syncthreads();
-    int x = -1;
-    if (t < nk) {
-      x = k1 + t;
-    } else if (t >= MAXRUN && t - MAXRUN < nk) {
-      const int q = t - MAXRUN;
-      x = piv[q];
-      bool dup = x < k2;
-      for (int j = 0; j < q && !dup; ++j) dup = piv[j] == x;
-      if (dup) x = -1;
-    }
-    int y = x;
-    for (int j = nk - 1; j >= 0 && x >= 0; --j) {
-      const int pj = piv[j], jj = k1 + j;
-      y = (y == jj) ? pj : (y == pj ? jj : y);
-    }
-    const bool mv = x >= 0 && y != x;
+    c = counts[r];
+    m = t < c ? moves[(long)r * 2 * MAXRUN + t] : make_int2(0, 0);
+  };
+  int2 m;
+  int c;
+  fetch(0, m, c);
+  for (int i = 0; i < nruns; ++i) {
+    int2 mn = make_int2(0, 0);
+    int cn = 0;
+    if (i + 1 < nruns) fetch(i + 1, mn, cn);
     // forward: new[x] = old[y]; inverse: new[y] = old[x]
     T val = (T)0;
-    if (mv) val = v[inverse ? x : y];
-    __syncthreads();
-    if (mv) v[inverse ? y : x] = val;
+    if (t < c) val = v[inverse ? m.x : m.y];
+    wait_all_mem();
+    __syncthreads();  // every source of this run read
+    if (t < c) v[inverse ? m.y : m.x] = val;
+    wait_all_mem();
+    __syncthreads();  // written before the next run reads
+    m = mn;
+    c = cn;
   }
 }
 
 template <typename T>
-int permute_vec(const int *ipiv, int n, T *v, bool inverse, hipStream_t s) {
+int permute_vec(const int *ipiv, int n, T *v, bool inverse, int2 *moves, int *counts,
+                hipStream_t s) {
   if (n <= 0) return CLU_OK;
-  hipLaunchKernelGGL((permute_vec_kernel<T>), dim3(1), dim3(2 * MAXRUN), 0, s, ipiv, n, v,
-                     inverse ? 1 : 0);
+  int rc = pivots_to_moves_all(ipiv, n, moves, counts, s);
+  if (rc) return rc;
+  const int nruns = (n + MAXRUN - 1) / MAXRUN;
+  hipLaunchKernelGGL((permute_vec_kernel<T>), dim3(1), dim3(2 * MAXRUN), 0, s, moves, counts,
+                     nruns, v, inverse ? 1 : 0);
   CLU_LAUNCHED("permute_vec_kernel");
   return CLU_OK;
 }
-template int permute_vec<double>(const int *, int, double *, bool, hipStream_t);
-template int permute_vec<float>(const int *, int, float *, bool, hipStream_t);
+template int permute_vec<double>(const int *, int, double *, bool, int2 *, int *, hipStream_t);
+template int permute_vec<float>(const int *, int, float *, bool, int2 *, int *, hipStream_t);
 
 template int apply_moves<double>(const int2 *, const int *, int, double *, long, int, int,
                                  int, int, bool, hipStream_t);

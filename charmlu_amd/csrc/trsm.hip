@@ -187,48 +187,6 @@ static int trsm_launch(int M, int N, T alpha, const T *F, long ldf, T *B, long l
   return CLU_OK;
 }
 
-__global__ void identity_batch_kernel(void *P, int esize, int m, long count) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= count * m * m) return;
-  const long r = (e / m) % m, c = e % m;
-  if (esize == 8) reinterpret_cast<double *>(P)[e] = r == c ? 1.0 : 0.0;
-  else reinterpret_cast<float *>(P)[e] = r == c ? 1.0f : 0.0f;
-}
-
-// Inverses of the diagonal blocks of a unit lower (upper = false) or a
-// non-unit upper (upper = true) triangle: block k (rows and columns
-// [k*S, min(n, (k+1)*S))) goes to Inv + k*S*S, row-major with ld S.  All
-// blocks are solved at once (one grid row per block: X_k = T_kk^{-1} I).
-template <typename T>
-int tri_inv_blocks(bool upper, int n, int S, const T *A, long lda, T *Inv, hipStream_t s) {
-  if (n <= 0) return CLU_OK;
-  if (S > 256) return -1;
-  const int nfull = n / S, last = n - nfull * S;
-  const long cnt = nfull + (last ? 1 : 0);
-  hipLaunchKernelGGL(identity_batch_kernel, dim3((unsigned)((cnt * S * S + 255) / 256)),
-                     dim3(256), 0, s, (void *)Inv, (int)sizeof(T), S, cnt);
-  CLU_LAUNCHED("identity_batch_kernel");
-  auto launch = [&](int M, int nbatch, const T *F, T *B) -> int {
-    const size_t shm = (size_t)M * 32 * sizeof(T) + (size_t)16 * M * sizeof(T);
-    const dim3 grid((M + 31) / 32, nbatch);
-    if (upper)
-      hipLaunchKernelGGL((trsm_kernel<T, false, false, false>), grid, dim3(256), shm, s, M, M,
-                         (T)1, F, lda, B, (long)S, (long)S * (lda + 1), (long)S * S);
-    else
-      hipLaunchKernelGGL((trsm_kernel<T, true, false, true>), grid, dim3(256), shm, s, M, M,
-                         (T)1, F, lda, B, (long)S, (long)S * (lda + 1), (long)S * S);
-    CLU_LAUNCHED("trsm_kernel (batched)");
-    return CLU_OK;
-  };
-  int rc;
-  if (nfull > 0 && (rc = launch(S, nfull, A, Inv))) return rc;
-  if (last > 0 && (rc = launch(last, 1, A + (long)nfull * S * (lda + 1), Inv + (long)nfull * S * S)))
-    return rc;
-  return CLU_OK;
-}
-template int tri_inv_blocks<double>(bool, int, int, const double *, long, double *, hipStream_t);
-template int tri_inv_blocks<float>(bool, int, int, const float *, long, float *, hipStream_t);
-
 // ---------------------------------------------------------------------------
 // Full inverse of a triangle by recursive doubling on the matrix cores.
 // The 16 x 16 diagonal blocks are inverted first, one wave each (lane j
