@@ -92,8 +92,13 @@ struct PanelWork {
   int max_groups;
   int *info;          // device int, first near-singular column + 1
 };
-// Factor rows [j0, m_end) x columns [j0, j0+w) (w <= 16) of A in place with
-// partial pivoting, one column at a time; ipiv[j0..j0+w) get global rows.
+// Widest strip panel_base takes for a strip of m rows: 16 columns, or 32
+// for FP32 strips of at most 65536 rows.
+template <typename T>
+int panel_strip_width(int m);
+// Factor rows [j0, m_end) x columns [j0, j0+w) (w <= panel_strip_width) of A
+// in place with partial pivoting, one column at a time; ipiv[j0..j0+w) get
+// global rows.
 // The strip's row interchanges are also applied to the other columns
 // [pc0, j0) and [j0+w, pc1) of the panel.  pivot = false: no interchanges
 // (the diagonal is the pivot; one workgroup, m_end - j0 <= 1024).

@@ -309,13 +309,15 @@ static int trsm_llnu_big(int M, int N, const T *L, long ldl, T *B, long ldb, hip
 }
 
 // Factor rows [c0, n) x columns [c0, c0+w) of the panel [pc0, pc0+pw)
-// (recursive halving down to 16-column strips factored by panel_base, which
+// (recursive halving down to strips of panel_strip_width columns -- 16, or
+// 32 for FP32 -- factored by panel_base, which
 // also applies each strip's interchanges to the rest of the panel, so both
 // halves of every split are already permuted when they are used).
 template <typename T>
 static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, int *ipiv,
                         Workspace &ws, hipStream_t s, bool pivot = true) {
-  if (w <= 16) return panel_base<T>(A, lda, c0, n, w, pc0, pc0 + pcw, ipiv, ws.pw, s, pivot);
+  if (w <= panel_strip_width<T>(n - c0))
+    return panel_base<T>(A, lda, c0, n, w, pc0, pc0 + pcw, ipiv, ws.pw, s, pivot);
   int w1 = (w / 2 + 15) / 16 * 16;
   if (w1 >= w) w1 = w / 2;
   const int w2 = w - w1;

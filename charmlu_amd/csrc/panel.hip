@@ -10,12 +10,14 @@
 // the rest of the strip (updateLsubBlock, lu.C:230-254).
 //
 // The strip lives in registers for the whole kernel: workgroup g, thread t
-// holds rows g*256*R + r*256 + t (r < R), 16 values each.  With one
-// workgroup a column costs two LDS barriers.  With G > 1 workgroups each
+// of NT holds rows g*NT*R + r*NT + t (r < R), 16 values each.  With one
+// workgroup a column costs one LDS barrier.  With G > 1 workgroups each
 // column needs one grid-wide exchange: every workgroup publishes its best
-// candidate (value, row, and the row's 16 values) and the owner of the
-// current top row publishes that row, into a double-buffered slot array
-// written with relaxed agent-scope atomics; a counter barrier follows; every
+// candidate (value, row, and the row's 16 values), and workgroup 0 the
+// current top row, into a double-buffered slot array with relaxed
+// agent-scope stores; each 8-byte word carries the column's sequence number
+// beside its payload, so readers poll until every word they need carries
+// the current one (one memory round trip, no counter barrier).  Every
 // workgroup then reduces the G candidates itself (same data, same order, so
 // all agree on the pivot) and swaps/eliminates its own rows.
 //
@@ -23,8 +25,9 @@
 // and cycle application (recordPivot / announceAgglomeratedPivots /
 // sendPendingPivots, lu.C:46-163, lu.C:300-352): a run of sequential swaps
 // becomes a list of (destination row, source row) moves, applied to whole
-// column ranges as a gather through LDS (one read and one write per moved
-// row segment, coalesced along the row).
+// column ranges (each workgroup gathers the moved row segments of its
+// columns into registers, then writes them: one read and one write per
+// moved row segment, coalesced along the row).
 #include <type_traits>
 
 #include "common.h"
@@ -32,7 +35,13 @@
 
 namespace clu {
 
-constexpr int PW = 16;  // max strip width
+// Strip width SW: 16 columns, or 32 for FP32 strips of at most 32 * 2048
+// rows (4 rows of 32 floats per thread; taller FP32 strips need 8 rows per
+// thread and stay 16 wide).  A 32-column strip halves the strip launches of
+// a panel and removes the 32-column nodes of its recursion (a triangular
+// solve and a GEMM each).
+constexpr int PW = 16;      // the strip width of FP64
+constexpr int PW32 = 32;    // the widest FP32 strip
 
 // Cross-workgroup records.  Every 8-byte word carries a 32-bit payload in
 // its low half and the column's sequence number in its high half, so a
@@ -40,10 +49,10 @@ constexpr int PW = 16;  // max strip width
 // trip (stores are not waited for, nothing is ordered by fences).  Record
 // of workgroup g for column parity par at recs + (par * GMAX + g) * RECW,
 // in 32-bit payload words (VW per value, FP64 low half first):
-//   [0, PW*VW)          the workgroup's best row (its PW values; zeros if none)
-//   [PW*VW, PW*VW+VW)   |value| of that row in the current column
-//   PW*VW+VW            its row (local index, -1: none)
-//   [TOPW, TOPW+PW*VW)  workgroup 0 only: row c (the top row) before the swap
+//   [0, SW*VW)          the workgroup's best row (its SW values; zeros if none)
+//   [SW*VW, SW*VW+VW)   |value| of that row in the current column
+//   SW*VW+VW            its row (local index, -1: none)
+//   [TOPW, TOPW+SW*VW)  workgroup 0 only: row c (the top row) before the swap
 constexpr int GMAX = 64;  // workgroups in one exchange
 constexpr int RECW = 72;  // words per record
 constexpr int SPIN_MAX = 1 << 22;  // polling rounds before giving up (seconds)
@@ -59,13 +68,13 @@ CLU_DEV unsigned row_word(const T *row, int k) {  // word k of an LDS row
   return reinterpret_cast<const unsigned *>(row)[k];
 }
 
-template <typename T, int R, int NT, bool MULTI, bool PIV>
+template <typename T, int R, int NT, bool MULTI, bool PIV, int SW>
 __global__ __launch_bounds__(NT) void panel_base_kernel(
     T *__restrict__ A, long lda, int j0, int m, int w, int pc0, int pc1,
     int *__restrict__ ipiv, unsigned long long *__restrict__ recs, unsigned seq0, int *info) {
   static_assert(PIV || !MULTI, "the unpivoted strip is one workgroup");
   constexpr int VW = sizeof(T) / 4;      // payload words per value
-  constexpr int CW = PW * VW;            // words of a row
+  constexpr int CW = SW * VW;            // words of a row
   constexpr int L1 = CW + VW + 1;        // words of a record without the top row
   constexpr int TOPW = (L1 + 1) / 2 * 2; // the top row, 8-byte aligned
   constexpr int RPW = NT * R;            // rows per workgroup
@@ -79,18 +88,18 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   // per-column LDS slots, double-buffered by column parity
   __shared__ T s_val[2][NW];
   __shared__ int s_idx[2][NW];
-  __shared__ T s_wcand[2][NW][PW];  // each wave's candidate row
-  __shared__ T s_top[2][PW];        // row c before the swap (in the workgroup holding it)
+  __shared__ T s_wcand[2][NW][SW];  // each wave's candidate row
+  __shared__ T s_top[2][SW];        // row c before the swap (in the workgroup holding it)
   __shared__ __attribute__((aligned(16))) unsigned s_rec[2][MULTI ? GMAX : 1][RECW];
-  __shared__ int s_piv[PW];
-  __shared__ T s_pmag[PW];
+  __shared__ int s_piv[SW];
+  __shared__ T s_pmag[SW];
 
-  T a[R][PW];
+  T a[R][SW];
 #pragma clang loop unroll(full)
   for (int r = 0; r < R; ++r) {
     const int li = g * RPW + r * NT + t;
 #pragma clang loop unroll(full)
-    for (int q = 0; q < PW; ++q)
+    for (int q = 0; q < SW; ++q)
       a[r][q] = (li < m && q < w) ? A[(long)(j0 + li) * lda + j0 + q] : (T)0;
   }
   // Drain the strip loads here: otherwise the compiler waits for them at the
@@ -126,9 +135,9 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
 #pragma clang loop unroll(full)
       for (int q = 0; q < 4; ++q) keep[q] = a[r][q];
 #pragma clang loop unroll(full)
-      for (int q = 0; q < PW - 4; ++q) a[r][q] = a[r][q + 4];
+      for (int q = 0; q < SW - 4; ++q) a[r][q] = a[r][q + 4];
 #pragma clang loop unroll(full)
-      for (int q = 0; q < 4; ++q) a[r][PW - 4 + q] = keep[q];
+      for (int q = 0; q < 4; ++q) a[r][SW - 4 + q] = keep[q];
     }
   };
   // this thread's candidate for column c over its rows >= c: strictly
@@ -150,11 +159,11 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   // values by selection and update with a zero multiplier.
   auto eliminate = [&](int c, int lim, const T *u, auto kc) {
     constexpr int K = decltype(kc)::value;
-    T uq[PW];
+    T uq[SW];
 #pragma clang loop unroll(full)
-    for (int q = 0; q < PW; ++q) uq[q] = u[q];  // all of the row: no per-word branches
+    for (int q = 0; q < SW; ++q) uq[q] = u[q];  // all of the row: no per-word branches
 #pragma clang loop unroll(full)
-    for (int q = K + 1; q < PW; ++q) uq[q] = q < lim ? uq[q] : (T)0;
+    for (int q = K + 1; q < SW; ++q) uq[q] = q < lim ? uq[q] : (T)0;
     const T piv = uq[K];
     if (piv == (T)0) return;
 #pragma clang loop unroll(full)
@@ -164,7 +173,7 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
       const T le = below ? l : (T)0;
       a[r][K] = below ? l : a[r][K];
 #pragma clang loop unroll(full)
-      for (int q = K + 1; q < PW; ++q) a[r][q] -= le * uq[q];
+      for (int q = K + 1; q < SW; ++q) a[r][q] -= le * uq[q];
     }
   };
   using K0 = std::integral_constant<int, 0>;
@@ -188,11 +197,11 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
         const int li = g * RPW + r * NT + t;
         if (PIV && bi >= 0 && li == bi) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) s_wcand[par][wave][q] = a[r][q];
+          for (int q = 0; q < SW; ++q) s_wcand[par][wave][q] = a[r][q];
         }
         if (li == c) {
 #pragma clang loop unroll(full)
-          for (int q = 0; q < PW; ++q) s_top[par][q] = a[r][q];
+          for (int q = 0; q < SW; ++q) s_top[par][q] = a[r][q];
         }
       }
       __syncthreads();
@@ -286,10 +295,10 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
           const int li = g * RPW + r * NT + t;
           if (li == c) {
 #pragma clang loop unroll(full)
-            for (int q = 0; q < PW; ++q) a[r][q] = u[q];
+            for (int q = 0; q < SW; ++q) a[r][q] = u[q];
           } else if (li == p) {
 #pragma clang loop unroll(full)
-            for (int q = 0; q < PW; ++q) a[r][q] = top[q];
+            for (int q = 0; q < SW; ++q) a[r][q] = top[q];
           }
         }
       }
@@ -306,14 +315,14 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
     rotate4();
   }
 #pragma clang loop unroll(disable)
-  for (int base = (w + 3) / 4 * 4; base < PW; base += 4) rotate4();
+  for (int base = (w + 3) / 4 * 4; base < SW; base += 4) rotate4();
 
 #pragma clang loop unroll(full)
   for (int r = 0; r < R; ++r) {
     const int li = g * RPW + r * NT + t;
     if (li < m) {
 #pragma clang loop unroll(full)
-      for (int q = 0; q < PW; ++q)
+      for (int q = 0; q < SW; ++q)
         if (q < w) A[(long)(j0 + li) * lda + j0 + q] = a[r][q];
     }
   }
@@ -331,16 +340,16 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   //      (the reference exchanges whole b-wide block rows, lu.C:166-177) ----
   // Each candidate position (the strip's rows and their pivot rows) finds
   // the row that ends up there by walking the swaps backwards.
-  __shared__ int s_pos[2 * PW], s_src[2 * PW];
+  __shared__ int s_pos[2 * SW], s_src[2 * SW];
   __shared__ int s_nmv;
-  if (t < 2 * PW) {
+  if (t < 2 * SW) {
     int x = -1;
     if (t < w) {
       x = t;
-    } else if (t >= PW && t - PW < w) {
-      x = s_piv[t - PW];
+    } else if (t >= SW && t - SW < w) {
+      x = s_piv[t - SW];
       bool dup = x < w;
-      for (int j = 0; j < t - PW; ++j) dup = dup || s_piv[j] == x;
+      for (int j = 0; j < t - SW; ++j) dup = dup || s_piv[j] == x;
       if (dup) x = -1;
     }
     int y = x;
@@ -363,22 +372,30 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   // columns of the rest of the panel, spread over all threads of the grid
   for (int k = g * NT + t; nmv > 0 && k < left + right; k += G * NT) {
     const int col = k < left ? pc0 + k : j0 + w + (k - left);
-    T v[2 * PW];
+    T v[2 * SW];
 #pragma clang loop unroll(full)
-    for (int i = 0; i < 2 * PW; ++i)
+    for (int i = 0; i < 2 * SW; ++i)
       if (i < nmv) v[i] = A[(long)(j0 + s_src[i]) * lda + col];
 #pragma clang loop unroll(full)
-    for (int i = 0; i < 2 * PW; ++i)
+    for (int i = 0; i < 2 * SW; ++i)
       if (i < nmv) A[(long)(j0 + s_pos[i]) * lda + col] = v[i];
   }
 }
+
+template <typename T>
+int panel_strip_width(int m) {
+  return sizeof(T) == 4 && m <= 32 * 2048 ? PW32 : PW;
+}
+template int panel_strip_width<double>(int);
+template int panel_strip_width<float>(int);
 
 template <typename T>
 int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *ipiv,
                PanelWork &pw, hipStream_t s, bool pivot) {
   const int m = m_end - j0;
   if (m <= 0 || w <= 0) return CLU_OK;
-  if (w > PW) return -1;
+  const int sw = w <= PW ? PW : PW32;
+  if (w > panel_strip_width<T>(m) && w > PW) return -1;
   if (m < w) return -4;  // row c of every column lies in the strip
   if (!pivot && m > 1024) return -3;  // the unpivoted strip is one workgroup
   // One 256-thread workgroup up to 1024 rows, one 512-thread workgroup up
@@ -387,12 +404,12 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   // 1024 rows while one per panel CU suffices, else 512-thread ones of 2048
   // rows (4-wave workgroups spend 23 % less per column, but an exchange
   // among 64 of them costs more than it saves); FP32 strips too tall for
-  // either use 8 rows per thread.  All workgroups of a strip must be
-  // resident at once (one per panel CU), and the polling waves must cover G
-  // records in one round.
-  const int vw = (int)sizeof(T) / 4, l1 = PW * vw + vw + 1;
+  // either use 8 rows per thread (16 columns only).  All workgroups of a
+  // strip must be resident at once (one per panel CU), and the polling
+  // waves must cover G records in one round.
+  const int vw = (int)sizeof(T) / 4, l1 = sw * vw + vw + 1;
   auto poll_ok = [&](int g, int nt) {
-    return g * l1 + PW * vw <= (nt / 64) * 64 * (nt >= 512 ? 4 : 5);
+    return g * l1 + sw * vw <= (nt / 64) * 64 * (nt >= 512 ? 4 : 5);
   };
   int R, G, NT;
   if (m <= 256) { R = 1; G = 1; NT = 256; }
@@ -406,33 +423,42 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   }
   if (G > pw.max_groups || G > GMAX) return -2;
   if (G > 1 && !poll_ok(G, NT)) return -2;
+  if (R == 8 && sw != PW) return -2;
   unsigned long long *recs = reinterpret_cast<unsigned long long *>(pw.slots);
   const unsigned seq0 = pw.epoch;
-#define CLU_PANEL(TT, RR, NN, MU, PV)                                                         \
-  hipLaunchKernelGGL((panel_base_kernel<TT, RR, NN, MU, PV>), dim3(G), dim3(NN), 0, s,        \
+#define CLU_PANEL(TT, RR, NN, MU, PV, SWW)                                                      \
+  hipLaunchKernelGGL((panel_base_kernel<TT, RR, NN, MU, PV, SWW>), dim3(G), dim3(NN), 0, s,     \
                      (TT *)A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info)
-  if (!pivot) {
-    switch (R) {
-      case 1: CLU_PANEL(T, 1, 256, false, false); break;
-      case 2: CLU_PANEL(T, 2, 256, false, false); break;
-      default: CLU_PANEL(T, 4, 256, false, false); break;
-    }
-  } else if (G > 1 && R == 8) {
-    if (sizeof(T) != 4) return -2;
-    CLU_PANEL(float, 8, 512, true, true);
-  } else if (G > 1 && NT == 256) {
-    CLU_PANEL(T, 4, 256, true, true);
-  } else if (G > 1) {
-    CLU_PANEL(T, 4, 512, true, true);
-  } else if (NT == 512) {
-    CLU_PANEL(T, 4, 512, false, true);
-  } else {
-    switch (R) {
-      case 1: CLU_PANEL(T, 1, 256, false, true); break;
-      case 2: CLU_PANEL(T, 2, 256, false, true); break;
-      default: CLU_PANEL(T, 4, 256, false, true); break;
-    }
+#define CLU_PANEL_ALL(TT, SWW)                                                                 \
+  if (!pivot) {                                                                                \
+    switch (R) {                                                                               \
+      case 1: CLU_PANEL(TT, 1, 256, false, false, SWW); break;                                 \
+      case 2: CLU_PANEL(TT, 2, 256, false, false, SWW); break;                                 \
+      default: CLU_PANEL(TT, 4, 256, false, false, SWW); break;                                \
+    }                                                                                          \
+  } else if (G > 1 && NT == 256) {                                                             \
+    CLU_PANEL(TT, 4, 256, true, true, SWW);                                                    \
+  } else if (G > 1) {                                                                          \
+    CLU_PANEL(TT, 4, 512, true, true, SWW);                                                    \
+  } else if (NT == 512) {                                                                      \
+    CLU_PANEL(TT, 4, 512, false, true, SWW);                                                   \
+  } else {                                                                                     \
+    switch (R) {                                                                               \
+      case 1: CLU_PANEL(TT, 1, 256, false, true, SWW); break;                                  \
+      case 2: CLU_PANEL(TT, 2, 256, false, true, SWW); break;                                  \
+      default: CLU_PANEL(TT, 4, 256, false, true, SWW); break;                                 \
+    }                                                                                          \
   }
+  if (R == 8) {
+    if (sizeof(T) != 4 || G <= 1) return -2;
+    CLU_PANEL(float, 8, 512, true, true, PW);
+  } else if (sw == PW) {
+    CLU_PANEL_ALL(T, PW)
+  } else {
+    if (sizeof(T) != 4) return -1;
+    CLU_PANEL_ALL(float, PW32)
+  }
+#undef CLU_PANEL_ALL
 #undef CLU_PANEL
   CLU_LAUNCHED("panel_base_kernel");
   pw.epoch += (unsigned)w;
