@@ -351,3 +351,56 @@ def test_tall_panel_mixed_refines_to_fp64(dev):
     print(f"n={n}: {st}, residual {res['residual']:.3f}, error vs FP64 LU {err:.2e}")
     assert st["residual"] <= 1.0 and res["residual"] < 16
     assert err < 1e-10
+
+
+def _strip_lu_fp32(S):
+    """Partial-pivoting LU of an FP32 m x w strip, column by column, in the
+    strip kernel's per-element order (the row restated from the reference's
+    panel loop, lu.C:259-288 and lu.C:230-254): the first largest |a| from
+    row c down (lowest row on ties), the row exchange, the multipliers by
+    FP32 division, then a[i][q] - l[i]*u[q] rounded once as the kernel's FMA
+    does (the product of two FP32 values is exact in FP64)."""
+    S = S.astype(np.float32).copy()
+    m, w = S.shape
+    piv = np.zeros(w, dtype=np.int64)
+    for c in range(w):
+        p = c + int(np.argmax(np.abs(S[c:, c])))
+        piv[c] = p
+        if p != c:
+            S[[c, p]] = S[[p, c]]
+        u = S[c].copy()
+        if u[c] == 0:
+            continue
+        l = (S[c + 1:, c] / u[c]).astype(np.float32)
+        S[c + 1:, c] = l
+        if c + 1 < w:
+            prod = l.astype(np.float64)[:, None] * u[c + 1:].astype(np.float64)[None, :]
+            S[c + 1:, c + 1:] = (S[c + 1:, c + 1:].astype(np.float64) - prod).astype(np.float32)
+    return S, piv
+
+
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
+def test_fp32_first_strip_matches_restatement(dev, orc, n):
+    """With block 32 the first panel of the FP32 LU is one 32-column strip
+    (one workgroup at n <= 2048, four exchanging through memory at 4096)
+    applied to the input itself: its pivots equal the restatement's, and its
+    columns -- after the later steps' interchanges, replayed from the GPU's
+    ipiv -- are bit-identical."""
+    import torch
+    A = orc.gen_uniform(orc.SEED_A + n, n).astype(np.float32)
+    A32 = torch.from_numpy(A.copy()).cuda()
+    ipiv, info = dev.getrf_mixed(A32, 32, "fp32")
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    ip = ipiv.cpu().numpy()
+    S, piv = _strip_lu_fp32(A[:, :32])
+    assert np.array_equal(ip[:32], piv), np.nonzero(ip[:32] != piv)[0]
+    for k in range(32, n):
+        if ip[k] != k:
+            S[[k, ip[k]]] = S[[ip[k], k]]
+    G = A32.cpu().numpy()[:, :32]
+    err = np.abs(G.astype(np.float64) - S).max() / np.abs(S).max()
+    print(f"n={n}: first strip max relative difference {err:.2e}")
+    # same operations in the same order: bit-identical (measured at all
+    # three sizes)
+    assert np.array_equal(G, S), err

@@ -230,3 +230,28 @@ def test_golden_fixtures_reproduce(orc):
         assert np.array_equal(r["x"], g[key + "_x"])
         assert np.array_equal(r["LU"], g[key + "_lu"])
     assert np.array_equal(orc.gen_rhs(4096), g["rhs_4096"])
+
+
+# ---- the FP32 strip restatement of the mixed-path strip test ---------------------
+def test_fp32_strip_restatement_pinned():
+    """The FP32 strip restatement (tests/test_gpu_mixed.py) is itself pinned:
+    its pivots are LAPACK's for the same strip (getrf on the FP64 copy of the
+    FP32 values; no near-ties at this seed), |l_ij| <= 1, and P S = L U to
+    FP32 rounding.  A wrong tie rule, a skipped swap or a dropped term in the
+    update fails one of these."""
+    from tests.test_gpu_mixed import _strip_lu_fp32
+    rng = np.random.default_rng(11)
+    S0 = (rng.random((300, 32)) - 0.5).astype(np.float32)
+    S, piv = _strip_lu_fp32(S0)
+    _, lp = sl.lu_factor(S0.astype(np.float64))
+    assert np.array_equal(piv, lp[:32])
+    L = np.tril(S.astype(np.float64), -1)[:, :32]
+    L[np.arange(32), np.arange(32)] = 1.0
+    U = np.triu(S.astype(np.float64)[:32])
+    assert np.abs(np.tril(S, -1)).max() <= 1.0
+    P = S0.astype(np.float64).copy()
+    for k, p in enumerate(piv):
+        if p != k:
+            P[[k, p]] = P[[p, k]]
+    err = np.abs(P - L @ U).max() / np.abs(S0).max()
+    assert err < 32 * 2.0 ** -24, err
