@@ -157,11 +157,11 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   // padding get a zero pivot-row entry, so they are left as they are.
   // Branch-free: every row computes its multiplier, rows <= c keep their
   // values by selection and update with a zero multiplier.
-  auto eliminate = [&](int c, int lim, const T *u, auto kc) {
+  auto eliminate = [&](int c, int lim, const auto &u, auto kc) {
     constexpr int K = decltype(kc)::value;
     T uq[SW];
 #pragma clang loop unroll(full)
-    for (int q = 0; q < SW; ++q) uq[q] = u[q];  // all of the row: no per-word branches
+    for (int q = 0; q < SW; ++q) uq[q] = u[q];
 #pragma clang loop unroll(full)
     for (int q = K + 1; q < SW; ++q) uq[q] = q < lim ? uq[q] : (T)0;
     const T piv = uq[K];
@@ -287,22 +287,54 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
 
       // pivots go to global memory after the loop: a store here would make
       // the next barrier wait for its acknowledgement from memory
-      if (t == 0) { s_piv[c] = p; s_pmag[c] = fabs(u[kk]); }
-      // ---- row exchange inside the strip ----
-      if (p != c) {
+      // ---- row exchange inside the strip, then the elimination ----
+      if constexpr (NT <= 256) {
+        // the pivot row in registers, read from LDS once for the new row c
+        // and the update (the wave holding row c then needs no LDS read of
+        // its own: 1000 of the 3800 cycles of a column at m = 1024 were
+        // that swap); 512-thread workgroups lack the registers for it.
+        // Two loops for the swap: as one if/else the compiler turns the two
+        // copies into one through a pointer that selects the source, which
+        // puts the register row in scratch memory.  (A compile-time index
+        // for |pivot|: a captured constexpr would index the row at run time.)
+        T ur[SW];
 #pragma clang loop unroll(full)
-        for (int r = 0; r < R; ++r) {
-          const int li = g * RPW + r * NT + t;
-          if (li == c) {
+        for (int q = 0; q < SW; ++q) ur[q] = u[q];
+        if (t == 0) { s_piv[c] = p; s_pmag[c] = fabs(ur[decltype(kc)::value]); }
+        if (p != c) {
 #pragma clang loop unroll(full)
-            for (int q = 0; q < SW; ++q) a[r][q] = u[q];
-          } else if (li == p) {
+          for (int r = 0; r < R; ++r) {
+            if (g * RPW + r * NT + t == c) {
 #pragma clang loop unroll(full)
-            for (int q = 0; q < SW; ++q) a[r][q] = top[q];
+              for (int q = 0; q < SW; ++q) a[r][q] = ur[q];
+            }
+          }
+#pragma clang loop unroll(full)
+          for (int r = 0; r < R; ++r) {
+            if (g * RPW + r * NT + t == p) {
+#pragma clang loop unroll(full)
+              for (int q = 0; q < SW; ++q) a[r][q] = top[q];
+            }
           }
         }
+        eliminate(c, lim, ur, kc);
+      } else {
+        if (t == 0) { s_piv[c] = p; s_pmag[c] = fabs(u[kk]); }
+        if (p != c) {
+#pragma clang loop unroll(full)
+          for (int r = 0; r < R; ++r) {
+            const int li = g * RPW + r * NT + t;
+            if (li == c) {
+#pragma clang loop unroll(full)
+              for (int q = 0; q < SW; ++q) a[r][q] = u[q];
+            } else if (li == p) {
+#pragma clang loop unroll(full)
+              for (int q = 0; q < SW; ++q) a[r][q] = top[q];
+            }
+          }
+        }
+        eliminate(c, lim, u, kc);
       }
-      eliminate(c, lim, u, kc);
     }
   };
 #pragma clang loop unroll(disable)
