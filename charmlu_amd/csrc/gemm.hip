@@ -253,7 +253,7 @@ CLU_DEV void wait_vm() {
 // into the sign of the A fragments, so the epilogue is stores only and the
 // C reads overlap the first stages' loads instead of following the last
 // MFMA.  Otherwise the epilogue reads C once the products are done.
-template <int NST, bool CINIT, int PIPE = 0>
+template <int NST, bool CINIT>
 CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const double *__restrict__ A,
                       long lda, const double *__restrict__ B, long ldb, double beta,
                       double *__restrict__ C, long ldc) {
@@ -322,47 +322,7 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
 
 #pragma unroll
   for (int st = 0; st < NST - 1; ++st) issue(st);
-  if (PIPE) {
-    // Software-pipelined fragments: the LDS reads of each half-stage are
-    // issued one half-stage (16 MFMAs) before their use.  Iteration kc:
-    // read F1 (stage kc, k 4..7); MFMAs on F0 (stage kc, k 0..3); wait for
-    // stage kc+1 and barrier; issue stage kc+NST-1 into the buffer of
-    // stage kc-1 (every wave finished reading it before this barrier);
-    // read F0 of stage kc+1; MFMAs on F1.
-    double f0a[4], f0b[4], f1a[4], f1b[4];
-    auto frag = [&](int kc, int s2, double (&fa)[4], double (&fb)[4]) {
-      const char *st = smem + (kc % NST) * STAGE;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) fa[i] = *reinterpret_cast<const double *>(st + aoff[s2][i]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) fb[j] = *reinterpret_cast<const double *>(st + boff[s2][j]);
-    };
-    auto mfma16 = [&](const double (&fa)[4], const double (&fb)[4]) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(PIPE == 2 ? -fa[i] : fa[i], fb[j],
-                                                           acc[i][j], 0, 0, 0);
-    };
-    wait_vm<4 * (NST - 2)>();
-    __asm__ volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    __asm__ volatile("" ::: "memory");
-    frag(0, 0, f0a, f0b);
-    for (int kc = 0; kc < nk; ++kc) {
-      frag(kc, 1, f1a, f1b);
-      mfma16(f0a, f0b);
-      wait_vm<4 * (NST - 3)>();  // stage kc+1 (NST - 3 later stages may stay in flight)
-      __asm__ volatile("" ::: "memory");
-      __builtin_amdgcn_s_barrier();
-      __asm__ volatile("" ::: "memory");
-      issue(kc + NST - 1);
-      if (kc + 1 < nk) frag(kc + 1, 0, f0a, f0b);
-      mfma16(f1a, f1b);
-    }
-  }
-  for (int kc = 0; !PIPE && kc < nk; ++kc) {
+  for (int kc = 0; kc < nk; ++kc) {
     // This thread's loads of stage kc have landed; after the barrier every
     // thread's have, and every wave is done reading stage kc-1, whose buffer
     // the next issue overwrites.  A bare s_barrier: __syncthreads() would
@@ -402,108 +362,6 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
           Cw[(long)(i * 16 + lq + 4 * v) * ldc + j * 16 + lr] = acc[i][j][v];
   } else {
     store_tile<double, 4, 4>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, 0, 0, true);
-  }
-}
-
-// The same 128 x 128 tile with 512 threads: 8 waves of 32 x 64 (a quarter of
-// the accumulators of the 256-thread form), so two workgroups give each
-// SIMD four waves to switch between while one waits on LDS, a barrier or
-// memory.  Same LDS layout and swizzles; each thread moves one 16-byte
-// chunk of A and one of B per stage.
-template <int NST, bool CINIT>
-CLU_DEV void lds_tile8(char *smem, int m0, int n0, int K, double alpha,
-                       const double *__restrict__ A, long lda, const double *__restrict__ B,
-                       long ldb, double beta, double *__restrict__ C, long ldc) {
-  constexpr int BM = 128, BN = 128, BK = 8;
-  constexpr int STAGE = (BM * BK + BK * BN) * 8;  // bytes
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm0 = (wave >> 1) * 32, wn0 = (wave & 1) * 64;
-  const int lr = lane & 15, lq = lane >> 4;
-
-  const int c = wave * 64 + lane;  // LDS chunk slot of A and of B
-  const int am = c >> 2, akq = (c & 3) ^ ((am >> 2) & 3);
-  const double *ga = A + (long)(m0 + am) * lda + 2 * akq;
-  const int la_off = (wave * 64) * 16;
-  const int bk = c >> 6, bnq = (c & 63) ^ ((bk & 1) << 3);
-  const double *gb = B + (long)bk * ldb + n0 + 2 * bnq;
-  const int lb_off = BM * BK * 8 + (wave * 64) * 16;
-  const int nk = K / BK;
-  auto issue = [&](int kc) {
-    const int kk = (kc < nk ? kc : nk - 1) * BK;  // past the end: reload the last stage
-    char *st = smem + (kc % NST) * STAGE;
-    load16_to_lds(ga + kk, st + la_off);
-    load16_to_lds(gb + (long)kk * ldb, st + lb_off);
-  };
-  int aoff[2][2], boff[2][4];
-#pragma unroll
-  for (int s2 = 0; s2 < 2; ++s2) {
-    const int k = 4 * s2 + lq;
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int m = wm0 + 16 * i + lr;
-      aoff[s2][i] = m * 64 + (((k >> 1) ^ ((m >> 2) & 3)) << 4) + ((k & 1) << 3);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = wn0 + 16 * j + lr;
-      boff[s2][j] = BM * BK * 8 + k * 1024 + (((n >> 1) ^ ((k & 1) << 3)) << 4) + ((n & 1) << 3);
-    }
-  }
-  double *Cw = C + (long)(m0 + wm0) * ldc + n0 + wn0;
-  double4_t acc[2][4];
-  if (CINIT && beta != 0.0) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          acc[i][j][v] = beta * Cw[(long)(i * 16 + lq + 4 * v) * ldc + j * 16 + lr];
-  } else {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  }
-  const bool neg = CINIT && alpha < 0.0;
-#pragma unroll
-  for (int st = 0; st < NST - 1; ++st) issue(st);
-  for (int kc = 0; kc < nk; ++kc) {
-    wait_vm<2 * (NST - 2)>();
-    __asm__ volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    __asm__ volatile("" ::: "memory");
-    issue(kc + NST - 1);
-    const char *st = smem + (kc % NST) * STAGE;
-#pragma unroll
-    for (int s2 = 0; s2 < 2; ++s2) {
-      double a[2], b[4];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const double *>(st + aoff[s2][i]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const double *>(st + boff[s2][j]);
-      if (neg) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = -a[i];
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-  }
-  wait_vm<0>();
-  if (CINIT) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          Cw[(long)(i * 16 + lq + 4 * v) * ldc + j * 16 + lr] = acc[i][j][v];
-  } else {
-    store_tile<double, 2, 4>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, 0, 0, true);
   }
 }
 
