@@ -144,7 +144,7 @@ template <typename T>
 int gemm_batched(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
                  T beta, T *C, long ldc, int ny, int nz, long a_y, long a_z, long b_y, long b_z,
                  long c_y, long c_z, hipStream_t s);
-// y = op(G) x, G one m x m block inverse (m <= 256)
+// y = op(G) x, G one m x m block inverse (m <= 1024)
 template <typename T>
 int blk_gemv(bool trans, int m, const T *G, long ldg, const T *x, T *y, hipStream_t s);
 template <typename T>

@@ -387,15 +387,22 @@ static int apply_runs(int nruns, const int2 *moves, const int *counts, T *A, lon
 
 // Solve op(A) X = B from P A = L U (forwardSolve / backwardSolve /
 // offDiagSolve, lu.ci:387-458): the row interchanges, then block forward and
-// backward substitution over 256-row blocks.  The diagonal blocks of L and U
+// backward substitution over S-row blocks.  The diagonal blocks of L and U
 // are inverted up front, all in one batched launch each, so the sequential
 // part of each block step is one small matrix-vector (or matrix) product
-// with the block's inverse instead of a 256-step substitution; the block's
+// with the block's inverse instead of an S-step substitution; the block's
 // contribution is then subtracted from every row still to be solved (a GEMV
-// for one right-hand side, a GEMM for several).
+// for one right-hand side, a GEMM for several).  S = 256; FP32 systems of
+// 8192 and more use S = 1024: a vector solve is two launches per block, and
+// at n = 65536 its 1024 launches of 256-row blocks spent a third of the
+// solve between kernels (the FP32 solve is the preconditioner of the
+// refinement, which corrects its rounding).
+// inv_ready: the inverses of the diagonal blocks are still in the workspace
+// from the previous call with the same factors (the refinement's repeated
+// solves)
 template <typename T>
 static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const int *dipiv, T *dB,
-                      long ldb, hipStream_t s) {
+                      long ldb, hipStream_t s, bool inv_ready = false) {
   if (trans != CHARMLU_NO_TRANS && trans != CHARMLU_TRANS) return -1;
   if (n < 0) return -2;
   if (nrhs < 0) return -3;
@@ -409,26 +416,26 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
   int rc = get_ws(&wsp);
   if (rc) return rc;
   Workspace &ws = *wsp;
-  constexpr int S = 256;
-  static_assert(S == RUN, "one move list per block");
+  const int S = sizeof(T) == 4 && n >= 8192 ? 1024 : 256;
   const int nblk = (n + S - 1) / S;
+  const int nruns = (n + RUN - 1) / RUN;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_bytes(&ws.sinv, &ws.sinv_b, sizeof(T) * 3 * (size_t)nblk * S * S))) return rc;
     if ((rc = ensure_bytes(&ws.sy, &ws.sy_b, sizeof(T) * (size_t)n * nrhs))) return rc;
     if ((rc = ensure_bytes(&ws.smv, &ws.smv_b,
-                           (sizeof(int2) * 2 * RUN + sizeof(int)) * (size_t)nblk)))
+                           (sizeof(int2) * 2 * RUN + sizeof(int)) * (size_t)nruns)))
       return rc;
   }
   T *Linv = (T *)ws.sinv, *Uinv = Linv + (size_t)nblk * S * S, *tbuf = Uinv + (size_t)nblk * S * S;
   T *Y = (T *)ws.sy;
   const long ldy = nrhs;
   int2 *vmoves = (int2 *)ws.smv;
-  int *vcounts = (int *)(vmoves + (size_t)nblk * 2 * RUN);
+  int *vcounts = (int *)(vmoves + (size_t)nruns * 2 * RUN);
   // inverses of the diagonal blocks, by recursive doubling, all blocks of a
   // triangle in one batch (the last, shorter block on its own)
   const int nfull = n / S, last = n - nfull * S;
-  for (int up = 0; up < 2; ++up) {
+  for (int up = 0; up < 2 && !inv_ready; ++up) {
     T *Inv = up ? Uinv : Linv;
     if (nfull > 0 && (rc = tri_inv<T>(up, !up, S, nfull, dLU, lda, (long)S * (lda + 1), Inv, S,
                                       (long)S * S, tbuf, s)))
@@ -437,7 +444,6 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
                                      Inv + (size_t)nfull * S * S, S, 0, tbuf, s)))
       return rc;
   }
-  const int nruns = (n + RUN - 1) / RUN;
   const bool vec = nrhs == 1 && ldb == 1;
   auto interchanges = [&](bool inverse) -> int {
     if (vec) return permute_vec<T>(dipiv, n, dB, inverse, vmoves, vcounts, s);
@@ -1336,12 +1342,14 @@ int charmlu_dgesv_ir(int n, const double *dA, long lda, const float *dLU, long l
   double *dres = dnorm + 2;    // residual maxima (5)
   float *z32 = (float *)(V + dbl);
   const double eps = 0x1.0p-53;
+  bool inv_ready = false;  // the block inverses of the factors, after the first solve
   auto precond = [&](const double *src, double scale, double *dst) -> int {
     // dst = (LU)^{-1} src, with src scaled by 1/scale into FP32 range
     int e = vec_d2s(n, 1.0 / scale, src, z32, s);
     if (e) return e;
-    e = getrs_impl<float>(CHARMLU_NO_TRANS, n, 1, dLU, ldlu, dipiv, z32, 1, s);
+    e = getrs_impl<float>(CHARMLU_NO_TRANS, n, 1, dLU, ldlu, dipiv, z32, 1, s, inv_ready);
     if (e) return e;
+    inv_ready = true;
     return vec_s2d(n, scale, z32, dst, s);
   };
   double hres[5];

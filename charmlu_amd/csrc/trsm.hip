@@ -329,16 +329,19 @@ __global__ __launch_bounds__(256) void blk_gemv_kernel(int m, const T *__restric
   if (!TR) {
     const int i = blockIdx.x * 4 + wave;
     if (i >= m) return;
-    T g[4], xv[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int c = lane + 64 * k;
-      g[k] = c < m ? G[(long)i * ldg + c] : (T)0;
-      xv[k] = c < m ? x[c] : (T)0;
-    }
     double acc = 0.0;
+    // 4 values per lane per pass, all loads of a pass before its products
+    for (int c0 = 0; c0 < m; c0 += 256) {
+      T g[4], xv[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc += (double)g[k] * (double)xv[k];
+      for (int k = 0; k < 4; ++k) {
+        const int c = c0 + lane + 64 * k;
+        g[k] = c < m ? G[(long)i * ldg + c] : (T)0;
+        xv[k] = c < m ? x[c] : (T)0;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc += (double)g[k] * (double)xv[k];
+    }
     acc = wave_sum(acc);
     if (lane == 0) y[i] = (T)acc;
   } else {
@@ -357,7 +360,7 @@ __global__ __launch_bounds__(256) void blk_gemv_kernel(int m, const T *__restric
 template <typename T>
 int blk_gemv(bool trans, int m, const T *G, long ldg, const T *x, T *y, hipStream_t s) {
   if (m <= 0) return CLU_OK;
-  if (m > 256) return -1;
+  if (m > 1024) return -1;
   if (!trans)
     hipLaunchKernelGGL((blk_gemv_kernel<T, false>), dim3((m + 3) / 4), dim3(256), 0, s, m, G, ldg,
                        x, y);
