@@ -68,6 +68,7 @@ struct Workspace {
   int panel_cus = 0, ncu = 0;
   hipEvent_t ev_panel = nullptr, ev_la = nullptr, ev_start = nullptr, ev_end = nullptr;
   hipEvent_t ev_urow = nullptr;  // the step's interchanges and U row done (update stream)
+  hipEvent_t ev_ula = nullptr;   // the same for the look-ahead block column only
   hipEvent_t ev_full = nullptr;
   // Durations of the last FP64 factorization's panels (begin/end events on
   // the panel stream, per step), read back at the next call with the same
@@ -207,6 +208,7 @@ static int get_ws(Workspace **out) {
     CLU_HIP(hipEventCreateWithFlags(&w.ev_panel, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_la, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_urow, hipEventDisableTiming));
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_ula, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming));
     // panel candidate records: 2 parities x 64 workgroups x 72 words of 8
     // bytes, zeroed once; their sequence numbers only ever grow (pw.epoch),
@@ -941,11 +943,11 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     return r;
   };
   // the look-ahead block column, timed when profiling is on
-  auto update_la = [&](int M, int N, const double *L, const double *U, long ldu,
-                       double *C) -> int {
-    const int pid = prof_begin(PK_GEMM_OTHER, 2.0 * M * N * nb, sm);
-    int r = dgemm_lookahead(M, N, nb, -1.0, L, lda, U, ldu, 1.0, C, lda, sm);
-    prof_end(pid, sm);
+  auto update_la = [&](int M, int N, const double *L, const double *U, long ldu, double *C,
+                       hipStream_t st) -> int {
+    const int pid = prof_begin(PK_GEMM_OTHER, 2.0 * M * N * nb, st);
+    int r = dgemm_lookahead(M, N, nb, -1.0, L, lda, U, ldu, 1.0, C, lda, st);
+    prof_end(pid, st);
     return r;
   };
   // The rest of the trailing update (updateMatrix, lu.C:26-43).  With the
@@ -1059,10 +1061,21 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     }
     if ((rc = swap_cols(r1, r1 + la, r1 + la, r1 + la))) return rc;
     if ((rc = u_row(0, la))) return rc;
-    // look-ahead block column, then the panel of step t+1 on the panel stream
+    // look-ahead block column, then the panel of step t+1 on the panel
+    // stream.  While the trailing matrix is wide, the look-ahead GEMM runs
+    // on the stream of every CU, beside the rest of the interchanges and of
+    // the U row (other columns), which part 1 of the update waits for; in
+    // the narrow steps the panel waits on it, and an extra stream hop there
+    // costs more than the overlap saves.
     const double *L21 = A11 + (long)nb * lda;
-    if ((rc = update_la(Nt, la, L21, U, ldu, A11 + (long)nb * lda + nb))) return rc;
-    CLU_HIP(hipEventRecord(ws.ev_la, sm));
+    const bool la_beside = Nt >= 16384;
+    if (la_beside) {
+      CLU_HIP(hipEventRecord(ws.ev_ula, sm));
+      CLU_HIP(hipStreamWaitEvent(ws.s_full, ws.ev_ula, 0));
+    }
+    hipStream_t sla = la_beside ? ws.s_full : sm;
+    if ((rc = update_la(Nt, la, L21, U, ldu, A11 + (long)nb * lda + nb, sla))) return rc;
+    CLU_HIP(hipEventRecord(ws.ev_la, sla));
     if ((rc = swap_cols(0, n, r0, r1 + la))) return rc;
     if ((rc = u_row(la, Nt))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_urow, sm));
