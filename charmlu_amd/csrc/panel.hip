@@ -580,7 +580,9 @@ int pivots_to_moves(const int *ipiv, int k1, int k2, int2 *moves, int *count,
   return CLU_OK;
 }
 
-// One workgroup per chunk of CW columns (128 bytes of every row): all moves'
+// One workgroup per chunk of CW columns (128 bytes of every row; 64 and 256
+// measured slower: 1.7 and 2.1 TB/s over a step's interchanges at
+// n = 32768 against 2.5): all moves'
 // source values of the chunk are gathered into registers (up to 2*MAXRUN
 // rows, NP per thread, every load issued before the first is used), then,
 // once the whole workgroup holds them, written to the destination rows.
