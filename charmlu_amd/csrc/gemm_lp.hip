@@ -264,7 +264,9 @@ static void hgemm_launch2(bool cinit, bool vec, int M, int N, int K, float alpha
   // 16-bit operands with K >= 1024 (the mixed LU's block 1024): 3 stages,
   // 3 workgroups per CU (546 TFLOP/s at 64512^2 x 1024 BF16 against 526
   // with 2 stages and 4 per CU, 522 with 4 and 2).  FP8 stays at 2 and 4:
-  // its update at n = 131072 took 3.5 s with 3 and 3 against 2.7 s.
+  // its update at n = 131072 took 3.5 s with 3 and 3 against 2.7 s, and at
+  // n = 65536 it ran at 533 TFLOP/s with 2 and 4 (532 with 2 and 3, 426
+  // with 4 and 2, 413 with 3 and 2).
   if (cinit && vec && K >= 1024 && sizeof(T16) == 2) {
     hgemm_launch<T16, 128, 3, 3, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
                                              escA, escB);
