@@ -58,6 +58,12 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 constexpr int MAX_GROUPS = 256;  // panel workgroups (rows / 2048)
 constexpr int RUN = 256;         // pivots per move list
 
+struct PanelHistory {
+  int n = 0, nb = 0, kind = -1;
+  std::vector<hipEvent_t> ev;  // 2 per step
+  std::vector<float> ms;       // per step, < 0 when unknown
+};
+
 struct Workspace {
   int device = -1;
   hipStream_t s_panel = nullptr;  // panels: the reserved CUs only
@@ -70,13 +76,12 @@ struct Workspace {
   hipEvent_t ev_urow = nullptr;  // the step's interchanges and U row done (update stream)
   hipEvent_t ev_ula = nullptr;   // the same for the look-ahead block column only
   hipEvent_t ev_full = nullptr;
-  // Durations of the last FP64 factorization's panels (begin/end events on
-  // the panel stream, per step), read back at the next call with the same
-  // n and block size to size the part of each trailing update that runs
-  // beside the panel.
-  int hist_n = 0, hist_nb = 0;
-  std::vector<hipEvent_t> hist_ev;  // 2 per step
-  std::vector<float> hist_ms;       // per step, < 0 when unknown
+  // Durations of the last factorization's panels (begin/end events on the
+  // panel stream, per step), read back at the next call of the same kind
+  // with the same n and block size to size the part of each trailing update
+  // that runs beside the panel: one history for FP64, one for the mixed
+  // form.
+  PanelHistory hist64, histmx;
   // panel exchange slots (sized for doubles, reused for floats)
   void *slots = nullptr;
   unsigned *counter = nullptr;
@@ -362,6 +367,32 @@ static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, 
     }
   }
   return factor_panel<T>(A, lda, n, c0 + w1, w2, pc0, pcw, ipiv, ws, s, pivot);
+}
+
+// Panel durations of the previous call of the same kind and shape (events
+// that are not complete, or a different kind or shape, leave -1), then
+// events for this call's NB steps.
+static int history_prepare(PanelHistory &h, int n, int nb, int kind, int NB) {
+  const bool same = h.n == n && h.nb == nb && h.kind == kind && (int)h.ev.size() >= 2 * NB;
+  if (!same) h.ms.assign(NB, -1.0f);
+  if (same) {
+    for (int t = 0; t < NB; ++t) {
+      float ms = -1.0f;
+      if (hipEventQuery(h.ev[2 * t + 1]) == hipSuccess &&
+          hipEventElapsedTime(&ms, h.ev[2 * t], h.ev[2 * t + 1]) == hipSuccess)
+        h.ms[t] = ms;
+      (void)hipGetLastError();
+    }
+  }
+  while ((int)h.ev.size() < 2 * NB) {
+    hipEvent_t e;
+    CLU_HIP(hipEventCreate(&e));
+    h.ev.push_back(e);
+  }
+  h.n = n;
+  h.nb = nb;
+  h.kind = kind;
+  return CLU_OK;
 }
 
 // move lists of ipiv[k1, k2) in runs of RUN pivots
@@ -962,7 +993,7 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   auto split_rows = [&](int t, int M, int N, const double *L, const double *U, long ldu) -> int {
     if (!split || !dgemm_trailing_lds(M, N, nb, L, lda, U, ldu)) return M;
     const int m = n - (t + 1) * nb;  // rows of panel t+1
-    double p_ms = (t + 1 < (int)ws.hist_ms.size()) ? ws.hist_ms[t + 1] : -1.0;
+    double p_ms = (t + 1 < (int)ws.hist64.ms.size()) ? ws.hist64.ms[t + 1] : -1.0;
     if (p_ms <= 0.0) p_ms = nb / 256.0 * (1.3 + 2.0 * (m < 32768 ? m / 32768.0 : 1.0));
     // FP64 matrix-core rate of the update stream's CUs at 80 % of peak
     // (optimistic, so the first part errs long), plus 10 % margin
@@ -998,25 +1029,7 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     full_pending = true;
     return CLU_OK;
   };
-  // panel durations of the previous call with the same shape, then events
-  // for this call's
-  if (ws.hist_n != n || ws.hist_nb != nb) ws.hist_ms.assign(NB, -1.0f);
-  if (ws.hist_n == n && ws.hist_nb == nb && (int)ws.hist_ev.size() == 2 * NB) {
-    for (int t = 0; t < NB; ++t) {
-      float ms = -1.0f;
-      if (hipEventQuery(ws.hist_ev[2 * t + 1]) == hipSuccess &&
-          hipEventElapsedTime(&ms, ws.hist_ev[2 * t], ws.hist_ev[2 * t + 1]) == hipSuccess)
-        ws.hist_ms[t] = ms;
-      (void)hipGetLastError();
-    }
-  }
-  while ((int)ws.hist_ev.size() < 2 * NB) {
-    hipEvent_t e;
-    CLU_HIP(hipEventCreate(&e));
-    ws.hist_ev.push_back(e);
-  }
-  ws.hist_n = n;
-  ws.hist_nb = nb;
+  if ((rc = history_prepare(ws.hist64, n, nb, 0, NB))) return rc;
 
   if ((rc = panel(0))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_panel, sp));
@@ -1083,9 +1096,9 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     const int Ma = Nt > la ? split_rows(t, Nt, Nt - la, L21, U + la, ldu) : 0;
     if (Nt > la && (rc = update_part1(Ma, Nt - la, L21, U + la, ldu, C2))) return rc;
     CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
-    CLU_HIP(hipEventRecord(ws.hist_ev[2 * (t + 1)], sp));
+    CLU_HIP(hipEventRecord(ws.hist64.ev[2 * (t + 1)], sp));
     if ((rc = panel(t + 1))) return rc;
-    CLU_HIP(hipEventRecord(ws.hist_ev[2 * (t + 1) + 1], sp));
+    CLU_HIP(hipEventRecord(ws.hist64.ev[2 * (t + 1) + 1], sp));
     CLU_HIP(hipEventRecord(ws.ev_panel, sp));
     if (Nt > la && (rc = update_part2(Ma, Nt, Nt - la, L21, U + la, ldu, C2))) return rc;
     // the U row block into the factor storage (nothing reads A12 before)
@@ -1208,33 +1221,65 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     prof_end(pid, sp);
     return r;
   };
-  // C (M x N) -= L21 (M x nb) * U (nb x N); U column c0 is column c0 of the
-  // packed transposed operand; the 16-bit L21 is packed negated, so the
-  // low-precision kernel accumulates onto C (C += (-L21) U)
-  auto update = [&](int M, int N, const float *L, long ldl, int c0, float *C, int Nt) -> int {
-    const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb, sm);
+  // C (M x N) -= L21 (M x nb) * U (nb x N) for the rows from r_off of L21
+  // (C already offset); U column c0 is column c0 of the packed transposed
+  // operand; the 16-bit L21 is packed negated, so the low-precision kernel
+  // accumulates onto C (C += (-L21) U).  FP8 row offsets are multiples of
+  // 128 (one scale per 128-row tile).
+  auto update = [&](int M, int N, int r_off, const float *L, long ldl, int c0, float *C, int Nt,
+                    hipStream_t st) -> int {
+    if (M <= 0 || N <= 0) return CLU_OK;
+    const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb, st);
+    const size_t ro = (size_t)r_off * Kp;
     int r;
     if (!lp)
-      r = sgemm(false, false, M, N, nb, -1.0f, L, ldl, ubuf + c0, Nt, 1.0f, C, lda, sm);
+      r = sgemm(false, false, M, N, nb, -1.0f, L + (long)r_off * ldl, ldl, ubuf + c0, Nt, 1.0f, C,
+                lda, st);
     else if (fp8)
-      r = fgemm_nt(M, N, Kp, 1.0f, (const uint8_t *)ws.pa16, Kp, (const int *)ws.esc,
-                   (const uint8_t *)ws.pu16 + (size_t)c0 * Kp, Kp,
-                   (const int *)ws.esc + (n / 128 + 2) + c0 / 128, 1.0f, C, lda, sm);
+      r = fgemm_nt(M, N, Kp, 1.0f, (const uint8_t *)ws.pa16 + ro, Kp,
+                   (const int *)ws.esc + r_off / 128, (const uint8_t *)ws.pu16 + (size_t)c0 * Kp,
+                   Kp, (const int *)ws.esc + (n / 128 + 2) + c0 / 128, 1.0f, C, lda, st);
     else if (prec == CHARMLU_PREC_BF16)
-      r = hgemm_nt<__bf16>(M, N, Kp, 1.0f, (const __bf16 *)ws.pa16, Kp,
-                           (const __bf16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, sm);
+      r = hgemm_nt<__bf16>(M, N, Kp, 1.0f, (const __bf16 *)ws.pa16 + ro, Kp,
+                           (const __bf16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, st);
     else
-      r = hgemm_nt<_Float16>(M, N, Kp, 1.0f, (const _Float16 *)ws.pa16, Kp,
-                             (const _Float16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, sm);
-    prof_end(pid, sm);
+      r = hgemm_nt<_Float16>(M, N, Kp, 1.0f, (const _Float16 *)ws.pa16 + ro, Kp,
+                             (const _Float16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, st);
+    prof_end(pid, st);
     return r;
   };
+  // Rows of the trailing update (after the look-ahead column) that run on
+  // the update stream beside the panel of the next step; the rest waits for
+  // that panel and runs on every CU.  Sized as in the FP64 form: the panel's
+  // duration in the previous call of the same kind and shape, else a model
+  // (FP32 1024-column panels: 1.5 + 14 m/65536 ms pivoted, 3.5 ms
+  // unpivoted), times the update rate of the stream's CUs, plus 10 %.
+  const bool split = ws.panel_cus > 0;
+  bool full_pending = false;
+  const double chip_rate = !lp ? 0.6 * 157.3e9 : fp8 ? 530e9 : 450e9;  // flop per ms
+  auto split_rows = [&](int t, int M, int N) -> int {
+    if (!split) return M;
+    const int m = n - (t + 1) * nb;
+    double p_ms = (t + 1 < (int)ws.histmx.ms.size()) ? ws.histmx.ms[t + 1] : -1.0;
+    if (p_ms <= 0.0)
+      p_ms = nb / 1024.0 * (nopiv ? 3.5 : 1.5 + 14.0 * (m < 65536 ? m / 65536.0 : 1.0));
+    const double rate = chip_rate * (ws.ncu - ws.panel_cus) / ws.ncu;
+    const double rows = 1.1 * p_ms * rate / (2.0 * N * nb);
+    if (rows >= M) return M;
+    const int Ma = ((int)rows + 127) / 128 * 128;
+    return Ma > M ? M : Ma;
+  };
+  if ((rc = history_prepare(ws.histmx, n, nb, 1 + 2 * prec + (nopiv ? 1 : 0), NB))) return rc;
 
   if ((rc = panel(0))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_panel, sp));
   for (int t = 0; t < NB; ++t) {
     const int r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
     CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));
+    if (full_pending) {  // the previous update's second part
+      CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+      full_pending = false;
+    }
     int pid = prof_begin(PK_SWAP, (double)(n - nb), sm);
     if (!nopiv && (rc = apply_runs<float>(nruns, moves_of(t), counts_of(t), dA, lda, 0, n, r0, r1,
                                           false, sm)))
@@ -1278,19 +1323,34 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       }
     }
     prof_end(pid, sm);
+    CLU_HIP(hipEventRecord(ws.ev_urow, sm));  // operands of this step's update ready
     const int la = Nt < nb ? Nt : nb;
     float *C = A11 + (long)nb * lda + nb;
-    if ((rc = update(Nt, la, L21, ldl21, 0, C, Nt))) return rc;
+    if ((rc = update(Nt, la, 0, L21, ldl21, 0, C, Nt, sm))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_la, sm));
+    // part 1 beside the panel (enqueued before the panel's launches), part 2
+    // on every CU after it
+    const int Ma = Nt > la ? split_rows(t, Nt, Nt - la) : 0;
+    if ((rc = update(Ma, Nt - la, 0, L21, ldl21, la, C + la, Nt, sm))) return rc;
     CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
+    CLU_HIP(hipEventRecord(ws.histmx.ev[2 * (t + 1)], sp));
     if ((rc = panel(t + 1))) return rc;
+    CLU_HIP(hipEventRecord(ws.histmx.ev[2 * (t + 1) + 1], sp));
     CLU_HIP(hipEventRecord(ws.ev_panel, sp));
-    if (Nt > la)
-      if ((rc = update(Nt, Nt - la, L21, ldl21, la, C + la, Nt))) return rc;
+    if (Nt > la && Ma < Nt) {
+      hipStream_t sf = ws.s_full;
+      CLU_HIP(hipStreamWaitEvent(sf, ws.ev_urow, 0));
+      CLU_HIP(hipStreamWaitEvent(sf, ws.ev_panel, 0));
+      if ((rc = update(Nt - Ma, Nt - la, Ma, L21, ldl21, la, C + (long)Ma * lda + la, Nt, sf)))
+        return rc;
+      CLU_HIP(hipEventRecord(ws.ev_full, sf));
+      full_pending = true;
+    }
     if ((rc = convert<float, float>(nb, Nt, ubuf, Nt, A11 + nb, lda, sm))) return rc;
     if (nopiv && (rc = convert<float, float>(Nt, nb, L21, ldl21, A11 + (long)nb * lda, lda, sm)))
       return rc;
   }
+  if (full_pending) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
   CLU_HIP(hipEventRecord(ws.ev_end, sm));
   CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
   return CLU_OK;
