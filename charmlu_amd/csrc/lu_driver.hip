@@ -1171,7 +1171,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_runs(ws, nruns))) return rc;
-    if ((rc = ensure_bytes(&ws.winv32, &ws.winv32_b, sizeof(float) * nb * nb))) return rc;
+    if ((rc = ensure_bytes(&ws.winv32, &ws.winv32_b, sizeof(float) * 2 * nb * nb))) return rc;
     if ((rc = ensure_bytes(&ws.ubuf32, &ws.ubuf32_b, sizeof(float) * nb * n))) return rc;
     if ((rc = ensure_bytes(&ws.tinv, &ws.tinv_b, sizeof(float) * nb * nb))) return rc;
     if ((rc = ensure_bytes(&ws.pinv, &ws.pinv_b, sizeof(float) * nb * nb / 4))) return rc;
@@ -1187,7 +1187,10 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if ((rc = ensure_bytes(&ws.l21buf, &ws.l21buf_b, sizeof(float) * nb * n))) return rc;
     }
   }
-  float *winv = (float *)ws.winv32, *ubuf = (float *)ws.ubuf32;
+  float *ubuf = (float *)ws.ubuf32;
+  // inv(L11) of step t, by step parity: the panel of step t+1 writes its
+  // own while the rest of step t's U row still reads this one
+  auto winv_of = [&](int t) { return (float *)ws.winv32 + (size_t)(t & 1) * nb * nb; };
   ws.pw.info = dinfo;
   CLU_HIP(hipEventRecord(ws.ev_start, su));
   CLU_HIP(hipStreamWaitEvent(sp, ws.ev_start, 0));
@@ -1216,7 +1219,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if ((r = build_moves(dipiv, r0, r0 + nb, moves_of(t), counts_of(t), sp))) return r;
     }
     if (r0 + nb < n)
-      r = tri_inv<float>(false, true, nb, 1, dA + (long)r0 * lda + r0, lda, 0, winv, nb, 0,
+      r = tri_inv<float>(false, true, nb, 1, dA + (long)r0 * lda + r0, lda, 0, winv_of(t), nb, 0,
                          (float *)ws.tinv, sp);
     prof_end(pid, sp);
     return r;
@@ -1280,16 +1283,28 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
       full_pending = false;
     }
-    int pid = prof_begin(PK_SWAP, (double)(n - nb), sm);
-    if (!nopiv && (rc = apply_runs<float>(nruns, moves_of(t), counts_of(t), dA, lda, 0, n, r0, r1,
-                                          false, sm)))
-      return rc;
-    prof_end(pid, sm);
-    if (Nt <= 0) break;
+    const int la = Nt < nb ? Nt : nb;
+    // Interchanges, U row and packed operands: first for the look-ahead
+    // block column (whose update the next panel waits for), the rest after
+    // that update, while the panel runs (as in the FP64 form)
+    auto swap_cols = [&](int c0, int c1, int skip0, int skip1) -> int {
+      if (nopiv) return CLU_OK;
+      const int p = prof_begin(PK_SWAP, (double)(c1 - c0 - (skip1 - skip0)), sm);
+      int e = apply_runs<float>(nruns, moves_of(t), counts_of(t), dA, lda, c0, c1, skip0, skip1,
+                                false, sm);
+      prof_end(p, sm);
+      return e;
+    };
+    if (Nt <= 0) {
+      if ((rc = swap_cols(0, n, r0, r1))) return rc;
+      break;
+    }
+    if ((rc = swap_cols(r1, r1 + la, r1 + la, r1 + la))) return rc;
     float *A11 = dA + (long)r0 * lda + r0;
     // L21: in the matrix, or (unpivoted form) A21 inv(U11) in l21buf
     const float *L21 = A11 + (long)nb * lda;
     long ldl21 = lda;
+    int pid;
     if (nopiv) {
       float *l21 = (float *)ws.l21buf;
       pid = prof_begin(PK_GEMM_OTHER, 2.0 * Nt * nb * nb, sm);
@@ -1300,34 +1315,46 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       L21 = l21;
       ldl21 = nb;
     }
-    pid = prof_begin(PK_TRSM, (double)Nt * nb * (nb - 1), sm);
-    if ((rc = sgemm(false, false, nb, Nt, nb, 1.0f, winv, nb, A11 + nb, lda, 0.0f, ubuf, Nt, sm)))
-      return rc;
-    if (fp8) {
-      // FP8 operands with power-of-two scales per 128-row tile of L21 and
-      // per 128-column tile of U12
-      int *escL = (int *)ws.esc, *escU = escL + (n / 128 + 2);
-      if ((rc = quant_rows_fp8(Nt, nb, Kp, -1.0f, L21, ldl21, (uint8_t *)ws.pa16, escL, sm)))
-        return rc;
-      if ((rc = quant_cols_fp8(nb, Nt, Kp, ubuf, Nt, (uint8_t *)ws.pu16, escU, sm))) return rc;
-    } else if (lp) {
-      // operands of the trailing update, rounded to 16 bits once per step
-      if (prec == CHARMLU_PREC_BF16) {
-        if ((rc = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, L21, ldl21, (__bf16 *)ws.pa16, sm)))
-          return rc;
-        if ((rc = pack_cols<__bf16>(nb, Nt, Kp, ubuf, Nt, (__bf16 *)ws.pu16, sm))) return rc;
-      } else {
-        if ((rc = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, L21, ldl21, (_Float16 *)ws.pa16, sm)))
-          return rc;
-        if ((rc = pack_cols<_Float16>(nb, Nt, Kp, ubuf, Nt, (_Float16 *)ws.pu16, sm))) return rc;
+    // U row columns [j0, j1) and their packed operand (the packed L21 with
+    // the first part)
+    auto u_row = [&](int j0, int j1) -> int {
+      if (j1 <= j0) return CLU_OK;
+      const int p = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
+      int e = sgemm(false, false, nb, j1 - j0, nb, 1.0f, winv_of(t), nb, A11 + nb + j0, lda, 0.0f,
+                    ubuf + j0, Nt, sm);
+      if (!e && fp8) {
+        // FP8 operands with power-of-two scales per 128-row tile of L21 and
+        // per 128-column tile of U12
+        int *escL = (int *)ws.esc, *escU = escL + (n / 128 + 2);
+        if (j0 == 0) e = quant_rows_fp8(Nt, nb, Kp, -1.0f, L21, ldl21, (uint8_t *)ws.pa16, escL, sm);
+        if (!e)
+          e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)ws.pu16 + (size_t)j0 * Kp,
+                             escU + j0 / 128, sm);
+      } else if (!e && lp) {
+        // operands of the trailing update, rounded to 16 bits once per step
+        if (prec == CHARMLU_PREC_BF16) {
+          if (j0 == 0) e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, L21, ldl21, (__bf16 *)ws.pa16, sm);
+          if (!e)
+            e = pack_cols<__bf16>(nb, j1 - j0, Kp, ubuf + j0, Nt,
+                                  (__bf16 *)ws.pu16 + (size_t)j0 * Kp, sm);
+        } else {
+          if (j0 == 0)
+            e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, L21, ldl21, (_Float16 *)ws.pa16, sm);
+          if (!e)
+            e = pack_cols<_Float16>(nb, j1 - j0, Kp, ubuf + j0, Nt,
+                                    (_Float16 *)ws.pu16 + (size_t)j0 * Kp, sm);
+        }
       }
-    }
-    prof_end(pid, sm);
-    CLU_HIP(hipEventRecord(ws.ev_urow, sm));  // operands of this step's update ready
-    const int la = Nt < nb ? Nt : nb;
+      prof_end(p, sm);
+      return e;
+    };
+    if ((rc = u_row(0, la))) return rc;
     float *C = A11 + (long)nb * lda + nb;
     if ((rc = update(Nt, la, 0, L21, ldl21, 0, C, Nt, sm))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_la, sm));
+    if ((rc = swap_cols(0, n, r0, r1 + la))) return rc;
+    if ((rc = u_row(la, Nt))) return rc;
+    CLU_HIP(hipEventRecord(ws.ev_urow, sm));  // operands of this step's update ready
     // part 1 beside the panel (enqueued before the panel's launches), part 2
     // on every CU after it
     const int Ma = Nt > la ? split_rows(t, Nt, Nt - la) : 0;
