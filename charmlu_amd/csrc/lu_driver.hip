@@ -369,6 +369,19 @@ static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, 
   return factor_panel<T>(A, lda, n, c0 + w1, w2, pc0, pcw, ipiv, ws, s, pivot);
 }
 
+// CHARMLU_FORCE_SPLIT=1 (read at every factorization): every trailing
+// update splits its rows in two halves (part 1 rounded up to 128 rows), so
+// that small test problems run both launches and their events; results do
+// not depend on the split.
+static bool split_forced() {
+  const char *e = getenv("CHARMLU_FORCE_SPLIT");
+  return e && e[0] == '1';
+}
+static int forced_split(int M) {
+  const int Ma = (M / 2 + 127) / 128 * 128;
+  return Ma > M ? M : Ma;
+}
+
 // Panel durations of the previous call of the same kind and shape (events
 // that are not complete, or a different kind or shape, leave -1), then
 // events for this call's NB steps.
@@ -989,9 +1002,11 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   // this n and block size, else a model), so neither set of CUs waits for
   // long.  Each tile is computed the same way whatever the split.
   const bool split = ws.panel_cus > 0;
+  const bool force_split = split_forced();
   bool full_pending = false;
   auto split_rows = [&](int t, int M, int N, const double *L, const double *U, long ldu) -> int {
     if (!split || !dgemm_trailing_lds(M, N, nb, L, lda, U, ldu)) return M;
+    if (force_split) return forced_split(M);
     const int m = n - (t + 1) * nb;  // rows of panel t+1
     double p_ms = (t + 1 < (int)ws.hist64.ms.size()) ? ws.hist64.ms[t + 1] : -1.0;
     if (p_ms <= 0.0) p_ms = nb / 256.0 * (1.3 + 2.0 * (m < 32768 ? m / 32768.0 : 1.0));
@@ -1258,10 +1273,12 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   // (FP32 1024-column panels: 1.5 + 14 m/65536 ms pivoted, 3.5 ms
   // unpivoted), times the update rate of the stream's CUs, plus 10 %.
   const bool split = ws.panel_cus > 0;
+  const bool force_split = split_forced();
   bool full_pending = false;
   const double chip_rate = !lp ? 0.6 * 157.3e9 : fp8 ? 530e9 : 450e9;  // flop per ms
   auto split_rows = [&](int t, int M, int N) -> int {
     if (!split) return M;
+    if (force_split) return forced_split(M);
     const int m = n - (t + 1) * nb;
     double p_ms = (t + 1 < (int)ws.histmx.ms.size()) ? ws.histmx.ms[t + 1] : -1.0;
     if (p_ms <= 0.0)

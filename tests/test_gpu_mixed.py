@@ -404,3 +404,31 @@ def test_fp32_first_strip_matches_restatement(dev, orc, n):
     # same operations in the same order: bit-identical (measured at all
     # three sizes)
     assert np.array_equal(G, S), err
+
+
+@pytest.mark.parametrize("prec,pivot", [("bf16", True), ("fp8", True), ("fp32", True),
+                                        ("bf16", False), ("fp8", False)])
+def test_forced_split_is_bitwise_unsplit_mixed(dev, orc, prec, pivot):
+    """The trailing update's split (rows beside the next panel, the rest on
+    every CU after it) computes every tile as the unsplit launch does: with
+    the split forced to half the rows at every step, the factors and pivots
+    are bit-identical (the FP8 half starts at a multiple of 128 rows, so its
+    per-tile scales line up)."""
+    import os
+    import torch
+    n, nb = 2048, 512
+    A = (orc.gen_uniform if pivot else orc.gen_diagdom)(orc.SEED_A + 5, n)
+    runs = []
+    for forced in (False, True):
+        A32 = torch.from_numpy(A.astype(np.float32)).cuda()
+        if forced:
+            os.environ["CHARMLU_FORCE_SPLIT"] = "1"
+        try:
+            ipiv, info = dev.getrf_mixed(A32, nb, prec, pivot=pivot)
+            torch.cuda.synchronize()
+        finally:
+            os.environ.pop("CHARMLU_FORCE_SPLIT", None)
+        assert int(info.item()) == 0
+        runs.append((A32.cpu().numpy(), ipiv.cpu().numpy()))
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(runs[0][0], runs[1][0])

@@ -331,3 +331,35 @@ def test_tall_panel_strips_residual(dev):
     out = dev.run(dev.LUConfig(40960, 256), verbose=True)
     assert out["info"] == 0
     assert out["residual"] < 16 and out["residual_ns"] < 16
+
+
+@pytest.mark.parametrize("n,b", [(1024, 128), (2048, 256)])
+def test_forced_split_matches_oracle_and_unsplit(dev, orc, n, b):
+    """Small problems never split a trailing update (it fits beside the
+    panel); forced to half the rows at every step, the factorization is
+    bit-identical to the unsplit one and still matches the oracle."""
+    import torch
+    from charmlu_amd import _lib
+    lib = _lib.load()
+    prof = (ctypes.c_double * 20)()
+    A = dev.gen_matrix(n, seed=orc.SEED_A)
+    LU1 = A.clone()
+    lib.charmlu_profile(1)
+    ip1, _ = dev.getrf(LU1, b)
+    assert lib.charmlu_profile_read(prof, 20) == 0
+    launches_unsplit = prof[0]  # trailing-update launches of the 128x128 kernel
+    LU2 = A.clone()
+    os.environ["CHARMLU_FORCE_SPLIT"] = "1"
+    try:
+        ip2, info = dev.getrf(LU2, b)
+        _sync()
+        assert lib.charmlu_profile_read(prof, 20) == 0
+    finally:
+        os.environ.pop("CHARMLU_FORCE_SPLIT", None)
+        lib.charmlu_profile(0)
+    assert prof[0] > launches_unsplit, (prof[0], launches_unsplit)
+    assert int(info.item()) == 0
+    assert np.array_equal(ip1.cpu().numpy(), ip2.cpu().numpy())
+    assert torch.equal(LU1, LU2)
+    o = orc.lu_solve_problem(n, b, 0, nthreads=8, A=A.cpu().numpy().copy())
+    assert np.array_equal(ip2.cpu().numpy(), o["ipiv"])
