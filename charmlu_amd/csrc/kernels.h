@@ -86,6 +86,9 @@ int permute_vec(const int *ipiv, int n, T *v, bool inverse, int2 *moves, int *co
                 hipStream_t s);
 
 // ---- panel factorization (FP64 / FP32) ------------------------------------
+// 8-byte words of the panel exchange records (2 parities x 64 workgroups x
+// 136 words): PanelWork::slots holds at least this many.
+constexpr long PANEL_SLOT_WORDS = 2L * 64 * 136;
 struct PanelWork {
   void *slots;        // per-workgroup candidate records, 2 parities x 256
   unsigned epoch;     // sequence number of the next column (host-side)

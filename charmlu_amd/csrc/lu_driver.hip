@@ -215,10 +215,10 @@ static int get_ws(Workspace **out) {
     CLU_HIP(hipEventCreateWithFlags(&w.ev_urow, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_ula, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming));
-    // panel candidate records: 2 parities x 64 workgroups x 72 words of 8
+    // panel candidate records: 2 parities x 64 workgroups x 136 words of 8
     // bytes, zeroed once; their sequence numbers only ever grow (pw.epoch),
     // so a record left by an earlier call can never look current
-    const size_t slot_bytes = (size_t)2 * 64 * 72 * 8;
+    const size_t slot_bytes = (size_t)PANEL_SLOT_WORDS * 8;
     CLU_HIP(hipMalloc(&w.slots, slot_bytes));
     CLU_HIP(hipMemset(w.slots, 0, slot_bytes));
     CLU_HIP(hipMalloc(&w.counter, 64));

@@ -35,26 +35,33 @@
 
 namespace clu {
 
-// Strip width SW: 16 columns, or 32 for FP32 strips of at most 32 * 2048
-// rows (4 rows of 32 floats per thread; taller FP32 strips need 8 rows per
-// thread and stay 16 wide).  A 32-column strip halves the strip launches of
-// a panel and removes the 32-column nodes of its recursion (a triangular
-// solve and a GEMM each).
-constexpr int PW = 16;      // the strip width of FP64
-constexpr int PW32 = 32;    // the widest FP32 strip
+// Strip width SW: 16 or 32 columns.  A 32-column strip halves the strip
+// launches of a panel and removes the 32-column nodes of its recursion (a
+// triangular solve and a GEMM each).  FP32 strips of at most 32 * 2048 rows
+// are 32 wide (4 rows of 32 floats per thread; taller ones need 8 rows per
+// thread).  FP64 strips are 32 wide up to 512 rows (one workgroup) and from
+// 1025 to 9216 rows (2 rows of 32 doubles per thread, at most 18
+// workgroups: their records fill the polling waves): per 32 columns 79 us
+// against 2 x 36 us plus the node at 512 rows, 120-131 us against
+// 2 x 62-66 us plus the node at 2048-8192 rows; from 513 to 1024 rows the
+// one workgroup of 512 threads this would need is slower (119 us against
+// 2 x 42).
+constexpr int PW = 16;      // the narrow strip
+constexpr int PW32 = 32;    // the wide strip
 
 // Cross-workgroup records.  Every 8-byte word carries a 32-bit payload in
 // its low half and the column's sequence number in its high half, so a
 // reader can check each word on its own: a column costs one memory round
 // trip (stores are not waited for, nothing is ordered by fences).  Record
-// of workgroup g for column parity par at recs + (par * GMAX + g) * RECW,
+// of workgroup g for column parity par at recs + (par * GMAX + g) * RW,
 // in 32-bit payload words (VW per value, FP64 low half first):
 //   [0, SW*VW)          the workgroup's best row (its SW values; zeros if none)
 //   [SW*VW, SW*VW+VW)   |value| of that row in the current column
 //   SW*VW+VW            its row (local index, -1: none)
 //   [TOPW, TOPW+SW*VW)  workgroup 0 only: row c (the top row) before the swap
-constexpr int GMAX = 64;  // workgroups in one exchange
-constexpr int RECW = 72;  // words per record
+constexpr int GMAX = 64;       // workgroups in one exchange
+constexpr int RECW_MAX = 136;  // words per record, at most (RW: each kernel's own)
+static_assert(2L * GMAX * RECW_MAX == PANEL_SLOT_WORDS, "slot array size");
 constexpr int SPIN_MAX = 1 << 22;  // polling rounds before giving up (seconds)
 
 // Payload word k of a value: FP64 low half first (little-endian), so the
@@ -80,7 +87,8 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   constexpr int RPW = NT * R;            // rows per workgroup
   constexpr int NW = NT / 64;            // waves
   constexpr int CHW = NW >= 8 ? 4 : 5;   // record words per polling lane
-  static_assert(TOPW + CW <= RECW, "record size");
+  constexpr int RW = TOPW + CW;          // words per record
+  static_assert(RW <= RECW_MAX, "record size");
   const int G = MULTI ? (int)gridDim.x : 1, g = MULTI ? (int)blockIdx.x : 0;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const T tiny = (T)100 * (sizeof(T) == 8 ? (T)2.220446049250313e-16 : (T)1.1920929e-07);
@@ -90,7 +98,7 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   __shared__ int s_idx[2][NW];
   __shared__ T s_wcand[2][NW][SW];  // each wave's candidate row
   __shared__ T s_top[2][SW];        // row c before the swap (in the workgroup holding it)
-  __shared__ __attribute__((aligned(16))) unsigned s_rec[2][MULTI ? GMAX : 1][RECW];
+  __shared__ __attribute__((aligned(16))) unsigned s_rec[2][MULTI ? GMAX : 1][RW];
   __shared__ int s_piv[SW];
   __shared__ T s_pmag[SW];
 
@@ -118,7 +126,7 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
     const int e = wave * 64 * CHW + lane + 64 * i;
     const int rg = e < G * L1 ? e / L1 : 0;
     const int rw = e < G * L1 ? e % L1 : TOPW + (e - G * L1);
-    poff[i] = (MULTI && e < G * L1 + CW && rg != g) ? rg * RECW + rw : -1;
+    poff[i] = (MULTI && e < G * L1 + CW && rg != g) ? rg * RW + rw : -1;
     poller = poller || poff[i] >= 0;
   }
   poller = __any(poller);  // wave-uniform
@@ -228,7 +236,7 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
         if (wave == 0) {
           // publish: lane k writes word k of this workgroup's record (and
           // of the top row in workgroup 0), straight from LDS
-          unsigned long long *mine = recs + ((long)par * GMAX + g) * RECW;
+          unsigned long long *mine = recs + ((long)par * GMAX + g) * RW;
           unsigned *own = &s_rec[par][g][0];
           for (int k = lane; k < (g == 0 ? L1 + CW : L1); k += 64) {
             unsigned pl;
@@ -245,7 +253,7 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
           // poll the other records (and workgroup 0's top row) until every
           // word carries this column's sequence number; bounded, so a lost
           // record cannot hang the kernel (info then reports -1)
-          const unsigned long long *prec = recs + (long)par * GMAX * RECW;
+          const unsigned long long *prec = recs + (long)par * GMAX * RW;
           unsigned long long wv[CHW];
           bool ok = false;
           for (int spin = 0; !ok; ++spin) {
@@ -416,7 +424,8 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
 
 template <typename T>
 int panel_strip_width(int m) {
-  return sizeof(T) == 4 && m <= 32 * 2048 ? PW32 : PW;
+  if (sizeof(T) == 4) return m <= 32 * 2048 ? PW32 : PW;
+  return m <= 512 || (m > 1024 && m <= 18 * 512) ? PW32 : PW;
 }
 template int panel_strip_width<double>(int);
 template int panel_strip_width<float>(int);
@@ -453,6 +462,13 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
     if (G > pw.max_groups || !poll_ok(G, 256)) { NT = 512; G = (m + 2047) / 2048; }
     if (pivot && sizeof(T) == 4 && G > pw.max_groups) { R = 8; NT = 512; G = (m + 4095) / 4096; }
   }
+  if (sizeof(T) == 8 && sw == PW32) {
+    // FP64 32-column strips: 2 rows per thread, one workgroup up to 512
+    // rows, else 256-thread workgroups of 512 rows
+    if (!pivot) return -2;
+    R = 2; NT = 256; G = m <= 512 ? 1 : (m + 511) / 512;
+    if (G > 1 && m <= 1024) return -2;
+  }
   if (G > pw.max_groups || G > GMAX) return -2;
   if (G > 1 && !poll_ok(G, NT)) return -2;
   if (R == 8 && sw != PW) return -2;
@@ -484,6 +500,9 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   if (R == 8) {
     if (sizeof(T) != 4 || G <= 1) return -2;
     CLU_PANEL(float, 8, 512, true, true, PW);
+  } else if (sizeof(T) == 8 && sw == PW32) {
+    if (G > 1) CLU_PANEL(double, 2, 256, true, true, PW32);
+    else CLU_PANEL(double, 2, 256, false, true, PW32);
   } else if (sw == PW) {
     CLU_PANEL_ALL(T, PW)
   } else {
