@@ -439,16 +439,6 @@ static int dispatch(int M, int N, int K, T alpha, const T *A, long lda, const T 
   return launch<T, 128, 128, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
-int dgemm_variant(bool ta, bool tb, int M, int N, int K, const double *A, long lda,
-                  const double *B, long ldb) {
-  if (!ta && !tb && lds_path_ok(M, N, K, A, lda, B, ldb)) return 0;
-  if (N <= 32) return 1;
-  if (N <= 64) return 4;
-  if (M <= 32) return 3;
-  if ((long)((M + 127) / 128) * ((N + 127) / 128) < 512) return 4;
-  return 2;
-}
-
 template <typename T>
 static int gemm_any(bool ta, bool tb, int M, int N, int K, T alpha, const T *A, long lda,
                     const T *B, long ldb, T beta, T *C, long ldc, hipStream_t s, int role = 0) {

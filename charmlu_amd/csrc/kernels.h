@@ -57,12 +57,6 @@ int pack_rows(int M, int K, int Kp, float scale, const float *src, long lds, T16
 template <typename T16>
 int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s);
 
-// Kernel dgemm() launches for these operands: 0 = the direct-to-LDS 128x128
-// NN kernel (the trailing update), 1 = 256x32, 2 = 128x128, 3 = 32x256,
-// 4 = 64x64 register-staged kernels.
-int dgemm_variant(bool ta, bool tb, int M, int N, int K, const double *A, long lda,
-                  const double *B, long ldb);
-
 // ---- pivoting -------------------------------------------------------------
 // Row move list of a run of sequential swaps ipiv[k1..k2) (global rows):
 // moves[i] = {dst, src} means "row dst receives old row src"; count[0] is
@@ -166,8 +160,6 @@ int residual(int n, const double *A, long lda, const double *x, const double *b,
 template <typename TD, typename TS>
 int convert(long m, long n, const TS *src, long lds, TD *dst, long ldd, hipStream_t s);
 int fill_int(int *p, int v, long n, hipStream_t s);
-int set_identity(double *A, long n, long lda, hipStream_t s);
-int fill_u32(unsigned *p, unsigned v, long n, hipStream_t s);
 
 // ---- FP64 vector kernels of the iterative refinement ----------------------
 // h[0..k) = V w (V: k rows of n); w += alpha * V^T h; out = {||x||_2^2,
@@ -180,7 +172,5 @@ int vec_norms(long n, const double *x, double *out, hipStream_t s);
 int vec_scale(long n, double a, const double *x, double *y, hipStream_t s);
 int vec_d2s(long n, double a, const double *x, float *y, hipStream_t s);
 int vec_s2d(long n, double a, const float *x, double *y, hipStream_t s);
-template <typename T>
-int set_identity_t(T *A, long n, long lda, hipStream_t s);
 
 }  // namespace clu

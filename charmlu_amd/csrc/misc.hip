@@ -175,40 +175,7 @@ template int convert<double, float>(long, long, const float *, long, double *, l
 template int convert<double, double>(long, long, const double *, long, double *, long, hipStream_t);
 template int convert<float, float>(long, long, const float *, long, float *, long, hipStream_t);
 
-__global__ void set_identity_kernel(double *A, long n, long lda) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n * n) A[(e / n) * lda + e % n] = (e / n == e % n) ? 1.0 : 0.0;
-}
-
-int set_identity(double *A, long n, long lda, hipStream_t s) {
-  if (n <= 0) return CLU_OK;
-  hipLaunchKernelGGL(set_identity_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, s,
-                     A, n, lda);
-  CLU_LAUNCHED("set_identity_kernel");
-  return CLU_OK;
-}
-
-template <typename T>
-__global__ void set_identity_t_kernel(T *A, long n, long lda) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n * n) A[(e / n) * lda + e % n] = (e / n == e % n) ? (T)1 : (T)0;
-}
-template <typename T>
-int set_identity_t(T *A, long n, long lda, hipStream_t s) {
-  if (n <= 0) return CLU_OK;
-  hipLaunchKernelGGL((set_identity_t_kernel<T>), dim3((unsigned)((n * n + 255) / 256)), dim3(256),
-                     0, s, A, n, lda);
-  CLU_LAUNCHED("set_identity_kernel");
-  return CLU_OK;
-}
-template int set_identity_t<float>(float *, long, long, hipStream_t);
-template int set_identity_t<double>(double *, long, long, hipStream_t);
-
 __global__ void fill_int_kernel(int *p, int v, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-__global__ void fill_u32_kernel(unsigned *p, unsigned v, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -217,12 +184,6 @@ int fill_int(int *p, int v, long n, hipStream_t s) {
   if (n <= 0) return CLU_OK;
   hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
   CLU_LAUNCHED("fill_int_kernel");
-  return CLU_OK;
-}
-int fill_u32(unsigned *p, unsigned v, long n, hipStream_t s) {
-  if (n <= 0) return CLU_OK;
-  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
-  CLU_LAUNCHED("fill_u32_kernel");
   return CLU_OK;
 }
 
