@@ -65,14 +65,16 @@ int pivots_to_moves(const int *ipiv, int k1, int k2, int2 *moves, int *count,
                     hipStream_t s);
 // Inverse direction: row src receives old row dst (undoes the permutation).
 // Apply a move list to columns [c0, c1) of a row-major matrix, skipping the
-// columns [skip0, skip1).
+// columns [skip0, skip1); nruns > 1: the lists at moves + r * run_stride
+// (lengths count[r]) in order, in one launch (reverse order when inverse).
 template <typename T>
 int apply_moves(const int2 *moves, const int *count, int max_moves, T *A,
                 long lda, int c0, int c1, int skip0, int skip1, bool inverse,
-                hipStream_t s);
-// Move lists of every run of 256 pivots of ipiv[0..n) at once (one
+                hipStream_t s, int nruns = 1, long run_stride = 0);
+// Move lists of every run of 256 pivots of ipiv[k1..k2) at once (one
 // workgroup per run): run r at moves + r*512, its length at counts[r].
-int pivots_to_moves_all(const int *ipiv, int n, int2 *moves, int *counts, hipStream_t s);
+int pivots_to_moves_all(const int *ipiv, int k1, int k2, int2 *moves, int *counts,
+                        hipStream_t s);
 // the interchanges ipiv[0..n) (or their inverse) applied to one vector;
 // moves / counts: scratch for the move lists of all (n + 255) / 256 runs
 template <typename T>

@@ -408,26 +408,19 @@ static int history_prepare(PanelHistory &h, int n, int nb, int kind, int NB) {
   return CLU_OK;
 }
 
-// move lists of ipiv[k1, k2) in runs of RUN pivots
+// move lists of ipiv[k1, k2) in runs of RUN pivots: one launch of one
+// workgroup per run (one run: the single-run kernel)
 static int build_moves(const int *ipiv, int k1, int k2, int2 *moves, int *counts, hipStream_t s) {
-  for (int r = 0, k = k1; k < k2; ++r, k += RUN) {
-    const int ke = k2 - k < RUN ? k2 : k + RUN;
-    int rc = pivots_to_moves(ipiv, k, ke, moves + (size_t)r * 2 * RUN, counts + r, s);
-    if (rc) return rc;
-  }
-  return CLU_OK;
+  if (k2 - k1 <= RUN) return pivots_to_moves(ipiv, k1, k2, moves, counts, s);
+  return pivots_to_moves_all(ipiv, k1, k2, moves, counts, s);
 }
 
+// the runs' move lists applied in one launch
 template <typename T>
 static int apply_runs(int nruns, const int2 *moves, const int *counts, T *A, long lda, int c0,
                       int c1, int skip0, int skip1, bool inverse, hipStream_t s) {
-  for (int i = 0; i < nruns; ++i) {
-    const int r = inverse ? nruns - 1 - i : i;
-    int rc = apply_moves<T>(moves + (size_t)r * 2 * RUN, counts + r, 2 * RUN, A, lda, c0, c1,
-                            skip0, skip1, inverse, s);
-    if (rc) return rc;
-  }
-  return CLU_OK;
+  return apply_moves<T>(moves, counts, 2 * RUN, A, lda, c0, c1, skip0, skip1, inverse, s, nruns,
+                        2L * RUN);
 }
 
 
@@ -493,13 +486,11 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
   const bool vec = nrhs == 1 && ldb == 1;
   auto interchanges = [&](bool inverse) -> int {
     if (vec) return permute_vec<T>(dipiv, n, dB, inverse, vmoves, vcounts, s);
-    // every run's move list in one launch, then the runs in order
-    int e = pivots_to_moves_all(dipiv, n, vmoves, vcounts, s);
-    for (int i = 0; !e && i < nruns; ++i) {
-      const int r = inverse ? nruns - 1 - i : i;
-      e = apply_moves<T>(vmoves + (size_t)r * 2 * RUN, vcounts + r, 2 * RUN, dB, ldb, 0, nrhs,
-                         nrhs, nrhs, inverse, s);
-    }
+    // every run's move list in one launch, then all runs in order in another
+    int e = pivots_to_moves_all(dipiv, 0, n, vmoves, vcounts, s);
+    if (!e)
+      e = apply_moves<T>(vmoves, vcounts, 2 * RUN, dB, ldb, 0, nrhs, nrhs, nrhs, inverse, s, nruns,
+                         2L * RUN);
     return e;
   };
   // dst = op(G) src for the inverse G of one diagonal block

@@ -644,30 +644,85 @@ __global__ __launch_bounds__(256) void apply_moves_kernel(
   }
 }
 
+// Several runs' move lists in one launch: each workgroup applies them to
+// its own columns in order (reverse order for the inverse permutation), a
+// barrier between runs (its columns belong to it alone).  Kept apart from
+// the one-run kernel, which needs fewer registers.
+template <typename T, int CW>
+__global__ __launch_bounds__(256) void apply_runs_kernel(
+    const int2 *__restrict__ moves, const int *__restrict__ counts, int nruns, long mstride,
+    T *__restrict__ A, long lda, int c0, int c1, int skip0, int skip1, int inverse) {
+  constexpr int RPP = 256 / CW;          // rows per pass
+  constexpr int NP = 2 * MAXRUN / RPP;   // passes for the longest list
+  __shared__ int2 s_mv[2 * MAXRUN];
+  const int t = threadIdx.x;
+  const int left = skip0 - c0;  // columns before the skipped range
+  const int logical = blockIdx.x * CW + t % CW;
+  const int col = logical < left ? c0 + logical : logical - left + skip1;
+  const bool colok = col < c1;
+  const int r0 = t / CW;
+  for (int i = 0; i < nruns; ++i) {
+    const int run = inverse ? nruns - 1 - i : i;
+    const int cnt = counts[run];
+    const int2 *mv = moves + (long)run * mstride;
+    if (i > 0) __syncthreads();  // the previous run's writes and list reads are done
+    for (int k = t; k < cnt; k += 256) s_mv[k] = mv[k];
+    __syncthreads();
+    T v[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int e = p * RPP + r0;
+      if (colok && e < cnt) {
+        const int2 m2 = s_mv[e];
+        v[p] = A[(long)(inverse ? m2.x : m2.y) * lda + col];
+      }
+    }
+    wait_all_mem();
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int e = p * RPP + r0;
+      if (colok && e < cnt) {
+        const int2 m2 = s_mv[e];
+        A[(long)(inverse ? m2.y : m2.x) * lda + col] = v[p];
+      }
+    }
+    wait_all_mem();
+  }
+}
+
 template <typename T>
 int apply_moves(const int2 *moves, const int *count, int max_moves, T *A, long lda,
-                int c0, int c1, int skip0, int skip1, bool inverse, hipStream_t s) {
+                int c0, int c1, int skip0, int skip1, bool inverse, hipStream_t s,
+                int nruns, long run_stride) {
   // normalise the skipped range to lie inside [c0, c1]
   skip0 = max(c0, min(skip0, c1));
   skip1 = max(skip0, min(skip1, c1));
   const int ncols = (c1 - c0) - (skip1 - skip0);
-  if (ncols <= 0 || max_moves <= 0) return CLU_OK;
+  if (ncols <= 0 || max_moves <= 0 || nruns <= 0) return CLU_OK;
   constexpr int CW = 128 / (int)sizeof(T);
   if (max_moves > 2 * MAXRUN) return -1;
+  if (nruns > 1 && run_stride < max_moves) return -1;
   const int nchunks = (ncols + CW - 1) / CW;
-  hipLaunchKernelGGL((apply_moves_kernel<T, CW>), dim3(nchunks), dim3(256), 0, s, moves, count,
-                     A, lda, c0, c1, skip0, skip1, inverse ? 1 : 0);
+  if (nruns == 1)
+    hipLaunchKernelGGL((apply_moves_kernel<T, CW>), dim3(nchunks), dim3(256), 0, s, moves, count,
+                       A, lda, c0, c1, skip0, skip1, inverse ? 1 : 0);
+  else
+    hipLaunchKernelGGL((apply_runs_kernel<T, CW>), dim3(nchunks), dim3(256), 0, s, moves, count,
+                       nruns, run_stride, A, lda, c0, c1, skip0, skip1, inverse ? 1 : 0);
   CLU_LAUNCHED("apply_moves_kernel");
   return CLU_OK;
 }
 
-// One workgroup per run of MAXRUN pivots: the run's move list, as
-// pivots_to_moves_kernel computes it, at moves + r * 2 * MAXRUN.
+// One workgroup per run of MAXRUN pivots of ipiv[k0, kend): run r covers
+// [k0 + r * MAXRUN, ...), its move list (as pivots_to_moves_kernel computes
+// it) at moves + r * 2 * MAXRUN.
 __global__ __launch_bounds__(2 * MAXRUN) void pivots_to_moves_all_kernel(
-    const int *__restrict__ ipiv, int n, int2 *__restrict__ moves, int *__restrict__ counts) {
+    const int *__restrict__ ipiv, int k0, int kend, int2 *__restrict__ moves,
+    int *__restrict__ counts) {
   __shared__ __attribute__((aligned(16))) int piv[MAXRUN];
   __shared__ int wave_cnt[2 * MAXRUN / 64];
-  const int r = blockIdx.x, k1 = r * MAXRUN, k2 = min(n, k1 + MAXRUN), nk = k2 - k1;
+  const int r = blockIdx.x, k1 = k0 + r * MAXRUN, k2 = min(kend, k1 + MAXRUN), nk = k2 - k1;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   if (t < MAXRUN) piv[t] = t < nk ? ipiv[k1 + t] : k1 + t;
   __syncthreads();
@@ -688,11 +743,12 @@ __global__ __launch_bounds__(2 * MAXRUN) void pivots_to_moves_all_kernel(
   }
 }
 
-int pivots_to_moves_all(const int *ipiv, int n, int2 *moves, int *counts, hipStream_t s) {
-  if (n <= 0) return CLU_OK;
-  const int nruns = (n + MAXRUN - 1) / MAXRUN;
-  hipLaunchKernelGGL(pivots_to_moves_all_kernel, dim3(nruns), dim3(2 * MAXRUN), 0, s, ipiv, n,
-                     moves, counts);
+int pivots_to_moves_all(const int *ipiv, int k1, int k2, int2 *moves, int *counts,
+                        hipStream_t s) {
+  if (k2 <= k1) return CLU_OK;
+  const int nruns = (k2 - k1 + MAXRUN - 1) / MAXRUN;
+  hipLaunchKernelGGL(pivots_to_moves_all_kernel, dim3(nruns), dim3(2 * MAXRUN), 0, s, ipiv, k1,
+                     k2, moves, counts);
   CLU_LAUNCHED("pivots_to_moves_all_kernel");
   return CLU_OK;
 }
@@ -736,7 +792,7 @@ template <typename T>
 int permute_vec(const int *ipiv, int n, T *v, bool inverse, int2 *moves, int *counts,
                 hipStream_t s) {
   if (n <= 0) return CLU_OK;
-  int rc = pivots_to_moves_all(ipiv, n, moves, counts, s);
+  int rc = pivots_to_moves_all(ipiv, 0, n, moves, counts, s);
   if (rc) return rc;
   const int nruns = (n + MAXRUN - 1) / MAXRUN;
   hipLaunchKernelGGL((permute_vec_kernel<T>), dim3(1), dim3(2 * MAXRUN), 0, s, moves, counts,
@@ -748,8 +804,8 @@ template int permute_vec<double>(const int *, int, double *, bool, int2 *, int *
 template int permute_vec<float>(const int *, int, float *, bool, int2 *, int *, hipStream_t);
 
 template int apply_moves<double>(const int2 *, const int *, int, double *, long, int, int,
-                                 int, int, bool, hipStream_t);
+                                 int, int, bool, hipStream_t, int, long);
 template int apply_moves<float>(const int2 *, const int *, int, float *, long, int, int,
-                                int, int, bool, hipStream_t);
+                                int, int, bool, hipStream_t, int, long);
 
 }  // namespace clu
