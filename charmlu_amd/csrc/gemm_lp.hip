@@ -351,24 +351,45 @@ CLU_DEV unsigned quant4(float a, float b, float c, float d) {
 __global__ __launch_bounds__(256) void quant_rows_fp8_kernel(int M, int K, int Kp, float sign,
                                                              const float *__restrict__ src,
                                                              long lds, uint8_t *__restrict__ dst,
-                                                             int *__restrict__ esc) {
+                                                             int *__restrict__ esc, int vec) {
+  // rows one after another, 4 consecutive k per thread (a 16-byte load when
+  // the source rows are 16-byte aligned): no index division per element
   __shared__ float red[4];
   const int g = blockIdx.x, t = threadIdx.x, r0 = g * 128;
   const int rows = min(128, M - r0);
+  auto load4 = [&](const float *sp, int k, float (&v)[4]) {
+    if (vec && k + 3 < K) {
+      const float4_t f = *reinterpret_cast<const float4_t *>(sp + k);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = f[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sp[k + q] : 0.f;
+    }
+  };
   float mx = 0.f;
-  for (int e = t; e < rows * K; e += 256)
-    mx = fmaxf(mx, fabsf(src[(long)(r0 + e / K) * lds + e % K]));
+  for (int i = 0; i < rows; ++i) {
+    const float *sp = src + (long)(r0 + i) * lds;
+    for (int k = 4 * t; k < K; k += 1024) {
+      float v[4];
+      load4(sp, k, v);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mx = fmaxf(mx, fabsf(v[q]));
+    }
+  }
   mx = block_max_256(mx, red);
   const int ex = tile_exponent(mx);
   if (t == 0) esc[g] = ex;
-  const int kq = Kp / 4;
-  for (int e = t; e < rows * kq; e += 256) {
-    const int i = e / kq, k = 4 * (e % kq);
+  for (int i = 0; i < rows; ++i) {
     const float *sp = src + (long)(r0 + i) * lds;
-    float v[4];
+    for (int k = 4 * t; k < Kp; k += 1024) {
+      float v[4];
+      load4(sp, k, v);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sign * ldexpf(sp[k + q], ex) : 0.f;
-    *reinterpret_cast<unsigned *>(dst + (long)(r0 + i) * Kp + k) = quant4(v[0], v[1], v[2], v[3]);
+      for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sign * ldexpf(v[q], ex) : 0.f;
+      *reinterpret_cast<unsigned *>(dst + (long)(r0 + i) * Kp + k) =
+          quant4(v[0], v[1], v[2], v[3]);
+    }
   }
 }
 
@@ -418,8 +439,9 @@ int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds,
                    int *esc, hipStream_t s) {
   if (M <= 0) return CLU_OK;
   if (Kp % 64 != 0 || Kp < K) return -1;
+  const int vec = lds % 4 == 0 && ((uintptr_t)src & 15) == 0;
   hipLaunchKernelGGL(quant_rows_fp8_kernel, dim3((M + 127) / 128), dim3(256), 0, s, M, K, Kp,
-                     sign, src, lds, dst, esc);
+                     sign, src, lds, dst, esc, vec);
   CLU_LAUNCHED("quant_rows_fp8_kernel");
   return CLU_OK;
 }
