@@ -14,6 +14,7 @@ typedef float float16_t __attribute__((ext_vector_type(16)));
 #define CLU_OK 0
 #define CLU_ERR_HIP (-100)
 #define CLU_ERR_NOMEM (-101)
+#define CLU_ERR_CONFIG (-102)
 
 namespace clu {
 

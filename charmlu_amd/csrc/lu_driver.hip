@@ -55,7 +55,12 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 // ---------------------------------------------------------------------------
 // per-device workspace
 // ---------------------------------------------------------------------------
-constexpr int MAX_GROUPS = 256;  // panel workgroups (rows / 2048)
+// Workgroups of one multi-workgroup panel strip, at most: every one of them
+// must be resident at once, and no strip of more than 32 has been run (a
+// mixed-form run with 48 panel CUs, whose 256-thread strips took up to 36
+// workgroups, stopped making progress; not analysed).
+constexpr int MAX_GROUPS = 32;
+constexpr int MAX_PANEL_CUS = 32;
 constexpr int RUN = 256;         // pivots per move list
 
 struct PanelHistory {
@@ -182,8 +187,16 @@ static int get_ws(Workspace **out) {
     CLU_HIP(hipGetDeviceProperties(&prop, dev));
     const int ncu = prop.multiProcessorCount;
     int reserve = 32;
-    if (const char *e = getenv("CHARMLU_PANEL_CUS")) reserve = atoi(e);
-    if (reserve < 0 || reserve >= ncu) reserve = 0;
+    if (const char *e = getenv("CHARMLU_PANEL_CUS")) {
+      char *end = nullptr;
+      const long v = strtol(e, &end, 10);
+      if (end == e || *end != '\0' || v < 0 || v > MAX_PANEL_CUS) {
+        set_error("CHARMLU_PANEL_CUS=%s: expected a CU count from 0 to %d", e, MAX_PANEL_CUS);
+        return CLU_ERR_CONFIG;
+      }
+      reserve = (int)v;
+    }
+    if (reserve >= ncu) reserve = 0;
     w.s_main = w.s_panel = nullptr;
     if (reserve > 0) {
       std::vector<uint32_t> mm((ncu + 31) / 32, 0u), pm((ncu + 31) / 32, 0u);

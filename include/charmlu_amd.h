@@ -19,8 +19,9 @@
  *   - stream is a hipStream_t passed as void* (NULL = the null stream).
  *   - Return value: 0 on success, -i if argument i is invalid (LAPACK
  *     style), CHARMLU_ERR_HIP if a HIP call failed, CHARMLU_ERR_NOMEM if the
- *     library could not allocate its workspace.  charmlu_last_error() gives
- *     a message.  Factorization status (singular pivot) is reported through
+ *     library could not allocate its workspace, CHARMLU_ERR_CONFIG if an
+ *     environment setting of the library is out of range (CHARMLU_PANEL_CUS:
+ *     0 to 32).  charmlu_last_error() gives a message.  Factorization status (singular pivot) is reported through
  *     an info word, never by aborting (the reference aborts, lu.ci:339-341).
  *   - Device-resident calls on one device share that device's workspace:
  *     calls that overlap in time on one device race.
@@ -37,6 +38,7 @@ extern "C" {
 #define CHARMLU_OK 0
 #define CHARMLU_ERR_HIP (-100)
 #define CHARMLU_ERR_NOMEM (-101)
+#define CHARMLU_ERR_CONFIG (-102)
 
 /* CBLAS enumerations (public CBLAS values) */
 #define CHARMLU_ROW_MAJOR 101

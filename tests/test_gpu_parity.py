@@ -363,3 +363,25 @@ def test_forced_split_matches_oracle_and_unsplit(dev, orc, n, b):
     assert torch.equal(LU1, LU2)
     o = orc.lu_solve_problem(n, b, 0, nthreads=8, A=A.cpu().numpy().copy())
     assert np.array_equal(ip2.cpu().numpy(), o["ipiv"])
+
+
+@pytest.mark.parametrize("value", ["48", "-1", "x"])
+def test_panel_cus_out_of_range_is_an_error(value):
+    """CHARMLU_PANEL_CUS takes 0 to 32: any other value makes the first call
+    on a device fail with CHARMLU_ERR_CONFIG and a message, before any
+    stream or workspace is made (a fresh process: the workspace is made once
+    per device)."""
+    import subprocess
+    import sys
+    code = ("import torch, charmlu_amd as c\n"
+            "A = torch.eye(64, dtype=torch.float64, device='cuda')\n"
+            "try:\n"
+            "    c.getrf(A, 32)\n"
+            "except c.CharmluError as e:\n"
+            "    print('ERR', e)\n")
+    env = dict(os.environ, CHARMLU_PANEL_CUS=value)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], env=env, cwd=root, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "ERR" in r.stdout and "-102" in r.stdout and "CHARMLU_PANEL_CUS" in r.stdout, r.stdout
