@@ -426,9 +426,11 @@ static int dispatch(int M, int N, int K, T alpha, const T *A, long lda, const T 
     CLU_LAUNCHED("dgemm_nn_lds_kernel");
     return CLU_OK;
   }
-  // narrow right-hand sides (panel-internal updates, multi-RHS solves)
+  // narrow right-hand sides (panel-internal updates, multi-RHS solves):
+  // 64-row tiles, four times the workgroups of 256-row ones (these updates
+  // are memory- and latency-bound, and the panel's run on 32 CUs)
   if (N <= 32)
-    return launch<T, 256, 32, 4, 1, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch<T, 64, 32, 4, 1, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   if (N <= 64)
     return launch<T, 64, 64, 2, 2, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   if (M <= 32)
