@@ -70,6 +70,8 @@ def _gemm_tol(A, B, C, alpha, beta):
 
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (128, 128, 256), (300, 200, 256),
                                    (1000, 64, 256), (64, 1000, 32), (257, 31, 100),
+                                   # narrow products (64 x 32 tiles): ragged M and K
+                                   (1000, 16, 16), (4097, 32, 33),
                                    (2048, 2048, 256), (513, 1031, 77),
                                    # >= 512 full 128x128 tiles: the direct-to-LDS path
                                    (4096, 2048, 256), (2816, 3072, 264)])
@@ -92,6 +94,34 @@ def test_dgemm_dev(dev, M, N, K, ta, tb):
     _sync()
     got = dC.cpu().numpy()
     assert np.all(np.abs(got - ref) <= _gemm_tol(opA, opB, C, alpha, beta))
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (1000, 16, 16), (4097, 32, 33), (777, 64, 40),
+                                   (20, 300, 64), (300, 200, 256), (1030, 1100, 100)])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_sgemm_dev(dev, M, N, K, ta, tb):
+    """charmlu_sgemm_dev on every tile shape it dispatches to (N <= 32,
+    N <= 64, M <= 32, small and large grids), against a float64 product of
+    the same FP32 inputs: |C - C_ref| <= 8 K u32 (|A||B| + |C|)."""
+    from charmlu_amd import _lib
+    rng = np.random.default_rng(M * 5 + N * 11 + K)
+    A = rng.uniform(-0.5, 0.5, (K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.uniform(-0.5, 0.5, (N, K) if tb else (K, N)).astype(np.float32)
+    C = rng.uniform(-0.5, 0.5, (M, N)).astype(np.float32)
+    alpha, beta = -1.0, 1.0
+    opA = (A.T if ta else A).astype(np.float64)
+    opB = (B.T if tb else B).astype(np.float64)
+    ref = alpha * opA @ opB + beta * C.astype(np.float64)
+    dA, dB, dC = _t(A), _t(B), _t(C)
+    rc = _lib.load().charmlu_sgemm_dev(_lib.TRANS if ta else _lib.NO_TRANS,
+                                       _lib.TRANS if tb else _lib.NO_TRANS, M, N, K, alpha, _p(dA),
+                                       A.shape[1], _p(dB), B.shape[1], beta, _p(dC), N, None)
+    assert rc == 0
+    _sync()
+    got = dC.cpu().numpy().astype(np.float64)
+    u32 = float(np.finfo(np.float32).eps)
+    tol = 8 * max(K, 1) * u32 * (np.abs(opA) @ np.abs(opB) + np.abs(C)) + 1e-30
+    assert np.all(np.abs(got - ref) <= tol)
 
 
 def test_dgemm_beta_zero_ignores_nan(dev):
