@@ -415,3 +415,38 @@ def test_panel_cus_out_of_range_is_an_error(value):
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "ERR" in r.stdout and "-102" in r.stdout and "CHARMLU_PANEL_CUS" in r.stdout, r.stdout
+
+
+def test_panel_cus_zero_is_bitwise_the_default(dev, orc, tmp_path):
+    """CHARMLU_PANEL_CUS=0 puts every stream on all CUs (no CU masks, no
+    split update, no in-panel GEMMs on a second stream): the factors are bit
+    for bit those of the default layout, for FP64 and the mixed BF16 form
+    (a fresh process: the streams are made once per device)."""
+    import subprocess
+    import sys
+    code = ("import sys, numpy as np, torch, charmlu_amd as c\n"
+            "A = c.gen_matrix(4096, seed=%d)\n"
+            "ip, info = c.getrf(A, 256)\n"
+            "S = c.gen_matrix(8192, seed=5).float()\n"
+            "ips, infos = c.getrf_mixed(S, 1024, 'bf16')\n"
+            "torch.cuda.synchronize()\n"
+            "np.savez(sys.argv[1], A=A.cpu().numpy(), ip=ip.cpu().numpy(), S=S.cpu().numpy(),\n"
+            "         ips=ips.cpu().numpy(), info=[int(info.item()), int(infos.item())])\n"
+            % orc.SEED_A)
+    out = str(tmp_path / "zero.npz")
+    env = dict(os.environ, CHARMLU_PANEL_CUS="0")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code, out], env=env, cwd=root, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    z = np.load(out)
+    A = dev.gen_matrix(4096, seed=orc.SEED_A)
+    ip, info = dev.getrf(A, 256)
+    S = dev.gen_matrix(8192, seed=5).float()
+    ips, infos = dev.getrf_mixed(S, 1024, "bf16")
+    _sync()
+    assert list(z["info"]) == [int(info.item()), int(infos.item())] == [0, 0]
+    assert np.array_equal(z["ip"], ip.cpu().numpy())
+    assert np.array_equal(z["A"], A.cpu().numpy())
+    assert np.array_equal(z["ips"], ips.cpu().numpy())
+    assert np.array_equal(z["S"], S.cpu().numpy())
