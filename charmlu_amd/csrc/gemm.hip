@@ -12,7 +12,9 @@
 // chunk.  FP64 LDS tiles are stored k-major (As[k][m], Bs[k][n]) so that
 // the 16 lanes that share an MFMA k index read 16 consecutive doubles; FP32
 // tiles k-minor with rows of 17 (bank-conflict-free stores and fragment
-// reads: 94-99 -> 97-104 TFLOP/s at 1024 x 64512 x 1024 and 65024 x 512 x 512).
+// reads), and FP32 NN operands with aligned rows are fetched 16 bytes per
+// lane: 94-99 -> 102-110 TFLOP/s at 1024 x 64512 x 1024, 65024 x 512 x 512
+// and 8192^2 x 1024, 77 -> 90 at 64512 x 256 x 256.
 //
 // MFMA 16x16x4 f64 operand layout (per lane l, 0..63):
 //   A[i][k] from lane i + 16k,  B[k][j] from lane j + 16k,
@@ -157,7 +159,35 @@ __global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(
   const int lr = lane & 15, lq = lane >> 4;
 
   T ra[A_PER], rb[B_PER];
+  // FP32 NN with 16-byte aligned rows and K, N multiples of 4: each thread
+  // loads 4 consecutive values per instruction (a quarter of the loads)
+  constexpr bool V4 = sizeof(T) == 4 && !TA && !TB && A_PER % 4 == 0 && B_PER % 4 == 0;
+  const bool vec = V4 && ((lda | ldb | K | N) & 3) == 0 &&
+                   (((unsigned long)A | (unsigned long)B) & 15) == 0;
   auto load = [&](int k0) {
+    if constexpr (V4) {
+      if (vec) {
+#pragma unroll
+        for (int r = 0; r < A_PER / 4; ++r) {
+          const int e = t + 256 * r, m = e / (GBK / 4), k = 4 * (e % (GBK / 4));
+          const int gm = m0 + m, gk = k0 + k;
+          const float4 v = (gm < M && gk < K)
+                               ? *reinterpret_cast<const float4 *>(&A[(long)gm * lda + gk])
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+          ra[4 * r] = v.x; ra[4 * r + 1] = v.y; ra[4 * r + 2] = v.z; ra[4 * r + 3] = v.w;
+        }
+#pragma unroll
+        for (int r = 0; r < B_PER / 4; ++r) {
+          const int e = t + 256 * r, k = e / (BN / 4), n = 4 * (e % (BN / 4));
+          const int gn = n0 + n, gk = k0 + k;
+          const float4 v = (gn < N && gk < K)
+                               ? *reinterpret_cast<const float4 *>(&B[(long)gk * ldb + gn])
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+          rb[4 * r] = v.x; rb[4 * r + 1] = v.y; rb[4 * r + 2] = v.z; rb[4 * r + 3] = v.w;
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < A_PER; ++r) {
       const int e = t + 256 * r;
@@ -180,6 +210,21 @@ __global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(
     }
   };
   auto stash = [&](int buf) {
+    if constexpr (V4) {
+      if (vec) {
+#pragma unroll
+        for (int r = 0; r < A_PER; ++r) {
+          const int e = t + 256 * (r / 4), m = e / (GBK / 4), k = 4 * (e % (GBK / 4)) + r % 4;
+          As[buf][ai(k, m)] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < B_PER; ++r) {
+          const int e = t + 256 * (r / 4), k = e / (BN / 4), n = 4 * (e % (BN / 4)) + r % 4;
+          Bs[buf][bi(k, n)] = rb[r];
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < A_PER; ++r) {
       const int e = t + 256 * r;
