@@ -12,9 +12,8 @@
 // chunk.  FP64 LDS tiles are stored k-major (As[k][m], Bs[k][n]) so that
 // the 16 lanes that share an MFMA k index read 16 consecutive doubles; FP32
 // tiles k-minor with rows of 17 (bank-conflict-free stores and fragment
-// reads), and FP32 NN operands with aligned rows are fetched 16 bytes per
-// lane: 94-99 -> 102-110 TFLOP/s at 1024 x 64512 x 1024, 65024 x 512 x 512
-// and 8192^2 x 1024, 77 -> 90 at 64512 x 256 x 256.
+// reads); NN operands with aligned rows are fetched 16 bytes per lane (FP32: 94-99 -> 102-110 TFLOP/s at 1024 x 64512 x 1024, 65024 x 512 x 512
+// and 8192^2 x 1024, 77 -> 90 at 64512 x 256 x 256).
 //
 // MFMA 16x16x4 f64 operand layout (per lane l, 0..63):
 //   A[i][k] from lane i + 16k,  B[k][j] from lane j + 16k,
@@ -159,31 +158,35 @@ __global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(
   const int lr = lane & 15, lq = lane >> 4;
 
   T ra[A_PER], rb[B_PER];
-  // FP32 NN with 16-byte aligned rows and K, N multiples of 4: each thread
-  // loads 4 consecutive values per instruction (a quarter of the loads)
-  constexpr bool V4 = sizeof(T) == 4 && !TA && !TB && A_PER % 4 == 0 && B_PER % 4 == 0;
-  const bool vec = V4 && ((lda | ldb | K | N) & 3) == 0 &&
+  // NN with 16-byte aligned rows and K, N multiples of VE: each thread
+  // loads VE consecutive values per instruction (4 floats or 2 doubles)
+  constexpr int VE = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(VE)));
+  // (FP64 128x128 tiles would spill: they keep the 8-byte loads)
+  constexpr bool V4 = !TA && !TB && A_PER % VE == 0 && B_PER % VE == 0 &&
+                      (sizeof(T) == 4 || BM * BN <= 64 * 64);
+  const bool vec = V4 && ((lda | ldb | K | N) & (VE - 1)) == 0 &&
                    (((unsigned long)A | (unsigned long)B) & 15) == 0;
   auto load = [&](int k0) {
     if constexpr (V4) {
       if (vec) {
 #pragma unroll
-        for (int r = 0; r < A_PER / 4; ++r) {
-          const int e = t + 256 * r, m = e / (GBK / 4), k = 4 * (e % (GBK / 4));
+        for (int r = 0; r < A_PER / VE; ++r) {
+          const int e = t + 256 * r, m = e / (GBK / VE), k = VE * (e % (GBK / VE));
           const int gm = m0 + m, gk = k0 + k;
-          const float4 v = (gm < M && gk < K)
-                               ? *reinterpret_cast<const float4 *>(&A[(long)gm * lda + gk])
-                               : make_float4(0.f, 0.f, 0.f, 0.f);
-          ra[4 * r] = v.x; ra[4 * r + 1] = v.y; ra[4 * r + 2] = v.z; ra[4 * r + 3] = v.w;
+          vec_t v = (vec_t)(T)0;
+          if (gm < M && gk < K) v = *reinterpret_cast<const vec_t *>(&A[(long)gm * lda + gk]);
+#pragma unroll
+          for (int j = 0; j < VE; ++j) ra[VE * r + j] = v[j];
         }
 #pragma unroll
-        for (int r = 0; r < B_PER / 4; ++r) {
-          const int e = t + 256 * r, k = e / (BN / 4), n = 4 * (e % (BN / 4));
+        for (int r = 0; r < B_PER / VE; ++r) {
+          const int e = t + 256 * r, k = e / (BN / VE), n = VE * (e % (BN / VE));
           const int gn = n0 + n, gk = k0 + k;
-          const float4 v = (gn < N && gk < K)
-                               ? *reinterpret_cast<const float4 *>(&B[(long)gk * ldb + gn])
-                               : make_float4(0.f, 0.f, 0.f, 0.f);
-          rb[4 * r] = v.x; rb[4 * r + 1] = v.y; rb[4 * r + 2] = v.z; rb[4 * r + 3] = v.w;
+          vec_t v = (vec_t)(T)0;
+          if (gn < N && gk < K) v = *reinterpret_cast<const vec_t *>(&B[(long)gk * ldb + gn]);
+#pragma unroll
+          for (int j = 0; j < VE; ++j) rb[VE * r + j] = v[j];
         }
         return;
       }
@@ -214,12 +217,12 @@ __global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(
       if (vec) {
 #pragma unroll
         for (int r = 0; r < A_PER; ++r) {
-          const int e = t + 256 * (r / 4), m = e / (GBK / 4), k = 4 * (e % (GBK / 4)) + r % 4;
+          const int e = t + 256 * (r / VE), m = e / (GBK / VE), k = VE * (e % (GBK / VE)) + r % VE;
           As[buf][ai(k, m)] = ra[r];
         }
 #pragma unroll
         for (int r = 0; r < B_PER; ++r) {
-          const int e = t + 256 * (r / 4), k = e / (BN / 4), n = 4 * (e % (BN / 4)) + r % 4;
+          const int e = t + 256 * (r / VE), k = e / (BN / VE), n = VE * (e % (BN / VE)) + r % VE;
           Bs[buf][bi(k, n)] = rb[r];
         }
         return;
