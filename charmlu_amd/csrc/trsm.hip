@@ -464,13 +464,26 @@ __global__ __launch_bounds__(256) void gemv_sub_nv_kernel(int M, int K, const T 
   const int sub = threadIdx.x & 15;
   double s = 0.0;
   if (row < M) {
+    // two loads in flight and two independent sums per lane
     const vec_t *a = reinterpret_cast<const vec_t *>(A + (long)row * lda);
     const vec_t *xv = reinterpret_cast<const vec_t *>(x);
-    for (int k = sub; k < K / V; k += 16) {
-      const vec_t av = a[k], xx = xv[k];
+    const int KV = K / V;
+    double s1 = 0.0;
+    int k = sub;
+    for (; k + 16 < KV; k += 32) {
+      const vec_t a0 = a[k], a1 = a[k + 16], x0 = xv[k], x1 = xv[k + 16];
 #pragma unroll
-      for (int v = 0; v < V; ++v) s += (double)av[v] * (double)xx[v];
+      for (int v = 0; v < V; ++v) {
+        s += (double)a0[v] * (double)x0[v];
+        s1 += (double)a1[v] * (double)x1[v];
+      }
     }
+    if (k < KV) {
+      const vec_t a0 = a[k], x0 = xv[k];
+#pragma unroll
+      for (int v = 0; v < V; ++v) s += (double)a0[v] * (double)x0[v];
+    }
+    s += s1;
   }
 #pragma unroll
   for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
