@@ -444,9 +444,8 @@ static int apply_runs(int nruns, const int2 *moves, const int *counts, T *A, lon
 // part of each block step is one small matrix-vector (or matrix) product
 // with the block's inverse instead of an S-step substitution; the block's
 // contribution is then subtracted from every row still to be solved (a GEMV
-// for one right-hand side, a GEMM for several).  S = 256, 512 for FP64
-// systems of 16384 and more (n = 32768: 3.48 -> 3.12 ms per vector solve,
-// half the launches); FP32 systems of 8192 and more use S = 1024: a vector solve is two launches per block, and
+// for one right-hand side, a GEMM for several).  S = 256; FP32 systems of
+// 8192 and more use S = 1024: a vector solve is two launches per block, and
 // at n = 65536 its 1024 launches of 256-row blocks spent a third of the
 // solve between kernels (the FP32 solve is the preconditioner of the
 // refinement, which corrects its rounding).
@@ -469,7 +468,7 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
   int rc = get_ws(&wsp);
   if (rc) return rc;
   Workspace &ws = *wsp;
-  const int S = sizeof(T) == 4 && n >= 8192 ? 1024 : (n >= 16384 ? 512 : 256);
+  const int S = sizeof(T) == 4 && n >= 8192 ? 1024 : 256;
   const int nblk = (n + S - 1) / S;
   const int nruns = (n + RUN - 1) / RUN;
   {
