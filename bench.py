@@ -18,7 +18,9 @@ The JSON line also carries:
                 matrix peak of 78.6 TFLOP/s.
   cpu_baseline  the CPU oracle (a restatement of the reference's block
                 algorithm with the vendor BLAS per block) timed on this host
-                on a bounded sample, in the same unit.
+                on a bounded sample of the same workload (its first block
+                steps), in the same unit, with C1 in full and the vendor
+                LAPACK's LU beside it; core count and CPU model included.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--nb B]
 """
@@ -37,28 +39,109 @@ FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 matrix (dense)
 HBM_PEAK = 8000.0  # GB/s
 
 
-def cpu_baseline(n: int, nb: int, threads: int, timeout: float = 240.0):
-    """Time the CPU oracle (vendor BLAS per block when MKL is present) on a
-    bounded sample in a child process; returns the cpu_baseline object."""
-    code = f"""
-import json, oracle
-mkl = oracle.use_mkl()
-r = oracle.lu_solve_problem({n}, {nb}, 0, nthreads={threads})
-print(json.dumps(dict(seconds=r['seconds'], residual=r['residual'], mkl=mkl)))
-"""
+def host_cpu():
+    """(threads to use, nproc, CPU model): the threads this process may run
+    on, capped by OMP_NUM_THREADS where the host sets it (a GPU host grants
+    each GPU a share of its cores)."""
+    nproc = os.cpu_count() or 1
     try:
-        out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True,
-                             text=True, timeout=timeout)
-        rec = json.loads(out.stdout.strip().splitlines()[-1])
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = nproc
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(avail, int(omp)) if omp.isdigit() and int(omp) > 0 else avail
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, nproc, model
+
+
+# The restatement's run on the host: C1 in full, and the first SAMPLE_STEPS
+# block steps of the single-GPU configuration (the most expensive ones) as
+# its bounded sample; then, in a second process (the vendor library's
+# threading is chosen once per process), the vendor LAPACK's multithreaded
+# LU + solve at C1 and at n = 16384.
+_PORT_CODE = """
+import json, sys, oracle
+T, n, nb, steps = (int(a) for a in sys.argv[1:5])
+mkl = oracle.use_mkl()
+c1 = oracle.lu_solve_problem(4096, 128, 0, nthreads=T)
+A = oracle.gen_uniform(oracle.SEED_A, n)
+oracle.set_max_steps(steps)
+t0 = oracle.wtime()
+oracle.getrf(A, nb, 0, None, T)
+sec = oracle.wtime() - t0
+print(json.dumps(dict(mkl=mkl, c1_seconds=c1['seconds'], c1_residual=c1['residual'],
+                      c1_info=c1['info'], sample_seconds=sec)))
+"""
+_LAPACK_CODE = """
+import json, sys, oracle
+T = int(sys.argv[1])
+out = {}
+for n in (4096, 16384):
+    A = oracle.gen_uniform(oracle.SEED_A, n)
+    b = oracle.gen_rhs(n)
+    info, x, sec = oracle.lapack_gesv(A, b, T)
+    res = oracle.residual(A, x, b)[0] if n <= 4096 else None
+    out[str(n)] = dict(info=info, seconds=sec, residual=res)
+print(json.dumps(out))
+"""
+SAMPLE_STEPS = 8
+
+
+def _child(code, args, timeout):
+    out = subprocess.run([sys.executable, "-c", code, *map(str, args)], cwd=ROOT,
+                         capture_output=True, text=True, timeout=timeout)
+    if out.returncode != 0:
+        raise RuntimeError(out.stderr[-500:])
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(n: int, nb: int, timeout: float = 300.0):
+    """The CPU baselines on this host's cores (reported beside the GPU
+    figure, not its target): the oracle (a restatement of the reference's
+    block algorithm with the vendor BLAS per block, the work of a step
+    spread over the threads as the reference spreads it over one PE per
+    core) at C1 (n = 4096, block 128) and on the first block steps of the
+    n x n factorization, plus the vendor LAPACK's LU; returns the
+    cpu_baseline object."""
+    threads, nproc, model = host_cpu()
+    base = {"unit": "TFLOP/s", "cores": threads, "kind": "port", "nproc": nproc,
+            "cpu_model": model}
+    try:
+        rec = _child(_PORT_CODE, [threads, n, nb, SAMPLE_STEPS], timeout)
     except Exception as e:  # the baseline is reported, never required
-        return {"value": None, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-                "sample": f"failed: {e}"}
-    tf = (2.0 / 3.0 * n ** 3) / rec["seconds"] / 1e12
+        return dict(base, value=None, sample=f"failed: {e}")
+    flops = 2.0 / 3.0 * (n ** 3 - (n - SAMPLE_STEPS * nb) ** 3)
+    tf = flops / rec["sample_seconds"] / 1e12
     blas = "MKL sequential dgemm/dtrsm/dger per block" if rec["mkl"] else "plain C loops"
-    return {"value": round(tf, 5), "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle restatement of the reference block LU + solve, n={n}, b={nb}, "
-                       f"{blas}, {threads} threads over the blocks of a step, "
-                       f"{rec['seconds']:.2f} s, residual {rec['residual']:.3f}")}
+    out = dict(base, value=round(tf, 5), sample=(
+        f"oracle restatement of the reference block LU ({blas}, {threads} threads over the "
+        f"blocks of a step): the first {SAMPLE_STEPS} of {n // nb} block steps of n={n}, b={nb} "
+        f"(2/3 n^3 - 2/3 (n - {SAMPLE_STEPS}*{nb})^3 = {flops:.3e} flop) in "
+        f"{rec['sample_seconds']:.2f} s"))
+    c1_tf = (2.0 / 3.0 * 4096 ** 3) / rec["c1_seconds"] / 1e12
+    out["c1"] = {"n": 4096, "block": 128, "seconds": round(rec["c1_seconds"], 3),
+                 "tflops": round(c1_tf, 5), "residual": rec["c1_residual"],
+                 "info": rec["c1_info"]}
+    try:
+        lap = _child(_LAPACK_CODE, [threads], timeout)
+        out["library"] = {"kind": "vendor LAPACK dgetrf + dgetrs (MKL, multithreaded)",
+                          "cores": threads}
+        for k, v in lap.items():
+            m = int(k)
+            out["library"][f"n{m}"] = {"seconds": round(v["seconds"], 3),
+                                       "tflops": round(2.0 / 3.0 * m ** 3 / v["seconds"] / 1e12, 5),
+                                       "info": v["info"], "residual": v["residual"]}
+    except Exception as e:
+        out["library"] = {"failed": str(e)}
+    return out
 
 
 def gemm_only(lib, _lib, torch, n, nb, steps, warmup):
@@ -143,13 +226,43 @@ def lp_gemm_only(lib, _lib, torch, n, nb, steps, warmup, prec="bf16"):
     print(json.dumps({"lp_gemm": f"{m}x{m}x{nb} {prec}", **res}), flush=True)
 
 
-def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, fp64_nb=None):
+def fp64_in_place(clu, lib, _lib, torch, A, b, nb64, n):
+    """One FP64 LU + solve factoring A itself (no copy: at n = 131072 the
+    FP64 matrix and a copy do not both fit beside the other buffers), then A
+    regenerated from its seed for the residual.  Returns (record, x64)."""
+    import ctypes
+    flops = 2.0 / 3.0 * n ** 3
+    x64 = b.clone()
+    torch.cuda.synchronize()
+    lib.charmlu_profile(1)
+    t0 = time.perf_counter()
+    ipiv, info = clu.getrf(A, nb64)
+    clu.getrs(A, ipiv, x64)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    prof = (ctypes.c_double * 20)()
+    _lib.check(lib.charmlu_profile_read(prof, 20), "charmlu_profile_read")
+    lib.charmlu_profile(0)
+    _lib.check(lib.charmlu_dgen_matrix(1, clu.lu.SEED_A, n, ctypes.c_void_p(A.data_ptr()), n,
+                                       clu.lu._stream_ptr()), "charmlu_dgen_matrix")
+    res = clu.residual(A, x64, b)
+    upd = prof[1] / (prof[15] * 1e-3) / 1e12 if prof[15] > 0 else 0.0
+    return {"tflops": round(flops / sec / 1e12, 3), "ms": round(sec * 1e3, 1), "block": nb64,
+            "residual": res["residual"], "info": int(info.item()), "steps": 1,
+            "trailing_update_tflops": round(upd, 3),
+            "trailing_update_frac": round(upd / FP64_PEAK, 4),
+            "note": "one step factoring A in place (no copy of A), after the mixed runs"}, x64
+
+
+def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, fp64_nb=None,
+                fp64_after=False):
     """Mixed-precision LU + FP64 GMRES-IR against the plain FP64 LU on the
     same diagonally dominant system (A U[-0.5,0.5) with a_ii = sum_j |a_ij|,
     b from MatGen).  A mixed step converts A to FP32, factors it with the
     low-precision trailing update and refines x in FP64 to the reference's
-    scaled residual <= 1; a FP64 step copies A, factors and solves.  Returns
-    the "mixed" object of the JSON line."""
+    scaled residual <= 1; a FP64 step copies A, factors and solves
+    (fp64_after: one step after the mixed runs, factoring A in place).
+    Returns the "mixed" object of the JSON line."""
     import ctypes
     flops = 2.0 / 3.0 * n ** 3
     A = clu.gen_matrix(n, kind="diagdom")
@@ -159,7 +272,8 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
     # plain FP64 LU on the same system: the speed reference and the solution
     # the mixed results are compared with
     x64 = None
-    if fp64_steps > 0:
+    xs = {}
+    if fp64_steps > 0 and not fp64_after:
         # the FP64 LU with its own best block size (1024 measured fastest at
         # n = 65536: 57.0 TFLOP/s against 56.4 with 512 and 52.8 with 256)
         nb64 = fp64_nb or nb
@@ -252,8 +366,28 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
         if x64 is not None:
             rec["error_vs_fp64"] = float(((x - x64).abs().max() / x64.abs().max()).item())
             rec["speedup_vs_fp64"] = round(rec["tflops"] / out["fp64"]["tflops"], 3)
+        elif fp64_after:
+            xs[key] = x.clone()
         out[key] = rec
+    if fp64_after and fp64_steps > 0:
+        del A32
+        torch.cuda.empty_cache()
+        out["fp64"], x64 = fp64_in_place(clu, lib, _lib, torch, A, b, fp64_nb or nb, n)
+        for key, xk in xs.items():
+            out[key]["error_vs_fp64"] = float(((xk - x64).abs().max() / x64.abs().max()).item())
+            out[key]["speedup_vs_fp64"] = round(out[key]["tflops"] / out["fp64"]["tflops"], 3)
     return out
+
+
+def c5_bench(clu, lib, _lib, torch, nb):
+    """C5: the FP8 form at n = 131072 (FP64 A 137 GB + FP32 factors 69 GB in
+    HBM), unpivoted and pivoted, one step each after no warmup, then one
+    FP64 LU + solve of the same system in place for the error and speed
+    comparison."""
+    rec = mixed_bench(clu, lib, _lib, torch, 131072, nb, ["fp8-nopiv", "fp8"], 1, 0, 1, nb,
+                      fp64_after=True)
+    rec["steps_note"] = "one timed step per form, no warmup (the first call of its shape)"
+    return rec
 
 
 def main():
@@ -263,8 +397,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--nb", type=int, default=256)
-    ap.add_argument("--cpu-n", type=int, default=16384)
-    ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gemm-only", action="store_true",
                     help="time only the trailing-update GEMM of step 0 (M=N=n-nb, K=nb)")
@@ -275,8 +407,10 @@ def main():
     ap.add_argument("--mixed-fp64-nb", type=int, default=1024)
     ap.add_argument("--mixed-precs", default="bf16,fp16,bf16-nopiv")
     ap.add_argument("--mixed-steps", type=int, default=2)
-    ap.add_argument("--mixed-fp64-steps", type=int, default=1)
+    ap.add_argument("--mixed-fp64-steps", type=int, default=2)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the FP8 form at n=131072 (C5) in the default run")
     ap.add_argument("--c5", action="store_true",
                     help="only the FP8 form at n=131072 (block --mixed-nb, diagonally dominant; "
                          "FP64 A 137 GB + FP32 factors 69 GB in HBM), one step each")
@@ -313,8 +447,7 @@ def main():
         lp_gemm_only(lib, _lib, torch, args.mixed_n, args.mixed_nb, args.steps, args.warmup)
         return
     if args.c5:
-        print(json.dumps({"c5": mixed_bench(clu, lib, _lib, torch, 131072, args.mixed_nb,
-                                             ["fp8-nopiv", "fp8"], 1, 0, 0)}), flush=True)
+        print(json.dumps({"c5": c5_bench(clu, lib, _lib, torch, args.mixed_nb)}), flush=True)
         return
     if args.mixed_only:
         print(json.dumps(mixed_bench(clu, lib, _lib, torch, args.mixed_n, args.mixed_nb,
@@ -439,8 +572,11 @@ def main():
         out["mixed"] = mixed_bench(clu, lib, _lib, torch, args.mixed_n, args.mixed_nb,
                                    args.mixed_precs.split(","), args.mixed_steps, args.warmup,
                                    args.mixed_fp64_steps, args.mixed_fp64_nb)
+    if not args.no_mixed and not args.no_c5:
+        torch.cuda.empty_cache()
+        out["c5"] = c5_bench(clu, lib, _lib, torch, args.mixed_nb)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n, nb, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(n, nb)
     print(json.dumps(out), flush=True)
 
 

@@ -10,10 +10,11 @@ from ._lib import CharmluError, load
 
 load()
 
-from .lu import (LUConfig, gen_matrix, gen_rhs, gesv_ir, getrf, getrf_mixed,  # noqa: E402
-                 getrs, getrs32, hpl_gflops, lag2s, residual, run)
+from .lu import (LUConfig, check_info, gen_matrix, gen_rhs, gesv_ir, gesv_ir_multi,  # noqa: E402
+                 getrf, getrf_mixed, getrs, getrs32, hpl_gflops, lag2s, residual, run)
 
-from .autograd import LUSolve, solve  # noqa: E402
+from .autograd import LUSolve, MixedLUSolve, solve  # noqa: E402
 
-__all__ = ["LUSolve", "solve", "CharmluError", "LUConfig", "gen_matrix", "gen_rhs", "gesv_ir", "getrf",
-           "getrf_mixed", "getrs", "getrs32", "hpl_gflops", "lag2s", "residual", "run", "load"]
+__all__ = ["LUSolve", "MixedLUSolve", "solve", "CharmluError", "LUConfig", "check_info",
+           "gen_matrix", "gen_rhs", "gesv_ir", "gesv_ir_multi", "getrf", "getrf_mixed", "getrs",
+           "getrs32", "hpl_gflops", "lag2s", "residual", "run", "load"]

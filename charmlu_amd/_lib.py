@@ -1,10 +1,11 @@
 """ctypes binding of libcharmlu_amd.so (the C ABI of include/charmlu_amd.h).
 
-The library must have been built (``python -m charmlu_amd.build`` or
+The library must have been built (``python charmlu_amd/build.py`` or
 ``__graft_entry__.build()``); there is no fallback path.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 
@@ -19,6 +20,9 @@ SIGNATURES = {
     "charmlu_version": (_c_int, []),
     "charmlu_last_error": (ctypes.c_char_p, []),
     "charmlu_init": (_c_int, [_c_int]),
+    "charmlu_finalize": (_c_int, []),
+    "charmlu_panel_plan": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                    ctypes.POINTER(ctypes.c_int)]),
     "charmlu_profile": (_c_int, [_c_int]),
     "charmlu_profile_read": (_c_int, [ctypes.POINTER(ctypes.c_double), _c_int]),
     "charmlu_cblas_dgemm": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double,
@@ -48,6 +52,9 @@ SIGNATURES = {
     "charmlu_sgetrs": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_long, _vp, _vp, _c_long, _vp]),
     "charmlu_dgesv_ir": (_c_int, [_c_int, _vp, _c_long, _vp, _c_long, _vp, _vp, _vp,
                                   ctypes.c_double, _c_int, _c_int, _vp, _vp]),
+    "charmlu_dgesv_ir_ex": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp, _vp,
+                                     _c_long, _vp, _c_long, ctypes.c_double, _c_int, _c_int, _vp,
+                                     _vp]),
     "charmlu_dgen_matrix": (_c_int, [_c_int, _c_u64, _c_long, _vp, _c_long, _vp]),
     "charmlu_dgen_rhs": (_c_int, [_c_int, _c_long, _c_long, _vp, _vp]),
     "charmlu_dresidual": (_c_int, [_c_int, _vp, _c_long, _vp, _vp, _vp, _vp, _vp]),
@@ -60,6 +67,7 @@ UPPER, LOWER = 121, 122
 NON_UNIT, UNIT = 131, 132
 LEFT, RIGHT = 141, 142
 PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16, PREC_FP8 = 0, 1, 2, 3, 4
+ERR_HIP, ERR_NOMEM, ERR_CONFIG, ERR_UNSUPPORTED = -100, -101, -102, -103
 MIXED_NOPIV = 0x100
 
 _lib = None
@@ -75,7 +83,7 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(
-                f"{LIB_PATH} is missing: build it with `python -m charmlu_amd.build` "
+                f"{LIB_PATH} is missing: build it with `python charmlu_amd/build.py` "
                 "(there is no CPU fallback)")
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
@@ -83,6 +91,9 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = lib
+        # streams, events and workspaces go back before the HIP runtime
+        # tears down at process exit
+        atexit.register(lib.charmlu_finalize)
     return _lib
 
 

@@ -15,6 +15,7 @@ typedef float float16_t __attribute__((ext_vector_type(16)));
 #define CLU_ERR_HIP (-100)
 #define CLU_ERR_NOMEM (-101)
 #define CLU_ERR_CONFIG (-102)
+#define CLU_ERR_UNSUPPORTED (-103)
 
 namespace clu {
 

@@ -88,13 +88,24 @@ constexpr long PANEL_SLOT_WORDS = 2L * 64 * 136;
 struct PanelWork {
   void *slots;        // per-workgroup candidate records, 2 parities x 256
   unsigned epoch;     // sequence number of the next column (host-side)
-  int max_groups;
+  int max_groups;     // workgroups of one strip that are resident at once
   int *info;          // device int, first near-singular column + 1
+  unsigned *abortw;   // device word: a strip's exchange gave up (cleared per factorization)
 };
-// Widest strip panel_base takes for a strip of m rows: 16 columns, or 32
-// for FP32 strips of at most 65536 rows.
+// Launch shape of one strip: G workgroups of NT threads, R rows per thread,
+// sw columns (the kernel's strip width, >= the strip's own width).
+struct PanelShape {
+  int sw, R, NT, G;
+};
+// Widest strip (32, 16 or 8 columns) panel_base takes for a strip of m rows
+// when at most max_groups workgroups can be resident at once; 0 if none
+// fits (every narrower or shorter strip then fits too: the factorizations
+// check the tallest, first, strip before they enqueue anything).
 template <typename T>
-int panel_strip_width(int m);
+int panel_strip_width(int m, int max_groups, bool pivot = true);
+// The launch shape panel_base uses for an m x w strip (0), or -1 if none.
+template <typename T>
+int panel_plan(int m, int w, int max_groups, bool pivot, PanelShape *ps);
 // Factor rows [j0, m_end) x columns [j0, j0+w) (w <= panel_strip_width) of A
 // in place with partial pivoting, one column at a time; ipiv[j0..j0+w) get
 // global rows.
@@ -159,9 +170,13 @@ int gen_rhs(int seed, long skip, long n, double *b, hipStream_t s);
 // r (optional) = b - A x.
 int residual(int n, const double *A, long lda, const double *x, const double *b,
              double *r, double *out, hipStream_t s);
+// out[0] = max|a_ij|, out[1] = max_j sum_i |a_ij| (out: 5 doubles, zeroed first)
+int abs_colsum(int n, const double *A, long lda, double *out, hipStream_t s);
 template <typename TD, typename TS>
 int convert(long m, long n, const TS *src, long lds, TD *dst, long ldd, hipStream_t s);
 int fill_int(int *p, int v, long n, hipStream_t s);
+// info[0] = -1 if abortw[0] != 0 (a panel strip's exchange gave up)
+int finish_info(int *info, const unsigned *abortw, hipStream_t s);
 
 // ---- FP64 vector kernels of the iterative refinement ----------------------
 // h[0..k) = V w (V: k rows of n); w += alpha * V^T h; out = {||x||_2^2,

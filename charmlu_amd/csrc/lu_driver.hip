@@ -56,12 +56,21 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 // per-device workspace
 // ---------------------------------------------------------------------------
 // Workgroups of one multi-workgroup panel strip, at most: every one of them
-// must be resident at once, and no strip of more than 32 has been run (a
-// mixed-form run with 48 panel CUs, whose 256-thread strips took up to 36
-// workgroups, stopped making progress; not analysed).
+// must be resident at once (resident_groups), and no strip of more than 32
+// has been run.
 constexpr int MAX_GROUPS = 32;
 constexpr int MAX_PANEL_CUS = 32;
 constexpr int RUN = 256;         // pivots per move list
+
+// Workgroups of one panel strip that can be resident at once.  The strip
+// kernels that exchange records across workgroups hold one workgroup per
+// CU (their register use allows one wave per SIMD, or two of a 512-thread
+// workgroup), so at most one workgroup per panel CU; with no CU masks
+// (panel_cus = 0) the strip runs on the whole chip.
+static int resident_groups(int panel_cus, int ncu) {
+  const int cus = panel_cus > 0 ? panel_cus : ncu;
+  return cus < MAX_GROUPS ? cus : MAX_GROUPS;
+}
 
 struct PanelHistory {
   int n = 0, nb = 0, kind = -1;
@@ -89,8 +98,9 @@ struct Workspace {
   PanelHistory hist64, histmx;
   // panel exchange slots (sized for doubles, reused for floats)
   void *slots = nullptr;
-  unsigned *counter = nullptr;
-  int *info_scratch = nullptr;
+  unsigned *abortw = nullptr;  // a strip's exchange gave up (cleared per factorization)
+  // fence of the library's streams before an error returns
+  hipEvent_t ev_fence[3] = {nullptr, nullptr, nullptr};
   // move lists: in-panel (one run) and per-step (nruns runs, two parities)
   int2 *pmoves = nullptr;
   int *pcount = nullptr;
@@ -234,18 +244,18 @@ static int get_ws(Workspace **out) {
     const size_t slot_bytes = (size_t)PANEL_SLOT_WORDS * 8;
     CLU_HIP(hipMalloc(&w.slots, slot_bytes));
     CLU_HIP(hipMemset(w.slots, 0, slot_bytes));
-    CLU_HIP(hipMalloc(&w.counter, 64));
-    CLU_HIP(hipMalloc(&w.info_scratch, 64));
+    CLU_HIP(hipMalloc(&w.abortw, 64));
+    CLU_HIP(hipMemset(w.abortw, 0, 64));
+    for (int i = 0; i < 3; ++i)
+      CLU_HIP(hipEventCreateWithFlags(&w.ev_fence[i], hipEventDisableTiming));
     CLU_HIP(hipMalloc(&w.pmoves, sizeof(int2) * 2 * RUN));
     CLU_HIP(hipMalloc(&w.pcount, 64));
     int rc = ensure_runs(w, 1);
     if (rc) return rc;
     w.pw.slots = w.slots;
+    w.pw.abortw = w.abortw;
     w.pw.epoch = 1;
-    // a multi-workgroup panel needs all its workgroups resident at once:
-    // at most one 512-thread workgroup per CU of the panel stream
-    w.pw.max_groups = reserve > 0 ? (reserve < MAX_GROUPS ? reserve : MAX_GROUPS)
-                                  : (ncu < MAX_GROUPS ? ncu : MAX_GROUPS);
+    w.pw.max_groups = resident_groups(reserve, ncu);
     w.device = dev;
   }
   *out = &w;
@@ -336,8 +346,13 @@ static int trsm_llnu_big(int M, int N, const T *L, long ldl, T *B, long ldb, hip
 template <typename T>
 static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, int *ipiv,
                         Workspace &ws, hipStream_t s, bool pivot = true) {
-  if (w <= panel_strip_width<T>(n - c0))
-    return panel_base<T>(A, lda, c0, n, w, pc0, pc0 + pcw, ipiv, ws.pw, s, pivot);
+  const int sw = panel_strip_width<T>(n - c0, ws.pw.max_groups, pivot);
+  if (sw == 0) {
+    set_error("a panel of %d rows needs more than %d resident workgroups", n - c0,
+              ws.pw.max_groups);
+    return CLU_ERR_UNSUPPORTED;
+  }
+  if (w <= sw) return panel_base<T>(A, lda, c0, n, w, pc0, pc0 + pcw, ipiv, ws.pw, s, pivot);
   int w1 = (w / 2 + 15) / 16 * 16;
   if (w1 >= w) w1 = w / 2;
   const int w2 = w - w1;
@@ -421,6 +436,31 @@ static int history_prepare(PanelHistory &h, int n, int nb, int kind, int NB) {
   return CLU_OK;
 }
 
+// On an error in the middle of a factorization: order the caller's stream
+// after everything already enqueued on the library's streams, so that no
+// kernel the call launched still writes the caller's buffers once the
+// error has been returned and the caller synchronizes its stream.
+static void fence_streams(Workspace &ws, hipStream_t su) {
+  hipStream_t ss[3] = {ws.s_panel, ws.s_full, ws.s_pgemm};
+  for (int i = 0; i < 3; ++i)
+    if (hipEventRecord(ws.ev_fence[i], ss[i]) == hipSuccess)
+      (void)hipStreamWaitEvent(ws.s_main, ws.ev_fence[i], 0);
+  if (hipEventRecord(ws.ev_end, ws.s_main) == hipSuccess)
+    (void)hipStreamWaitEvent(su, ws.ev_end, 0);
+  (void)hipGetLastError();
+}
+
+// The tallest strip of a factorization is its first (n rows): if a launch
+// shape exists for it, every later strip has one too.
+template <typename T>
+static int check_panel_shape(const Workspace &ws, int n, bool pivot) {
+  if (n <= 0 || panel_strip_width<T>(n, ws.pw.max_groups, pivot) > 0) return CLU_OK;
+  set_error("n = %d: the first panel needs more than the %d workgroups that can be resident at "
+            "once on %d panel CUs (largest FP64 n with 32: 131072, FP32: 262144)",
+            n, ws.pw.max_groups, ws.panel_cus);
+  return CLU_ERR_UNSUPPORTED;
+}
+
 // move lists of ipiv[k1, k2) in runs of RUN pivots: one launch of one
 // workgroup per run (one run: the single-run kernel)
 static int build_moves(const int *ipiv, int k1, int k2, int2 *moves, int *counts, hipStream_t s) {
@@ -444,17 +484,18 @@ static int apply_runs(int nruns, const int2 *moves, const int *counts, T *A, lon
 // part of each block step is one small matrix-vector (or matrix) product
 // with the block's inverse instead of an S-step substitution; the block's
 // contribution is then subtracted from every row still to be solved (a GEMV
-// for one right-hand side, a GEMM for several).  S = 256; FP32 systems of
-// 8192 and more use S = 1024: a vector solve is two launches per block, and
-// at n = 65536 its 1024 launches of 256-row blocks spent a third of the
-// solve between kernels (the FP32 solve is the preconditioner of the
-// refinement, which corrects its rounding).
+// for one right-hand side, a GEMM for several).  S = 256 (explicit inverses
+// of 512-row triangles doubled the scaled residual of the FP64 solve at
+// n = 32768).  The refinement's FP32 preconditioner solves of systems of
+// 8192 and more pass S = 1024: a vector solve is two launches per block, at
+// n = 65536 its 1024 launches of 256-row blocks spent a third of the solve
+// between kernels, and the refinement corrects the extra rounding.
 // inv_ready: the inverses of the diagonal blocks are still in the workspace
-// from the previous call with the same factors (the refinement's repeated
-// solves)
+// from the previous call with the same factors and S (the refinement's
+// repeated solves)
 template <typename T>
 static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const int *dipiv, T *dB,
-                      long ldb, hipStream_t s, bool inv_ready = false) {
+                      long ldb, hipStream_t s, bool inv_ready = false, int S = 256) {
   if (trans != CHARMLU_NO_TRANS && trans != CHARMLU_TRANS) return -1;
   if (n < 0) return -2;
   if (nrhs < 0) return -3;
@@ -468,7 +509,7 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
   int rc = get_ws(&wsp);
   if (rc) return rc;
   Workspace &ws = *wsp;
-  const int S = sizeof(T) == 4 && n >= 8192 ? 1024 : 256;
+  if (S != 256 && S != 1024) return -1;
   const int nblk = (n + S - 1) / S;
   const int nruns = (n + RUN - 1) / RUN;
   {
@@ -627,6 +668,77 @@ int charmlu_init(int device) {
   CLU_HIP(hipSetDevice(device));
   Workspace *ws = nullptr;
   return get_ws(&ws);
+}
+
+// Everything a workspace holds, released on its device; the workspace is
+// then as new (the next call on that device makes it again).
+static int release_ws(Workspace &w) {
+  if (w.device < 0) return CLU_OK;
+  CLU_HIP(hipSetDevice(w.device));
+  CLU_HIP(hipDeviceSynchronize());
+  hipStream_t ss[4] = {w.s_panel, w.s_main, w.s_full, w.s_pgemm};
+  for (hipStream_t st : ss)
+    if (st) CLU_HIP(hipStreamDestroy(st));
+  hipEvent_t ev[] = {w.ev_pg_in, w.ev_pg_out, w.ev_panel, w.ev_la,  w.ev_start, w.ev_end,
+                     w.ev_urow,  w.ev_ula,    w.ev_full,  w.ev_fence[0], w.ev_fence[1],
+                     w.ev_fence[2]};
+  for (hipEvent_t e : ev)
+    if (e) CLU_HIP(hipEventDestroy(e));
+  for (PanelHistory *h : {&w.hist64, &w.histmx})
+    for (hipEvent_t e : h->ev) CLU_HIP(hipEventDestroy(e));
+  void *bufs[] = {w.slots,  w.abortw, w.pmoves, w.pcount, w.smoves, w.scount, w.stage,
+                  w.winv,   w.ubuf,   w.winv32, w.ubuf32, w.pa16,   w.pu16,   w.uinv32,
+                  w.l21buf, w.esc,    w.irbuf,  w.sinv,   w.sy,     w.smv,    w.tinv,
+                  w.pinv,   w.ptmp};
+  for (void *b : bufs)
+    if (b) CLU_HIP(hipFree(b));
+  w = Workspace();
+  return CLU_OK;
+}
+
+int charmlu_finalize(void) {
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = CLU_OK;
+  for (Workspace &w : g_ws) {
+    const int r = release_ws(w);
+    if (r && !rc) rc = r;
+  }
+  for (Prof::Rec &r : g_prof.recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  g_prof.recs.clear();
+  g_prof.used = 0;
+  g_prof.on = false;
+  if (prev >= 0) (void)hipSetDevice(prev);
+  return rc;
+}
+
+int charmlu_panel_plan(int prec, int m, int w, int panel_cus, int ncu, int pivot, int *out) {
+  if (prec != CHARMLU_PREC_FP64 && prec != CHARMLU_PREC_FP32) return -1;
+  if (m < 1) return -2;
+  if (w < 1) return -3;
+  if (panel_cus < 0) return -4;
+  if (ncu < 1) return -5;
+  if (!out) return -7;
+  const int groups = resident_groups(panel_cus, ncu);
+  PanelShape ps{0, 0, 0, 0};
+  const int r = prec == CHARMLU_PREC_FP64 ? panel_plan<double>(m, w, groups, pivot != 0, &ps)
+                                          : panel_plan<float>(m, w, groups, pivot != 0, &ps);
+  out[0] = ps.sw;
+  out[1] = ps.R;
+  out[2] = ps.NT;
+  out[3] = ps.G;
+  out[4] = groups;
+  out[5] = prec == CHARMLU_PREC_FP64 ? panel_strip_width<double>(m, groups, pivot != 0)
+                                     : panel_strip_width<float>(m, groups, pivot != 0);
+  if (r) {
+    set_error("no launch shape for a %d x %d strip with %d resident workgroups", m, w, groups);
+    return CLU_ERR_UNSUPPORTED;
+  }
+  return CLU_OK;
 }
 
 // ---- device-pointer block ops ----------------------------------------------
@@ -949,7 +1061,8 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   Workspace *wsp;
   if ((rc = get_ws(&wsp))) return rc;
   Workspace &ws = *wsp;
-  // all work runs on the library's two streams, fenced to the caller's
+  if ((rc = check_panel_shape<double>(ws, n, true))) return rc;
+  // all work runs on the library's streams, fenced to the caller's
   hipStream_t sm = ws.s_main;
   const int nruns = (nb + RUN - 1) / RUN;
   // inv(L11) * A12 on the matrix cores replaces the substitution
@@ -970,6 +1083,7 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   auto winv_of = [&](int t) { return ws.winv + (size_t)(t & 1) * nb * nb; };
   hipStream_t sp = ws.s_panel;
   ws.pw.info = dinfo;
+  if ((rc = fill_int((int *)ws.abortw, 0, 1, su))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_start, su));
   CLU_HIP(hipStreamWaitEvent(sp, ws.ev_start, 0));
   CLU_HIP(hipStreamWaitEvent(sm, ws.ev_start, 0));
@@ -1048,8 +1162,12 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     full_pending = true;
     return CLU_OK;
   };
-  if ((rc = history_prepare(ws.hist64, n, nb, 0, NB))) return rc;
-
+  if ((rc = history_prepare(ws.hist64, n, nb, 0, NB))) {
+    fence_streams(ws, su);
+    return rc;
+  }
+  rc = [&]() -> int {  // the steps; an error returns from here to the fence below
+  int rc;
   if ((rc = panel(0))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_panel, sp));
   for (int t = 0; t < NB; ++t) {
@@ -1125,9 +1243,14 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
       return rc;
   }
   if (full_pending) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+  // info = -1 if a strip's exchange gave up
+  if ((rc = finish_info(dinfo, ws.abortw, sm))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_end, sm));
   CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
   return CLU_OK;
+  }();
+  if (rc) fence_streams(ws, su);
+  return rc;
 }
 
 // ---- solve ---------------------------------------------------------------------
@@ -1181,6 +1304,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   Workspace *wsp;
   if ((rc = get_ws(&wsp))) return rc;
   Workspace &ws = *wsp;
+  if ((rc = check_panel_shape<float>(ws, n, true))) return rc;
   hipStream_t sm = ws.s_main, sp = ws.s_panel;
   const int nruns = (nb + RUN - 1) / RUN;
   const bool lp = prec != CHARMLU_PREC_FP32;
@@ -1211,6 +1335,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   // own while the rest of step t's U row still reads this one
   auto winv_of = [&](int t) { return (float *)ws.winv32 + (size_t)(t & 1) * nb * nb; };
   ws.pw.info = dinfo;
+  if ((rc = fill_int((int *)ws.abortw, 0, 1, su))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_start, su));
   CLU_HIP(hipStreamWaitEvent(sp, ws.ev_start, 0));
   CLU_HIP(hipStreamWaitEvent(sm, ws.ev_start, 0));
@@ -1293,8 +1418,12 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     const int Ma = ((int)rows + 127) / 128 * 128;
     return Ma > M ? M : Ma;
   };
-  if ((rc = history_prepare(ws.histmx, n, nb, 1 + 2 * prec + (nopiv ? 1 : 0), NB))) return rc;
-
+  if ((rc = history_prepare(ws.histmx, n, nb, 1 + 2 * prec + (nopiv ? 1 : 0), NB))) {
+    fence_streams(ws, su);
+    return rc;
+  }
+  rc = [&]() -> int {  // the steps; an error returns from here to the fence below
+  int rc;
   if ((rc = panel(0))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_panel, sp));
   for (int t = 0; t < NB; ++t) {
@@ -1399,9 +1528,13 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       return rc;
   }
   if (full_pending) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+  if ((rc = finish_info(dinfo, ws.abortw, sm))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_end, sm));
   CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
   return CLU_OK;
+  }();
+  if (rc) fence_streams(ws, su);
+  return rc;
 }
 
 // FP64 -> FP32 copy of a matrix (the working matrix of the mixed path)
@@ -1416,86 +1549,102 @@ int charmlu_dlag2s(long m, long n, const double *dA, long lda, float *dS, long l
   return convert<float, double>(m, n, dA, lda, dS, lds, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
 // ---- FP64 GMRES-based iterative refinement ------------------------------------
-// x0 = (LU)^{-1} b in FP32; then, while the reference's scaled residual
-// max|b - A x| / ((max|a_ij| max|x| + max|b|) n eps) (benchmark.C:385-400)
-// exceeds tol: r = b - A x in FP64, one GMRES cycle on (LU)^{-1} A d =
-// (LU)^{-1} r (FP64 Arnoldi basis, classical Gram-Schmidt applied twice,
-// Givens rotations on the host), x += d.  The FP32 factors only apply the
-// preconditioner; residuals, the basis, inner products and x stay in FP64.
-// (Starting from x0 = 0 instead saves one solve and one residual pass but
-// measured one more cycle at n = 65536.)
-int charmlu_dgesv_ir(int n, const double *dA, long lda, const float *dLU, long ldlu,
-                     const int *dipiv, const double *db, double *dx, double tol, int max_iter,
-                     int restart, double *stats, void *stream) {
-  if (n < 0) return -1;
-  if (!dA && n > 0) return -2;
-  if (lda < (n > 1 ? n : 1)) return -3;
-  if (!dLU && n > 0) return -4;
-  if (ldlu < (n > 1 ? n : 1)) return -5;
-  if (!dipiv && n > 0) return -6;
-  if (!db && n > 0) return -7;
-  if (!dx && n > 0) return -8;
-  if (!(tol > 0.0)) return -9;
-  if (max_iter < 0) return -10;
-  if (restart < 1 || restart > 200) return -11;
-  hipStream_t s = (hipStream_t)stream;
-  if (stats) for (int i = 0; i < 4; ++i) stats[i] = 0.0;
-  if (n == 0) return CLU_OK;
-  Workspace *wsp;
-  int rc;
-  if ((rc = get_ws(&wsp))) return rc;
-  Workspace &ws = *wsp;
+// x0 = op(LU)^{-1} b in FP32; then, while the reference's scaled residual
+// max|b - op(A) x| / ((max|a_ij| max|x| + max|b|) n eps) (benchmark.C:385-400)
+// exceeds tol: r = b - op(A) x in FP64, one GMRES cycle on
+// op(LU)^{-1} op(A) d = op(LU)^{-1} r (FP64 Arnoldi basis, classical
+// Gram-Schmidt applied twice, Givens rotations on the host), x += d.  op is
+// the identity or the transpose (the adjoint solve A^T lambda = g of the
+// differentiable solve, from the same stored factors).  The FP32 factors
+// only apply the preconditioner; residuals, the basis, inner products and x
+// stay in FP64.  (Starting from x0 = 0 instead saves one solve and one
+// residual pass but measured one more cycle at n = 65536.)
+// b and x: contiguous n-vectors; stats[4] as charmlu_dgesv_ir.
+namespace clu {
+static int gmres_ir(bool trans, int n, const double *dA, long lda, const float *dLU, long ldlu,
+                    const int *dipiv, const double *db, double *dx, double tol, int max_iter,
+                    int restart, double *stats, double *irbuf, bool &inv_ready, hipStream_t s) {
   const int m = restart;
-  // [V: (m+1) x n][r: n][w: n][small: 2*(m+1)+16][z32: n floats]
+  // irbuf: [V: (m+1) x n][r: n][w: n][small: 2*(m+1)+16][z32: n floats]
   const size_t small = 2 * (size_t)(m + 1) + 16;
   const size_t dbl = (size_t)(m + 3) * n + small;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if ((rc = ensure_bytes(&ws.irbuf, &ws.irbuf_b, dbl * sizeof(double) + sizeof(float) * n + 256)))
-      return rc;
-  }
-  double *V = (double *)ws.irbuf;
+  double *V = irbuf;
   double *r = V + (size_t)(m + 1) * n;
   double *w = r + n;
-  double *dh = w + n;          // h (m+1), then h2 (m+1)
-  double *dnorm = dh + 2 * (m + 1);  // 2 norms
-  double *dres = dnorm + 2;    // residual maxima (5)
+  double *dh = w + n;                // h (m+1), then h2 (m+1)
+  double *dnorm = dh + 2 * (m + 1);  // 2 norms (then 2 more for the transposed check)
+  double *dres = dnorm + 4;          // residual maxima (5), column maxima (5)
   float *z32 = (float *)(V + dbl);
   const double eps = 0x1.0p-53;
-  bool inv_ready = false;  // the block inverses of the factors, after the first solve
+  const int tr = trans ? CHARMLU_TRANS : CHARMLU_NO_TRANS;
+  int rc;
   auto precond = [&](const double *src, double scale, double *dst) -> int {
-    // dst = (LU)^{-1} src, with src scaled by 1/scale into FP32 range
+    // dst = op(LU)^{-1} src, with src scaled by 1/scale into FP32 range
     int e = vec_d2s(n, 1.0 / scale, src, z32, s);
     if (e) return e;
-    e = getrs_impl<float>(CHARMLU_NO_TRANS, n, 1, dLU, ldlu, dipiv, z32, 1, s, inv_ready);
+    e = getrs_impl<float>(tr, n, 1, dLU, ldlu, dipiv, z32, 1, s, inv_ready,
+                          n >= 8192 ? 1024 : 256);
     if (e) return e;
     inv_ready = true;
     return vec_s2d(n, scale, z32, dst, s);
   };
-  double hres[5];
-  auto check = [&](double *ref, double *ns, double *rmax) -> int {
-    int e = residual(n, dA, lda, dx, db, r, dres, s);
-    if (e) return e;
-    CLU_HIP(hipMemcpyAsync(hres, dres, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+  // op(A)^T's fixed maxima for the transposed check: max|a_ij|, ||A^T||_inf, max|b|
+  double amax = 0.0, anorm_t = 0.0, bmax = 0.0;
+  if (trans) {
+    double h3[3];
+    if ((rc = abs_colsum(n, dA, lda, dres + 5, s))) return rc;
+    if ((rc = vec_norms(n, db, dnorm + 2, s))) return rc;
+    CLU_HIP(hipMemcpyAsync(h3, dres + 5, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    CLU_HIP(hipMemcpyAsync(h3 + 2, dnorm + 3, sizeof(double), hipMemcpyDeviceToHost, s));
     CLU_HIP(hipStreamSynchronize(s));
-    const double den = (hres[0] * hres[2] + hres[1]) * n * eps;
-    *ref = den > 0 ? hres[3] / den : (hres[3] > 0 ? INFINITY : 0.0);
-    const double den2 = hres[4] * hres[2] * n * eps;
-    *ns = den2 > 0 ? hres[3] / den2 : (hres[3] > 0 ? INFINITY : 0.0);
-    *rmax = hres[3];
+    amax = h3[0];
+    anorm_t = h3[1];
+    bmax = h3[2];
+  }
+  double hres[5];
+  // r = b - op(A) x and the two scaled residuals
+  auto check = [&](double *ref, double *ns, double *rmax) -> int {
+    double xmax;
+    if (!trans) {
+      int e = residual(n, dA, lda, dx, db, r, dres, s);
+      if (e) return e;
+      CLU_HIP(hipMemcpyAsync(hres, dres, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+      CLU_HIP(hipStreamSynchronize(s));
+      amax = hres[0];
+      bmax = hres[1];
+      xmax = hres[2];
+      *rmax = hres[3];
+      anorm_t = hres[4];
+    } else {
+      int e = vec_scale(n, 1.0, db, r, s);
+      if (!e) e = dgemv(true, n, n, -1.0, dA, lda, dx, 1, 1.0, r, 1, s);
+      if (!e) e = vec_norms(n, r, dnorm, s);
+      if (!e) e = vec_norms(n, dx, dnorm + 2, s);
+      if (e) return e;
+      CLU_HIP(hipMemcpyAsync(hres, dnorm, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+      CLU_HIP(hipStreamSynchronize(s));
+      *rmax = hres[1];
+      xmax = hres[3];
+    }
+    const double den = (amax * xmax + bmax) * n * eps;
+    *ref = den > 0 ? *rmax / den : (*rmax > 0 ? INFINITY : 0.0);
+    const double den2 = anorm_t * xmax * n * eps;
+    *ns = den2 > 0 ? *rmax / den2 : (*rmax > 0 ? INFINITY : 0.0);
     return CLU_OK;
   };
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hh(2 * (m + 1) + 2), y(m);
   int outer = 0, inner = 0;
   double ref = 0, ns = 0, rmax = 0;
-  // x0 = (LU)^{-1} b
+  // x0 = op(LU)^{-1} b
   if ((rc = precond(db, 1.0, dx))) return rc;
   if ((rc = check(&ref, &ns, &rmax))) return rc;
   for (;;) {
     if (ref <= tol || outer >= max_iter || !(rmax > 0)) break;
     ++outer;
-    // v0 = (LU)^{-1} r / beta
+    // v0 = op(LU)^{-1} r / beta
     if ((rc = precond(r, rmax, V))) return rc;
     if ((rc = vec_norms(n, V, dnorm, s))) return rc;
     CLU_HIP(hipMemcpyAsync(hh.data(), dnorm, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1511,8 +1660,8 @@ int charmlu_dgesv_ir(int n, const double *dA, long lda, const float *dLU, long l
     int k = 0;
     for (int j = 0; j < m; ++j) {
       double *vj = V + (size_t)j * n, *vn = V + (size_t)(j + 1) * n;
-      // w = A v_j in FP64, then v_{j+1} = (LU)^{-1} w
-      if ((rc = dgemv(false, n, n, 1.0, dA, lda, vj, 1, 0.0, w, 1, s))) return rc;
+      // w = op(A) v_j in FP64, then v_{j+1} = op(LU)^{-1} w
+      if ((rc = dgemv(trans, n, n, 1.0, dA, lda, vj, 1, 0.0, w, 1, s))) return rc;
       if ((rc = precond(w, 1.0, vn))) return rc;
       // classical Gram-Schmidt, twice
       if ((rc = basis_dot(j + 1, n, V, n, vn, dh, s))) return rc;
@@ -1567,6 +1716,108 @@ int charmlu_dgesv_ir(int n, const double *dA, long lda, const float *dLU, long l
   }
   return CLU_OK;
 }
+
+// bytes of the refinement's buffer (plus two n-vectors for a column of a
+// multi-RHS call)
+static size_t irbuf_bytes(int n, int restart) {
+  const size_t small = 2 * (size_t)(restart + 1) + 16;
+  return ((size_t)(restart + 3) * n + small + 2 * (size_t)n) * sizeof(double) +
+         sizeof(float) * n + 256;
+}
+}  // namespace clu
+
+extern "C" {
+
+int charmlu_dgesv_ir(int n, const double *dA, long lda, const float *dLU, long ldlu,
+                     const int *dipiv, const double *db, double *dx, double tol, int max_iter,
+                     int restart, double *stats, void *stream) {
+  if (n < 0) return -1;
+  if (!dA && n > 0) return -2;
+  if (lda < (n > 1 ? n : 1)) return -3;
+  if (!dLU && n > 0) return -4;
+  if (ldlu < (n > 1 ? n : 1)) return -5;
+  if (!dipiv && n > 0) return -6;
+  if (!db && n > 0) return -7;
+  if (!dx && n > 0) return -8;
+  if (!(tol > 0.0)) return -9;
+  if (max_iter < 0) return -10;
+  if (restart < 1 || restart > 200) return -11;
+  hipStream_t s = (hipStream_t)stream;
+  if (stats) for (int i = 0; i < 4; ++i) stats[i] = 0.0;
+  if (n == 0) return CLU_OK;
+  Workspace *wsp;
+  int rc;
+  if ((rc = get_ws(&wsp))) return rc;
+  Workspace &ws = *wsp;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure_bytes(&ws.irbuf, &ws.irbuf_b, irbuf_bytes(n, restart)))) return rc;
+  }
+  bool inv_ready = false;
+  return gmres_ir(false, n, dA, lda, dLU, ldlu, dipiv, db, dx, tol, max_iter, restart, stats,
+                  (double *)ws.irbuf, inv_ready, s);
+}
+
+int charmlu_dgesv_ir_ex(int trans, int n, int nrhs, const double *dA, long lda, const float *dLU,
+                        long ldlu, const int *dipiv, const double *dB, long ldb, double *dX,
+                        long ldx, double tol, int max_iter, int restart, double *stats,
+                        void *stream) {
+  if (trans != CHARMLU_NO_TRANS && trans != CHARMLU_TRANS) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (!dA && n > 0) return -4;
+  if (lda < (n > 1 ? n : 1)) return -5;
+  if (!dLU && n > 0) return -6;
+  if (ldlu < (n > 1 ? n : 1)) return -7;
+  if (!dipiv && n > 0) return -8;
+  if (!dB && n > 0 && nrhs > 0) return -9;
+  if (ldb < (nrhs > 1 ? nrhs : 1)) return -10;
+  if (!dX && n > 0 && nrhs > 0) return -11;
+  if (ldx < (nrhs > 1 ? nrhs : 1)) return -12;
+  if (!(tol > 0.0)) return -13;
+  if (max_iter < 0) return -14;
+  if (restart < 1 || restart > 200) return -15;
+  hipStream_t s = (hipStream_t)stream;
+  if (stats) for (int i = 0; i < 4; ++i) stats[i] = 0.0;
+  if (n == 0 || nrhs == 0) return CLU_OK;
+  Workspace *wsp;
+  int rc;
+  if ((rc = get_ws(&wsp))) return rc;
+  Workspace &ws = *wsp;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure_bytes(&ws.irbuf, &ws.irbuf_b, irbuf_bytes(n, restart)))) return rc;
+  }
+  double *buf = (double *)ws.irbuf;
+  // a column's right-hand side and solution, contiguous, after the
+  // refinement's own vectors
+  const size_t small = 2 * (size_t)(restart + 1) + 16;
+  double *bcol = buf + (size_t)(restart + 3) * n + small + (size_t)(n + 1) / 2 + 1;
+  double *xcol = bcol + n;
+  // the single-vector layout puts n floats right after the doubles: keep the
+  // columns past them (the float area rounded up to whole doubles, plus one)
+  bool inv_ready = false;
+  const bool tr = trans == CHARMLU_TRANS;
+  for (int k = 0; k < nrhs; ++k) {
+    double st[4] = {0, 0, 0, 0};
+    if ((rc = convert<double, double>(n, 1, dB + k, ldb, bcol, 1, s))) return rc;
+    if ((rc = gmres_ir(tr, n, dA, lda, dLU, ldlu, dipiv, bcol, xcol, tol, max_iter, restart, st,
+                       buf, inv_ready, s)))
+      return rc;
+    if ((rc = convert<double, double>(n, 1, xcol, 1, dX + k, ldx, s))) return rc;
+    if (stats) {
+      stats[0] = std::max(stats[0], st[0]);
+      stats[1] += st[1];
+      stats[2] = std::max(stats[2], st[2]);
+      stats[3] = std::max(stats[3], st[3]);
+    }
+  }
+  return CLU_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 // ---- generation and validation ---------------------------------------------------
 int charmlu_dgen_matrix(int kind, uint64_t seed, long n, double *dA, long lda, void *stream) {

@@ -149,6 +149,37 @@ int residual(int n, const double *A, long lda, const double *x, const double *b,
   return CLU_OK;
 }
 
+// out[0] = max_ij |a_ij|, out[1] = max_j sum_i |a_ij| (the infinity norm
+// of A^T): one thread per column, rows in order (coalesced across lanes).
+__global__ __launch_bounds__(256) void abs_colsum_kernel(int n, const double *__restrict__ A,
+                                                         long lda, double *__restrict__ out) {
+  const long j = (long)blockIdx.x * 256 + threadIdx.x;
+  double cs = 0.0, cm = 0.0;
+  if (j < n) {
+    for (long i = 0; i < n; ++i) {
+      const double a = fabs(A[i * lda + j]);
+      cs += a;
+      cm = fmax(cm, a);
+    }
+  }
+  cs = wave_max(cs);
+  cm = wave_max(cm);
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_nonneg(&out[0], cm);
+    atomic_max_nonneg(&out[1], cs);
+  }
+}
+
+int abs_colsum(int n, const double *A, long lda, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(residual_init_kernel, dim3(1), dim3(64), 0, s, out);
+  CLU_LAUNCHED("residual_init_kernel");
+  if (n <= 0) return CLU_OK;
+  hipLaunchKernelGGL(abs_colsum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, A,
+                     lda, out);
+  CLU_LAUNCHED("abs_colsum_kernel");
+  return CLU_OK;
+}
+
 template <typename TD, typename TS>
 __global__ void convert_kernel(long m, long n, const TS *__restrict__ src, long lds,
                                TD *__restrict__ dst, long ldd) {
@@ -184,6 +215,16 @@ int fill_int(int *p, int v, long n, hipStream_t s) {
   if (n <= 0) return CLU_OK;
   hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
   CLU_LAUNCHED("fill_int_kernel");
+  return CLU_OK;
+}
+
+__global__ void finish_info_kernel(int *info, const unsigned *abortw) {
+  if (threadIdx.x == 0 && abortw[0] != 0u) info[0] = -1;
+}
+
+int finish_info(int *info, const unsigned *abortw, hipStream_t s) {
+  hipLaunchKernelGGL(finish_info_kernel, dim3(1), dim3(64), 0, s, info, abortw);
+  CLU_LAUNCHED("finish_info_kernel");
   return CLU_OK;
 }
 

@@ -35,7 +35,7 @@
 
 namespace clu {
 
-// Strip width SW: 16 or 32 columns.  A 32-column strip halves the strip
+// Strip width SW: 8, 16 or 32 columns.  A 32-column strip halves the strip
 // launches of a panel and removes the 32-column nodes of its recursion (a
 // triangular solve and a GEMM each).  FP32 strips of at most 32 * 2048 rows
 // are 32 wide (4 rows of 32 floats per thread; taller ones need 8 rows per
@@ -45,7 +45,10 @@ namespace clu {
 // against 2 x 36 us plus the node at 512 rows, 120-131 us against
 // 2 x 62-66 us plus the node at 2048-8192 rows; from 513 to 1024 rows the
 // one workgroup of 512 threads this would need is slower (119 us against
-// 2 x 42).
+// 2 x 42).  8-column strips (8 rows of 8 doubles or 16 rows of 8 floats per
+// thread, 512-thread workgroups) are the fallback for strips too tall for
+// the wider ones on the workgroups that can be resident at once.
+constexpr int PW8 = 8;      // the fallback strip
 constexpr int PW = 16;      // the narrow strip
 constexpr int PW32 = 32;    // the wide strip
 
@@ -78,7 +81,8 @@ CLU_DEV unsigned row_word(const T *row, int k) {  // word k of an LDS row
 template <typename T, int R, int NT, bool MULTI, bool PIV, int SW>
 __global__ __launch_bounds__(NT) void panel_base_kernel(
     T *__restrict__ A, long lda, int j0, int m, int w, int pc0, int pc1,
-    int *__restrict__ ipiv, unsigned long long *__restrict__ recs, unsigned seq0, int *info) {
+    int *__restrict__ ipiv, unsigned long long *__restrict__ recs, unsigned seq0, int *info,
+    unsigned *abortw) {
   static_assert(PIV || !MULTI, "the unpivoted strip is one workgroup");
   constexpr int VW = sizeof(T) / 4;      // payload words per value
   constexpr int CW = SW * VW;            // words of a row
@@ -251,20 +255,25 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
         }
         if (poller) {
           // poll the other records (and workgroup 0's top row) until every
-          // word carries this column's sequence number; bounded, so a lost
-          // record cannot hang the kernel (info then reports -1)
+          // word carries this column's sequence number.  Bounded: after
+          // SPIN_MAX rounds the wave raises the factorization's abort word
+          // and goes on with what it has; every poll of every strip of the
+          // same factorization then stops at its first incomplete round, so
+          // a lost record costs one bound per factorization, not one per
+          // column (the factorization then reports info = -1).
           const unsigned long long *prec = recs + (long)par * GMAX * RW;
           unsigned long long wv[CHW];
           bool ok = false;
           for (int spin = 0; !ok; ++spin) {
 #pragma clang loop unroll(full)
             for (int i = 0; i < CHW; ++i) wv[i] = poff[i] >= 0 ? ld_agent(prec + poff[i]) : sq;
+            const bool aborted = __any(ld_agent(abortw) != 0u);
             ok = true;
 #pragma clang loop unroll(full)
             for (int i = 0; i < CHW; ++i) ok = ok && (wv[i] & 0xffffffff00000000ull) == sq;
             ok = __all(ok);
-            if (!ok && spin >= SPIN_MAX) {
-              if (lane == 0) *info = -1;
+            if (!ok && (aborted || spin >= SPIN_MAX)) {
+              if (lane == 0 && !aborted) st_agent(abortw, 1u);
               ok = true;
             }
             if (!ok) __builtin_amdgcn_s_sleep(1);
@@ -422,95 +431,154 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
   }
 }
 
+// ---- launch shapes ---------------------------------------------------------
+// A strip of m rows and sw columns runs as G workgroups of NT threads with R
+// rows per thread (R * NT rows each).  G > 1 needs every workgroup resident
+// at once (they exchange a record per column) and the polling waves must
+// cover the G records in one round.  Candidates in order of preference
+// (measured fastest first); the first that fits is taken.
+namespace {
+struct Cand { int R, NT; };
+constexpr int poll_words(int NT) { return (NT / 64) * 64 * (NT >= 512 ? 4 : 5); }
+
 template <typename T>
-int panel_strip_width(int m) {
-  if (sizeof(T) == 4) return m <= 32 * 2048 ? PW32 : PW;
-  return m <= 512 || (m > 1024 && m <= 18 * 512) ? PW32 : PW;
+bool shape_fits(int m, int sw, Cand c, int max_groups, PanelShape *ps) {
+  const int rows = c.R * c.NT;
+  const int G = (m + rows - 1) / rows;
+  const int vw = (int)sizeof(T) / 4, l1 = sw * vw + vw + 1;
+  if (G > max_groups || G > GMAX) return false;
+  if (G > 1 && G * l1 + sw * vw > poll_words(c.NT)) return false;
+  // FP64 32-column strips between 513 and 1024 rows would need one
+  // 512-thread workgroup: not instantiated (slower than two 16-column strips)
+  if (sizeof(T) == 8 && sw == PW32 && G > 1 && m <= 1024) return false;
+  ps->sw = sw;
+  ps->R = c.R;
+  ps->NT = c.NT;
+  ps->G = G;
+  return true;
 }
-template int panel_strip_width<double>(int);
-template int panel_strip_width<float>(int);
+
+template <typename T>
+bool plan_sw(int m, int sw, int max_groups, bool pivot, PanelShape *ps) {
+  if (m <= 0) return false;
+  if (!pivot) {  // one workgroup of 256 threads (the unpivoted strip)
+    if (m > 1024 || sw == PW8 || (sizeof(T) == 8 && sw == PW32)) return false;
+    return shape_fits<T>(m, sw, Cand{m <= 256 ? 1 : m <= 512 ? 2 : 4, 256}, 1, ps);
+  }
+  if (sw == PW8) {  // tall strips only
+    return shape_fits<T>(m, sw, Cand{sizeof(T) == 8 ? 8 : 16, 512}, max_groups, ps);
+  }
+  if (sizeof(T) == 8 && sw == PW32)
+    return shape_fits<T>(m, sw, Cand{2, 256}, max_groups, ps);
+  // one workgroup up to 2048 rows (256 threads up to 1024, then 512)
+  if (m <= 256) return shape_fits<T>(m, sw, Cand{1, 256}, 1, ps);
+  if (m <= 512) return shape_fits<T>(m, sw, Cand{2, 256}, 1, ps);
+  if (m <= 1024) return shape_fits<T>(m, sw, Cand{4, 256}, 1, ps);
+  if (m <= 2048) return shape_fits<T>(m, sw, Cand{4, 512}, 1, ps);
+  // taller: 256-thread workgroups of 1024 rows while they fit, else
+  // 512-thread ones of 2048 rows, else (FP32, 16 columns) 4096 rows
+  if (shape_fits<T>(m, sw, Cand{4, 256}, max_groups, ps)) return true;
+  if (shape_fits<T>(m, sw, Cand{4, 512}, max_groups, ps)) return true;
+  return sizeof(T) == 4 && sw == PW && shape_fits<T>(m, sw, Cand{8, 512}, max_groups, ps);
+}
+}  // namespace
+
+template <typename T>
+int panel_strip_width(int m, int max_groups, bool pivot) {
+  PanelShape ps;
+  // the measured policy for 32 columns (FP32 up to 65536 rows; FP64 up to
+  // 512 and from 1025 to 9216), then 16, then the 8-column fallback
+  const bool want32 = sizeof(T) == 4 ? m <= 32 * 2048 : (m <= 512 || (m > 1024 && m <= 18 * 512));
+  if (want32 && plan_sw<T>(m, PW32, max_groups, pivot, &ps)) return PW32;
+  if (plan_sw<T>(m, PW, max_groups, pivot, &ps)) return PW;
+  if (plan_sw<T>(m, PW8, max_groups, pivot, &ps)) return PW8;
+  return 0;
+}
+template int panel_strip_width<double>(int, int, bool);
+template int panel_strip_width<float>(int, int, bool);
+
+template <typename T>
+int panel_plan(int m, int w, int max_groups, bool pivot, PanelShape *ps) {
+  if (m <= 0 || w <= 0 || w > PW32 || !ps) return -1;
+  const int widest = panel_strip_width<T>(m, max_groups, pivot);
+  if (widest == 0 || w > widest) return -1;
+  const int sw = w <= PW8 && widest == PW8 ? PW8 : (w <= PW ? PW : PW32);
+  if (!plan_sw<T>(m, sw, max_groups, pivot, ps)) return -1;
+  return CLU_OK;
+}
+template int panel_plan<double>(int, int, int, bool, PanelShape *);
+template int panel_plan<float>(int, int, int, bool, PanelShape *);
 
 template <typename T>
 int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *ipiv,
                PanelWork &pw, hipStream_t s, bool pivot) {
   const int m = m_end - j0;
   if (m <= 0 || w <= 0) return CLU_OK;
-  const int sw = w <= PW ? PW : PW32;
-  if (w > panel_strip_width<T>(m) && w > PW) return -1;
   if (m < w) return -4;  // row c of every column lies in the strip
-  if (!pivot && m > 1024) return -3;  // the unpivoted strip is one workgroup
-  // One 256-thread workgroup up to 1024 rows, one 512-thread workgroup up
-  // to 2048 (3.2-3.9 us per column against 4.0 for two workgroups, whose
-  // exchange goes through memory).  Taller strips: 256-thread workgroups of
-  // 1024 rows while one per panel CU suffices, else 512-thread ones of 2048
-  // rows (4-wave workgroups spend 23 % less per column, but an exchange
-  // among 64 of them costs more than it saves); FP32 strips too tall for
-  // either use 8 rows per thread (16 columns only).  All workgroups of a
-  // strip must be resident at once (one per panel CU), and the polling
-  // waves must cover G records in one round.
-  const int vw = (int)sizeof(T) / 4, l1 = sw * vw + vw + 1;
-  auto poll_ok = [&](int g, int nt) {
-    return g * l1 + sw * vw <= (nt / 64) * 64 * (nt >= 512 ? 4 : 5);
-  };
-  int R, G, NT;
-  if (m <= 256) { R = 1; G = 1; NT = 256; }
-  else if (m <= 512) { R = 2; G = 1; NT = 256; }
-  else if (m <= 1024) { R = 4; G = 1; NT = 256; }
-  else if (m <= 2048 && pivot) { R = 4; G = 1; NT = 512; }
-  else {
-    R = 4; NT = 256; G = (m + 1023) / 1024;
-    if (G > pw.max_groups || !poll_ok(G, 256)) { NT = 512; G = (m + 2047) / 2048; }
-    if (pivot && sizeof(T) == 4 && G > pw.max_groups) { R = 8; NT = 512; G = (m + 4095) / 4096; }
+  PanelShape ps;
+  if (panel_plan<T>(m, w, pw.max_groups, pivot, &ps)) {
+    set_error("no launch shape for a %d x %d panel strip with %d resident workgroups", m, w,
+              pw.max_groups);
+    return CLU_ERR_UNSUPPORTED;
   }
-  if (sizeof(T) == 8 && sw == PW32) {
-    // FP64 32-column strips: 2 rows per thread, one workgroup up to 512
-    // rows, else 256-thread workgroups of 512 rows
-    if (!pivot) return -2;
-    R = 2; NT = 256; G = m <= 512 ? 1 : (m + 511) / 512;
-    if (G > 1 && m <= 1024) return -2;
-  }
-  if (G > pw.max_groups || G > GMAX) return -2;
-  if (G > 1 && !poll_ok(G, NT)) return -2;
-  if (R == 8 && sw != PW) return -2;
+  const int G = ps.G;
   unsigned long long *recs = reinterpret_cast<unsigned long long *>(pw.slots);
   const unsigned seq0 = pw.epoch;
-#define CLU_PANEL(TT, RR, NN, MU, PV, SWW)                                                      \
-  hipLaunchKernelGGL((panel_base_kernel<TT, RR, NN, MU, PV, SWW>), dim3(G), dim3(NN), 0, s,     \
-                     (TT *)A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info)
-#define CLU_PANEL_ALL(TT, SWW)                                                                 \
-  if (!pivot) {                                                                                \
-    switch (R) {                                                                               \
-      case 1: CLU_PANEL(TT, 1, 256, false, false, SWW); break;                                 \
-      case 2: CLU_PANEL(TT, 2, 256, false, false, SWW); break;                                 \
-      default: CLU_PANEL(TT, 4, 256, false, false, SWW); break;                                \
-    }                                                                                          \
-  } else if (G > 1 && NT == 256) {                                                             \
-    CLU_PANEL(TT, 4, 256, true, true, SWW);                                                    \
-  } else if (G > 1) {                                                                          \
-    CLU_PANEL(TT, 4, 512, true, true, SWW);                                                    \
-  } else if (NT == 512) {                                                                      \
-    CLU_PANEL(TT, 4, 512, false, true, SWW);                                                   \
-  } else {                                                                                     \
-    switch (R) {                                                                               \
-      case 1: CLU_PANEL(TT, 1, 256, false, true, SWW); break;                                  \
-      case 2: CLU_PANEL(TT, 2, 256, false, true, SWW); break;                                  \
-      default: CLU_PANEL(TT, 4, 256, false, true, SWW); break;                                 \
-    }                                                                                          \
+  bool launched = true;
+#define CLU_PANEL(RR, NN, MU, PV, SWW)                                                         \
+  hipLaunchKernelGGL((panel_base_kernel<T, RR, NN, MU, PV, SWW>), dim3(G), dim3(NN), 0, s,     \
+                     A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info, pw.abortw)
+  // the instantiated shapes: (R, NT, multi-workgroup, pivoting, strip width)
+  const bool multi = G > 1;
+  if (!pivot) {
+    if (ps.sw == PW) {
+      if (ps.R == 1) CLU_PANEL(1, 256, false, false, PW);
+      else if (ps.R == 2) CLU_PANEL(2, 256, false, false, PW);
+      else CLU_PANEL(4, 256, false, false, PW);
+    } else if constexpr (sizeof(T) == 4) {
+      if (ps.R == 1) CLU_PANEL(1, 256, false, false, PW32);
+      else if (ps.R == 2) CLU_PANEL(2, 256, false, false, PW32);
+      else CLU_PANEL(4, 256, false, false, PW32);
+    } else {
+      launched = false;
+    }
+  } else if (ps.sw == PW8) {
+    if (!multi) launched = false;
+    else if constexpr (sizeof(T) == 8) CLU_PANEL(8, 512, true, true, PW8);
+    else CLU_PANEL(16, 512, true, true, PW8);
+  } else if (ps.sw == PW32) {
+    if constexpr (sizeof(T) == 8) {
+      if (multi) CLU_PANEL(2, 256, true, true, PW32);
+      else CLU_PANEL(2, 256, false, true, PW32);
+    } else {
+      if (multi && ps.NT == 256) CLU_PANEL(4, 256, true, true, PW32);
+      else if (multi) CLU_PANEL(4, 512, true, true, PW32);
+      else if (ps.NT == 512) CLU_PANEL(4, 512, false, true, PW32);
+      else if (ps.R == 1) CLU_PANEL(1, 256, false, true, PW32);
+      else if (ps.R == 2) CLU_PANEL(2, 256, false, true, PW32);
+      else CLU_PANEL(4, 256, false, true, PW32);
+    }
+  } else {  // 16 columns
+    if (ps.R == 8) {
+      if constexpr (sizeof(T) == 4) {
+        if (multi) CLU_PANEL(8, 512, true, true, PW);
+        else launched = false;
+      } else {
+        launched = false;
+      }
+    } else if (multi && ps.NT == 256) CLU_PANEL(4, 256, true, true, PW);
+    else if (multi) CLU_PANEL(4, 512, true, true, PW);
+    else if (ps.NT == 512) CLU_PANEL(4, 512, false, true, PW);
+    else if (ps.R == 1) CLU_PANEL(1, 256, false, true, PW);
+    else if (ps.R == 2) CLU_PANEL(2, 256, false, true, PW);
+    else CLU_PANEL(4, 256, false, true, PW);
   }
-  if (R == 8) {
-    if (sizeof(T) != 4 || G <= 1) return -2;
-    CLU_PANEL(float, 8, 512, true, true, PW);
-  } else if (sizeof(T) == 8 && sw == PW32) {
-    if (G > 1) CLU_PANEL(double, 2, 256, true, true, PW32);
-    else CLU_PANEL(double, 2, 256, false, true, PW32);
-  } else if (sw == PW) {
-    CLU_PANEL_ALL(T, PW)
-  } else {
-    if (sizeof(T) != 4) return -1;
-    CLU_PANEL_ALL(float, PW32)
-  }
-#undef CLU_PANEL_ALL
 #undef CLU_PANEL
+  if (!launched) {
+    set_error("panel strip shape (R %d, NT %d, G %d, width %d) not instantiated", ps.R, ps.NT, G,
+              ps.sw);
+    return CLU_ERR_UNSUPPORTED;
+  }
   CLU_LAUNCHED("panel_base_kernel");
   pw.epoch += (unsigned)w;
   return CLU_OK;

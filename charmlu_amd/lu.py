@@ -43,15 +43,46 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else None
 
 
-def _require(t, dtype, name):
+def _require(t, dtype, name, shape=None):
+    """Check a tensor argument before its device pointer goes to the C ABI:
+    a CUDA tensor of ``dtype``, row-major (unit stride along rows; a vector
+    contiguous), of ``shape`` when given."""
     torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
     if not t.is_cuda:
         raise ValueError(f"{name} must be a CUDA tensor")
     if t.dtype != dtype:
         raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
     if t.dim() == 2 and t.stride(1) != 1:
         raise ValueError(f"{name} must be row-major (stride(1) == 1)")
+    if t.dim() == 1 and t.numel() > 1 and t.stride(0) != 1:
+        raise ValueError(f"{name} must be contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
     return t
+
+
+def _require_ipiv(ipiv, n):
+    """Pivot vectors are int32 CUDA tensors of n entries."""
+    torch = _torch()
+    return _require(ipiv, torch.int32, "ipiv", (n,))
+
+
+def check_info(info, what: str = "factorization"):
+    """Raise on a nonzero info word: j+1 > 0 a near-singular pivot at column
+    j (the reference aborts there, lu.ci:339-341), -1 a panel exchange that
+    timed out (the factors are invalid)."""
+    torch = _torch()
+    v = int(info.item()) if isinstance(info, torch.Tensor) else int(info)
+    if v > 0:
+        raise torch.linalg.LinAlgError(
+            f"{what}: singular pivot at column {v - 1} (|u_jj| <= 100 eps)")
+    if v < 0:
+        raise _lib.CharmluError(
+            f"{what}: a panel strip's cross-workgroup exchange timed out (info {v}); "
+            "the factors are invalid")
+    return v
 
 
 # ---------------------------------------------------------------------------
@@ -98,8 +129,12 @@ def getrs(LU, ipiv, B, trans: bool = False):
     torch = _torch()
     _require(LU, torch.float64, "LU")
     n = LU.shape[0]
+    _require(LU, torch.float64, "LU", (n, n))
+    _require_ipiv(ipiv, n)
+    _require(B, torch.float64, "B")
+    if B.shape[0] != n or B.dim() not in (1, 2):
+        raise ValueError(f"B must be ({n},) or ({n}, nrhs)")
     B2 = B.view(n, 1) if B.dim() == 1 else B
-    _require(B2, torch.float64, "B")
     nrhs = B2.shape[1]
     check(_lib.load().charmlu_dgetrs(_lib.TRANS if trans else _lib.NO_TRANS, n, nrhs, _ptr(LU),
                                      LU.stride(0), _ptr(ipiv), _ptr(B2), B2.stride(0),
@@ -152,8 +187,12 @@ def getrs32(LU32, ipiv, B, trans: bool = False):
     torch = _torch()
     _require(LU32, torch.float32, "LU32")
     n = LU32.shape[0]
+    _require(LU32, torch.float32, "LU32", (n, n))
+    _require_ipiv(ipiv, n)
+    _require(B, torch.float32, "B")
+    if B.shape[0] != n or B.dim() not in (1, 2):
+        raise ValueError(f"B must be ({n},) or ({n}, nrhs)")
     B2 = B.view(n, 1) if B.dim() == 1 else B
-    _require(B2, torch.float32, "B")
     check(_lib.load().charmlu_sgetrs(_lib.TRANS if trans else _lib.NO_TRANS, n, B2.shape[1],
                                      _ptr(LU32), LU32.stride(0), _ptr(ipiv), _ptr(B2),
                                      B2.stride(0), _stream_ptr()), "charmlu_sgetrs")
@@ -167,8 +206,13 @@ def gesv_ir(A, LU32, ipiv, b, x=None, tol: float = 1.0, max_iter: int = 30,
     torch = _torch()
     _require(A, torch.float64, "A")
     n = A.shape[0]
+    _require(A, torch.float64, "A", (n, n))
+    _require(LU32, torch.float32, "LU32", (n, n))
+    _require_ipiv(ipiv, n)
+    _require(b, torch.float64, "b", (n,))
     if x is None:
         x = torch.empty(n, dtype=torch.float64, device=A.device)
+    _require(x, torch.float64, "x", (n,))
     st = np.zeros(4, dtype=np.float64)
     check(_lib.load().charmlu_dgesv_ir(n, _ptr(A), A.stride(0), _ptr(LU32), LU32.stride(0),
                                        _ptr(ipiv), _ptr(b), _ptr(x), float(tol), int(max_iter),
@@ -178,11 +222,48 @@ def gesv_ir(A, LU32, ipiv, b, x=None, tol: float = 1.0, max_iter: int = 30,
                    residual_ns=float(st[3]))
 
 
+def gesv_ir_multi(A, LU32, ipiv, B, X=None, trans: bool = False, tol: float = 1.0,
+                  max_iter: int = 30, restart: int = 50):
+    """FP64 GMRES-IR for op(A) X = B with several right-hand sides (B: n x
+    nrhs FP64), preconditioned by the FP32 factors of ``getrf_mixed``;
+    trans=True solves A^T X = B from the same factors.  Returns (X, stats
+    dict: most cycles of any column, inner iterations of all columns,
+    largest final residuals)."""
+    torch = _torch()
+    _require(A, torch.float64, "A")
+    n = A.shape[0]
+    _require(A, torch.float64, "A", (n, n))
+    _require(LU32, torch.float32, "LU32", (n, n))
+    _require_ipiv(ipiv, n)
+    _require(B, torch.float64, "B")
+    if B.shape[0] != n or B.dim() not in (1, 2):
+        raise ValueError(f"B must be ({n},) or ({n}, nrhs)")
+    B2 = B.view(n, 1) if B.dim() == 1 else B
+    if X is None:
+        X = torch.empty_like(B, memory_format=torch.contiguous_format)
+    _require(X, torch.float64, "X", tuple(B.shape))
+    X2 = X.view(n, 1) if X.dim() == 1 else X
+    st = np.zeros(4, dtype=np.float64)
+    check(_lib.load().charmlu_dgesv_ir_ex(_lib.TRANS if trans else _lib.NO_TRANS, n,
+                                          B2.shape[1], _ptr(A), A.stride(0), _ptr(LU32),
+                                          LU32.stride(0), _ptr(ipiv), _ptr(B2),
+                                          max(B2.stride(0), 1), _ptr(X2), max(X2.stride(0), 1),
+                                          float(tol), int(max_iter), int(restart),
+                                          ctypes_void(st.ctypes.data), _stream_ptr()),
+          "charmlu_dgesv_ir_ex")
+    return X, dict(cycles=int(st[0]), iterations=int(st[1]), residual=float(st[2]),
+                   residual_ns=float(st[3]))
+
+
 def residual(A, x, b):
     """Reference scaled residual (benchmark.C:385-400) and the normwise form
     ||Ax-b||_inf / (||A||_inf ||x||_inf n eps).  Returns a dict."""
     torch = _torch()
+    _require(A, torch.float64, "A")
     n = A.shape[0]
+    _require(A, torch.float64, "A", (n, n))
+    _require(x, torch.float64, "x", (n,))
+    _require(b, torch.float64, "b", (n,))
     out = torch.zeros(5, dtype=torch.float64, device=A.device)
     check(_lib.load().charmlu_dresidual(n, _ptr(A), A.stride(0), _ptr(x), _ptr(b), None,
                                         _ptr(out), _stream_ptr()), "charmlu_dresidual")
@@ -262,6 +343,8 @@ def run(cfg: LUConfig, seed_a: int = SEED_A, seed_b: int = SEED_B, kind: str = "
     stop.record()
     torch.cuda.synchronize()
     seconds = start.elapsed_time(stop) / 1e3
+    if int(info.item()) < 0:
+        check_info(info, "getrf")  # a timed-out panel exchange: the factors are invalid
     res = residual(A, x, rhs)
     out = dict(n=n, block=b, seconds=seconds, info=int(info.item()),
                gflops_hpl=hpl_gflops(n, seconds),

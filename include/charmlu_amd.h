@@ -21,8 +21,14 @@
  *     style), CHARMLU_ERR_HIP if a HIP call failed, CHARMLU_ERR_NOMEM if the
  *     library could not allocate its workspace, CHARMLU_ERR_CONFIG if an
  *     environment setting of the library is out of range (CHARMLU_PANEL_CUS:
- *     0 to 32).  charmlu_last_error() gives a message.  Factorization status (singular pivot) is reported through
- *     an info word, never by aborting (the reference aborts, lu.ci:339-341).
+ *     0 to 32), CHARMLU_ERR_UNSUPPORTED if the problem is too large for the
+ *     panel CUs (the first panel strip needs more workgroups than can be
+ *     resident at once: with 32 panel CUs, n <= 131072 in FP64 and
+ *     n <= 262144 in FP32; charmlu_panel_plan tells).  charmlu_last_error()
+ *     gives a message.  A factorization that fails after it has enqueued
+ *     work orders the caller's stream after all of it before returning.
+ *     Factorization status (singular pivot) is reported through an info
+ *     word, never by aborting (the reference aborts, lu.ci:339-341).
  *   - Device-resident calls on one device share that device's workspace:
  *     calls that overlap in time on one device race.
  */
@@ -39,6 +45,7 @@ extern "C" {
 #define CHARMLU_ERR_HIP (-100)
 #define CHARMLU_ERR_NOMEM (-101)
 #define CHARMLU_ERR_CONFIG (-102)
+#define CHARMLU_ERR_UNSUPPORTED (-103)
 
 /* CBLAS enumerations (public CBLAS values) */
 #define CHARMLU_ROW_MAJOR 101
@@ -79,6 +86,19 @@ int charmlu_profile(int on);
 int charmlu_profile_read(double *out, int nout);
 /* Make device `device` current for this thread and set up its workspace. */
 int charmlu_init(int device);
+/* Wait for every device the library has used, then release its streams,
+ * events and workspaces (no counterpart in the reference, whose Charm++
+ * runtime owns its resources).  The next call sets them up again.  The
+ * Python binding calls it at interpreter exit. */
+int charmlu_finalize(void);
+/* Launch shape of one panel strip of m rows and w columns (no counterpart
+ * in the reference; host arithmetic only, callable without a device):
+ * prec CHARMLU_PREC_FP64 or CHARMLU_PREC_FP32, panel_cus as
+ * CHARMLU_PANEL_CUS (0: no CU masks), ncu the device's CU count, pivot 0 or
+ * 1.  out[6] = {kernel strip width, rows per thread, threads per
+ * workgroup, workgroups, workgroups resident at once, widest strip for m
+ * rows (0: none)}.  Returns 0 or CHARMLU_ERR_UNSUPPORTED. */
+int charmlu_panel_plan(int prec, int m, int w, int panel_cus, int ncu, int pivot, int *out);
 
 /* ---- 1. platformBlas.h surface: host pointers, synchronous ------------- */
 
@@ -131,7 +151,9 @@ int charmlu_dger_dev(int M, int N, double alpha, const double *dx, long incx,
  * factorDiagBlock/factorBelowDiagBlock/factorAboveDiagBlock, lu.ci:31-370).
  * nb: block size (the reference's b; n % nb must be 0, benchmark.C:143-147).
  * dipiv: n device ints.  dinfo: one device int, 0 or the first column j+1
- * whose pivot has |u_jj| <= 100*eps (lu.ci:339-341).  Asynchronous. */
+ * whose pivot has |u_jj| <= 100*eps (lu.ci:339-341), or -1 if the
+ * cross-workgroup exchange of a panel strip timed out (a workgroup of the
+ * strip was not resident; the factors are then invalid).  Asynchronous. */
 int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo,
                    void *stream);
 
@@ -205,6 +227,20 @@ int charmlu_sgetrs(int trans, int n, int nrhs, const float *dLU, long lda,
 int charmlu_dgesv_ir(int n, const double *dA, long lda, const float *dLU, long ldlu,
                      const int *dipiv, const double *db, double *dx, double tol,
                      int max_iter, int restart, double *stats, void *stream);
+
+/* charmlu_dgesv_ir for op(A) X = B with several right-hand sides: trans
+ * NO_TRANS or TRANS (A^T X = B, the adjoint solve of the differentiable
+ * solve, from the same stored low-precision factors; the transposed
+ * block solve of forwardSolve/backwardSolve, lu.ci:387-458).  B and X are
+ * n x nrhs row-major (ldb, ldx >= nrhs); each column is refined on its own
+ * to the same stopping test, with r = b - op(A) x in FP64.  stats
+ * (optional, host, 4 doubles) = {most cycles of any column, inner
+ * iterations of all columns, largest final reference residual, largest
+ * final normwise residual (with ||A^T||_inf for TRANS)}.  Synchronous. */
+int charmlu_dgesv_ir_ex(int trans, int n, int nrhs, const double *dA, long lda, const float *dLU,
+                        long ldlu, const int *dipiv, const double *dB, long ldb, double *dX,
+                        long ldx, double tol, int max_iter, int restart, double *stats,
+                        void *stream);
 
 /* Seeded inputs of the driver (benchmark.C:430-460).
  * gen_matrix: entry (i,j) of the library's counter generator, kind 0 =

@@ -63,6 +63,7 @@ def lib():
         L.orc_getrf.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long,
                                 _dp, _ip, ctypes.c_int]
         L.orc_getrf.restype = ctypes.c_int
+        L.orc_set_tie_log.argtypes = [_ip, _dp, _dp]
         L.orc_getf2.argtypes = [ctypes.c_int, _dp, ctypes.c_long, _ip]
         L.orc_getf2.restype = ctypes.c_int
         L.orc_solve.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long, _dp]
@@ -70,6 +71,10 @@ def lib():
         L.orc_residual.argtypes = [ctypes.c_int, _dp, ctypes.c_long, _dp, _dp, _dp, _dp]
         L.orc_residual.restype = ctypes.c_double
         L.orc_wtime.restype = ctypes.c_double
+        L.orc_set_max_steps.argtypes = [ctypes.c_int]
+        L.orc_lapack_gesv.argtypes = [ctypes.c_char_p, ctypes.c_int, _dp, ctypes.c_long, _dp,
+                                      _ip, ctypes.c_int, _dp]
+        L.orc_lapack_gesv.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -180,13 +185,42 @@ def apply_cycles(nc, seq_index, seq, A, c0=0, c1=None):
 
 # ---- factorization / solve --------------------------------------------------
 def getrf(A: np.ndarray, b: int, batch: int = 0, rhs: np.ndarray | None = None,
-          nthreads: int = 1):
-    """In-place blocked LU; returns (info, ipiv)."""
+          nthreads: int = 1, ties: dict | None = None):
+    """In-place blocked LU; returns (info, ipiv).  With a dict ``ties``, fills
+    it with the near-tie log: ``mag1`` (|pivot| per column), ``row2`` and
+    ``mag2`` (the best other candidate row per column and its |value|)."""
     n = A.shape[0]
     ipiv = np.zeros(max(n, 1), np.int32)
-    info = lib().orc_getrf(n, b, batch, _d(A), A.shape[1],
-                           _d(rhs) if rhs is not None else None, _i(ipiv), nthreads)
+    if ties is not None:
+        row2 = np.full(max(n, 1), -1, np.int32)
+        mag2 = np.zeros(max(n, 1))
+        mag1 = np.zeros(max(n, 1))
+        lib().orc_set_tie_log(_i(row2), _d(mag2), _d(mag1))
+    try:
+        info = lib().orc_getrf(n, b, batch, _d(A), A.shape[1],
+                               _d(rhs) if rhs is not None else None, _i(ipiv), nthreads)
+    finally:
+        if ties is not None:
+            lib().orc_set_tie_log(None, None, None)
+    if ties is not None:
+        ties.update(row2=row2[:n], mag2=mag2[:n], mag1=mag1[:n])
     return info, ipiv[:n]
+
+
+def near_ties(ties: dict, ulps: float = 4.0) -> np.ndarray:
+    """Columns whose two best pivot candidates lie within ``ulps`` ulp of
+    each other (the pivot choice there is decided by rounding)."""
+    m1, m2 = ties["mag1"], ties["mag2"]
+    return np.nonzero((ties["row2"] >= 0) & (m1 - m2 <= ulps * np.spacing(m1)))[0]
+
+
+def is_near_tie(ties: dict, j: int, row: int, ulps: float = 4.0) -> bool:
+    """Is choosing ``row`` instead of the oracle's pivot at column ``j`` a
+    near-tie: the oracle's runner-up row, within ``ulps`` ulp of the pivot?"""
+    if int(ties["row2"][j]) != int(row):
+        return False
+    m1, m2 = float(ties["mag1"][j]), float(ties["mag2"][j])
+    return m1 - m2 <= ulps * float(np.spacing(m1))
 
 
 def getf2(A: np.ndarray):
@@ -221,18 +255,52 @@ def wtime() -> float:
     return lib().orc_wtime()
 
 
+def set_max_steps(k: int) -> None:
+    """Run only the first k block steps in later ``getrf`` calls (0: all)."""
+    lib().orc_set_max_steps(int(k))
+
+
+def sample_flops(n: int, b: int, steps: int) -> float:
+    """LU flops of the first ``steps`` block steps of an n x n factorization
+    with block b: 2/3 n^3 minus the 2/3 m^3 of the m = n - steps*b trailing
+    matrix left (the leading term of each)."""
+    m = max(n - steps * b, 0)
+    return 2.0 / 3.0 * (float(n) ** 3 - float(m) ** 3)
+
+
+def lapack_gesv(A: np.ndarray, bvec: np.ndarray, nthreads: int, path: str = MKL_PATH):
+    """The vendor LAPACK's multithreaded LU + solve of A x = b (in place on
+    copies); returns (info, x, seconds).  Call it in a process where the
+    oracle's sequential BLAS (``use_mkl``) is not loaded: the library's
+    threading can be chosen once per process."""
+    Aw = np.ascontiguousarray(A).copy()
+    x = np.ascontiguousarray(bvec).copy()
+    n = Aw.shape[0]
+    ipiv = np.zeros(max(n, 1), np.int32)
+    sec = np.zeros(1)
+    info = lib().orc_lapack_gesv(path.encode(), n, _d(Aw), Aw.shape[1], _d(x), _i(ipiv),
+                                 int(nthreads), _d(sec))
+    return info, x, float(sec[0])
+
+
 def lu_solve_problem(n: int, b: int, batch: int = 0, nthreads: int = 1,
-                     seed_a: int = SEED_A, seed_b: int = SEED_B, A: np.ndarray | None = None):
+                     seed_a: int = SEED_A, seed_b: int = SEED_B, A: np.ndarray | None = None,
+                     ties: bool = False, validate: bool = True):
     """The reference driver's whole run on the CPU: generate, factor, solve,
-    validate (benchmark.C:231-400).  Returns a dict."""
+    validate (benchmark.C:231-400).  Returns a dict (with ``ties``, the
+    near-tie log of ``getrf`` under the key "ties")."""
     A0 = gen_uniform(seed_a, n) if A is None else A
     bvec = gen_rhs(n, seed_b)
     LU = A0.copy()
     beta = bvec.copy()
+    tlog = {} if ties else None
     t0 = wtime()
-    info, ipiv = getrf(LU, b, batch, beta, nthreads)
+    info, ipiv = getrf(LU, b, batch, beta, nthreads, ties=tlog)
     solve(LU, b, beta)
     t1 = wtime()
-    r, ns, m4 = residual(A0, beta, bvec)
-    return dict(info=info, ipiv=ipiv, x=beta, LU=LU, residual=r, residual_ns=ns,
-                maxima=m4, seconds=t1 - t0, A=A0, b=bvec)
+    r, ns, m4 = residual(A0, beta, bvec) if validate else (None, None, None)
+    out = dict(info=info, ipiv=ipiv, x=beta, LU=LU, residual=r, residual_ns=ns,
+               maxima=m4, seconds=t1 - t0, A=A0, b=bvec)
+    if ties:
+        out["ties"] = tlog
+    return out

@@ -74,12 +74,14 @@ double orc_gen_value(uint64_t seed, long i, long j) {
 
 void orc_gen_uniform(uint64_t seed, long row0, long col0, long m, long n,
                      double *A, long lda) {
+#pragma omp parallel for schedule(static) if (m * n >= (1L << 22))
   for (long i = 0; i < m; ++i)
     for (long j = 0; j < n; ++j)
       A[i * lda + j] = orc_gen_value(seed, row0 + i, col0 + j);
 }
 
 void orc_gen_diagdom(uint64_t seed, long n, double *A, long lda) {
+#pragma omp parallel for schedule(static) if (n * n >= (1L << 22))
   for (long i = 0; i < n; ++i) {
     double s = 0.0;
     for (long j = 0; j < n; ++j) {
@@ -285,6 +287,35 @@ void orc_apply_cycles(int ncycles, const int *seq_index, const int *seq,
 
 #define AT(i, j) A[(long)(i) * lda + (j)]
 
+/* Optional near-tie log of orc_getrf: per global column j, |value| of the
+ * chosen pivot, and the row and |value| of the best other candidate (rows
+ * >= j, the same values the argmax saw; -1 / 0 if none).  A device pivot
+ * that differs from the oracle's is a near-tie when it is that row and the
+ * two magnitudes lie within a few ulp. */
+static int *g_tie_row = NULL;
+static double *g_tie_mag = NULL, *g_piv_mag = NULL;
+
+void orc_set_tie_log(int *row2, double *mag2, double *mag1) {
+  g_tie_row = row2;
+  g_tie_mag = mag2;
+  g_piv_mag = mag1;
+}
+
+/* argmax step with the runner-up: strict '>' keeps the first maximum
+ * (MaxElm, lu.C:197-218); an equal later value becomes the runner-up */
+static inline void cand2(double v, int i, double *cval, int *cloc, double *c2, int *c2loc) {
+  const double a = fabs(v);
+  if (a > fabs(*cval)) {
+    *c2 = fabs(*cval);
+    *c2loc = *cloc;
+    *cval = v;
+    *cloc = i;
+  } else if (a > *c2) {
+    *c2 = a;
+    *c2loc = i;
+  }
+}
+
 /* Emit one pivot batch and apply it to every column block but the active
  * panel (lu.ci:92-123 on the receiving blocks; the rhs copies were already
  * permuted inside the panel, lu.ci:290-323). */
@@ -305,6 +336,11 @@ static void flush_batch(int n, int b, int t, int count, const int *pos,
   free(seq);
 }
 
+/* Number of block steps orc_getrf runs (0: all): the CPU baseline times the
+ * first steps of a large factorization as a bounded sample of it. */
+static int g_max_steps = 0;
+void orc_set_max_steps(int k) { g_max_steps = k > 0 ? k : 0; }
+
 int orc_getrf(int n, int b, int batch, double *A, long lda, double *rhs,
               int *ipiv, int nthreads) {
   if (n < 0 || b <= 0) return -1;
@@ -318,21 +354,26 @@ int orc_getrf(int n, int b, int batch, double *A, long lda, double *rhs,
   int *bpos = (int *)malloc(sizeof(int) * b);
   int *brow = (int *)malloc(sizeof(int) * b);
 
-  for (int t = 0; t < nb; ++t) {
+  const int steps = g_max_steps > 0 && g_max_steps < nb ? g_max_steps : nb;
+  for (int t = 0; t < steps; ++t) {
     const int r0 = t * b, r1 = r0 + b;
     /* ---- panel: factorDiagBlock (lu.ci:256-370) + factorBelowDiagBlock
      *      (lu.ci:186-251) together over rows [r0, n) ---- */
     /* findMaxElm(0, 0) on every panel block (lu.C:197-205), reduced with
      * MaxElm_max (lu.C:207-218): strict '>' from {0.0, -1}. */
-    double cval = 0.0;
-    int cloc = -1;
-    for (int i = r0; i < n; ++i)
-      if (fabs(AT(i, r0)) > fabs(cval)) { cval = AT(i, r0); cloc = i; }
+    double cval = 0.0, c2 = 0.0;
+    int cloc = -1, c2loc = -1;
+    for (int i = r0; i < n; ++i) cand2(AT(i, r0), i, &cval, &cloc, &c2, &c2loc);
     int nrec = 0;
     for (int c = 0; c < b; ++c) {
       const int j = r0 + c;
       int p = cloc < 0 ? j : cloc; /* all-zero column: no row exchange */
       ipiv[j] = p;
+      if (g_tie_row) {
+        g_tie_row[j] = c2loc;
+        g_tie_mag[j] = c2;
+        g_piv_mag[j] = fabs(cval);
+      }
       /* swapLocal / pivotRowSwap + applySwap: full panel-width rows and
        * the rhs entry (lu.C:166-177, lu.ci:290-323) */
       if (p != j) {
@@ -351,6 +392,8 @@ int orc_getrf(int n, int b, int batch, double *A, long lda, double *rhs,
       if (fabs(piv) <= tiny && info == 0) info = j + 1;
       cval = 0.0;
       cloc = -1;
+      c2 = 0.0;
+      c2loc = -1;
       if (piv == 0.0) continue; /* nothing to eliminate with */
       /* computeMultipliersAndFindColMax (lu.C:259-288): division, eager
        * update of column j+1, candidate for the next column. */
@@ -359,7 +402,7 @@ int orc_getrf(int n, int b, int batch, double *A, long lda, double *rhs,
         for (int i = j + 1; i < n; ++i) {
           AT(i, j) = AT(i, j) / piv;
           AT(i, j + 1) -= AT(i, j) * u1;
-          if (fabs(AT(i, j + 1)) > fabs(cval)) { cval = AT(i, j + 1); cloc = i; }
+          cand2(AT(i, j + 1), i, &cval, &cloc, &c2, &c2loc);
         }
       } else {
         for (int i = j + 1; i < n; ++i) AT(i, j) = AT(i, j) / piv;
@@ -489,6 +532,35 @@ double orc_residual(int n, const double *A, long lda, const double *x,
   if (out4) { out4[0] = amax; out4[1] = bmax; out4[2] = xmax; out4[3] = rmax; }
   if (ns) *ns = rmax / (arow * xmax * n * DBL_EPSILON);
   return rmax / ((amax * xmax + bmax) * n * DBL_EPSILON);
+}
+
+/* The library LU the CPU baseline also reports (not the reference's
+ * algorithm): the vendor LAPACK's dgetrf + dgetrs on all given threads.
+ * The row-major A is A^T in column-major storage, so dgetrf factors A^T
+ * in place and dgetrs with trans = 'T' solves A x = b from those factors
+ * (no transposed copy).  Returns LAPACK's info (or -100 if the library or
+ * a symbol is missing); *seconds = factor + solve wall time. */
+typedef void (*dgetrf_t)(const int *, const int *, double *, const int *, int *, int *);
+typedef void (*dgetrs_t)(const char *, const int *, const int *, const double *, const int *,
+                         const int *, double *, const int *, int *, size_t);
+int orc_lapack_gesv(const char *libpath, int n, double *A, long lda, double *b, int *ipiv,
+                    int nthreads, double *seconds) {
+  void *h = dlopen(libpath, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return -100;
+  int (*set_layer)(int) = (int (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
+  if (set_layer) set_layer(3 /* MKL_THREADING_GNU: the oracle's own OpenMP runtime */);
+  void (*set_threads)(int) = (void (*)(int))dlsym(h, "MKL_Set_Num_Threads");
+  if (set_threads) set_threads(nthreads);
+  dgetrf_t getrf = (dgetrf_t)dlsym(h, "dgetrf_");
+  dgetrs_t getrs = (dgetrs_t)dlsym(h, "dgetrs_");
+  if (!getrf || !getrs) return -100;
+  const int ni = n, ld = (int)lda, one = 1;
+  int info = 0;
+  const double t0 = orc_wtime();
+  getrf(&ni, &ni, A, &ld, ipiv, &info);
+  if (info == 0) getrs("T", &ni, &one, A, &ld, ipiv, b, &ni, &info, 1);
+  *seconds = orc_wtime() - t0;
+  return info;
 }
 
 double orc_wtime(void) {

@@ -86,6 +86,10 @@ void orc_apply_cycles(int ncycles, const int *seq_index, const int *seq,
  * Charm++ PE per core). */
 int orc_getrf(int n, int b, int batch, double *A, long lda, double *rhs,
               int *ipiv, int nthreads);
+/* Near-tie log of the next orc_getrf calls (NULL pointers: off): for each
+ * global column j, mag1[j] = |pivot|, row2[j] / mag2[j] = the best other
+ * candidate row at that column (-1 / 0 if none).  Arrays of n entries. */
+void orc_set_tie_log(int *row2, double *mag2, double *mag1);
 /* Unblocked textbook LU with partial pivoting (LAPACK dgetf2 with division),
  * lowest index on ties.  Used as a brute-force pin for orc_getrf. */
 int orc_getf2(int n, double *A, long lda, int *ipiv);
@@ -102,6 +106,13 @@ double orc_residual(int n, const double *A, long lda, const double *x,
                     const double *b, double *out4, double *ns);
 
 double orc_wtime(void);
+/* Block steps of the next orc_getrf calls (0: all); a bounded sample. */
+void orc_set_max_steps(int k);
+/* The vendor LAPACK's multithreaded dgetrf + dgetrs (dlopen'ed from
+ * libpath) on row-major A and b: the library baseline beside the
+ * restatement.  Returns LAPACK's info; *seconds = factor + solve time. */
+int orc_lapack_gesv(const char *libpath, int n, double *A, long lda, double *b, int *ipiv,
+                    int nthreads, double *seconds);
 
 #ifdef __cplusplus
 }

@@ -6,7 +6,13 @@ checks residual < 16); these fixtures freeze the oracle's outputs on the
 reference's own test grid (test_lu.pl:9-52) at the sizes that are cheap,
 so later changes to the oracle or the device path are caught.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [--large]
+
+--large writes large_fixtures.npz instead: the oracle's pivots, x and
+near-tie log for the single-GPU FP64 configuration (n = 32768, block 256)
+and for block 1024 on a pivoting matrix (n = 8192), at sizes the oracle
+needs minutes for (the GPU tests compare against these files instead of
+running the oracle on the GPU host).
 """
 import os
 import sys
@@ -26,7 +32,39 @@ GRID = [
 ]
 
 
+# (n, b) of the large fixtures: C2 and the FP64 block of the mixed-precision
+# comparison (1024) on a uniform, pivoting matrix
+LARGE = [(32768, 256), (8192, 1024)]
+TIE_WINDOW = 1e-6  # columns whose two best candidates lie this close (relative) are kept
+
+
+def large(nthreads: int):
+    out = {}
+    oracle.use_mkl()
+    for n, b in LARGE:
+        r = oracle.lu_solve_problem(n, b, 0, nthreads=nthreads, ties=True)
+        t = r["ties"]
+        key = f"n{n}_b{b}"
+        out[key + "_ipiv"] = r["ipiv"].astype(np.int32)
+        out[key + "_x"] = r["x"]
+        out[key + "_res"] = np.array([r["residual"], r["residual_ns"]])
+        out[key + "_info"] = np.array([r["info"]], np.int32)
+        close = np.nonzero((t["row2"] >= 0) & (t["mag1"] - t["mag2"] <= TIE_WINDOW * t["mag1"]))[0]
+        out[key + "_tie_cols"] = close.astype(np.int32)
+        out[key + "_tie_row2"] = t["row2"][close].astype(np.int32)
+        out[key + "_tie_mag1"] = t["mag1"][close]
+        out[key + "_tie_mag2"] = t["mag2"][close]
+        print(f"n={n} b={b}: {r['seconds']:.1f} s, info {r['info']}, residual "
+              f"{r['residual']:.3f}, {len(close)} columns within {TIE_WINDOW:g}, "
+              f"{len(oracle.near_ties(t))} within 4 ulp", flush=True)
+    np.savez_compressed(os.path.join(HERE, "large_fixtures.npz"), **out)
+    print("wrote", len(out), "arrays")
+
+
 def main():
+    if "--large" in sys.argv:
+        large(os.cpu_count() or 1)
+        return
     out = {}
     for n, b, batch in GRID:
         r = oracle.lu_solve_problem(n, b, batch)

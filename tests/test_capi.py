@@ -89,3 +89,104 @@ def test_product_has_no_oracle_dependency():
             txt = open(path, errors="replace").read()
             for needle in ("import oracle", "from oracle", "liboracle", "lu_oracle", "orc_"):
                 assert needle not in txt, (path, needle)
+
+
+# ---------------------------------------------------------------------------
+# panel strip launch shapes (host arithmetic of charmlu_panel_plan)
+# ---------------------------------------------------------------------------
+def _plan(prec, m, w, cus, ncu=256, pivot=1):
+    from charmlu_amd import _lib
+    out = (ctypes.c_int * 6)()
+    rc = _lib.load().charmlu_panel_plan(prec, m, w, cus, ncu, pivot, out)
+    return rc, list(out)
+
+
+def _poll_words(nt):
+    return (nt // 64) * 64 * (4 if nt >= 512 else 5)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("cus", [0, 1, 4, 8, 16, 24, 32])
+def test_panel_plan_never_exceeds_resident_workgroups(prec, cus):
+    """Every strip shape the library launches keeps its workgroups within
+    what can be resident at once (one per panel CU, at most 32), covers the
+    strip's rows, and lets the polling waves cover all records in one round;
+    and whenever the tallest strip (the first, n rows) has a shape, every
+    shorter one has too, so the factorizations' up-front check suffices."""
+    from charmlu_amd import _lib
+    vw = 2 if prec == 0 else 1
+    resident = min(cus if cus > 0 else 256, 32)
+    widest_prev = None
+    for m in sorted(set(range(1, 4200, 37)) | set(range(4096, 300000, 4093))):
+        rc, o = _plan(prec, m, 8, cus)
+        widest = o[5]
+        assert o[4] == resident
+        if widest == 0:
+            assert rc == _lib.ERR_UNSUPPORTED
+        else:
+            assert rc == 0
+        # shorter strips never lose their shape
+        if widest_prev == 0:
+            assert widest == 0, (m, cus)
+        widest_prev = widest
+        for w in (8, 16, 32):
+            rc, o = _plan(prec, m, w, cus)
+            if w > widest or m < 1:
+                assert rc == _lib.ERR_UNSUPPORTED
+                continue
+            assert rc == 0, (m, w, cus)
+            sw, R, NT, G = o[:4]
+            assert sw >= w and R * NT * G >= m and R * NT * (G - 1) < m
+            assert G <= resident
+            if G > 1:
+                assert G * (sw * vw + vw + 1) + sw * vw <= _poll_words(NT)
+
+
+def test_panel_plan_known_shapes():
+    """The shapes of the measured default (32 panel CUs) and the fallbacks."""
+    from charmlu_amd import _lib
+    FP64, FP32 = _lib.PREC_FP64, _lib.PREC_FP32
+    assert _plan(FP64, 32768, 16, 32)[1][:4] == [16, 4, 256, 32]   # C2's first strips
+    assert _plan(FP64, 65536, 16, 32)[1][:4] == [16, 4, 512, 32]
+    assert _plan(FP64, 8192, 32, 32)[1][:4] == [32, 2, 256, 16]    # 32-column FP64 strips
+    assert _plan(FP64, 131072, 8, 32)[1][:4] == [8, 8, 512, 32]    # beyond 65536 rows
+    assert _plan(FP64, 131072 + 256, 8, 32)[1][5] == 0             # too tall: rejected
+    assert _plan(FP32, 65536, 32, 32)[1][:4] == [32, 4, 512, 32]   # C3's first strips
+    assert _plan(FP32, 131072, 16, 32)[1][:4] == [16, 8, 512, 32]  # C5's first strips
+    assert _plan(FP32, 262144, 8, 32)[1][:4] == [8, 16, 512, 32]
+    # 16 panel CUs at n = 9216: 32-column FP64 strips would need 18
+    # workgroups, so 16-column ones run
+    rc, o = _plan(FP64, 9216, 32, 16)
+    assert rc == _lib.ERR_UNSUPPORTED and o[5] == 16
+    assert _plan(FP64, 9216, 16, 16)[1][:4] == [16, 4, 256, 9]
+    # 8 panel CUs at C3 (FP32, 65536 rows): 8-column strips of 8192 rows each
+    assert _plan(FP32, 65536, 8, 8)[1][:4] == [8, 16, 512, 8]
+    # unpivoted strips are one workgroup of at most 1024 rows
+    assert _plan(FP32, 1024, 32, 32, pivot=0)[1][:4] == [32, 4, 256, 1]
+    assert _plan(FP32, 1025, 16, 32, pivot=0)[0] == _lib.ERR_UNSUPPORTED
+    assert _plan(FP64, 512, 32, 32, pivot=0)[0] == _lib.ERR_UNSUPPORTED  # FP64: 16 columns
+    assert _plan(FP64, 512, 16, 32, pivot=0)[1][:4] == [16, 2, 256, 1]
+    # argument errors
+    assert _plan(7, 100, 8, 32)[0] == -1
+    assert _plan(FP64, 0, 8, 32)[0] == -2
+    assert _plan(FP64, 100, 0, 32)[0] == -3
+
+
+def test_dgesv_ir_ex_argument_errors():
+    from charmlu_amd import _lib
+    lib = _lib.load()
+    p = ctypes.c_void_p(16)
+    args = lambda **kw: [kw.get("trans", _lib.NO_TRANS), kw.get("n", 4), kw.get("nrhs", 2), p,
+                         kw.get("lda", 4), p, 4, p, p, kw.get("ldb", 2), p, kw.get("ldx", 2),
+                         kw.get("tol", 1.0), 5, kw.get("restart", 10), None, None]
+    assert lib.charmlu_dgesv_ir_ex(*args(trans=9)) == -1
+    assert lib.charmlu_dgesv_ir_ex(*args(n=-1)) == -2
+    assert lib.charmlu_dgesv_ir_ex(*args(nrhs=-1)) == -3
+    assert lib.charmlu_dgesv_ir_ex(*args(lda=3)) == -5
+    assert lib.charmlu_dgesv_ir_ex(*args(ldb=1)) == -10
+    assert lib.charmlu_dgesv_ir_ex(*args(ldx=1)) == -12
+    assert lib.charmlu_dgesv_ir_ex(*args(tol=0.0)) == -13
+    assert lib.charmlu_dgesv_ir_ex(*args(restart=0)) == -15
+    # nothing to solve: no device touched
+    assert lib.charmlu_dgesv_ir_ex(*args(n=0)) == 0
+    assert lib.charmlu_dgesv_ir_ex(*args(nrhs=0)) == 0

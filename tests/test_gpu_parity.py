@@ -20,6 +20,8 @@ import scipy.linalg as sl
 pytestmark = pytest.mark.gpu
 EPS = np.finfo(np.float64).eps
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "oracle_fixtures.npz")
+LARGE = os.path.join(os.path.dirname(__file__), "golden", "large_fixtures.npz")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _t(a):
@@ -227,9 +229,44 @@ def test_dgemv_trans_and_colmajor(dev):
 # ---------------------------------------------------------------------------
 # whole factorization + solve vs the oracle
 # ---------------------------------------------------------------------------
-def _near_tie_ok(A_ref_panel_col, p_gpu, p_orc):
-    a, b = abs(A_ref_panel_col[p_gpu]), abs(A_ref_panel_col[p_orc])
-    return abs(a - b) <= 4 * np.spacing(max(a, b))
+def check_pivots(gp, op, ties, what, ulps=4.0):
+    """Pivots index-for-index with the oracle's.  The first difference is
+    accepted only at a near-tie: the device's row is the oracle's runner-up
+    at that column and their magnitudes (the oracle's values) lie within
+    ``ulps`` ulp; it is logged, and the columns after it are not compared
+    (a flipped pivot reorders every later row).  ``ties``: the oracle's
+    near-tie log, as a dict of full per-column arrays or of the close
+    columns only (cols, row2, mag1, mag2).  Also logs every column the
+    oracle itself saw as a near-tie.  Returns the number of columns
+    compared."""
+    gp = np.asarray(gp)
+    op = np.asarray(op)
+    if "cols" in ties:
+        cols = np.asarray(ties["cols"])
+        close = [int(c) for c, r2, m1, m2 in zip(cols, ties["row2"], ties["mag1"], ties["mag2"])
+                 if m1 - m2 <= ulps * np.spacing(m1)]
+    else:
+        close = [int(c) for c in np.nonzero((ties["row2"] >= 0) & (
+            ties["mag1"] - ties["mag2"] <= ulps * np.spacing(ties["mag1"])))[0]]
+    for c in close:
+        print(f"{what}: oracle near-tie within {ulps:g} ulp at column {c}")
+    mism = np.nonzero(gp != op)[0]
+    if len(mism) == 0:
+        return len(gp)
+    j = int(mism[0])
+    if "cols" in ties:
+        k = np.nonzero(np.asarray(ties["cols"]) == j)[0]
+        ok = len(k) == 1 and int(ties["row2"][k[0]]) == int(gp[j]) and \
+            ties["mag1"][k[0]] - ties["mag2"][k[0]] <= ulps * np.spacing(ties["mag1"][k[0]])
+    else:
+        ok = int(ties["row2"][j]) == int(gp[j]) and \
+            ties["mag1"][j] - ties["mag2"][j] <= ulps * np.spacing(ties["mag1"][j])
+    if not ok:
+        pytest.fail(f"{what}: pivot of column {j} is row {gp[j]}, the oracle's {op[j]} "
+                    "(not a near-tie)")
+    print(f"{what}: near-tie flip at column {j} (device row {gp[j]}, oracle row {op[j]}); "
+          "later columns not compared")
+    return j
 
 
 def _compare_run(dev, orc, n, b, batch=0, kind="uniform", nthreads=8):
@@ -243,24 +280,19 @@ def _compare_run(dev, orc, n, b, batch=0, kind="uniform", nthreads=8):
     res = dev.residual(A, x, rhs)
     _sync()
     A_h = A.cpu().numpy()
-    o = orc.lu_solve_problem(n, b, batch, nthreads=nthreads, A=A_h.copy())
-    gp = ipiv.cpu().numpy()
-    mism = np.nonzero(gp != o["ipiv"])[0]
-    if len(mism):
-        # only a near-tie may flip; check it on the oracle's own column values
-        j = int(mism[0])
-        print(f"pivot mismatch at column {j}: gpu {gp[j]} oracle {o['ipiv'][j]}")
-        pytest.fail(f"pivot sequence differs from the oracle at column {j} (n={n}, b={b})")
+    o = orc.lu_solve_problem(n, b, batch, nthreads=nthreads, A=A_h.copy(), ties=True)
+    check_pivots(ipiv.cpu().numpy(), o["ipiv"], o["ties"], f"n={n} b={b}")
     xg = x.cpu().numpy()
     rel = np.abs(xg - o["x"]).max() / np.abs(o["x"]).max()
     return dict(info=int(info.item()), oinfo=o["info"], rel=rel, res=res, ores=o["residual"],
                 LU=LU, o=o)
 
 
-# the reference's regression grid (test_lu.pl:9-52), minus the 4096 cases
-# covered below
+# the reference's regression grid (test_lu.pl:9-52) with its large cases
+# (4096, 256) and (4096, 512); (4096, 128) is C1, below
 GRID = [(48, 48), (48, 24), (48, 16), (48, 12), (32, 8), (32, 4), (512, 128), (512, 64),
-        (512, 256), (1024, 256), (1024, 128), (1024, 64), (2048, 256), (2048, 512)]
+        (512, 256), (1024, 256), (1024, 128), (1024, 64), (2048, 256), (2048, 512),
+        (4096, 256), (4096, 512)]
 
 
 @pytest.mark.parametrize("n,b", GRID)
@@ -344,13 +376,134 @@ def test_residual_kernel_matches_oracle(dev, orc):
     assert r["a_max"] == m4[0] and r["b_max"] == m4[1] and r["x_max"] == m4[2]
 
 
-def test_c2_n32768_residual(dev):
-    # configuration C2 at full size: factor + solve on the device, the
-    # reference's pass criterion (residual < 16) and the normwise form
-    out = dev.run(dev.LUConfig(32768, 256), verbose=True)
+def _fixture_run(dev, n, b):
+    """Factor + solve the seeded problem on the device and compare it with
+    the oracle's pivots and x stored in the large fixtures (written on the
+    CPU by tests/golden/make_golden.py --large)."""
+    g = np.load(LARGE)
+    key = f"n{n}_b{b}"
+    out = dev.run(dev.LUConfig(n, b), verbose=True)
+    assert out["info"] == 0 == int(g[key + "_info"][0])
+    ties = dict(cols=g[key + "_tie_cols"], row2=g[key + "_tie_row2"], mag1=g[key + "_tie_mag1"],
+                mag2=g[key + "_tie_mag2"])
+    ncmp = check_pivots(out["ipiv"].cpu().numpy(), g[key + "_ipiv"], ties, key)
+    xo = g[key + "_x"]
+    rel = float(np.abs(out["x"].cpu().numpy() - xo).max() / np.abs(xo).max())
+    print(f"{key}: {ncmp} pivots compared, x vs oracle {rel:.2e}, residual "
+          f"{out['residual']:.3f} (oracle {g[key + '_res'][0]:.3f}), "
+          f"{out['tflops_eff']:.2f} TFLOP/s")
+    assert rel <= 1e-10
+    assert out["residual"] < 16 and out["residual_ns"] < 16
+    return out
+
+
+def test_c2_n32768_pivots_x_residual_vs_oracle(dev):
+    """C2 at full size (n = 32768, block 256): all 32768 pivots index for
+    index with the oracle's (a difference only at a logged near-tie within
+    4 ulp), x within 1e-10 of the oracle's, the reference's residual < 16."""
+    _fixture_run(dev, 32768, 256)
+
+
+def test_fp64_block1024_pivoting_matrix_vs_oracle(dev):
+    """Block 1024 (the FP64 comparator of the mixed form, whose U row uses
+    an explicit inverse of the 1024 x 1024 L11) on the uniform, pivoting
+    matrix at n = 8192: pivots and x against the oracle at the same block."""
+    _fixture_run(dev, 8192, 1024)
+
+
+def _subprocess(code, env_extra, out_path, timeout=300):
+    import subprocess
+    import sys
+    env = dict(os.environ, **env_extra)
+    print(f"subprocess with {env_extra}", flush=True)
+    r = subprocess.run([sys.executable, "-c", code, out_path], env=env, cwd=ROOT,
+                       capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    return np.load(out_path)
+
+
+def test_panel_cus_16_fp64_matches_oracle(dev, orc, tmp_path):
+    """16 panel CUs (16 workgroups resident per strip) at n = 9216: the
+    strips that would need 18 workgroups run 16 columns wide instead; the
+    pivots and x match the oracle."""
+    code = ("import sys, numpy as np, torch, charmlu_amd as c\n"
+            "n, b = 9216, 256\n"
+            "A = c.gen_matrix(n); rhs = c.gen_rhs(n)\n"
+            "ip, info = c.getrf(A, b); x = rhs.clone(); c.getrs(A, ip, x)\n"
+            "torch.cuda.synchronize()\n"
+            "np.savez(sys.argv[1], ip=ip.cpu().numpy(), x=x.cpu().numpy(), info=[int(info.item())])\n")
+    z = _subprocess(code, {"CHARMLU_PANEL_CUS": "16"}, str(tmp_path / "c16.npz"))
+    o = orc.lu_solve_problem(9216, 256, nthreads=16, ties=True)
+    assert int(z["info"][0]) == 0 == o["info"]
+    check_pivots(z["ip"], o["ipiv"], o["ties"], "16 panel CUs, n=9216")
+    assert np.abs(z["x"] - o["x"]).max() / np.abs(o["x"]).max() <= 1e-10
+
+
+def test_panel_cus_4_uses_8_column_strips(dev, tmp_path):
+    """4 panel CUs: FP64 strips taller than 8192 rows and FP32 strips taller
+    than 16384 rows only fit as 8-column strips (8 rows of 8 doubles or 16
+    rows of 8 floats per thread, 4 workgroups).  FP64 n = 16384: pivots and
+    x as in the default layout (32 panel CUs, 16-column strips) and the
+    reference's residual < 16; mixed BF16 n = 32768 on a diagonally
+    dominant matrix: identity pivots and FP64 refinement to residual <= 1."""
+    import torch
+    code = ("import sys, numpy as np, torch, charmlu_amd as c\n"
+            "n, b = 16384, 256\n"
+            "A = c.gen_matrix(n); rhs = c.gen_rhs(n)\n"
+            "ip, info = c.getrf(A, b); x = rhs.clone(); c.getrs(A, ip, x)\n"
+            "D = c.gen_matrix(32768, kind='diagdom'); bd = c.gen_rhs(32768)\n"
+            "S = c.lag2s(D); ips, infos = c.getrf_mixed(S, 1024, 'bf16')\n"
+            "xd, st = c.gesv_ir(D, S, ips, bd, tol=1.0)\n"
+            "torch.cuda.synchronize()\n"
+            "np.savez(sys.argv[1], ip=ip.cpu().numpy(), x=x.cpu().numpy(),\n"
+            "         info=[int(info.item()), int(infos.item())], ips=ips.cpu().numpy(),\n"
+            "         res=[st['residual']])\n")
+    z = _subprocess(code, {"CHARMLU_PANEL_CUS": "4"}, str(tmp_path / "c4.npz"))
+    assert list(z["info"]) == [0, 0]
+    n = 16384
+    A = dev.gen_matrix(n)
+    rhs = dev.gen_rhs(n)
+    LU = A.clone()
+    ip, info = dev.getrf(LU, 256)
+    x = rhs.clone()
+    dev.getrs(LU, ip, x)
+    res = dev.residual(A, torch.from_numpy(z["x"]).cuda(), rhs)
+    _sync()
+    assert np.array_equal(z["ip"], ip.cpu().numpy())
+    xr = x.cpu().numpy()
+    assert np.abs(z["x"] - xr).max() / np.abs(xr).max() <= 1e-10
+    assert res["residual"] < 16
+    assert np.array_equal(z["ips"], np.arange(32768, dtype=np.int32))
+    assert z["res"][0] <= 1.0
+
+
+def test_fp64_beyond_65536_rows_uses_8_column_strips(dev):
+    """n = 73728: the first panels are taller than 32 workgroups of 2048
+    rows, so they run as 8-column strips of 4096 rows per workgroup; no
+    oracle at this size, so the reference's pass criterion (residual < 16)
+    and the normwise form."""
+    out = dev.run(dev.LUConfig(73728, 256), verbose=True)
     assert out["info"] == 0
     assert out["residual"] < 16 and out["residual_ns"] < 16
-    print(f"C2: {out['seconds']:.3f} s, {out['tflops_eff']:.2f} TFLOP/s effective")
+
+
+def test_too_tall_first_panel_is_rejected_before_any_work(dev):
+    """n = 131328 FP64 (first strip 131328 rows: more than 32 workgroups of
+    4096 rows): CHARMLU_ERR_UNSUPPORTED with a message, before any kernel
+    touches A or ipiv (so small stand-in buffers are safe here), and the
+    info word is 0."""
+    import torch
+    from charmlu_amd import _lib
+    lib = _lib.load()
+    A = torch.zeros(16, dtype=torch.float64, device="cuda")
+    ip = torch.zeros(16, dtype=torch.int32, device="cuda")
+    info = torch.full((1,), 7, dtype=torch.int32, device="cuda")
+    n = 131328
+    rc = lib.charmlu_dgetrf(n, 256, _p(A), n, _p(ip), _p(info), None)
+    _sync()
+    assert rc == _lib.ERR_UNSUPPORTED
+    assert "resident" in lib.charmlu_last_error().decode()
+    assert int(info.item()) == 0 and int(ip.abs().sum().item()) == 0
 
 
 def test_tall_panel_strips_residual(dev):

@@ -164,6 +164,69 @@ def test_getrf_matches_unblocked_brute_force(orc):
         assert np.allclose(L1, L2, rtol=0, atol=1e-13)
 
 
+def _getf2_runner_up(A):
+    """Unblocked LU with partial pivoting (division) in numpy, recording per
+    column the |pivot| and the best other candidate row and |value|."""
+    A = A.copy()
+    n = A.shape[0]
+    row2 = np.full(n, -1)
+    mag1 = np.zeros(n)
+    mag2 = np.zeros(n)
+    for j in range(n):
+        col = np.abs(A[j:, j])
+        p = j + int(np.argmax(col))  # first maximum
+        mag1[j] = col[p - j]
+        rest = col.copy()
+        rest[p - j] = -1.0
+        if n - j > 1:
+            q = int(np.argmax(rest))
+            row2[j], mag2[j] = j + q, rest[q]
+        A[[j, p]] = A[[p, j]]
+        A[j + 1:, j] /= A[j, j]
+        A[j + 1:, j + 1:] -= np.outer(A[j + 1:, j], A[j, j + 1:])
+    return row2, mag1, mag2
+
+
+def test_near_tie_log_matches_brute_force(orc):
+    """The runner-up candidates the oracle logs per column (the data that
+    classifies a device pivot differing from the oracle's as a near-tie)
+    are those of an unblocked LU on the same matrix."""
+    for n, b in [(48, 16), (64, 8), (96, 32)]:
+        A = orc.gen_uniform(200 + n, n)
+        t = {}
+        orc.getrf(A.copy(), b, ties=t)
+        row2, mag1, mag2 = _getf2_runner_up(A)
+        assert np.array_equal(t["row2"][:-1], row2[:-1])
+        assert np.allclose(t["mag1"], mag1, rtol=1e-12, atol=0)
+        assert np.allclose(t["mag2"][:-1], mag2[:-1], rtol=1e-12, atol=0)
+        assert t["row2"][-1] == -1  # the last column has one candidate
+
+
+def test_near_tie_log_flags_an_engineered_tie(orc):
+    """Column 0 holds 0.5 at row 3 and the next double below 0.5 at row 9:
+    a 1-ulp near-tie, classified as such only for row 9; row 3 wins (the
+    larger), and an exact tie keeps the lower row (strict '>')."""
+    n = 16
+    A = orc.gen_uniform(7, n) * 0.1
+    A[3, 0] = 0.5
+    A[9, 0] = -np.nextafter(0.5, 0.0)
+    t = {}
+    _, ipiv = orc.getrf(A.copy(), 8, ties=t)
+    assert ipiv[0] == 3
+    assert 0 in orc.near_ties(t)
+    assert orc.is_near_tie(t, 0, 9) and not orc.is_near_tie(t, 0, 5)
+    assert not orc.is_near_tie(t, 0, 9, ulps=0.25)  # the gap is half a spacing of 0.5
+    A[9, 0] = -0.5  # exact tie: the first (lower) row is the pivot
+    t = {}
+    _, ipiv = orc.getrf(A.copy(), 8, ties=t)
+    assert ipiv[0] == 3 and t["row2"][0] == 9 and t["mag1"][0] == t["mag2"][0]
+    # a clear winner is no near-tie
+    A[9, 0] = 0.25
+    t = {}
+    orc.getrf(A.copy(), 8, ties=t)
+    assert 0 not in orc.near_ties(t)
+
+
 def test_batch_size_does_not_change_result(orc):
     # cycles == sequential swaps, so the pivot batch size (test_lu.pl:16-26)
     # changes no bit of the factorization
