@@ -55,21 +55,45 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 // ---------------------------------------------------------------------------
 // per-device workspace
 // ---------------------------------------------------------------------------
-// Workgroups of one multi-workgroup panel strip, at most: every one of them
-// must be resident at once (resident_groups), and no strip of more than 32
-// has been run.
-constexpr int MAX_GROUPS = 32;
-constexpr int MAX_PANEL_CUS = 32;
+// Workgroups of one multi-workgroup panel strip, at most (the records of
+// one exchange), and panel CUs, at most.
+constexpr int MAX_GROUPS = 64;
+constexpr int MAX_PANEL_CUS = 64;
 constexpr int RUN = 256;         // pivots per move list
 
-// Workgroups of one panel strip that can be resident at once.  The strip
-// kernels that exchange records across workgroups hold one workgroup per
-// CU (their register use allows one wave per SIMD, or two of a 512-thread
-// workgroup), so at most one workgroup per panel CU; with no CU masks
-// (panel_cus = 0) the strip runs on the whole chip.
+// Workgroups of one panel strip that are certain to be resident at once on
+// the panel stream's CU mask (the top panel_cus of ncu CUs).  The strip
+// kernels that exchange records hold one workgroup per CU (their register
+// use allows one wave per SIMD, or two of a 512-thread workgroup).  CU mask
+// bit c lies on XCD c % X (X = ncu / 32 XCDs of 32 CUs) and there on shader
+// engine (c / X) % 4.  The hardware deals a launch's workgroups round-robin
+// to the XCDs (starting where the previous launch left off) and, inside an
+// XCD, round-robin to the shader engines that hold masked CUs, whatever
+// their free CUs: so every XCD must take ceil(G / X) workgroups with each of
+// its engines taking its share.  Measured with a bounded-sleep probe (one
+// workgroup per CU) on MI355X: 32 of 32 CUs (one per engine) hold 32
+// workgroups and not 36; 48 CUs (engines with 1 or 2) hold 32, not 36;
+// 24 CUs hold 24, not 28; 16 hold 16, not 24.  With no CU masks
+// (panel_cus = 0) the strip may run on any CU: 32.
 static int resident_groups(int panel_cus, int ncu) {
-  const int cus = panel_cus > 0 ? panel_cus : ncu;
-  return cus < MAX_GROUPS ? cus : MAX_GROUPS;
+  if (panel_cus <= 0) return 32;
+  const int X = ncu >= 64 ? ncu / 32 : 1;
+  int cap = 1 << 30;
+  for (int x = 0; x < X; ++x) {
+    int per_se[4] = {0, 0, 0, 0};
+    for (int c = ncu - panel_cus; c < ncu; ++c)
+      if (c % X == x) ++per_se[(c / X) % 4];
+    int engines = 0, least = 1 << 30;
+    for (int e = 0; e < 4; ++e)
+      if (per_se[e] > 0) {
+        ++engines;
+        least = per_se[e] < least ? per_se[e] : least;
+      }
+    const int cap_x = engines > 0 ? engines * least : 0;
+    cap = cap_x < cap ? cap_x : cap;
+  }
+  cap *= X;
+  return cap < MAX_GROUPS ? cap : MAX_GROUPS;
 }
 
 struct PanelHistory {
@@ -198,10 +222,17 @@ static int get_ws(Workspace **out) {
     const int ncu = prop.multiProcessorCount;
     int reserve = 32;
     if (const char *e = getenv("CHARMLU_PANEL_CUS")) {
+      // 0 (no masks) or a multiple of the XCD count: the launches of a
+      // masked stream are dealt to every XCD, so each XCD needs the same
+      // number of its CUs (a mask without CUs on some XCD never runs the
+      // workgroups dealt there)
       char *end = nullptr;
       const long v = strtol(e, &end, 10);
-      if (end == e || *end != '\0' || v < 0 || v > MAX_PANEL_CUS) {
-        set_error("CHARMLU_PANEL_CUS=%s: expected a CU count from 0 to %d", e, MAX_PANEL_CUS);
+      const int X = ncu >= 64 ? ncu / 32 : 1;
+      if (end == e || *end != '\0' || v < 0 || v > MAX_PANEL_CUS || v % X != 0 ||
+          (v > 0 && v >= ncu)) {
+        set_error("CHARMLU_PANEL_CUS=%s: expected 0 or a multiple of %d from %d to %d", e, X, X,
+                  MAX_PANEL_CUS);
         return CLU_ERR_CONFIG;
       }
       reserve = (int)v;
@@ -720,7 +751,7 @@ int charmlu_panel_plan(int prec, int m, int w, int panel_cus, int ncu, int pivot
   if (prec != CHARMLU_PREC_FP64 && prec != CHARMLU_PREC_FP32) return -1;
   if (m < 1) return -2;
   if (w < 1) return -3;
-  if (panel_cus < 0) return -4;
+  if (panel_cus < 0 || panel_cus > MAX_PANEL_CUS) return -4;
   if (ncu < 1) return -5;
   if (!out) return -7;
   const int groups = resident_groups(panel_cus, ncu);

@@ -21,7 +21,8 @@
  *     style), CHARMLU_ERR_HIP if a HIP call failed, CHARMLU_ERR_NOMEM if the
  *     library could not allocate its workspace, CHARMLU_ERR_CONFIG if an
  *     environment setting of the library is out of range (CHARMLU_PANEL_CUS:
- *     0 to 32), CHARMLU_ERR_UNSUPPORTED if the problem is too large for the
+ *     0, or a multiple of 8 from 8 to 64), CHARMLU_ERR_UNSUPPORTED if the
+ *     problem is too large for the
  *     panel CUs (the first panel strip needs more workgroups than can be
  *     resident at once: with 32 panel CUs, n <= 131072 in FP64 and
  *     n <= 262144 in FP32; charmlu_panel_plan tells).  charmlu_last_error()

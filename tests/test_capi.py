@@ -105,17 +105,30 @@ def _poll_words(nt):
     return (nt // 64) * 64 * (4 if nt >= 512 else 5)
 
 
+# Workgroups of a one-per-CU kernel resident at once on the top `cus` CUs
+# of MI355X (256 CUs, 8 XCDs of 4 shader engines), as measured with a
+# bounded-sleep probe for 16, 24, 32 and 48 (16, 24, 32 and 32: a 48-CU mask
+# holds 1 or 2 CUs per engine, and workgroups are dealt round-robin over the
+# engines); 0 is the unmasked layout's cap.
+RESIDENT = {0: 32, 8: 8, 16: 16, 20: 16, 24: 24, 32: 32, 40: 32, 48: 32, 56: 32, 64: 64}
+
+
+def test_resident_workgroups_follow_the_measured_layout():
+    for cus, want in RESIDENT.items():
+        assert _plan(1, 1024, 16, cus)[1][4] == want, cus
+
+
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("cus", [0, 1, 4, 8, 16, 24, 32])
+@pytest.mark.parametrize("cus", [0, 8, 16, 24, 32, 48, 64])
 def test_panel_plan_never_exceeds_resident_workgroups(prec, cus):
     """Every strip shape the library launches keeps its workgroups within
-    what can be resident at once (one per panel CU, at most 32), covers the
-    strip's rows, and lets the polling waves cover all records in one round;
-    and whenever the tallest strip (the first, n rows) has a shape, every
-    shorter one has too, so the factorizations' up-front check suffices."""
+    what can be resident at once, covers the strip's rows, and lets the
+    polling waves cover all records in one round; and whenever the tallest
+    strip (the first, n rows) has a shape, every shorter one has too, so
+    the factorizations' up-front check suffices."""
     from charmlu_amd import _lib
     vw = 2 if prec == 0 else 1
-    resident = min(cus if cus > 0 else 256, 32)
+    resident = RESIDENT[cus]
     widest_prev = None
     for m in sorted(set(range(1, 4200, 37)) | set(range(4096, 300000, 4093))):
         rc, o = _plan(prec, m, 8, cus)
@@ -170,6 +183,7 @@ def test_panel_plan_known_shapes():
     assert _plan(7, 100, 8, 32)[0] == -1
     assert _plan(FP64, 0, 8, 32)[0] == -2
     assert _plan(FP64, 100, 0, 32)[0] == -3
+    assert _plan(FP64, 100, 8, 65)[0] == -4
 
 
 def test_dgesv_ir_ex_argument_errors():

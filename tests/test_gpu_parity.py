@@ -439,32 +439,14 @@ def test_panel_cus_16_fp64_matches_oracle(dev, orc, tmp_path):
     assert np.abs(z["x"] - o["x"]).max() / np.abs(o["x"]).max() <= 1e-10
 
 
-def test_panel_cus_4_uses_8_column_strips(dev, tmp_path):
-    """4 panel CUs: FP64 strips taller than 8192 rows and FP32 strips taller
-    than 16384 rows only fit as 8-column strips (8 rows of 8 doubles or 16
-    rows of 8 floats per thread, 4 workgroups).  FP64 n = 16384: pivots and
-    x as in the default layout (32 panel CUs, 16-column strips) and the
-    reference's residual < 16; mixed BF16 n = 32768 on a diagonally
-    dominant matrix: identity pivots and FP64 refinement to residual <= 1."""
+def _fp64_vs_default(dev, z, n, b):
+    """The subprocess's FP64 pivots and x against this process's (default
+    layout, 32 panel CUs) run of the same problem."""
     import torch
-    code = ("import sys, numpy as np, torch, charmlu_amd as c\n"
-            "n, b = 16384, 256\n"
-            "A = c.gen_matrix(n); rhs = c.gen_rhs(n)\n"
-            "ip, info = c.getrf(A, b); x = rhs.clone(); c.getrs(A, ip, x)\n"
-            "D = c.gen_matrix(32768, kind='diagdom'); bd = c.gen_rhs(32768)\n"
-            "S = c.lag2s(D); ips, infos = c.getrf_mixed(S, 1024, 'bf16')\n"
-            "xd, st = c.gesv_ir(D, S, ips, bd, tol=1.0)\n"
-            "torch.cuda.synchronize()\n"
-            "np.savez(sys.argv[1], ip=ip.cpu().numpy(), x=x.cpu().numpy(),\n"
-            "         info=[int(info.item()), int(infos.item())], ips=ips.cpu().numpy(),\n"
-            "         res=[st['residual']])\n")
-    z = _subprocess(code, {"CHARMLU_PANEL_CUS": "4"}, str(tmp_path / "c4.npz"))
-    assert list(z["info"]) == [0, 0]
-    n = 16384
     A = dev.gen_matrix(n)
     rhs = dev.gen_rhs(n)
     LU = A.clone()
-    ip, info = dev.getrf(LU, 256)
+    ip, info = dev.getrf(LU, b)
     x = rhs.clone()
     dev.getrs(LU, ip, x)
     res = dev.residual(A, torch.from_numpy(z["x"]).cuda(), rhs)
@@ -473,8 +455,59 @@ def test_panel_cus_4_uses_8_column_strips(dev, tmp_path):
     xr = x.cpu().numpy()
     assert np.abs(z["x"] - xr).max() / np.abs(xr).max() <= 1e-10
     assert res["residual"] < 16
-    assert np.array_equal(z["ips"], np.arange(32768, dtype=np.int32))
+
+
+_LAYOUT_CODE = ("import sys, numpy as np, torch, charmlu_amd as c\n"
+                "n, b, nm = int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4])\n"
+                "A = c.gen_matrix(n); rhs = c.gen_rhs(n)\n"
+                "ip, info = c.getrf(A, b); x = rhs.clone(); c.getrs(A, ip, x)\n"
+                "del A\n"
+                "D = c.gen_matrix(nm, kind='diagdom'); bd = c.gen_rhs(nm)\n"
+                "S = c.lag2s(D); ips, infos = c.getrf_mixed(S, 1024, 'bf16')\n"
+                "xd, st = c.gesv_ir(D, S, ips, bd, tol=1.0)\n"
+                "torch.cuda.synchronize()\n"
+                "np.savez(sys.argv[1], ip=ip.cpu().numpy(), x=x.cpu().numpy(),\n"
+                "         info=[int(info.item()), int(infos.item())], ips=ips.cpu().numpy(),\n"
+                "         res=[st['residual']])\n")
+
+
+def _layout_run(tmp_path, cus, n, b, nm):
+    import subprocess
+    import sys
+    out = str(tmp_path / f"cus{cus}.npz")
+    env = dict(os.environ, CHARMLU_PANEL_CUS=str(cus))
+    print(f"subprocess: CHARMLU_PANEL_CUS={cus}, FP64 n={n} b={b}, mixed BF16 n={nm}", flush=True)
+    r = subprocess.run([sys.executable, "-c", _LAYOUT_CODE, out, str(n), str(b), str(nm)],
+                       env=env, cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    z = np.load(out)
+    assert list(z["info"]) == [0, 0]
+    assert np.array_equal(z["ips"], np.arange(nm, dtype=np.int32))
     assert z["res"][0] <= 1.0
+    return z
+
+
+def test_panel_cus_8_uses_8_column_strips(dev, tmp_path):
+    """8 panel CUs (8 resident workgroups): FP64 strips taller than 16384
+    rows and FP32 strips taller than 32768 rows only fit as 8-column strips
+    (8 rows of 8 doubles or 16 rows of 8 floats per thread).  FP64
+    n = 24576: pivots and x as in the default layout and residual < 16;
+    mixed BF16 n = 40960 (diagonally dominant): identity pivots and FP64
+    refinement to the reference's residual <= 1."""
+    z = _layout_run(tmp_path, 8, 24576, 256, 40960)
+    _fp64_vs_default(dev, z, 24576, 256)
+
+
+def test_panel_cus_48_stays_within_resident_workgroups(dev, tmp_path):
+    """48 panel CUs: a 48-CU mask keeps 32 one-per-CU workgroups resident,
+    not 48 (1 or 2 CUs per shader engine, workgroups dealt round-robin over
+    the engines), so strips stay at 32 workgroups.  (Strips of 33 to 36
+    workgroups on such a mask made a mixed BF16 run stop progressing: a
+    workgroup that is not resident leaves the others polling for its
+    record.)  FP64 n = 32768 against the default layout, mixed BF16
+    n = 65536 refined to residual <= 1."""
+    z = _layout_run(tmp_path, 48, 32768, 256, 65536)
+    _fp64_vs_default(dev, z, 32768, 256)
 
 
 def test_fp64_beyond_65536_rows_uses_8_column_strips(dev):
@@ -548,12 +581,12 @@ def test_forced_split_matches_oracle_and_unsplit(dev, orc, n, b):
     assert np.array_equal(ip2.cpu().numpy(), o["ipiv"])
 
 
-@pytest.mark.parametrize("value", ["48", "-1", "x"])
+@pytest.mark.parametrize("value", ["72", "4", "20", "-1", "x"])
 def test_panel_cus_out_of_range_is_an_error(value):
-    """CHARMLU_PANEL_CUS takes 0 to 32: any other value makes the first call
-    on a device fail with CHARMLU_ERR_CONFIG and a message, before any
-    stream or workspace is made (a fresh process: the workspace is made once
-    per device)."""
+    """CHARMLU_PANEL_CUS takes 0 or a multiple of 8 (the XCDs) up to 64: any
+    other value makes the first call on a device fail with
+    CHARMLU_ERR_CONFIG and a message, before any stream or workspace is made
+    (a fresh process: the workspace is made once per device)."""
     import subprocess
     import sys
     code = ("import torch, charmlu_amd as c\n"
