@@ -49,6 +49,8 @@ SIGNATURES = {
                                      _vp, _c_long, ctypes.c_float, _vp, _c_long, _vp]),
     "charmlu_gemm_fp8_dev": (_c_int, [_c_int, _c_int, _c_int, ctypes.c_float, _vp, _c_long, _vp,
                                       _c_long, ctypes.c_float, _vp, _c_long, _vp]),
+    "charmlu_gemm_fp8_dev_noalloc": (_c_int, [_c_int, _c_int, _c_int, ctypes.c_float, _vp, _c_long,
+                                             _vp, _c_long, ctypes.c_float, _vp, _c_long, _vp]),
     "charmlu_sgetrs": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_long, _vp, _vp, _c_long, _vp]),
     "charmlu_dgesv_ir": (_c_int, [_c_int, _vp, _c_long, _vp, _c_long, _vp, _vp, _vp,
                                   ctypes.c_double, _c_int, _c_int, _vp, _vp]),

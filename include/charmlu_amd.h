@@ -213,6 +213,22 @@ int charmlu_gemm_fp8_dev(int M, int N, int K, float alpha, const float *dA, long
                          const float *dB, long ldb, float beta, float *dC, long ldc,
                          void *stream);
 
+/* charmlu_gemm_fp8_dev without any allocation (no counterpart in the
+ * reference: for callers that cannot allocate device memory, such as code
+ * under HIP graph capture, tight serving loops and harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as charmlu_gemm_fp8_dev,
+ * but the quantized operands and their scales live in a static device
+ * array of CHARMLU_NOALLOC_SCRATCH_BYTES that every device using the
+ * library holds until the process ends: (M + N) * ceil(K / 64) * 64 +
+ * 4 * (ceil(M / 128) + ceil(N / 128)) bytes must fit, else
+ * CHARMLU_ERR_UNSUPPORTED.  It calls no allocating routine and never
+ * touches the library's workspace.  Calls on one device share that array:
+ * calls that overlap in time on one device race.  Asynchronous. */
+#define CHARMLU_NOALLOC_SCRATCH_BYTES (32u << 20)
+int charmlu_gemm_fp8_dev_noalloc(int M, int N, int K, float alpha, const float *dA, long lda,
+                                 const float *dB, long ldb, float beta, float *dC, long ldc,
+                                 void *stream);
+
 /* charmlu_dgetrs on FP32 factors and an FP32 right-hand side. */
 int charmlu_sgetrs(int trans, int n, int nrhs, const float *dLU, long lda,
                    const int *dipiv, float *dB, long ldb, void *stream);
