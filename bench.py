@@ -12,10 +12,12 @@ the timed region, as in the reference.
 
 The JSON line also carries:
   roofline      the trailing-update GEMM (dgemm_nn_lds_kernel, 128x128 tiles):
-                algorithmic flops of its launches / their summed duration,
-                both from HIP events the library records around each launch
-                on the stream it runs on (charmlu_profile), against the FP64
-                matrix peak of 78.6 TFLOP/s.
+                algorithmic flops per launch / average launch duration, both
+                from HIP events the library records around each launch on
+                the stream it runs on (charmlu_profile), against the FP64
+                matrix peak of 78.6 TFLOP/s; beside it the same flops over
+                the union of the launch intervals (the two launches of a
+                step overlap).
   cpu_baseline  the CPU oracle (a restatement of the reference's block
                 algorithm with the vendor BLAS per block) timed on this host
                 on a bounded sample of the same workload (its first block
@@ -489,10 +491,13 @@ def main():
     pan_cnt, pan_work, pan_ms = p[6:9]
     sw_cnt, sw_cols, sw_ms = p[9:12]
     tr_cnt, tr_work, tr_ms = p[12:15]
-    # the two parts of a split update overlap: flops over the time during
-    # which at least one trailing-update launch was running
+    # per launch: algorithmic flops of a launch over its average duration
+    # (HIP events on the stream each launch runs on); the two parts of a
+    # split update overlap, so the same flops over the time during which at
+    # least one launch was running is reported beside it
     g_busy = p[15]
-    achieved = g_work / (g_busy * 1e-3) / 1e12 if g_busy > 0 else 0.0
+    achieved = (g_work / g_cnt) / (g_ms / g_cnt * 1e-3) / 1e12 if g_cnt and g_ms > 0 else 0.0
+    achieved_union = g_work / (g_busy * 1e-3) / 1e12 if g_busy > 0 else 0.0
     # row interchanges: algorithmic bytes = 16 * (rows moved) * (columns moved)
     ip = ipiv.cpu().numpy()
     moved = 0
@@ -553,9 +558,14 @@ def main():
             "avg_launch_ms": round(g_ms / g_cnt, 4) if g_cnt else None,
             "flops_per_launch": g_work / g_cnt if g_cnt else None,
             "busy_ms": round(g_busy, 2),
-            "achieved_note": ("flops / busy_ms: each step's update runs as two launches (beside "
-                              "the panel, then on every CU) that overlap; launches x "
-                              "avg_launch_ms counts the overlap twice"),
+            "achieved_union": round(achieved_union, 3),
+            "frac_union": round(achieved_union / FP64_PEAK, 4),
+            "achieved_note": ("achieved = flops_per_launch / avg_launch_ms.  Each step's update "
+                              "runs as two launches: the rows beside the next panel on the "
+                              "update stream's 224 CUs, the rest on all 256 CUs once that panel "
+                              "is done; the two overlap for a while, so achieved_union (all "
+                              "flops / busy_ms, the time with at least one launch running) "
+                              "counts that overlap once"),
         },
         "phases_ms_per_step": {
             "gemm_trailing_lds": round(g_ms / args.steps, 2),

@@ -335,7 +335,7 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
   }
   const int nk = K / BK;
   auto issue = [&](int kc) {
-    const int kk = (kc < nk ? kc : nk - 1) * BK;  // past the end: reload the last stage
+    const int kk = kc * BK;
     char *st = smem + (kc % NST) * STAGE;
 #pragma unroll
     for (int r = 0; r < 2; ++r) load16_to_lds(ga[r] + kk, st + la_off[r]);
@@ -377,18 +377,26 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
   }
   const bool neg = CINIT && alpha < 0.0;
 
+  static_assert(NST >= 2 && NST <= 4, "stages");
 #pragma unroll
-  for (int st = 0; st < NST - 1; ++st) issue(st);
+  for (int st = 0; st < NST - 1; ++st)
+    if (st < nk) issue(st);
   for (int kc = 0; kc < nk; ++kc) {
-    // This thread's loads of stage kc have landed; after the barrier every
-    // thread's have, and every wave is done reading stage kc-1, whose buffer
-    // the next issue overwrites.  A bare s_barrier: __syncthreads() would
-    // also drain the direct-to-LDS loads of the later stages.
-    wait_vm<4 * (NST - 2)>();
+    // This thread's loads of stage kc have landed once at most the 4 loads
+    // of each later stage already issued (min(NST - 2, nk - 1 - kc) of
+    // them: no stage past the end is loaded) are outstanding; after the
+    // barrier every thread's have, and every wave is done reading stage
+    // kc-1, whose buffer the next issue overwrites.  A bare s_barrier:
+    // __syncthreads() would also drain the direct-to-LDS loads of the later
+    // stages.
+    const int ahead = nk - 1 - kc;
+    if (NST >= 4 && ahead >= 2) wait_vm<4 * (NST >= 4 ? 2 : 0)>();
+    else if (NST >= 3 && ahead >= 1) wait_vm<4>();
+    else wait_vm<0>();
     __asm__ volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     __asm__ volatile("" ::: "memory");
-    issue(kc + NST - 1);
+    if (kc + NST - 1 < nk) issue(kc + NST - 1);
     const char *st = smem + (kc % NST) * STAGE;
 #pragma unroll
     for (int s2 = 0; s2 < 2; ++s2) {
