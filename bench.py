@@ -185,15 +185,22 @@ def lp_gemm_only(lib, _lib, torch, n, nb, steps, warmup, prec="bf16"):
     K = nb; FP32 C, BF16/FP16 operands), timed with events on the stream."""
     import ctypes
     m = n - nb
-    dt = torch.bfloat16 if prec == "bf16" else torch.float16
+    fp8 = prec == "fp8"
+    dt = torch.float32 if fp8 else torch.bfloat16 if prec == "bf16" else torch.float16
     A = (torch.rand((m, nb), device="cuda") - 0.5).to(dt)
-    B = (torch.rand((m, nb), device="cuda") - 0.5).to(dt)
+    B = (torch.rand((nb, m) if fp8 else (m, nb), device="cuda") - 0.5).to(dt)
     C = torch.rand((m, m), device="cuda")
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     p = _lib.PREC_BF16 if prec == "bf16" else _lib.PREC_FP16
     res = {}
     for alpha, tag in ((1.0, "accumulate"), (-1.0, "alpha_beta")):
         def run():
+            if fp8:  # quantizes both operands (e4m3, per-tile scales), then the product
+                _lib.check(lib.charmlu_gemm_fp8_dev(m, m, nb, alpha, ctypes.c_void_p(A.data_ptr()),
+                                                    nb, ctypes.c_void_p(B.data_ptr()), m, 1.0,
+                                                    ctypes.c_void_p(C.data_ptr()), m, stream),
+                           "charmlu_gemm_fp8_dev")
+                return
             _lib.check(lib.charmlu_gemm_lp_dev(p, m, m, nb, alpha, ctypes.c_void_p(A.data_ptr()),
                                                nb, ctypes.c_void_p(B.data_ptr()), nb, 1.0,
                                                ctypes.c_void_p(C.data_ptr()), m, stream),
@@ -404,6 +411,8 @@ def main():
                     help="time only the trailing-update GEMM of step 0 (M=N=n-nb, K=nb)")
     ap.add_argument("--lp-gemm-only", action="store_true",
                     help="time only the low-precision trailing update of step 0 at --mixed-n")
+    ap.add_argument("--lp-prec", default="bf16", choices=["bf16", "fp16", "fp8"],
+                    help="operand format of --lp-gemm-only (fp8: quantization included)")
     ap.add_argument("--mixed-n", type=int, default=65536)
     ap.add_argument("--mixed-nb", type=int, default=1024)
     ap.add_argument("--mixed-fp64-nb", type=int, default=1024)
@@ -446,7 +455,8 @@ def main():
         gemm_only(lib, _lib, torch, n, nb, args.steps, args.warmup)
         return
     if args.lp_gemm_only:
-        lp_gemm_only(lib, _lib, torch, args.mixed_n, args.mixed_nb, args.steps, args.warmup)
+        lp_gemm_only(lib, _lib, torch, args.mixed_n, args.mixed_nb, args.steps, args.warmup,
+                     args.lp_prec)
         return
     if args.c5:
         print(json.dumps({"c5": c5_bench(clu, lib, _lib, torch, args.mixed_nb)}), flush=True)

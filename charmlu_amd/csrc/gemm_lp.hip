@@ -65,41 +65,48 @@ CLU_DEV void wait_vm_lp() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
 }
 
-template <typename T16, int BM, int NST, int OCC, bool CINIT, bool VEC>
-__global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K, float alpha,
-                                                          const T16 *__restrict__ A, long lda,
-                                                          const T16 *__restrict__ B, long ldb,
-                                                          float beta, float *__restrict__ C,
-                                                          long ldc, int tiles_m, int tiles_n,
-                                                          const int *__restrict__ escA,
-                                                          const int *__restrict__ escB) {
-  // BM x BN C tile, 4 waves of 64 x 64 (BM = 128: 2 x 2 waves; BM = 64:
-  // 1 x 4 waves, so every C row segment is 1 KB of contiguous FP32).
-  // A stage holds 64 bytes of K per row: 32 16-bit or 64 FP8 values.
+// XOR swizzle of the 4 16-byte chunks of an LDS operand row (64 bytes): the
+// chunk k of row r sits at chunk k ^ swz(r).  A fragment read takes one
+// chunk of 16 consecutive rows (16 bytes per lane), 8 lanes per LDS cycle:
+// swz gives every 8 consecutive rows of one parity, and every 4 rows of one
+// residue modulo 4, four different chunks, so neither 8- nor 16-lane groups
+// of a read share banks ((r >> 2) & 3 alone left rows r and r + 2 on the
+// same banks: half the LDS cycles of the BF16 update were bank conflicts).
+CLU_DEV int swz(int r) { return ((r >> 2) & 1) | ((((r >> 1) ^ (r >> 3)) & 1) << 1); }
+
+template <typename T16, int BM, int BN, int NST, int OCC, bool CINIT, bool VEC>
+__global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kernel(
+    int M, int N, int K, float alpha, const T16 *__restrict__ A, long lda,
+    const T16 *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
+    int tiles_m, int tiles_n, const int *__restrict__ escA, const int *__restrict__ escB) {
+  // BM x BN C tile, one wave per 64 x 64 block of it (BM / 64 x BN / 64
+  // waves).  A stage holds 64 bytes of K per row: 32 16-bit or 64 FP8 values.
   // FP8 operands carry power-of-two scales per 128-row tile (escA: A * 2^e
   // was quantized; escB likewise per 128 rows of B, i.e. columns of C), so
-  // the tile's products are 2^(escA[tm] + escB[tn]) too large.
+  // a wave's products are 2^(escA[..] + escB[..]) too large.
+  constexpr int NT = (BM / 64) * (BN / 64) * 64;  // threads
   constexpr int EPC = 16 / sizeof(T16);  // elements per 16-byte chunk
-  constexpr int BN = 16384 / BM, BK = 4 * EPC;
+  constexpr int BK = 4 * EPC;
   constexpr int WGN = BN / 64;
-  constexpr int HALF = BM * BK * 2;     // bytes of the A operand's stage
-  constexpr int STAGE = (BM + BN) * BK * 2;
-  constexpr int CA = BM * 4 / 256, CB = BN * 4 / 256;  // 16-byte chunks per thread
-  static_assert(BM == 64 || BM == 128, "tile");
+  constexpr int HALF = BM * 64;          // bytes of the A operand's stage
+  constexpr int STAGE = (BM + BN) * 64;
+  constexpr int CA = BM * 4 / NT, CB = BN * 4 / NT;  // 16-byte chunks per thread
+  static_assert(BM % 64 == 0 && BN % 64 == 0 && CA >= 1 && CB >= 1, "tile");
   __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
   typedef typename Lp<T16>::vec_t vec_t;
 
   // XCD-aware tile order: consecutive workgroups go to the 8 XCDs in turn,
   // so each XCD gets a contiguous range of tiles, walked in groups of GM
-  // tile rows (GM = 4 measured best: 325 TFLOP/s at 65024^2 x 512 BF16 vs
-  // 315 with 8 and 295 with 16; without the XCD remap 293 and 1.6x the
-  // fetched bytes; at K = 1024, 548 with 4 or 8, 530 with 2, 534 with 16)
+  // tile rows (512 rows of A; for 128-row tiles GM = 4 measured best: 325
+  // TFLOP/s at 65024^2 x 512 BF16 vs 315 with 8 and 295 with 16; without the
+  // XCD remap 293 and 1.6x the fetched bytes; at K = 1024, 548 with 4 or 8,
+  // 530 with 2, 534 with 16)
   int tm, tn;
   {
     const int bid = blockIdx.x, nblk = gridDim.x, per = nblk / 8;
     int lid = bid;
     if (bid < per * 8) lid = (bid % 8) * per + bid / 8;
-    constexpr int GM = 4;
+    constexpr int GM = 512 / BM;
     const int group_size = GM * tiles_n, group = lid / group_size, first_m = group * GM;
     const int gm = min(tiles_m - first_m, GM), in_group = lid - group * group_size;
     tm = first_m + in_group % gm;
@@ -110,8 +117,8 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
   const int wm0 = (wave / WGN) * 64, wn0 = (wave % WGN) * 64;
   const int lr = lane & 15, lq = lane >> 4;
   const bool full = m0 + BM <= M && n0 + BN <= N;
-  // products of this tile carry 2^escale (FP8 only; BM = BN = 128 there)
-  const int escale = escA ? escA[tm] + escB[tn] : 0;
+  // products of this wave's block carry 2^escale (FP8 only)
+  const int escale = escA ? escA[(m0 + wm0) >> 7] + escB[(n0 + wn0) >> 7] : 0;
 
   // The MFMA's first operand is the B tile (16 C columns), its second the A
   // tile (16 C rows), so D[r][c] of block (i, j) is C[row][col] with
@@ -154,21 +161,21 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
   int loffa[CA], loffb[CB];
 #pragma unroll
   for (int r = 0; r < CA; ++r) {
-    const int c = wave * 64 + 256 * r + lane;  // chunk slot in the A stage
-    const int row = c >> 2, kq = (c & 3) ^ ((row >> 2) & 3);
+    const int c = wave * 64 + NT * r + lane;  // chunk slot in the A stage
+    const int row = c >> 2, kq = (c & 3) ^ swz(row);
     ga[r] = A + (long)min(m0 + row, M - 1) * lda + EPC * kq;
-    loffa[r] = (wave * 64 + 256 * r) * 16;
+    loffa[r] = (wave * 64 + NT * r) * 16;
   }
 #pragma unroll
   for (int r = 0; r < CB; ++r) {
-    const int c = wave * 64 + 256 * r + lane;  // chunk slot in the B stage
-    const int row = c >> 2, kq = (c & 3) ^ ((row >> 2) & 3);
+    const int c = wave * 64 + NT * r + lane;  // chunk slot in the B stage
+    const int row = c >> 2, kq = (c & 3) ^ swz(row);
     gb[r] = B + (long)min(n0 + row, N - 1) * ldb + EPC * kq;
-    loffb[r] = HALF + (wave * 64 + 256 * r) * 16;
+    loffb[r] = HALF + (wave * 64 + NT * r) * 16;
   }
   const int nk = K / BK;
   auto issue = [&](int kc) {
-    const int kk = (kc < nk ? kc : nk - 1) * BK;  // past the end: reload the last stage
+    const int kk = kc * BK;
     char *st = smem + (kc % NST) * STAGE;
 #pragma unroll
     for (int r = 0; r < CA; ++r)
@@ -183,20 +190,28 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int m = wm0 + 16 * i + lr, n = wn0 + 16 * i + lr;
-    aoff[i] = m * 64 + ((lq ^ ((m >> 2) & 3)) << 4);
-    boff[i] = HALF + n * 64 + ((lq ^ ((n >> 2) & 3)) << 4);
+    aoff[i] = m * 64 + ((lq ^ swz(m)) << 4);
+    boff[i] = HALF + n * 64 + ((lq ^ swz(n)) << 4);
   }
 
+  static_assert(NST >= 2 && NST <= 5, "stages");
 #pragma unroll
-  for (int st = 0; st < NST - 1; ++st) issue(st);
+  for (int st = 0; st < NST - 1; ++st)
+    if (st < nk) issue(st);
   for (int kc = 0; kc < nk; ++kc) {
-    // the C prefetch (CINIT) was issued before the stages, so waiting for
-    // stage kc also waits for it on the first iteration
-    wait_vm_lp<(CA + CB) * (NST - 2)>();
+    // stage kc has landed once at most the CA + CB loads of each later stage
+    // already issued (min(NST - 2, nk - 1 - kc) of them; no stage past the
+    // end is loaded) are outstanding; the C prefetch (CINIT) was issued
+    // before the stages, so the first wait covers it too
+    const int ahead = nk - 1 - kc;
+    if (NST >= 5 && ahead >= 3) wait_vm_lp<(CA + CB) * (NST >= 5 ? 3 : 0)>();
+    else if (NST >= 4 && ahead >= 2) wait_vm_lp<(CA + CB) * (NST >= 4 ? 2 : 0)>();
+    else if (NST >= 3 && ahead >= 1) wait_vm_lp<(CA + CB) * (NST >= 3 ? 1 : 0)>();
+    else wait_vm_lp<0>();
     __asm__ volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     __asm__ volatile("" ::: "memory");
-    issue(kc + NST - 1);
+    if (kc + NST - 1 < nk) issue(kc + NST - 1);
     const char *st = smem + (kc % NST) * STAGE;
     vec_t a[4], b[4];
 #pragma unroll
@@ -245,14 +260,14 @@ __global__ __launch_bounds__(256, OCC) void hgemm_nt_kernel(int M, int N, int K,
     }
 }
 
-template <typename T16, int BMv, int NS, int OC, bool CI, bool V>
+template <typename T16, int BMv, int BNv, int NS, int OC, bool CI, bool V>
 static void hgemm_launch(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B,
                          long ldb, float beta, float *C, long ldc, hipStream_t s,
                          const int *escA = nullptr, const int *escB = nullptr) {
-  constexpr int BNv = 16384 / BMv;
+  constexpr int NT = (BMv / 64) * (BNv / 64) * 64;
   const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
-  hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, NS, OC, CI, V>), dim3(tm * tn), dim3(256), 0, s,
-                     M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB);
+  hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, BNv, NS, OC, CI, V>), dim3(tm * tn), dim3(NT), 0,
+                     s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB);
 }
 // 128 x 128 tiles, 2 stages, 4 workgroups per CU (measured at 65280^2 x 256
 // BF16: 212 TFLOP/s against 188 with 4 stages and 2 per CU, 198 with 64 x
@@ -261,28 +276,46 @@ template <typename T16>
 static void hgemm_launch2(bool cinit, bool vec, int M, int N, int K, float alpha, const T16 *A,
                           long lda, const T16 *B, long ldb, float beta, float *C, long ldc,
                           hipStream_t s, const int *escA = nullptr, const int *escB = nullptr) {
-  // 16-bit operands with K >= 1024 (the mixed LU's block 1024): 3 stages,
-  // 3 workgroups per CU (546 TFLOP/s at 64512^2 x 1024 BF16 against 526
-  // with 2 stages and 4 per CU, 522 with 4 and 2).  FP8 stays at 2 and 4:
-  // its update at n = 131072 took 3.5 s with 3 and 3 against 2.7 s, and at
-  // n = 65536 it ran at 533 TFLOP/s with 2 and 4 (532 with 2 and 3, 426
-  // with 4 and 2, 413 with 3 and 2).
+  // Smaller grids keep 128 x 128 tiles: 16-bit operands with K >= 1024 on
+  // 3 stages, 3 workgroups per CU (546 TFLOP/s at 64512^2 x 1024 BF16
+  // against 526 with 2 stages and 4 per CU, 522 with 4 and 2, before the
+  // bank-conflict-free swizzle); FP8 on 2 stages and 4 per CU (its update at
+  // n = 131072 took 3.5 s with 3 and 3 against 2.7 s).
+  // Large accumulate-into-C updates (the mixed LU's trailing update at
+  // block 1024): bigger tiles read fewer operand bytes per flop.  At
+  // 64512^2 x 1024 the BF16 update ran at 615 TFLOP/s standalone with
+  // 256 x 256 tiles (4 stages, one workgroup of 16 waves per CU) against
+  // 572 with 128 x 128 (3 stages, 3 per CU) and 607 with 128 x 256, and at
+  // 507 against 433 inside the LU at n = 65536; FP8 at 749 with 128 x 256
+  // tiles (3 stages, 2 per CU) against 695 with 128 x 128 (2 stages, 4 per
+  // CU) and 668 with 256 x 256 (639 against 562 inside the LU).  Only while
+  // the grid keeps every CU busy with them (at least 2^25 C elements).
+  const bool large = (long)M * N >= (1L << 25);
+  if (cinit && vec && K >= 1024 && large) {
+    if constexpr (sizeof(T16) == 2)
+      hgemm_launch<T16, 256, 256, 4, 1, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                    s, escA, escB);
+    else
+      hgemm_launch<T16, 128, 256, 3, 2, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                    s, escA, escB);
+    return;
+  }
   if (cinit && vec && K >= 1024 && sizeof(T16) == 2) {
-    hgemm_launch<T16, 128, 3, 3, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+    hgemm_launch<T16, 128, 128, 3, 3, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
                                              escA, escB);
     return;
   }
   if (cinit && vec)
-    hgemm_launch<T16, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+    hgemm_launch<T16, 128, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
                                              escA, escB);
   else if (cinit)
-    hgemm_launch<T16, 128, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+    hgemm_launch<T16, 128, 128, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
                                               escA, escB);
   else if (vec)
-    hgemm_launch<T16, 128, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+    hgemm_launch<T16, 128, 128, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
                                               escA, escB);
   else
-    hgemm_launch<T16, 128, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+    hgemm_launch<T16, 128, 128, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
                                                escA, escB);
 }
 

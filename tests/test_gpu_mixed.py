@@ -74,6 +74,49 @@ def test_gemm_lp_dev_vs_fp32(dev, prec, M, N, K, alpha, beta, ldc_pad):
     assert torch.equal(Cfull[:, N:], C0[:, N:])
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp8"])
+def test_large_update_tiles_vs_fp64(dev, prec):
+    """The trailing update's large-grid tiles (256 x 256 for 16-bit
+    operands, 128 x 256 for FP8; taken from 2^25 C elements with K >= 1024,
+    C accumulated in place) on a ragged 6144 x 5900 x 1024 product against
+    the FP64 product of the same operands: 16-bit within the FP32
+    summation bound, FP8 within the e4m3 rounding bound of its operands."""
+    import ctypes
+    import torch
+    from charmlu_amd import _lib
+    lib = _lib.load()
+    M, N, K = 6144, 5900, 1024
+    torch.manual_seed(7)
+    C0 = torch.rand(M, N, device="cuda")
+    C = C0.clone()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if prec == "fp8":
+        A = torch.rand(M, K, device="cuda") - 0.5
+        B = torch.rand(K, N, device="cuda") - 0.5
+        rc = lib.charmlu_gemm_fp8_dev(M, N, K, 1.0, ctypes.c_void_p(A.data_ptr()), K,
+                                      ctypes.c_void_p(B.data_ptr()), N, 1.0,
+                                      ctypes.c_void_p(C.data_ptr()), N, s)
+        torch.cuda.synchronize()
+        prod = A.double() @ B.double()
+        bound = 2 * 2.0 ** -4 * (A.double().abs() @ B.double().abs()) + 2.0 ** -22 * (
+            prod.abs() + C0.double().abs()) * 4 + 1e-30
+    else:
+        dt = torch.bfloat16 if prec == "bf16" else torch.float16
+        A = (torch.rand(M, K, device="cuda") - 0.5).to(dt)
+        B = (torch.rand(N, K, device="cuda") - 0.5).to(dt)
+        rc = lib.charmlu_gemm_lp_dev(_lib.PREC_BF16 if prec == "bf16" else _lib.PREC_FP16, M, N,
+                                     K, 1.0, ctypes.c_void_p(A.data_ptr()), K,
+                                     ctypes.c_void_p(B.data_ptr()), K, 1.0,
+                                     ctypes.c_void_p(C.data_ptr()), N, s)
+        torch.cuda.synchronize()
+        prod = A.double() @ B.double().T
+        bound = K * 2.0 ** -23 * (A.double().abs() @ B.double().abs().T) + 2.0 ** -23 * (
+            C0.double().abs() + prod.abs()) * 4 + 1e-30
+    assert rc == 0
+    err = (C.double() - (prod + C0.double())).abs()
+    assert bool((err <= bound).all()), float((err / bound).max())
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("n,nb", [(512, 128), (1024, 256), (768, 256), (960, 192), (1002, 167)])
 def test_mixed_factor_diagdom(dev, orc, prec, n, nb):
