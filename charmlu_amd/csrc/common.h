@@ -49,6 +49,16 @@ CLU_DEV T ld_agent(const T *p) {
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// XOR swizzle of the 4 16-byte chunks of a 64-byte LDS operand row: chunk k
+// of row r sits at chunk k ^ lds_swz(r).  A fragment read takes one chunk
+// of 16 consecutive rows (16 bytes per lane), 8 lanes per LDS cycle:
+// lds_swz gives every 8 consecutive rows of one parity, and every 4 rows of
+// one residue modulo 4, four different chunks, so neither 8- nor 16-lane
+// groups of a read share banks ((r >> 2) & 3 alone left rows r and r + 2 on
+// the same banks: half the LDS cycles of the BF16 update, 40 % of the FP64
+// one's, were bank conflicts).
+CLU_DEV int lds_swz(int r) { return ((r >> 2) & 1) | ((((r >> 1) ^ (r >> 3)) & 1) << 1); }
+
 // Wait until every outstanding memory operation of this thread completed.
 CLU_DEV void wait_all_mem() { __builtin_amdgcn_s_waitcnt(0); }
 

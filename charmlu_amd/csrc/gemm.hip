@@ -286,7 +286,7 @@ __global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(
 // LU trailing update.
 //
 // LDS layout per stage (16 KB): A as 128 rows x 4 chunks of 16 bytes, the
-// chunk index XORed with (row >> 2) & 3; B as 8 rows x 64 chunks, the chunk
+// chunk index XORed with lds_swz(row); B as 8 rows x 64 chunks, the chunk
 // index XORed with (k & 1) << 3.  A direct-to-LDS load writes lane i's 16
 // bytes at base + 16 i, so the swizzle is applied by choosing which global
 // chunk each lane fetches.
@@ -326,7 +326,7 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int c = wave * 64 + 256 * r + lane;  // LDS chunk slot
-    const int am = c >> 2, akq = (c & 3) ^ ((am >> 2) & 3);
+    const int am = c >> 2, akq = (c & 3) ^ lds_swz(am);
     ga[r] = A + (long)(m0 + am) * lda + 2 * akq;
     la_off[r] = (wave * 64 + 256 * r) * 16;  // wave-uniform slot base
     const int bk = c >> 6, bnq = (c & 63) ^ ((bk & 1) << 3);
@@ -344,31 +344,47 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
   };
 
   // per-lane fragment offsets inside a stage
-  int aoff[2][4], boff[2][4];
+  // The two matrix-core steps of a stage take k = 2 lq + s2 on lane group
+  // lq (s2 = 0, 1), so a lane's A values of both steps are the two doubles
+  // of one 16-byte LDS chunk (chunk lq of its row): one read per row block
+  // The 16 columns of matrix-core block j are not contiguous: lane lr of
+  // block pair p = j / 2 holds columns 32 p + 2 lr (block 2p) and
+  // 32 p + 2 lr + 1 (block 2p + 1), so a lane's B values of both blocks of a
+  // pair, and its C values, are 16-byte pairs
+  int aoff[4], boff[2][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = wm0 + 16 * i + lr;
+    aoff[i] = m * 64 + ((lq ^ lds_swz(m)) << 4);
+  }
 #pragma unroll
   for (int s2 = 0; s2 < 2; ++s2) {
-    const int k = 4 * s2 + lq;
+    const int k = 2 * lq + s2;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int m = wm0 + 16 * i + lr;
-      aoff[s2][i] = m * 64 + (((k >> 1) ^ ((m >> 2) & 3)) << 4) + ((k & 1) << 3);
-      const int n = wn0 + 16 * i + lr;
-      boff[s2][i] = BM * BK * 8 + k * 1024 + (((n >> 1) ^ ((k & 1) << 3)) << 4) + ((n & 1) << 3);
+    for (int p = 0; p < 2; ++p) {
+      const int n = wn0 + 32 * p + 2 * lr;
+      boff[s2][p] = BM * BK * 8 + k * 1024 + (((n >> 1) ^ ((k & 1) << 3)) << 4);
     }
   }
 
-  // accumulator element (i, j, v) is C row 16 i + lq + 4 v, column 16 j + lr
-  // of this wave's 64 x 64 block
+  // accumulator element (i, j, v) is C row 16 i + lq + 4 v, column
+  // 32 (j / 2) + 2 lr + (j % 2) of this wave's 64 x 64 block
   double *Cw = C + (long)(m0 + wm0) * ldc + n0 + wn0;
+  auto cpair = [&](int i, int p, int v) {
+    return reinterpret_cast<double2 *>(Cw + (long)(i * 16 + lq + 4 * v) * ldc + 32 * p + 2 * lr);
+  };
   double4_t acc[4][4];
   if (CINIT && beta != 0.0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          acc[i][j][v] = beta * Cw[(long)(i * 16 + lq + 4 * v) * ldc + j * 16 + lr];
+        for (int v = 0; v < 4; ++v) {
+          const double2 c2 = *cpair(i, p, v);
+          acc[i][2 * p][v] = beta * c2.x;
+          acc[i][2 * p + 1][v] = beta * c2.y;
+        }
   } else {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -398,13 +414,20 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
     __asm__ volatile("" ::: "memory");
     if (kc + NST - 1 < nk) issue(kc + NST - 1);
     const char *st = smem + (kc % NST) * STAGE;
+    double2 a2[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a2[i] = *reinterpret_cast<const double2 *>(st + aoff[i]);
 #pragma unroll
     for (int s2 = 0; s2 < 2; ++s2) {
       double a[4], b[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const double *>(st + aoff[s2][i]);
+      for (int i = 0; i < 4; ++i) a[i] = s2 ? a2[i].y : a2[i].x;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const double *>(st + boff[s2][j]);
+      for (int p = 0; p < 2; ++p) {
+        const double2 b2 = *reinterpret_cast<const double2 *>(st + boff[s2][p]);
+        b[2 * p] = b2.x;
+        b[2 * p + 1] = b2.y;
+      }
       if (neg) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) a[i] = -a[i];
@@ -421,12 +444,37 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-          Cw[(long)(i * 16 + lq + 4 * v) * ldc + j * 16 + lr] = acc[i][j][v];
+          *cpair(i, p, v) = double2{acc[i][2 * p][v], acc[i][2 * p + 1][v]};
   } else {
-    store_tile<double, 4, 4>(acc, C, ldc, m0 + wm0, n0 + wn0, lr, lq, alpha, beta, 0, 0, true);
+    // C = alpha acc + beta C, a row block's C pairs requested before the
+    // previous one is combined and stored
+    double2 cur[2][4], nxt[2][4];
+    auto fetch = [&](int i, double2 (&d)[2][4]) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) d[p][v] = beta != 0.0 ? *cpair(i, p, v) : double2{0.0, 0.0};
+    };
+    fetch(0, cur);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i + 1 < 4) fetch(i + 1, nxt);
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          *cpair(i, p, v) = double2{alpha * acc[i][2 * p][v] + beta * cur[p][v].x,
+                                    alpha * acc[i][2 * p + 1][v] + beta * cur[p][v].y};
+      if (i + 1 < 4) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) cur[p][v] = nxt[p][v];
+      }
+    }
   }
 }
 
@@ -459,15 +507,16 @@ static int launch(int M, int N, int K, T alpha, const T *A, long lda, const T *B
 }
 
 static bool lds_aligned(int M, int N, int K, const double *A, long lda, const double *B,
-                        long ldb) {
+                        long ldb, const double *C, long ldc) {
   return M % 128 == 0 && N % 128 == 0 && K % 8 == 0 && K > 0 && lda % 2 == 0 && ldb % 2 == 0 &&
-         ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
+         ldc % 2 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0 &&
+         ((uintptr_t)C & 15) == 0;
 }
 // the direct-to-LDS kernel for general callers: aligned and enough tiles to
 // fill the chip (smaller grids do better with 64 x 64 tiles)
 static bool lds_path_ok(int M, int N, int K, const double *A, long lda, const double *B,
-                        long ldb) {
-  return lds_aligned(M, N, K, A, lda, B, ldb) && (long)(M / 128) * (N / 128) >= 256;
+                        long ldb, const double *C, long ldc) {
+  return lds_aligned(M, N, K, A, lda, B, ldb, C, ldc) && (long)(M / 128) * (N / 128) >= 256;
 }
 
 template <typename T, bool TA, bool TB>
@@ -475,8 +524,10 @@ static int dispatch(int M, int N, int K, T alpha, const T *A, long lda, const T 
                     T beta, T *C, long ldc, hipStream_t s, int role) {
   const bool trailing = role == 1 && (alpha == (T)1 || alpha == (T)-1);
   if (sizeof(T) == 8 && !TA && !TB &&
-      (trailing ? lds_aligned(M, N, K, (const double *)A, lda, (const double *)B, ldb)
-                : lds_path_ok(M, N, K, (const double *)A, lda, (const double *)B, ldb))) {
+      (trailing ? lds_aligned(M, N, K, (const double *)A, lda, (const double *)B, ldb,
+                              (const double *)C, ldc)
+                : lds_path_ok(M, N, K, (const double *)A, lda, (const double *)B, ldb,
+                              (const double *)C, ldc))) {
     const int tiles_m = M / 128, tiles_n = N / 128;
     const bool cinit = alpha == (T)1 || alpha == (T)-1;
 #define CLU_LDS_LAUNCH(R, CI)                                                                   \
@@ -535,8 +586,8 @@ int dgemm_trailing(int M, int N, int K, double alpha, const double *A, long lda,
   return gemm_any<double>(false, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, 1);
 }
 bool dgemm_trailing_lds(int M, int N, int K, const double *A, long lda, const double *B,
-                        long ldb) {
-  return lds_aligned(M, N, K, A, lda, B, ldb);
+                        long ldb, const double *C, long ldc) {
+  return lds_aligned(M, N, K, A, lda, B, ldb, C, ldc);
 }
 
 int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A, long lda,

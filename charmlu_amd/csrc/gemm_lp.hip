@@ -65,15 +65,6 @@ CLU_DEV void wait_vm_lp() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
 }
 
-// XOR swizzle of the 4 16-byte chunks of an LDS operand row (64 bytes): the
-// chunk k of row r sits at chunk k ^ swz(r).  A fragment read takes one
-// chunk of 16 consecutive rows (16 bytes per lane), 8 lanes per LDS cycle:
-// swz gives every 8 consecutive rows of one parity, and every 4 rows of one
-// residue modulo 4, four different chunks, so neither 8- nor 16-lane groups
-// of a read share banks ((r >> 2) & 3 alone left rows r and r + 2 on the
-// same banks: half the LDS cycles of the BF16 update were bank conflicts).
-CLU_DEV int swz(int r) { return ((r >> 2) & 1) | ((((r >> 1) ^ (r >> 3)) & 1) << 1); }
-
 template <typename T16, int BM, int BN, int NST, int OCC, bool CINIT, bool VEC>
 __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kernel(
     int M, int N, int K, float alpha, const T16 *__restrict__ A, long lda,
@@ -162,14 +153,14 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
 #pragma unroll
   for (int r = 0; r < CA; ++r) {
     const int c = wave * 64 + NT * r + lane;  // chunk slot in the A stage
-    const int row = c >> 2, kq = (c & 3) ^ swz(row);
+    const int row = c >> 2, kq = (c & 3) ^ lds_swz(row);
     ga[r] = A + (long)min(m0 + row, M - 1) * lda + EPC * kq;
     loffa[r] = (wave * 64 + NT * r) * 16;
   }
 #pragma unroll
   for (int r = 0; r < CB; ++r) {
     const int c = wave * 64 + NT * r + lane;  // chunk slot in the B stage
-    const int row = c >> 2, kq = (c & 3) ^ swz(row);
+    const int row = c >> 2, kq = (c & 3) ^ lds_swz(row);
     gb[r] = B + (long)min(n0 + row, N - 1) * ldb + EPC * kq;
     loffb[r] = HALF + (wave * 64 + NT * r) * 16;
   }
@@ -190,8 +181,8 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int m = wm0 + 16 * i + lr, n = wn0 + 16 * i + lr;
-    aoff[i] = m * 64 + ((lq ^ swz(m)) << 4);
-    boff[i] = HALF + n * 64 + ((lq ^ swz(n)) << 4);
+    aoff[i] = m * 64 + ((lq ^ lds_swz(m)) << 4);
+    boff[i] = HALF + n * 64 + ((lq ^ lds_swz(n)) << 4);
   }
 
   static_assert(NST >= 2 && NST <= 5, "stages");

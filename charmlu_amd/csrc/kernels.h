@@ -19,12 +19,12 @@ int dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double *A,
 int dgemm_lookahead(int M, int N, int K, double alpha, const double *A, long lda,
                     const double *B, long ldb, double beta, double *C, long ldc, hipStream_t s);
 // The factorization's trailing update (alpha = +-1): the direct-to-LDS
-// kernel whenever dgemm_trailing_lds holds (aligned operands, M and N
-// multiples of 128, K of 8), whatever the number of tiles.
+// kernel whenever dgemm_trailing_lds holds (16-byte aligned operands and C,
+// M and N multiples of 128, K of 8), whatever the number of tiles.
 int dgemm_trailing(int M, int N, int K, double alpha, const double *A, long lda,
                    const double *B, long ldb, double beta, double *C, long ldc, hipStream_t s);
 bool dgemm_trailing_lds(int M, int N, int K, const double *A, long lda, const double *B,
-                        long ldb);
+                        long ldb, const double *C, long ldc);
 
 // The same on FP32 operands (FP32 matrix cores).
 int sgemm(bool ta, bool tb, int M, int N, int K, float alpha, const float *A, long lda,

@@ -1153,8 +1153,9 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
   const bool split = ws.panel_cus > 0;
   const bool force_split = split_forced();
   bool full_pending = false;
-  auto split_rows = [&](int t, int M, int N, const double *L, const double *U, long ldu) -> int {
-    if (!split || !dgemm_trailing_lds(M, N, nb, L, lda, U, ldu)) return M;
+  auto split_rows = [&](int t, int M, int N, const double *L, const double *U, long ldu,
+                        const double *C) -> int {
+    if (!split || !dgemm_trailing_lds(M, N, nb, L, lda, U, ldu, C, lda)) return M;
     if (force_split) return forced_split(M);
     const int m = n - (t + 1) * nb;  // rows of panel t+1
     double p_ms = (t + 1 < (int)ws.hist64.ms.size()) ? ws.hist64.ms[t + 1] : -1.0;
@@ -1261,7 +1262,7 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     if ((rc = u_row(la, Nt))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_urow, sm));
     double *C2 = A11 + (long)nb * lda + nb + la;
-    const int Ma = Nt > la ? split_rows(t, Nt, Nt - la, L21, U + la, ldu) : 0;
+    const int Ma = Nt > la ? split_rows(t, Nt, Nt - la, L21, U + la, ldu, C2) : 0;
     if (Nt > la && (rc = update_part1(Ma, Nt - la, L21, U + la, ldu, C2))) return rc;
     CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
     CLU_HIP(hipEventRecord(ws.hist64.ev[2 * (t + 1)], sp));
