@@ -21,6 +21,7 @@ SIGNATURES = {
     "charmlu_last_error": (ctypes.c_char_p, []),
     "charmlu_init": (_c_int, [_c_int]),
     "charmlu_finalize": (_c_int, []),
+    "charmlu_panel_probe": (_c_int, [_c_int, _c_int, _vp, _vp]),
     "charmlu_panel_plan": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                     ctypes.POINTER(ctypes.c_int)]),
     "charmlu_profile": (_c_int, [_c_int]),

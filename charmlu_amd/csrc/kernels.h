@@ -175,6 +175,9 @@ int abs_colsum(int n, const double *A, long lda, double *out, hipStream_t s);
 template <typename TD, typename TS>
 int convert(long m, long n, const TS *src, long lds, TD *dst, long ldd, hipStream_t s);
 int fill_int(int *p, int v, long n, hipStream_t s);
+// out[4 g + {0..3}] = {XCC id, HW_ID, start, end (100 MHz clock)} of each of
+// `groups` one-per-CU workgroups that sleep `sleeps` x 8128 cycles
+int residency_probe(int groups, int sleeps, unsigned long long *out, hipStream_t s);
 // info[0] = -1 if abortw[0] != 0 (a panel strip's exchange gave up)
 int finish_info(int *info, const unsigned *abortw, hipStream_t s);
 

@@ -747,6 +747,25 @@ int charmlu_finalize(void) {
   return rc;
 }
 
+int charmlu_panel_probe(int groups, int sleeps, unsigned long long *dout, void *stream) {
+  if (groups < 0 || groups > 4096) return -1;
+  if (sleeps < 0 || sleeps > 4096) return -2;
+  if (!dout && groups > 0) return -3;
+  if (groups == 0) return CLU_OK;
+  Workspace *wsp;
+  int rc;
+  if ((rc = get_ws(&wsp))) return rc;
+  Workspace &ws = *wsp;
+  hipStream_t su = (hipStream_t)stream;
+  // on the panel stream, ordered after the caller's stream and back
+  CLU_HIP(hipEventRecord(ws.ev_start, su));
+  CLU_HIP(hipStreamWaitEvent(ws.s_panel, ws.ev_start, 0));
+  if ((rc = residency_probe(groups, sleeps, dout, ws.s_panel))) return rc;
+  CLU_HIP(hipEventRecord(ws.ev_end, ws.s_panel));
+  CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
+  return CLU_OK;
+}
+
 int charmlu_panel_plan(int prec, int m, int w, int panel_cus, int ncu, int pivot, int *out) {
   if (prec != CHARMLU_PREC_FP64 && prec != CHARMLU_PREC_FP32) return -1;
   if (m < 1) return -2;

@@ -218,6 +218,39 @@ int fill_int(int *p, int v, long n, hipStream_t s) {
   return CLU_OK;
 }
 
+// Co-residency probe of a stream's CU mask: one workgroup per CU (96 KB of
+// LDS each), every workgroup stamps the constant 100 MHz clock, sleeps a
+// fixed time, stamps again and writes {XCC id, hardware id, start, end}.
+// Workgroups that were not resident together start after another one ended.
+// Every workgroup ends after its fixed sleep: nothing waits on another.
+__global__ __launch_bounds__(256) void residency_probe_kernel(unsigned long long *out, int sleeps) {
+  __shared__ int hold[96 * 1024 / 4];
+  // every thread touches its part of the 96 KB (the array must stay
+  // allocated: it is what holds one workgroup per CU)
+  hold[threadIdx.x * 96] = (int)threadIdx.x + sleeps;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int pick = (int)((unsigned)sleeps % 256u);
+    const int zero = hold[pick * 96] - pick - sleeps;  // 0
+    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    for (int i = 0; i < sleeps; ++i) __builtin_amdgcn_s_sleep(127);
+    const unsigned long long t1 = __builtin_amdgcn_s_memrealtime();
+    const unsigned hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    // HW_REG_HW_ID
+    const unsigned xcc = __builtin_amdgcn_s_getreg((31 << 11) | 20);  // HW_REG_XCC_ID
+    out[4 * blockIdx.x + 0] = xcc + (unsigned)zero;
+    out[4 * blockIdx.x + 1] = hw;
+    out[4 * blockIdx.x + 2] = t0;
+    out[4 * blockIdx.x + 3] = t1;
+  }
+}
+
+int residency_probe(int groups, int sleeps, unsigned long long *out, hipStream_t s) {
+  if (groups <= 0) return CLU_OK;
+  hipLaunchKernelGGL(residency_probe_kernel, dim3(groups), dim3(256), 0, s, out, sleeps);
+  CLU_LAUNCHED("residency_probe_kernel");
+  return CLU_OK;
+}
+
 __global__ void finish_info_kernel(int *info, const unsigned *abortw) {
   if (threadIdx.x == 0 && abortw[0] != 0u) info[0] = -1;
 }

@@ -100,6 +100,14 @@ int charmlu_finalize(void);
  * workgroup, workgroups, workgroups resident at once, widest strip for m
  * rows (0: none)}.  Returns 0 or CHARMLU_ERR_UNSUPPORTED. */
 int charmlu_panel_plan(int prec, int m, int w, int panel_cus, int ncu, int pivot, int *out);
+/* Co-residency check of the panel stream's CU mask (no counterpart in the
+ * reference): launches `groups` workgroups of a kernel that holds one
+ * workgroup per CU on the stream the panel strips run on; each sleeps
+ * `sleeps` x 8128 cycles and writes {XCC id, hardware id, start, end} (the
+ * end stamps on the 100 MHz constant clock) to dout[4 * groups] (device).
+ * A workgroup that starts after another one ended was not resident with
+ * it.  Ordered on `stream`; asynchronous. */
+int charmlu_panel_probe(int groups, int sleeps, unsigned long long *dout, void *stream);
 
 /* ---- 1. platformBlas.h surface: host pointers, synchronous ------------- */
 

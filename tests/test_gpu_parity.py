@@ -636,3 +636,38 @@ def test_panel_cus_zero_is_bitwise_the_default(dev, orc, tmp_path):
     assert np.array_equal(z["A"], A.cpu().numpy())
     assert np.array_equal(z["ips"], ips.cpu().numpy())
     assert np.array_equal(z["S"], S.cpu().numpy())
+
+
+def _probe(dev, groups, sleeps=400):
+    import torch
+    from charmlu_amd import _lib
+    out = torch.zeros(4 * groups, dtype=torch.int64, device="cuda")
+    rc = _lib.load().charmlu_panel_probe(groups, sleeps, _p(out), None)
+    assert rc == 0
+    _sync()
+    h = out.cpu().numpy().reshape(groups, 4).astype(np.uint64)
+    t0, t1 = h[:, 2], h[:, 3]
+    late = int((t0 >= t1.min()).sum())
+    return late, h
+
+
+def test_panel_stream_holds_the_resident_workgroups(dev):
+    """The capacity rule the strip shapes are planned with, on this device:
+    the panel stream (32 CUs, the default) holds as many one-per-CU
+    workgroups at once as charmlu_panel_plan says (every one starts before
+    any ends), and not 4 more (some start only after another one ended:
+    workgroups are dealt round-robin to the XCDs and their shader engines,
+    whatever the free CUs).  Each workgroup sleeps about 1.4 ms."""
+    import ctypes
+    from charmlu_amd import _lib
+    out = (ctypes.c_int * 6)()
+    assert _lib.load().charmlu_panel_plan(_lib.PREC_FP64, 1024, 16, 32, 256, 1, out) == 0
+    resident = out[4]
+    assert resident == 32
+    late, h = _probe(dev, resident)
+    print(f"{resident} workgroups: {late} started after the first end; XCCs "
+          f"{sorted(set(int(x) for x in h[:, 0]))}")
+    assert late == 0
+    late, _ = _probe(dev, resident + 4)
+    print(f"{resident + 4} workgroups: {late} started after the first end")
+    assert late > 0
