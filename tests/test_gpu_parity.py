@@ -671,3 +671,34 @@ def test_panel_stream_holds_the_resident_workgroups(dev):
     late, _ = _probe(dev, resident + 4)
     print(f"{resident + 4} workgroups: {late} started after the first end")
     assert late > 0
+
+
+@pytest.mark.parametrize("cus", [16, 24, 48])
+def test_panel_stream_resident_workgroups_other_masks(dev, tmp_path, cus):
+    """The same check on other panel CU counts (a fresh process each: the
+    streams are made once per device): 16 and 24 CUs hold 16 and 24
+    one-per-CU workgroups, 48 CUs (1 or 2 per shader engine) only 32."""
+    import subprocess
+    import sys
+    code = ("import sys, ctypes, numpy as np, torch, charmlu_amd as c\n"
+            "from charmlu_amd import _lib\n"
+            "L = _lib.load(); o = (ctypes.c_int * 6)()\n"
+            "assert L.charmlu_panel_plan(0, 1024, 16, int(sys.argv[2]), 256, 1, o) == 0\n"
+            "res = []\n"
+            "for g in (o[4], o[4] + 4):\n"
+            "    out = torch.zeros(4 * g, dtype=torch.int64, device='cuda')\n"
+            "    assert L.charmlu_panel_probe(g, 400, ctypes.c_void_p(out.data_ptr()), None) == 0\n"
+            "    torch.cuda.synchronize()\n"
+            "    h = out.cpu().numpy().reshape(g, 4).astype(np.uint64)\n"
+            "    res.append([g, int((h[:, 2] >= h[:, 3].min()).sum())])\n"
+            "np.save(sys.argv[1], np.array(res))\n")
+    out = str(tmp_path / "probe.npy")
+    env = dict(os.environ, CHARMLU_PANEL_CUS=str(cus))
+    print(f"subprocess: CHARMLU_PANEL_CUS={cus}", flush=True)
+    r = subprocess.run([sys.executable, "-c", code, out, str(cus)], env=env, cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-3000:]
+    (g0, late0), (g1, late1) = np.load(out).tolist()
+    print(f"{cus} panel CUs: {g0} workgroups, {late0} late; {g1} workgroups, {late1} late")
+    assert g0 == {16: 16, 24: 24, 48: 32}[cus]
+    assert late0 == 0 and late1 > 0
