@@ -563,7 +563,7 @@ def main():
             "frac": round(achieved / FP64_PEAK, 4),
             "traffic": traffic,
             "traffic_note": traffic_note,
-            "kernel": "dgemm_nn_lds_kernel<4, 1, true> (trailing update, 128x128 tiles)",
+            "kernel": "dgemm_nn_lds_kernel<3, 1, true> (trailing update, 128x128 tiles)",
             "launches": int(g_cnt),
             "avg_launch_ms": round(g_ms / g_cnt, 4) if g_cnt else None,
             "flops_per_launch": g_work / g_cnt if g_cnt else None,

@@ -285,6 +285,17 @@ __global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(
 // of 128, K of 8, and every row start is 16-byte aligned -- the case of the
 // LU trailing update.
 //
+// Three workgroups per CU (three waves per SIMD: at most 168 VGPRs, three
+// 48 KB stage rings): while one workgroup waits for its C tile at the start
+// or drains its stores at the end, the other two keep the matrix cores busy.
+// Two workgroups with four stages reached 67.8 % of the FP64 peak at
+// 32512^2 x 256, three with three stages 75.7 %.  C is read and written
+// with non-temporal accesses (each C element is touched once per launch), so
+// it does not evict the A and B tiles from L2: 79.5 %; with the first stages
+// requested before C, 82 %.  The global addresses
+// are uniform bases plus one 32-bit lane offset each, so that the 32 C
+// addresses of a lane are not kept live across the K loop (64 VGPRs).
+//
 // LDS layout per stage (16 KB): A as 128 rows x 4 chunks of 16 bytes, the
 // chunk index XORed with lds_swz(row); B as 8 rows x 64 chunks, the chunk
 // index XORed with (k & 1) << 3.  A direct-to-LDS load writes lane i's 16
@@ -296,6 +307,20 @@ typedef __attribute__((address_space(1))) void *gbl_ptr_t;
 
 CLU_DEV void load16_to_lds(const double *g, void *lds_base) {
   __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, (lds_ptr_t)lds_base, 16, 0, 0);
+}
+typedef double dvec2_t __attribute__((ext_vector_type(2)));
+CLU_DEV double2 ld_nt(const double2 *p) {
+  const dvec2_t v = __builtin_nontemporal_load(reinterpret_cast<const dvec2_t *>(p));
+  return double2{v.x, v.y};
+}
+CLU_DEV void st_nt(double2 *p, double x, double y) {
+  __builtin_nontemporal_store((dvec2_t){x, y}, reinterpret_cast<dvec2_t *>(p));
+}
+// p advanced by a byte offset (a 32-bit lane offset added to a uniform base)
+template <typename T>
+CLU_DEV T *byte_off(T *p, unsigned off) {
+  return reinterpret_cast<T *>(reinterpret_cast<char *>(const_cast<void *>(
+                                   reinterpret_cast<const void *>(p))) + off);
 }
 
 // s_waitcnt vmcnt(N) (N < 16) leaving the other counters unconstrained
@@ -316,31 +341,35 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
                       double *__restrict__ C, long ldc) {
   constexpr int BM = 128, BN = 128, BK = 8;
   constexpr int STAGE = (BM * BK + BK * BN) * 8;  // bytes
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
   const int lr = lane & 15, lq = lane >> 4;
 
-  // global source of this thread's chunks (two of A, two of B per stage)
+  // global source of this thread's chunks (two of A, two of B per stage):
+  // A chunk slot c = 64 wave + 256 r + lane is row c / 4 = 16 wave + 64 r +
+  // lane / 4, 16-byte chunk (c & 3) ^ lds_swz(row) (the swizzle depends on
+  // lane / 4 only); B chunk c is k row c / 64 = wave + 4 r, chunk
+  // (c & 63) ^ ((wave & 1) << 3).  Uniform row bases, one lane offset each.
   const double *ga[2], *gb[2];
   int la_off[2], lb_off[2];
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
-    const int c = wave * 64 + 256 * r + lane;  // LDS chunk slot
-    const int am = c >> 2, akq = (c & 3) ^ lds_swz(am);
-    ga[r] = A + (long)(m0 + am) * lda + 2 * akq;
+    ga[r] = A + (long)(m0 + wave * 16 + 64 * r) * lda;
+    gb[r] = B + (long)(wave + 4 * r) * ldb + n0;
     la_off[r] = (wave * 64 + 256 * r) * 16;  // wave-uniform slot base
-    const int bk = c >> 6, bnq = (c & 63) ^ ((bk & 1) << 3);
-    gb[r] = B + (long)bk * ldb + n0 + 2 * bnq;
     lb_off[r] = BM * BK * 8 + (wave * 64 + 256 * r) * 16;
   }
+  const unsigned a_lo = (unsigned)(((lane >> 2) * (int)lda + 2 * ((lane & 3) ^ lds_swz(lane >> 2))) * 8);
+  const unsigned b_lo = (unsigned)(2 * (lane ^ ((wave & 1) << 3)) * 8);
   const int nk = K / BK;
   auto issue = [&](int kc) {
     const int kk = kc * BK;
     char *st = smem + (kc % NST) * STAGE;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) load16_to_lds(ga[r] + kk, st + la_off[r]);
+    for (int r = 0; r < 2; ++r) load16_to_lds(byte_off(ga[r] + kk, a_lo), st + la_off[r]);
 #pragma unroll
-    for (int r = 0; r < 2; ++r) load16_to_lds(gb[r] + (long)kk * ldb, st + lb_off[r]);
+    for (int r = 0; r < 2; ++r) load16_to_lds(byte_off(gb[r] + (long)kk * ldb, b_lo), st + lb_off[r]);
   };
 
   // per-lane fragment offsets inside a stage
@@ -368,11 +397,20 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
   }
 
   // accumulator element (i, j, v) is C row 16 i + lq + 4 v, column
-  // 32 (j / 2) + 2 lr + (j % 2) of this wave's 64 x 64 block
+  // 32 (j / 2) + 2 lr + (j % 2) of this wave's 64 x 64 block: a uniform
+  // base per (i, j / 2, v) plus the lane's offset c_lo (bytes)
   double *Cw = C + (long)(m0 + wm0) * ldc + n0 + wn0;
+  unsigned c_lo = (unsigned)(lq * (int)ldc + 2 * lr) * 8u;
   auto cpair = [&](int i, int p, int v) {
-    return reinterpret_cast<double2 *>(Cw + (long)(i * 16 + lq + 4 * v) * ldc + 32 * p + 2 * lr);
+    return byte_off(reinterpret_cast<double2 *>(Cw + (long)(i * 16 + 4 * v) * ldc + 32 * p), c_lo);
   };
+  // The first operand stages are requested before the C tile: behind the
+  // tile's 32 C loads per lane they would reach LDS later than the C values
+  // reach the accumulators (79.5 -> 82 % of the FP64 peak at 32512^2 x 256).
+  static_assert(NST >= 2 && NST <= 4, "stages");
+#pragma unroll
+  for (int st = 0; st < NST - 1; ++st)
+    if (st < nk) issue(st);
   double4_t acc[4][4];
   if (CINIT && beta != 0.0) {
 #pragma unroll
@@ -381,7 +419,7 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
       for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const double2 c2 = *cpair(i, p, v);
+          const double2 c2 = ld_nt(cpair(i, p, v));
           acc[i][2 * p][v] = beta * c2.x;
           acc[i][2 * p + 1][v] = beta * c2.y;
         }
@@ -393,14 +431,11 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
   }
   const bool neg = CINIT && alpha < 0.0;
 
-  static_assert(NST >= 2 && NST <= 4, "stages");
-#pragma unroll
-  for (int st = 0; st < NST - 1; ++st)
-    if (st < nk) issue(st);
   for (int kc = 0; kc < nk; ++kc) {
     // This thread's loads of stage kc have landed once at most the 4 loads
     // of each later stage already issued (min(NST - 2, nk - 1 - kc) of
-    // them: no stage past the end is loaded) are outstanding; after the
+    // them: no stage past the end is loaded) are outstanding (at kc = 0 the
+    // C loads, issued after them, are waited for too); after the
     // barrier every thread's have, and every wave is done reading stage
     // kc-1, whose buffer the next issue overwrites.  A bare s_barrier:
     // __syncthreads() would also drain the direct-to-LDS loads of the later
@@ -440,14 +475,16 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
     }
   }
   wait_vm<0>();  // no direct-to-LDS write may be in flight past this tile
+  // a new value as far as the compiler knows: the C addresses are formed
+  // again here instead of being held in registers through the K loop
+  __asm__ volatile("" : "+v"(c_lo));
   if (CINIT) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          *cpair(i, p, v) = double2{acc[i][2 * p][v], acc[i][2 * p + 1][v]};
+        for (int v = 0; v < 4; ++v) st_nt(cpair(i, p, v), acc[i][2 * p][v], acc[i][2 * p + 1][v]);
   } else {
     // C = alpha acc + beta C, a row block's C pairs requested before the
     // previous one is combined and stored
@@ -456,7 +493,7 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) d[p][v] = beta != 0.0 ? *cpair(i, p, v) : double2{0.0, 0.0};
+        for (int v = 0; v < 4; ++v) d[p][v] = beta != 0.0 ? ld_nt(cpair(i, p, v)) : double2{0.0, 0.0};
     };
     fetch(0, cur);
 #pragma unroll
@@ -466,8 +503,8 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
       for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-          *cpair(i, p, v) = double2{alpha * acc[i][2 * p][v] + beta * cur[p][v].x,
-                                    alpha * acc[i][2 * p + 1][v] + beta * cur[p][v].y};
+          st_nt(cpair(i, p, v), alpha * acc[i][2 * p][v] + beta * cur[p][v].x,
+                alpha * acc[i][2 * p + 1][v] + beta * cur[p][v].y);
       if (i + 1 < 4) {
 #pragma unroll
         for (int p = 0; p < 2; ++p)
@@ -480,9 +517,11 @@ CLU_DEV void lds_tile(char *smem, int m0, int n0, int K, double alpha, const dou
 
 // Aligned full-tile NN kernel, one tile per workgroup.  ROLE only names the
 // launch site in profiles: 1 = the factorization's trailing update, 2 = its
-// look-ahead block column, 0 = every other caller.
+// look-ahead block column, 0 = every other caller.  The general-alpha form
+// (C read after the products) keeps two workgroups per CU: its epilogue
+// holds a row block of C beside the accumulators.
 template <int NST, int ROLE, bool CINIT>
-__global__ __launch_bounds__(256, 2) void dgemm_nn_lds_kernel(
+__global__ __launch_bounds__(256, CINIT ? 3 : 2) void dgemm_nn_lds_kernel(
     int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
     const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
     int tiles_m, int tiles_n) {
@@ -531,7 +570,7 @@ static int dispatch(int M, int N, int K, T alpha, const T *A, long lda, const T 
     const int tiles_m = M / 128, tiles_n = N / 128;
     const bool cinit = alpha == (T)1 || alpha == (T)-1;
 #define CLU_LDS_LAUNCH(R, CI)                                                                   \
-  hipLaunchKernelGGL((dgemm_nn_lds_kernel<4, R, CI>), dim3(tiles_m * tiles_n), dim3(256), 0, s, \
+  hipLaunchKernelGGL((dgemm_nn_lds_kernel<3, R, CI>), dim3(tiles_m * tiles_n), dim3(256), 0, s, \
                      M, N, K, (double)alpha, (const double *)A, lda, (const double *)B, ldb,    \
                      (double)beta, (double *)C, ldc, tiles_m, tiles_n)
     if (trailing) CLU_LDS_LAUNCH(1, true);

@@ -98,6 +98,32 @@ def test_dgemm_dev(dev, M, N, K, ta, tb):
     assert np.all(np.abs(got - ref) <= _gemm_tol(opA, opB, C, alpha, beta))
 
 
+@pytest.mark.parametrize("alpha,beta", [(0.5, -2.0), (1.0, 0.0), (-1.0, 0.25), (2.0, 0.0),
+                                        (1.0, 1.0)])
+def test_dgemm_dev_lds_path_alpha_beta(dev, alpha, beta):
+    """The direct-to-LDS kernel (2048 x 4096 x 264: 512 full tiles) with
+    alpha = +-1 (C in the accumulators, scaled by beta) and other alpha (C
+    read after the products), beta = 0 included; C holds NaN where beta = 0
+    must not read it."""
+    from charmlu_amd import _lib
+    M, N, K = 2048, 4096, 264
+    rng = np.random.default_rng(int(alpha * 8 + beta * 16) + 99)
+    A = rng.uniform(-0.5, 0.5, (M, K))
+    B = rng.uniform(-0.5, 0.5, (K, N))
+    C = rng.uniform(-0.5, 0.5, (M, N))
+    if beta == 0.0:
+        C[::7, ::5] = np.nan
+    ref = alpha * A @ B + (beta * C if beta != 0.0 else 0.0)
+    dA, dB, dC = _t(A), _t(B), _t(C)
+    rc = _lib.load().charmlu_dgemm_dev(_lib.NO_TRANS, _lib.NO_TRANS, M, N, K, alpha, _p(dA), K,
+                                       _p(dB), N, beta, _p(dC), N, None)
+    assert rc == 0
+    _sync()
+    got = dC.cpu().numpy()
+    Cz = np.nan_to_num(C) if beta == 0.0 else C
+    assert np.all(np.abs(got - ref) <= _gemm_tol(A, B, Cz, alpha, beta))
+
+
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (1000, 16, 16), (4097, 32, 33), (777, 64, 40),
                                    (20, 300, 64), (300, 200, 256), (1030, 1100, 100)])
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
