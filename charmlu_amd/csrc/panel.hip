@@ -305,11 +305,15 @@ __global__ __launch_bounds__(NT) void panel_base_kernel(
       // pivots go to global memory after the loop: a store here would make
       // the next barrier wait for its acknowledgement from memory
       // ---- row exchange inside the strip, then the elimination ----
-      if constexpr (NT <= 256) {
+      if constexpr (NT <= 256 || (R <= 4 && R * SW * sizeof(T) <= 512)) {
         // the pivot row in registers, read from LDS once for the new row c
         // and the update (the wave holding row c then needs no LDS read of
         // its own: 1000 of the 3800 cycles of a column at m = 1024 were
-        // that swap); 512-thread workgroups lack the registers for it.
+        // that swap).  512-thread workgroups (two waves per SIMD, at most
+        // 256 VGPRs) have the registers for it with 4 rows per thread of at
+        // most 512 bytes: FP32 strips of 32 columns, 7.15 -> 6.53 us per
+        // column at 65536 rows; FP64 strips of 16, 7.50 -> 7.26 (with 8 or
+        // 16 rows per thread the row would spill).
         // Two loops for the swap: as one if/else the compiler turns the two
         // copies into one through a pointer that selects the source, which
         // puts the register row in scratch memory.  (A compile-time index
