@@ -388,6 +388,36 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
     return out
 
 
+def small_case(clu, torch, n, nb, steps=5):
+    """The reference's own CPU-runnable case on the GPU: FP64 LU + solve of
+    n = 4096, block 128 (the same generators), one untimed step, then
+    ``steps`` timed ones; effective TFLOP/s over factor + solve, beside the
+    CPU figure of the same case in cpu_baseline.c1."""
+    A0 = clu.gen_matrix(n)
+    b0 = clu.gen_rhs(n)
+    A = torch.empty_like(A0)
+    x = torch.empty_like(b0)
+
+    def step():
+        A.copy_(A0)
+        x.copy_(b0)
+        ipiv, info = clu.getrf(A, nb)
+        clu.getrs(A, ipiv, x)
+        return info
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info = step()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    res = clu.residual(A0, x, b0)
+    return {"n": n, "block": nb, "steps": steps, "ms": round(sec * 1e3, 3),
+            "tflops": round(2.0 / 3.0 * n ** 3 / sec / 1e12, 3),
+            "residual": res["residual"], "info": int(info.item())}
+
+
 def c5_bench(clu, lib, _lib, torch, nb):
     """C5: the FP8 form at n = 131072 (FP64 A 137 GB + FP32 factors 69 GB in
     HBM), unpivoted and pivoted, one step each after no warmup, then one
@@ -586,6 +616,7 @@ def main():
         },
         "row_interchange_gbs": round(swap_bytes / (sw_ms * 1e-3) / 1e9, 1) if sw_ms > 0 else None,
     }
+    out["c1_gpu"] = small_case(clu, torch, 4096, 128)
     if not args.no_mixed:
         del A, A0
         torch.cuda.empty_cache()

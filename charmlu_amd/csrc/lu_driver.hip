@@ -1179,10 +1179,13 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
     const int m = n - (t + 1) * nb;  // rows of panel t+1
     double p_ms = (t + 1 < (int)ws.hist64.ms.size()) ? ws.hist64.ms[t + 1] : -1.0;
     if (p_ms <= 0.0) p_ms = nb / 256.0 * (1.3 + 2.0 * (m < 32768 ? m / 32768.0 : 1.0));
-    // FP64 matrix-core rate of the update stream's CUs at 80 % of peak
-    // (optimistic, so the first part errs long), plus 10 % margin
+    // FP64 matrix-core rate of the update stream's CUs at 80 % of peak,
+    // plus 5 % margin (with 10 % the first part outlasted the panel more
+    // often and overlapped the second for longer: per launch 0.73 against
+    // 0.75 of peak at n = 32768, the LU within 0.2 %; with 0 %, 0.74 and
+    // 0.9 % slower; with 20 %, 0.70 and 0.1 % faster)
     const double rate = 0.8 * 78.6e9 * (ws.ncu - ws.panel_cus) / ws.ncu;  // flop per ms
-    const double rows = 1.1 * p_ms * rate / (2.0 * N * nb);
+    const double rows = 1.05 * p_ms * rate / (2.0 * N * nb);
     if (rows >= M) return M;
     const int Ma = ((int)rows + 127) / 128 * 128;
     return Ma > M ? M : Ma;
