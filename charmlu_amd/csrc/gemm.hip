@@ -531,6 +531,161 @@ __global__ __launch_bounds__(256, CINIT ? 3 : 2) void dgemm_nn_lds_kernel(
   lds_tile<NST, CINIT>(smem, tm * 128, tn * 128, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
+// ---------------------------------------------------------------------------
+// The FP32 form of the direct-to-LDS NN kernel (the mixed LU's U row, its
+// unpivoted L21 and the larger updates inside its panels): a 128 x 128 C
+// tile, 4 waves of 64 x 64, K in stages of 16 (64-byte A rows as in the FP64
+// kernel, with the same XOR swizzle of the 16-byte chunks), three stages in
+// flight, three workgroups per CU, v_mfma_f32_16x16x4f32.  Matrix-core step
+// s of a stage takes k = 4 q + s on lane group q, so a lane's A values of
+// the four steps are one 16-byte chunk.  The 16 columns of matrix-core
+// block j are columns 4 r + j (r the lane in its group of 16) of the wave's
+// 64, so one 16-byte read gives a lane its B values of the four blocks at
+// one k, and its C values of one row are a 16-byte quad.
+// FP32 operand layout (measured on gfx950): A[i][k] from lane i + 16 k,
+// B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i / 4), element i % 4.
+// ---------------------------------------------------------------------------
+CLU_DEV void load16_to_lds(const float *g, void *lds_base) {
+  __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, (lds_ptr_t)lds_base, 16, 0, 0);
+}
+CLU_DEV float4_t ld_nt4(const float4_t *p) { return __builtin_nontemporal_load(p); }
+CLU_DEV void st_nt4(float4_t *p, float4_t v) { __builtin_nontemporal_store(v, p); }
+
+template <int NST, bool CINIT>
+CLU_DEV void lds_tile_f32(char *smem, int m0, int n0, int K, float alpha,
+                          const float *__restrict__ A, long lda, const float *__restrict__ B,
+                          long ldb, float beta, float *__restrict__ C, long ldc) {
+  constexpr int BM = 128, BN = 128, BK = 16;
+  constexpr int STAGE = (BM * BK + BK * BN) * 4;  // bytes
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
+  const int lr = lane & 15, lq = lane >> 4;
+
+  // this thread's chunks per stage: A slot c = 64 wave + 256 r + lane is
+  // row c / 4 = 16 wave + 64 r + lane / 4, chunk (c & 3) ^ lds_swz(row); B
+  // slot c is k row c / 32 = 2 wave + 8 r + lane / 32, chunk c & 31 (B rows
+  // of 128 floats, 512 bytes, unswizzled: a fragment read takes 16
+  // consecutive chunks of one row per lane group)
+  const float *ga[2], *gb[2];
+  int la_off[2], lb_off[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    ga[r] = A + (long)(m0 + wave * 16 + 64 * r) * lda;
+    gb[r] = B + (long)(2 * wave + 8 * r) * ldb + n0;
+    la_off[r] = (wave * 64 + 256 * r) * 16;
+    lb_off[r] = BM * BK * 4 + (wave * 64 + 256 * r) * 16;
+  }
+  const unsigned a_lo =
+      (unsigned)(((lane >> 2) * (int)lda + 4 * ((lane & 3) ^ lds_swz(lane >> 2))) * 4);
+  const unsigned b_lo = (unsigned)(((lane >> 5) * (int)ldb + 4 * (lane & 31)) * 4);
+  const int nk = K / BK;
+  auto issue = [&](int kc) {
+    const int kk = kc * BK;
+    char *st = smem + (kc % NST) * STAGE;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) load16_to_lds(byte_off(ga[r] + kk, a_lo), st + la_off[r]);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) load16_to_lds(byte_off(gb[r] + (long)kk * ldb, b_lo), st + lb_off[r]);
+  };
+  int aoff[4], boff[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = wm0 + 16 * i + lr;
+    aoff[i] = m * 64 + ((lq ^ lds_swz(m)) << 4);
+  }
+#pragma unroll
+  for (int s4 = 0; s4 < 4; ++s4) boff[s4] = BM * BK * 4 + (4 * lq + s4) * 512 + (wn0 + 4 * lr) * 4;
+
+  // accumulator element (i, j, e) is C row 16 i + 4 lq + e, column 4 lr + j
+  // of this wave's 64 x 64 block
+  float *Cw = C + (long)(m0 + wm0) * ldc + n0 + wn0;
+  unsigned c_lo = (unsigned)(4 * lq * (int)ldc + 4 * lr) * 4u;
+  auto cquad = [&](int i, int e) {
+    return byte_off(reinterpret_cast<float4_t *>(Cw + (long)(i * 16 + e) * ldc), c_lo);
+  };
+  static_assert(NST >= 2 && NST <= 4, "stages");
+#pragma unroll
+  for (int st = 0; st < NST - 1; ++st)
+    if (st < nk) issue(st);
+  float4_t acc[4][4];
+  if (CINIT && beta != 0.f) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float4_t c4 = ld_nt4(cquad(i, e));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j][e] = beta * c4[j];
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  }
+  const bool neg = CINIT && alpha < 0.f;
+
+  for (int kc = 0; kc < nk; ++kc) {
+    const int ahead = nk - 1 - kc;
+    if (NST >= 4 && ahead >= 2) wait_vm<4 * (NST >= 4 ? 2 : 0)>();
+    else if (NST >= 3 && ahead >= 1) wait_vm<4>();
+    else wait_vm<0>();
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    if (kc + NST - 1 < nk) issue(kc + NST - 1);
+    const char *st = smem + (kc % NST) * STAGE;
+    float4_t a4[4], b4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a4[i] = *reinterpret_cast<const float4_t *>(st + aoff[i]);
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) b4[s4] = *reinterpret_cast<const float4_t *>(st + boff[s4]);
+    if (neg) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a4[i] = -a4[i];
+    }
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][s4], b4[s4][j], acc[i][j], 0, 0, 0);
+  }
+  wait_vm<0>();
+  __asm__ volatile("" : "+v"(c_lo));
+  if (CINIT) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        st_nt4(cquad(i, e), (float4_t){acc[i][0][e], acc[i][1][e], acc[i][2][e], acc[i][3][e]});
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float4_t old = beta != 0.f ? ld_nt4(cquad(i, e)) : (float4_t){0.f, 0.f, 0.f, 0.f};
+        float4_t v;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = alpha * acc[i][j][e] + beta * old[j];
+        st_nt4(cquad(i, e), v);
+      }
+  }
+}
+
+template <int NST, bool CINIT>
+__global__ __launch_bounds__(256, 3) void sgemm_nn_lds_kernel(
+    int M, int N, int K, float alpha, const float *__restrict__ A, long lda,
+    const float *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
+    int tiles_m, int tiles_n) {
+  __shared__ __attribute__((aligned(1024))) char smem[NST * (128 * 16 + 16 * 128) * 4];
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  lds_tile_f32<NST, CINIT>(smem, tm * 128, tn * 128, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
 template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB>
 static int launch(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
                   T beta, T *C, long ldc, hipStream_t s, int ny = 1, int nz = 1,
@@ -558,9 +713,31 @@ static bool lds_path_ok(int M, int N, int K, const double *A, long lda, const do
   return lds_aligned(M, N, K, A, lda, B, ldb, C, ldc) && (long)(M / 128) * (N / 128) >= 256;
 }
 
+static bool lds_path_ok_f32(int M, int N, int K, const float *A, long lda, const float *B,
+                            long ldb, const float *C, long ldc) {
+  return M % 128 == 0 && N % 128 == 0 && K % 16 == 0 && K > 0 && lda % 4 == 0 &&
+         ldb % 4 == 0 && ldc % 4 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0 &&
+         ((uintptr_t)C & 15) == 0 && (long)(M / 128) * (N / 128) >= 256;
+}
+
 template <typename T, bool TA, bool TB>
 static int dispatch(int M, int N, int K, T alpha, const T *A, long lda, const T *B, long ldb,
                     T beta, T *C, long ldc, hipStream_t s, int role) {
+  if (sizeof(T) == 4 && !TA && !TB &&
+      lds_path_ok_f32(M, N, K, (const float *)A, lda, (const float *)B, ldb, (const float *)C,
+                      ldc)) {
+    const int tiles_m = M / 128, tiles_n = N / 128;
+    const bool cinit = alpha == (T)1 || alpha == (T)-1;
+#define CLU_SLDS_LAUNCH(CI)                                                                     \
+  hipLaunchKernelGGL((sgemm_nn_lds_kernel<3, CI>), dim3(tiles_m * tiles_n), dim3(256), 0, s,    \
+                     M, N, K, (float)alpha, (const float *)A, lda, (const float *)B, ldb,       \
+                     (float)beta, (float *)C, ldc, tiles_m, tiles_n)
+    if (cinit) CLU_SLDS_LAUNCH(true);
+    else CLU_SLDS_LAUNCH(false);
+#undef CLU_SLDS_LAUNCH
+    CLU_LAUNCHED("sgemm_nn_lds_kernel");
+    return CLU_OK;
+  }
   const bool trailing = role == 1 && (alpha == (T)1 || alpha == (T)-1);
   if (sizeof(T) == 8 && !TA && !TB &&
       (trailing ? lds_aligned(M, N, K, (const double *)A, lda, (const double *)B, ldb,

@@ -125,7 +125,10 @@ def test_dgemm_dev_lds_path_alpha_beta(dev, alpha, beta):
 
 
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (1000, 16, 16), (4097, 32, 33), (777, 64, 40),
-                                   (20, 300, 64), (300, 200, 256), (1030, 1100, 100)])
+                                   (20, 300, 64), (300, 200, 256), (1030, 1100, 100),
+                                   # >= 256 full 128x128 tiles, K a multiple of 16: the
+                                   # direct-to-LDS FP32 path (NN only)
+                                   (2048, 2048, 512), (1024, 4096, 1024), (2048, 4096, 272)])
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
 def test_sgemm_dev(dev, M, N, K, ta, tb):
     """charmlu_sgemm_dev on every tile shape it dispatches to (N <= 32,
@@ -149,6 +152,33 @@ def test_sgemm_dev(dev, M, N, K, ta, tb):
     got = dC.cpu().numpy().astype(np.float64)
     u32 = float(np.finfo(np.float32).eps)
     tol = 8 * max(K, 1) * u32 * (np.abs(opA) @ np.abs(opB) + np.abs(C)) + 1e-30
+    assert np.all(np.abs(got - ref) <= tol)
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.5, -2.0), (1.0, 0.0), (-1.0, 0.25), (2.0, 0.0)])
+def test_sgemm_dev_lds_path_alpha_beta(dev, alpha, beta):
+    """The FP32 direct-to-LDS kernel (2048 x 4096 x 528) with alpha = +-1 (C
+    in the accumulators) and other alpha (C read after the products), beta
+    = 0 included with NaN in C; bound 8 K u32 (|alpha| |A||B| + |beta||C|)."""
+    from charmlu_amd import _lib
+    M, N, K = 2048, 4096, 528
+    rng = np.random.default_rng(int(alpha * 8 + beta * 16) + 107)
+    A = rng.uniform(-0.5, 0.5, (M, K)).astype(np.float32)
+    B = rng.uniform(-0.5, 0.5, (K, N)).astype(np.float32)
+    C = rng.uniform(-0.5, 0.5, (M, N)).astype(np.float32)
+    if beta == 0.0:
+        C[::7, ::5] = np.nan
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    C64 = np.nan_to_num(C.astype(np.float64)) if beta == 0.0 else C.astype(np.float64)
+    ref = alpha * A64 @ B64 + beta * C64
+    dA, dB, dC = _t(A), _t(B), _t(C)
+    rc = _lib.load().charmlu_sgemm_dev(_lib.NO_TRANS, _lib.NO_TRANS, M, N, K, alpha, _p(dA), K,
+                                       _p(dB), N, beta, _p(dC), N, None)
+    assert rc == 0
+    _sync()
+    got = dC.cpu().numpy().astype(np.float64)
+    u32 = float(np.finfo(np.float32).eps)
+    tol = 8 * K * u32 * (abs(alpha) * np.abs(A64) @ np.abs(B64) + abs(beta) * np.abs(C64)) + 1e-30
     assert np.all(np.abs(got - ref) <= tol)
 
 
