@@ -535,8 +535,8 @@ __global__ __launch_bounds__(256, CINIT ? 3 : 2) void dgemm_nn_lds_kernel(
 // The FP32 form of the direct-to-LDS NN kernel (the mixed LU's U row, its
 // unpivoted L21 and the larger updates inside its panels): a 128 x 128 C
 // tile, 4 waves of 64 x 64, K in stages of 16 (64-byte A rows as in the FP64
-// kernel, with the same XOR swizzle of the 16-byte chunks), three stages in
-// flight, three workgroups per CU, v_mfma_f32_16x16x4f32.  Matrix-core step
+// kernel, with the same XOR swizzle of the 16-byte chunks), three stages and
+// three workgroups per CU or four and two, v_mfma_f32_16x16x4f32.  Matrix-core step
 // s of a stage takes k = 4 q + s on lane group q, so a lane's A values of
 // the four steps are one 16-byte chunk.  The 16 columns of matrix-core
 // block j are columns 4 r + j (r the lane in its group of 16) of the wave's
@@ -675,8 +675,8 @@ CLU_DEV void lds_tile_f32(char *smem, int m0, int n0, int K, float alpha,
   }
 }
 
-template <int NST, bool CINIT>
-__global__ __launch_bounds__(256, 3) void sgemm_nn_lds_kernel(
+template <int NST, bool CINIT, int OCC>
+__global__ __launch_bounds__(256, OCC) void sgemm_nn_lds_kernel(
     int M, int N, int K, float alpha, const float *__restrict__ A, long lda,
     const float *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
     int tiles_m, int tiles_n) {
@@ -728,12 +728,21 @@ static int dispatch(int M, int N, int K, T alpha, const T *A, long lda, const T 
                       ldc)) {
     const int tiles_m = M / 128, tiles_n = N / 128;
     const bool cinit = alpha == (T)1 || alpha == (T)-1;
-#define CLU_SLDS_LAUNCH(CI)                                                                     \
-  hipLaunchKernelGGL((sgemm_nn_lds_kernel<3, CI>), dim3(tiles_m * tiles_n), dim3(256), 0, s,    \
-                     M, N, K, (float)alpha, (const float *)A, lda, (const float *)B, ldb,       \
+    // K >= 512: four stages, two workgroups per CU (1024 x 64512 x 1024:
+    // 131.7 against 127.4 TFLOP/s as C = A B; 8192^2 x 1024: 130.5 against
+    // 125.7); shorter K: three stages, three per CU (32768^2 x 256: 123.7
+    // against 120.6)
+#define CLU_SLDS_LAUNCH(NS, CI, OC)                                                            \
+  hipLaunchKernelGGL((sgemm_nn_lds_kernel<NS, CI, OC>), dim3(tiles_m * tiles_n), dim3(256), 0, \
+                     s, M, N, K, (float)alpha, (const float *)A, lda, (const float *)B, ldb,   \
                      (float)beta, (float *)C, ldc, tiles_m, tiles_n)
-    if (cinit) CLU_SLDS_LAUNCH(true);
-    else CLU_SLDS_LAUNCH(false);
+    if (K >= 512) {
+      if (cinit) CLU_SLDS_LAUNCH(4, true, 2);
+      else CLU_SLDS_LAUNCH(4, false, 2);
+    } else {
+      if (cinit) CLU_SLDS_LAUNCH(3, true, 3);
+      else CLU_SLDS_LAUNCH(3, false, 3);
+    }
 #undef CLU_SLDS_LAUNCH
     CLU_LAUNCHED("sgemm_nn_lds_kernel");
     return CLU_OK;

@@ -155,13 +155,15 @@ def test_sgemm_dev(dev, M, N, K, ta, tb):
     assert np.all(np.abs(got - ref) <= tol)
 
 
-@pytest.mark.parametrize("alpha,beta", [(0.5, -2.0), (1.0, 0.0), (-1.0, 0.25), (2.0, 0.0)])
-def test_sgemm_dev_lds_path_alpha_beta(dev, alpha, beta):
-    """The FP32 direct-to-LDS kernel (2048 x 4096 x 528) with alpha = +-1 (C
-    in the accumulators) and other alpha (C read after the products), beta
-    = 0 included with NaN in C; bound 8 K u32 (|alpha| |A||B| + |beta||C|)."""
+@pytest.mark.parametrize("alpha,beta,K", [(0.5, -2.0, 528), (1.0, 0.0, 528), (-1.0, 0.25, 528),
+                                          (2.0, 0.0, 528), (0.5, -2.0, 272), (1.0, 0.0, 272)])
+def test_sgemm_dev_lds_path_alpha_beta(dev, alpha, beta, K):
+    """The FP32 direct-to-LDS kernel (2048 x 4096 x K: four stages for K >=
+    512, three below) with alpha = +-1 (C in the accumulators) and other
+    alpha (C read after the products), beta = 0 included with NaN in C;
+    bound 8 K u32 (|alpha| |A||B| + |beta||C|)."""
     from charmlu_amd import _lib
-    M, N, K = 2048, 4096, 528
+    M, N = 2048, 4096
     rng = np.random.default_rng(int(alpha * 8 + beta * 16) + 107)
     A = rng.uniform(-0.5, 0.5, (M, K)).astype(np.float32)
     B = rng.uniform(-0.5, 0.5, (K, N)).astype(np.float32)
