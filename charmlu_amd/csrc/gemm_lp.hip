@@ -20,6 +20,16 @@
 // MFMA 16x16x32 (BF16/FP16) layout, per lane l: A[i][k] in lane i + 16*(k/8),
 // element k%8; B[k][j] in lane j + 16*(k/8), element k%8; D[i][j] in lane
 // j + 16*(i/4), element i%4 (measured on gfx950).
+//
+// fgemm_nt_kernel (FP8 e4m3): the same staging, with gfx950's block-scaled
+// v_mfma_scale_f32_32x32x64_f8f6f4, whose rate (4.9 PFLOP/s measured in a
+// register-only loop) is twice that of v_mfma_f32_16x16x32_fp8_fp8 (2.5).
+// Lane l holds 32 consecutive k of row l % 32 (k block l / 32), one 64-byte
+// stage row is one instruction's K; D[i][j] in lane j + 32 * ((i / 4) % 2),
+// element 4 * (i / 8) + i % 4 (checked on gfx950 against a brute-force
+// product, with scales).  The per-tile power-of-two scales of the operands
+// are the instruction's E8M0 block scales (2^(s - 127)), so products come
+// out unscaled and C is used as it is.
 #include <hip/hip_bf16.h>
 
 #include "common.h"
@@ -47,17 +57,8 @@ template <> struct Lp<_Float16> {
   }
   static CLU_DEV _Float16 from(float x) { return (_Float16)x; }
 };
-// FP8 e4m3 (OCP, largest finite 448): a lane's 16-byte LDS read holds 16 k
-// values, consumed by two v_mfma_f32_16x16x32_fp8_fp8 (8 bytes each; lane
-// layout as for the 16-bit types, measured on gfx950)
-typedef long long2_t __attribute__((ext_vector_type(2)));
-template <> struct Lp<uint8_t> {
-  typedef long2_t vec_t;
-  static CLU_DEV float4_t mfma(vec_t a, vec_t b, float4_t c) {
-    c = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a.x, b.x, c, 0, 0, 0);
-    return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a.y, b.y, c, 0, 0, 0);
-  }
-};
+typedef int int8v_t __attribute__((ext_vector_type(8)));
+typedef int int4v_t __attribute__((ext_vector_type(4)));
 
 template <int N>
 CLU_DEV void wait_vm_lp() {
@@ -69,12 +70,9 @@ template <typename T16, int BM, int BN, int NST, int OCC, bool CINIT, bool VEC>
 __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kernel(
     int M, int N, int K, float alpha, const T16 *__restrict__ A, long lda,
     const T16 *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
-    int tiles_m, int tiles_n, const int *__restrict__ escA, const int *__restrict__ escB) {
+    int tiles_m, int tiles_n) {
   // BM x BN C tile, one wave per 64 x 64 block of it (BM / 64 x BN / 64
-  // waves).  A stage holds 64 bytes of K per row: 32 16-bit or 64 FP8 values.
-  // FP8 operands carry power-of-two scales per 128-row tile (escA: A * 2^e
-  // was quantized; escB likewise per 128 rows of B, i.e. columns of C), so
-  // a wave's products are 2^(escA[..] + escB[..]) too large.
+  // waves).  A stage holds 64 bytes of K per row: 32 16-bit values.
   constexpr int NT = (BM / 64) * (BN / 64) * 64;  // threads
   constexpr int EPC = 16 / sizeof(T16);  // elements per 16-byte chunk
   constexpr int BK = 4 * EPC;
@@ -108,8 +106,6 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
   const int wm0 = (wave / WGN) * 64, wn0 = (wave % WGN) * 64;
   const int lr = lane & 15, lq = lane >> 4;
   const bool full = m0 + BM <= M && n0 + BN <= N;
-  // products of this wave's block carry 2^escale (FP8 only)
-  const int escale = escA ? escA[(m0 + wm0) >> 7] + escB[(n0 + wn0) >> 7] : 0;
 
   // The MFMA's first operand is the B tile (16 C columns), its second the A
   // tile (16 C rows), so D[r][c] of block (i, j) is C[row][col] with
@@ -133,10 +129,6 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
 #pragma unroll
           for (int r = 0; r < 4; ++r)
             acc[i][j][r] = (row < M && col + r < N) ? C[(long)row * ldc + col + r] : 0.f;
-        }
-        if (escA) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc[i][j][r] = ldexpf(acc[i][j][r], escale);
         }
       }
   } else {
@@ -217,14 +209,6 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
   wait_vm_lp<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
 
   // epilogue: CINIT stores the accumulators; otherwise C = alpha acc + beta C
-  if (escA) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[i][j][r] = ldexpf(acc[i][j][r], -escale);
-  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -253,12 +237,11 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
 
 template <typename T16, int BMv, int BNv, int NS, int OC, bool CI, bool V>
 static void hgemm_launch(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B,
-                         long ldb, float beta, float *C, long ldc, hipStream_t s,
-                         const int *escA = nullptr, const int *escB = nullptr) {
+                         long ldb, float beta, float *C, long ldc, hipStream_t s) {
   constexpr int NT = (BMv / 64) * (BNv / 64) * 64;
   const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
   hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, BNv, NS, OC, CI, V>), dim3(tm * tn), dim3(NT), 0,
-                     s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB);
+                     s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
 }
 // 128 x 128 tiles, 2 stages, 4 workgroups per CU (measured at 65280^2 x 256
 // BF16: 212 TFLOP/s against 188 with 4 stages and 2 per CU, 198 with 64 x
@@ -266,48 +249,36 @@ static void hgemm_launch(int M, int N, int K, float alpha, const T16 *A, long ld
 template <typename T16>
 static void hgemm_launch2(bool cinit, bool vec, int M, int N, int K, float alpha, const T16 *A,
                           long lda, const T16 *B, long ldb, float beta, float *C, long ldc,
-                          hipStream_t s, const int *escA = nullptr, const int *escB = nullptr) {
-  // Smaller grids keep 128 x 128 tiles: 16-bit operands with K >= 1024 on
-  // 3 stages, 3 workgroups per CU (546 TFLOP/s at 64512^2 x 1024 BF16
-  // against 526 with 2 stages and 4 per CU, 522 with 4 and 2, before the
-  // bank-conflict-free swizzle); FP8 on 2 stages and 4 per CU (its update at
-  // n = 131072 took 3.5 s with 3 and 3 against 2.7 s).
+                          hipStream_t s) {
+  // Smaller grids keep 128 x 128 tiles: K >= 1024 on 3 stages, 3
+  // workgroups per CU (546 TFLOP/s at 64512^2 x 1024 BF16 against 526 with
+  // 2 stages and 4 per CU, 522 with 4 and 2, before the bank-conflict-free
+  // swizzle).
   // Large accumulate-into-C updates (the mixed LU's trailing update at
   // block 1024): bigger tiles read fewer operand bytes per flop.  At
   // 64512^2 x 1024 the BF16 update ran at 615 TFLOP/s standalone with
   // 256 x 256 tiles (4 stages, one workgroup of 16 waves per CU) against
   // 572 with 128 x 128 (3 stages, 3 per CU) and 607 with 128 x 256, and at
-  // 507 against 433 inside the LU at n = 65536; FP8 at 749 with 128 x 256
-  // tiles (3 stages, 2 per CU) against 695 with 128 x 128 (2 stages, 4 per
-  // CU) and 668 with 256 x 256 (639 against 562 inside the LU).  Only while
-  // the grid keeps every CU busy with them (at least 2^25 C elements).
+  // 507 against 433 inside the LU at n = 65536.  Only while the grid keeps
+  // every CU busy with them (at least 2^25 C elements).
   const bool large = (long)M * N >= (1L << 25);
   if (cinit && vec && K >= 1024 && large) {
-    if constexpr (sizeof(T16) == 2)
-      hgemm_launch<T16, 256, 256, 4, 1, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                                                    s, escA, escB);
-    else
-      hgemm_launch<T16, 128, 256, 3, 2, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                                                    s, escA, escB);
+    hgemm_launch<T16, 256, 256, 4, 1, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
     return;
   }
-  if (cinit && vec && K >= 1024 && sizeof(T16) == 2) {
-    hgemm_launch<T16, 128, 128, 3, 3, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                             escA, escB);
+  if (cinit && vec && K >= 1024) {
+    hgemm_launch<T16, 128, 128, 3, 3, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
     return;
   }
   if (cinit && vec)
-    hgemm_launch<T16, 128, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                             escA, escB);
+    hgemm_launch<T16, 128, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   else if (cinit)
-    hgemm_launch<T16, 128, 128, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                              escA, escB);
+    hgemm_launch<T16, 128, 128, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   else if (vec)
-    hgemm_launch<T16, 128, 128, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                              escA, escB);
+    hgemm_launch<T16, 128, 128, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
   else
-    hgemm_launch<T16, 128, 128, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                               escA, escB);
+    hgemm_launch<T16, 128, 128, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                    s);
 }
 
 template <typename T16>
@@ -324,8 +295,196 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
   return CLU_OK;
 }
 
-// FP8 operands with per-tile power-of-two scales (128 x 128 tiles only, so a
-// C tile sees one scale of A and one of B)
+// C = alpha A B^T + beta C (or C += A B^T with CINIT) for FP8 e4m3 operands
+// quantized with power-of-two scales per 128-row tile (escA[g]: rows 128 g ..
+// of A were multiplied by 2^escA[g] before rounding; escB likewise per 128
+// rows of B, i.e. per 128 columns of C).  BM x BN C tile, one wave per
+// 64 x 64 block (2 x 2 blocks of 32 x 32), K in stages of 64 bytes per row
+// staged as in hgemm_nt_kernel.
+template <int BM, int BN, int NST, int OCC, bool CINIT, bool VEC>
+__global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kernel(
+    int M, int N, int K, float alpha, const uint8_t *__restrict__ A, long lda,
+    const uint8_t *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
+    int tiles_m, int tiles_n, const int *__restrict__ escA, const int *__restrict__ escB) {
+  constexpr int NT = (BM / 64) * (BN / 64) * 64;
+  constexpr int BK = 64;  // K per stage (bytes per row)
+  constexpr int WGN = BN / 64;
+  constexpr int HALF = BM * 64;
+  constexpr int STAGE = (BM + BN) * 64;
+  constexpr int CA = BM * 4 / NT, CB = BN * 4 / NT;
+  static_assert(BM % 64 == 0 && BN % 64 == 0 && CA >= 1 && CB >= 1, "tile");
+  __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
+
+  int tm, tn;  // XCD-aware tile order, as in hgemm_nt_kernel
+  {
+    const int bid = blockIdx.x, nblk = gridDim.x, per = nblk / 8;
+    int lid = bid;
+    if (bid < per * 8) lid = (bid % 8) * per + bid / 8;
+    constexpr int GM = 512 / BM;
+    const int group_size = GM * tiles_n, group = lid / group_size, first_m = group * GM;
+    const int gm = min(tiles_m - first_m, GM), in_group = lid - group * group_size;
+    tm = first_m + in_group % gm;
+    tn = in_group / gm;
+  }
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm0 = (wave / WGN) * 64, wn0 = (wave % WGN) * 64;
+  const int lr = lane & 31, lh = lane >> 5;
+  const bool full = m0 + BM <= M && n0 + BN <= N;
+  // E8M0 block scales of the two operands (2^(s - 127) undoes 2^esc); a
+  // wave's 64 rows and 64 columns each lie in one 128-row scale tile
+  const int sa = 127 - escA[(m0 + wm0) >> 7], sb = 127 - escB[(n0 + wn0) >> 7];
+
+  // The matrix-core op's first operand is the B tile (32 C columns), its
+  // second the A tile (32 C rows): element r of block (i, j) is C[row][col]
+  // with row = m0 + wm0 + 32 j + lr and col = n0 + wn0 + 32 i + 8 (r / 4) +
+  // 4 lh + r % 4, so elements 4 s .. 4 s + 3 are 4 consecutive columns.
+  float16_t acc[2][2];
+  auto crow = [&](int j) { return m0 + wm0 + 32 * j + lr; };
+  auto ccol = [&](int i, int s4) { return n0 + wn0 + 32 * i + 8 * s4 + 4 * lh; };
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        float4_t v = {0.f, 0.f, 0.f, 0.f};
+        if (CINIT) {
+          const int row = crow(j), col = ccol(i, s4);
+          if (VEC && full) {
+            v = *reinterpret_cast<const float4_t *>(C + (long)row * ldc + col);
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              v[r] = (row < M && col + r < N) ? C[(long)row * ldc + col + r] : 0.f;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[i][j][4 * s4 + r] = v[r];
+      }
+
+  // this thread's chunks of A and of B per stage (rows past M or N are
+  // clamped to the last one; their products are never stored)
+  const uint8_t *ga[CA], *gb[CB];
+  int loffa[CA], loffb[CB];
+#pragma unroll
+  for (int r = 0; r < CA; ++r) {
+    const int c = wave * 64 + NT * r + lane;
+    const int row = c >> 2, kq = (c & 3) ^ lds_swz(row);
+    ga[r] = A + (long)min(m0 + row, M - 1) * lda + 16 * kq;
+    loffa[r] = (wave * 64 + NT * r) * 16;
+  }
+#pragma unroll
+  for (int r = 0; r < CB; ++r) {
+    const int c = wave * 64 + NT * r + lane;
+    const int row = c >> 2, kq = (c & 3) ^ lds_swz(row);
+    gb[r] = B + (long)min(n0 + row, N - 1) * ldb + 16 * kq;
+    loffb[r] = HALF + (wave * 64 + NT * r) * 16;
+  }
+  const int nk = K / BK;
+  auto issue = [&](int kc) {
+    const int kk = kc * BK;
+    char *st = smem + (kc % NST) * STAGE;
+#pragma unroll
+    for (int r = 0; r < CA; ++r)
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(ga[r] + kk), (lds_ptr_t)(st + loffa[r]), 16, 0,
+                                       0);
+#pragma unroll
+    for (int r = 0; r < CB; ++r)
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(gb[r] + kk), (lds_ptr_t)(st + loffb[r]), 16, 0,
+                                       0);
+  };
+  // a lane's operand: the two 16-byte chunks 2 lh and 2 lh + 1 of its row
+  int aoff[2][2], boff[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int m = wm0 + 32 * i + lr, n = wn0 + 32 * i + lr;
+      aoff[i][h] = m * 64 + (((2 * lh + h) ^ lds_swz(m)) << 4);
+      boff[i][h] = HALF + n * 64 + (((2 * lh + h) ^ lds_swz(n)) << 4);
+    }
+
+  static_assert(NST >= 2 && NST <= 4, "stages");
+#pragma unroll
+  for (int st = 0; st < NST - 1; ++st)
+    if (st < nk) issue(st);
+  for (int kc = 0; kc < nk; ++kc) {
+    const int ahead = nk - 1 - kc;
+    if (NST >= 4 && ahead >= 2) wait_vm_lp<(CA + CB) * (NST >= 4 ? 2 : 0)>();
+    else if (NST >= 3 && ahead >= 1) wait_vm_lp<(CA + CB) * (NST >= 3 ? 1 : 0)>();
+    else wait_vm_lp<0>();
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    if (kc + NST - 1 < nk) issue(kc + NST - 1);
+    const char *st = smem + (kc % NST) * STAGE;
+    int8v_t a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int4v_t a0 = *reinterpret_cast<const int4v_t *>(st + aoff[i][0]);
+      const int4v_t a1 = *reinterpret_cast<const int4v_t *>(st + aoff[i][1]);
+      const int4v_t b0 = *reinterpret_cast<const int4v_t *>(st + boff[i][0]);
+      const int4v_t b1 = *reinterpret_cast<const int4v_t *>(st + boff[i][1]);
+      a[i] = (int8v_t){a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+      b[i] = (int8v_t){b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(b[i], a[j], acc[i][j], 0, 0,
+                                                                    0, sb, 0, sa);
+  }
+  wait_vm_lp<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
+
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int row = crow(j), col = ccol(i, s4);
+        float *cp = C + (long)row * ldc + col;
+        float4_t v = {acc[i][j][4 * s4], acc[i][j][4 * s4 + 1], acc[i][j][4 * s4 + 2],
+                      acc[i][j][4 * s4 + 3]};
+        if (VEC && full) {
+          if (!CINIT) {
+            const float4_t old = beta != 0.f ? *reinterpret_cast<const float4_t *>(cp)
+                                             : (float4_t){0.f, 0.f, 0.f, 0.f};
+            v = alpha * v + beta * old;
+          }
+          *reinterpret_cast<float4_t *>(cp) = v;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (row < M && col + r < N) {
+              float x = v[r];
+              if (!CINIT) x = alpha * x + (beta != 0.f ? beta * cp[r] : 0.f);
+              cp[r] = x;
+            }
+        }
+      }
+}
+
+template <int BMv, int BNv, int NS, int OC, bool CI, bool V>
+static void fgemm_launch(int M, int N, int K, float alpha, const uint8_t *A, long lda,
+                         const uint8_t *B, long ldb, float beta, float *C, long ldc,
+                         hipStream_t s, const int *escA, const int *escB) {
+  constexpr int NT = (BMv / 64) * (BNv / 64) * 64;
+  const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
+  hipLaunchKernelGGL((fgemm_nt_kernel<BMv, BNv, NS, OC, CI, V>), dim3(tm * tn), dim3(NT), 0, s, M,
+                     N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB);
+}
+
+// FP8 operands with per-tile power-of-two scales; the large accumulate-into-C
+// updates (the mixed LU's trailing update at block 1024) take 128 x 256
+// tiles on 3 stages with 2 workgroups per CU (at most 128 VGPRs: four waves
+// per SIMD), everything else 128 x 128 tiles on 2 stages with 4 per CU.  At
+// 64512^2 x 1024: 781-787 TFLOP/s standalone (749 with two
+// v_mfma_f32_16x16x32_fp8_fp8 per 16-byte operand read and the scales
+// applied to C before and after the K loop); persistent workgroups, two per
+// CU with or without a start offset between them, measured slower (678-689).
 int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
              const uint8_t *B, long ldb, const int *escB, float beta, float *C, long ldc,
              hipStream_t s) {
@@ -334,9 +493,24 @@ int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const
       ((uintptr_t)B & 15) || !escA || !escB)
     return -1;
   const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;
-  hgemm_launch2<uint8_t>(alpha == 1.f && beta == 1.f, vec, M, N, K, alpha, A, lda, B, ldb, beta,
-                         C, ldc, s, escA, escB);
-  CLU_LAUNCHED("hgemm_nt_kernel (fp8)");
+  const bool cinit = alpha == 1.f && beta == 1.f;
+  const bool large = (long)M * N >= (1L << 25);
+  if (cinit && vec && K >= 1024 && large)
+    fgemm_launch<128, 256, 3, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA,
+                                             escB);
+  else if (cinit && vec)
+    fgemm_launch<128, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA,
+                                             escB);
+  else if (cinit)
+    fgemm_launch<128, 128, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+                                              escA, escB);
+  else if (vec)
+    fgemm_launch<128, 128, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+                                              escA, escB);
+  else
+    fgemm_launch<128, 128, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
+                                               escA, escB);
+  CLU_LAUNCHED("fgemm_nt_kernel");
   return CLU_OK;
 }
 
@@ -370,45 +544,86 @@ CLU_DEV unsigned quant4(float a, float b, float c, float d) {
   return (unsigned)w;
 }
 
-// dst (M x Kp bytes) = fp8(sign * src * 2^e_g) for the rows of 128-row tile
-// g, zero for k >= K; esc[g] = e_g.  One workgroup per tile.
-__global__ __launch_bounds__(256) void quant_rows_fp8_kernel(int M, int K, int Kp, float sign,
-                                                             const float *__restrict__ src,
-                                                             long lds, uint8_t *__restrict__ dst,
-                                                             int *__restrict__ esc, int vec) {
-  // rows one after another, 4 consecutive k per thread (a 16-byte load when
-  // the source rows are 16-byte aligned): no index division per element
+// Each 128-row (128-column) scale tile is split over QSPLIT workgroups: a
+// first kernel writes the tile's QSPLIT partial maxima of |v|, a second
+// converts, every workgroup of a tile deriving the same exponent from the
+// same partials (the maximum does not depend on the order), so the scales
+// and bytes equal those of one workgroup per tile.  part holds QSPLIT
+// floats per tile.  (One workgroup per tile, reading its tile twice,
+// streamed 0.8 TB/s at n = 65536: 45 ms of each FP8 LU.)
+constexpr int QSPLIT = FP8_PART_PER_TILE;
+
+// part[g * QSPLIT + s] = max |src| over rows 128 g + 16 s .. + 15 (K columns)
+__global__ __launch_bounds__(256) void absmax_rows_kernel(int M, int K,
+                                                          const float *__restrict__ src, long lds,
+                                                          float *__restrict__ part, int vec) {
   __shared__ float red[4];
-  const int g = blockIdx.x, t = threadIdx.x, r0 = g * 128;
-  const int rows = min(128, M - r0);
-  auto load4 = [&](const float *sp, int k, float (&v)[4]) {
-    if (vec && k + 3 < K) {
-      const float4_t f = *reinterpret_cast<const float4_t *>(sp + k);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] = f[q];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sp[k + q] : 0.f;
-    }
-  };
+  const int g = blockIdx.x, sl = blockIdx.y, t = threadIdx.x;
+  const int r0 = g * 128 + sl * (128 / QSPLIT);
+  const int rows = max(0, min(128 / QSPLIT, M - r0));
   float mx = 0.f;
   for (int i = 0; i < rows; ++i) {
     const float *sp = src + (long)(r0 + i) * lds;
     for (int k = 4 * t; k < K; k += 1024) {
-      float v[4];
-      load4(sp, k, v);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) mx = fmaxf(mx, fabsf(v[q]));
+      if (vec && k + 3 < K) {
+        const float4_t f = *reinterpret_cast<const float4_t *>(sp + k);
+        mx = fmaxf(mx, fmaxf(fmaxf(fabsf(f[0]), fabsf(f[1])), fmaxf(fabsf(f[2]), fabsf(f[3]))));
+      } else {
+        for (int q = 0; q < 4 && k + q < K; ++q) mx = fmaxf(mx, fabsf(sp[k + q]));
+      }
     }
   }
   mx = block_max_256(mx, red);
-  const int ex = tile_exponent(mx);
-  if (t == 0) esc[g] = ex;
+  if (t == 0) part[g * QSPLIT + sl] = mx;
+}
+
+// part[g * QSPLIT + s] = max |src| over columns 128 g .. + 127 of rows
+// s, s + QSPLIT, s + 2 QSPLIT, ... (< K)
+__global__ __launch_bounds__(256) void absmax_cols_kernel(int K, int N,
+                                                          const float *__restrict__ src, long lds,
+                                                          float *__restrict__ part) {
+  __shared__ float red[4];
+  const int g = blockIdx.x, sl = blockIdx.y, t = threadIdx.x;
+  const int c = g * 128 + (t & 127);
+  float mx = 0.f;
+  if (c < N)
+    for (int k = sl + QSPLIT * (t >> 7); k < K; k += 2 * QSPLIT)
+      mx = fmaxf(mx, fabsf(src[(long)k * lds + c]));
+  mx = block_max_256(mx, red);
+  if (t == 0) part[g * QSPLIT + sl] = mx;
+}
+
+CLU_DEV int tile_exponent_of(const float *part, int g) {
+  float mx = part[g * QSPLIT];
+#pragma unroll
+  for (int q = 1; q < QSPLIT; ++q) mx = fmaxf(mx, part[g * QSPLIT + q]);
+  return tile_exponent(mx);
+}
+
+// dst (M x Kp bytes) = fp8(sign * src * 2^e_g) for rows 128 g + 16 s .. + 15
+// of 128-row tile g, zero for k >= K; esc[g] = e_g
+__global__ __launch_bounds__(256) void quant_rows_fp8_kernel(int M, int K, int Kp, float sign,
+                                                             const float *__restrict__ src,
+                                                             long lds, uint8_t *__restrict__ dst,
+                                                             const float *__restrict__ part,
+                                                             int *__restrict__ esc, int vec) {
+  const int g = blockIdx.x, sl = blockIdx.y, t = threadIdx.x;
+  const int r0 = g * 128 + sl * (128 / QSPLIT);
+  const int rows = max(0, min(128 / QSPLIT, M - r0));
+  const int ex = tile_exponent_of(part, g);
+  if (sl == 0 && t == 0) esc[g] = ex;
   for (int i = 0; i < rows; ++i) {
     const float *sp = src + (long)(r0 + i) * lds;
     for (int k = 4 * t; k < Kp; k += 1024) {
       float v[4];
-      load4(sp, k, v);
+      if (vec && k + 3 < K) {
+        const float4_t f = *reinterpret_cast<const float4_t *>(sp + k);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = f[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sp[k + q] : 0.f;
+      }
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sign * ldexpf(v[q], ex) : 0.f;
       *reinterpret_cast<unsigned *>(dst + (long)(r0 + i) * Kp + k) =
@@ -418,63 +633,61 @@ __global__ __launch_bounds__(256) void quant_rows_fp8_kernel(int M, int K, int K
 }
 
 // dst (N x Kp bytes) = fp8(src^T * 2^e_g) for the columns of 128-column tile
-// g of src (K x N), zero for k >= K; esc[g] = e_g.  One workgroup per tile,
-// K in slabs of 64 transposed through LDS.
+// g of src (K x N), zero for k >= K; esc[g] = e_g.  Workgroup (g, j)
+// transposes the 64-row slab j through LDS.
 __global__ __launch_bounds__(256) void quant_cols_fp8_kernel(int K, int N, int Kp,
                                                              const float *__restrict__ src,
                                                              long lds, uint8_t *__restrict__ dst,
+                                                             const float *__restrict__ part,
                                                              int *__restrict__ esc) {
-  __shared__ float red[4];
   __shared__ float tile[64][129];
-  const int g = blockIdx.x, t = threadIdx.x, c0 = g * 128;
+  const int g = blockIdx.x, t = threadIdx.x, c0 = g * 128, k0 = blockIdx.y * 64;
   const int cols = min(128, N - c0);
-  float mx = 0.f;
-  for (int e = t; e < K * 128; e += 256) {
-    const int k = e / 128, c = e % 128;
-    if (c < cols) mx = fmaxf(mx, fabsf(src[(long)k * lds + c0 + c]));
-  }
-  mx = block_max_256(mx, red);
-  const int ex = tile_exponent(mx);
-  if (t == 0) esc[g] = ex;
+  const int ex = tile_exponent_of(part, g);
+  if (blockIdx.y == 0 && t == 0) esc[g] = ex;
   const int c = t & 127, h = t >> 7;  // column, half of the 64-row slab
-  for (int k0 = 0; k0 < Kp; k0 += 64) {
-    for (int e = t; e < 64 * 128; e += 256) {
-      const int kk = e / 128, cc = e % 128;
-      tile[kk][cc] = (k0 + kk < K && cc < cols) ? src[(long)(k0 + kk) * lds + c0 + cc] : 0.f;
-    }
-    __syncthreads();
-    if (c < cols) {
-      unsigned w[8];
+  for (int e = t; e < 64 * 128; e += 256) {
+    const int kk = e >> 7, cc = e & 127;
+    tile[kk][cc] = (k0 + kk < K && cc < cols) ? src[(long)(k0 + kk) * lds + c0 + cc] : 0.f;
+  }
+  __syncthreads();
+  if (c < cols) {
+    unsigned w[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int kk = 32 * h + 4 * q;
-        w[q] = quant4(ldexpf(tile[kk][c], ex), ldexpf(tile[kk + 1][c], ex),
-                      ldexpf(tile[kk + 2][c], ex), ldexpf(tile[kk + 3][c], ex));
-      }
-      uint4 *dp = reinterpret_cast<uint4 *>(dst + (long)(c0 + c) * Kp + k0 + 32 * h);
-      dp[0] = make_uint4(w[0], w[1], w[2], w[3]);
-      dp[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    for (int q = 0; q < 8; ++q) {
+      const int kk = 32 * h + 4 * q;
+      w[q] = quant4(ldexpf(tile[kk][c], ex), ldexpf(tile[kk + 1][c], ex),
+                    ldexpf(tile[kk + 2][c], ex), ldexpf(tile[kk + 3][c], ex));
     }
-    __syncthreads();
+    uint4 *dp = reinterpret_cast<uint4 *>(dst + (long)(c0 + c) * Kp + k0 + 32 * h);
+    dp[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    dp[1] = make_uint4(w[4], w[5], w[6], w[7]);
   }
 }
 
 int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds, uint8_t *dst,
-                   int *esc, hipStream_t s) {
+                   int *esc, float *part, hipStream_t s) {
   if (M <= 0) return CLU_OK;
   if (Kp % 64 != 0 || Kp < K) return -1;
   const int vec = lds % 4 == 0 && ((uintptr_t)src & 15) == 0;
-  hipLaunchKernelGGL(quant_rows_fp8_kernel, dim3((M + 127) / 128), dim3(256), 0, s, M, K, Kp,
-                     sign, src, lds, dst, esc, vec);
+  const dim3 grid((M + 127) / 128, QSPLIT);
+  hipLaunchKernelGGL(absmax_rows_kernel, grid, dim3(256), 0, s, M, K, src, lds, part, vec);
+  CLU_LAUNCHED("absmax_rows_kernel");
+  hipLaunchKernelGGL(quant_rows_fp8_kernel, grid, dim3(256), 0, s, M, K, Kp, sign, src, lds, dst,
+                     part, esc, vec);
   CLU_LAUNCHED("quant_rows_fp8_kernel");
   return CLU_OK;
 }
 int quant_cols_fp8(int K, int N, int Kp, const float *src, long lds, uint8_t *dst, int *esc,
-                   hipStream_t s) {
+                   float *part, hipStream_t s) {
   if (N <= 0) return CLU_OK;
   if (Kp % 64 != 0 || Kp < K) return -1;
-  hipLaunchKernelGGL(quant_cols_fp8_kernel, dim3((N + 127) / 128), dim3(256), 0, s, K, N, Kp, src,
-                     lds, dst, esc);
+  const int tiles = (N + 127) / 128;
+  hipLaunchKernelGGL(absmax_cols_kernel, dim3(tiles, QSPLIT), dim3(256), 0, s, K, N, src, lds,
+                     part);
+  CLU_LAUNCHED("absmax_cols_kernel");
+  hipLaunchKernelGGL(quant_cols_fp8_kernel, dim3(tiles, Kp / 64), dim3(256), 0, s, K, N, Kp, src,
+                     lds, dst, part, esc);
   CLU_LAUNCHED("quant_cols_fp8_kernel");
   return CLU_OK;
 }

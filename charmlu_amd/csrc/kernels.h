@@ -44,11 +44,14 @@ int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const
              hipStream_t s);
 // FP8 quantization with per-tile power-of-two scales: rows of src (M x K)
 // in 128-row tiles (times sign), or columns of src (K x N) in 128-column
-// tiles stored transposed; dst rows hold Kp bytes (Kp % 64 == 0, zero pad)
+// tiles stored transposed; dst rows hold Kp bytes (Kp % 64 == 0, zero pad);
+// esc gets one exponent per tile, part (scratch) FP8_PART_PER_TILE floats
+// per tile
+constexpr int FP8_PART_PER_TILE = 8;
 int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds, uint8_t *dst,
-                   int *esc, hipStream_t s);
+                   int *esc, float *part, hipStream_t s);
 int quant_cols_fp8(int K, int N, int Kp, const float *src, long lds, uint8_t *dst, int *esc,
-                   hipStream_t s);
+                   float *part, hipStream_t s);
 // dst (M x Kp) = 16-bit src (M x K) padded with zeros; dst (N x Kp) = 16-bit
 // transpose of src (K x N) padded with zeros.
 template <typename T16>

@@ -866,12 +866,16 @@ int charmlu_gemm_fp8_dev(int M, int N, int K, float alpha, const float *dA, long
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_bytes(&ws.pa16, &ws.pa16_b, (size_t)M * Kp))) return rc;
     if ((rc = ensure_bytes(&ws.pu16, &ws.pu16_b, (size_t)N * Kp))) return rc;
-    if ((rc = ensure_bytes(&ws.esc, &ws.esc_b, sizeof(int) * (size_t)(tm + tn)))) return rc;
+    if ((rc = ensure_bytes(&ws.esc, &ws.esc_b,
+                           (sizeof(int) + sizeof(float) * FP8_PART_PER_TILE) * (size_t)(tm + tn))))
+      return rc;
   }
   uint8_t *qa = (uint8_t *)ws.pa16, *qb = (uint8_t *)ws.pu16;
   int *ea = (int *)ws.esc, *eb = ea + tm;
-  if ((rc = quant_rows_fp8(M, K, Kp, 1.0f, dA, lda, qa, ea, s))) return rc;
-  if ((rc = quant_cols_fp8(K, N, Kp, dB, ldb, qb, eb, s))) return rc;
+  float *part = (float *)(eb + tn);
+  if ((rc = quant_rows_fp8(M, K, Kp, 1.0f, dA, lda, qa, ea, part, s))) return rc;
+  if ((rc = quant_cols_fp8(K, N, Kp, dB, ldb, qb, eb, part + (size_t)FP8_PART_PER_TILE * tm, s)))
+    return rc;
   return fgemm_nt(M, N, Kp, alpha, qa, Kp, ea, qb, Kp, eb, beta, dC, ldc, s);
 }
 
@@ -1377,7 +1381,11 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if ((rc = ensure_bytes(&ws.pa16, &ws.pa16_b, (size_t)2 * n * Kp))) return rc;
       if ((rc = ensure_bytes(&ws.pu16, &ws.pu16_b, (size_t)2 * n * Kp))) return rc;
     }
-    if (fp8 && (rc = ensure_bytes(&ws.esc, &ws.esc_b, sizeof(int) * 2 * (size_t)(n / 128 + 2))))
+    // per operand: one exponent and FP8_PART_PER_TILE partial maxima per
+    // 128-row tile
+    if (fp8 && (rc = ensure_bytes(&ws.esc, &ws.esc_b,
+                                  (sizeof(int) + sizeof(float) * FP8_PART_PER_TILE) * 2 *
+                                      (size_t)(n / 128 + 2))))
       return rc;
     if (nopiv) {
       if ((rc = ensure_bytes(&ws.uinv32, &ws.uinv32_b, sizeof(float) * nb * nb))) return rc;
@@ -1529,11 +1537,14 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if (!e && fp8) {
         // FP8 operands with power-of-two scales per 128-row tile of L21 and
         // per 128-column tile of U12
-        int *escL = (int *)ws.esc, *escU = escL + (n / 128 + 2);
-        if (j0 == 0) e = quant_rows_fp8(Nt, nb, Kp, -1.0f, L21, ldl21, (uint8_t *)ws.pa16, escL, sm);
+        const int T = n / 128 + 2;
+        int *escL = (int *)ws.esc, *escU = escL + T;
+        float *partL = (float *)(escU + T), *partU = partL + (size_t)FP8_PART_PER_TILE * T;
+        if (j0 == 0)
+          e = quant_rows_fp8(Nt, nb, Kp, -1.0f, L21, ldl21, (uint8_t *)ws.pa16, escL, partL, sm);
         if (!e)
           e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)ws.pu16 + (size_t)j0 * Kp,
-                             escU + j0 / 128, sm);
+                             escU + j0 / 128, partU + (size_t)FP8_PART_PER_TILE * (j0 / 128), sm);
       } else if (!e && lp) {
         // operands of the trailing update, rounded to 16 bits once per step
         if (prec == CHARMLU_PREC_BF16) {

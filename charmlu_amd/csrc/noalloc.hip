@@ -3,8 +3,9 @@
 // device memory (code under HIP graph capture, tight serving loops,
 // harnesses that forbid hipMalloc).
 //
-// The quantized operands (M x Kp and N x Kp bytes of e4m3) and their
-// per-tile power-of-two exponents live in `scratch`, a file-scope device
+// The quantized operands (M x Kp and N x Kp bytes of e4m3), their per-tile
+// power-of-two exponents and the quantizer's partial maxima live in
+// `scratch`, a file-scope device
 // array of NOALLOC_SCRATCH_BYTES that every device holding the library
 // keeps until the process ends.  In host code `scratch` names a host-side
 // stand-in, not device memory: its device address is looked up once per
@@ -73,11 +74,14 @@ extern "C" int charmlu_gemm_fp8_dev_noalloc(int M, int N, int K, float alpha, co
   }
   const long Kp = (K + 63) / 64 * 64;
   const long tm = (M + 127) / 128, tn = (N + 127) / 128;
-  const size_t need = (size_t)(M + (long)N) * Kp + sizeof(int) * (size_t)(tm + tn);
+  // per 128-row tile of either operand: one exponent and the partial maxima
+  // of the quantizer
+  const size_t per_tile = sizeof(int) + sizeof(float) * FP8_PART_PER_TILE;
+  const size_t need = (size_t)(M + (long)N) * Kp + per_tile * (size_t)(tm + tn);
   if (need > NOALLOC_SCRATCH_BYTES) {
-    set_error("charmlu_gemm_fp8_dev_noalloc: (M + N) * ceil64(K) + 4 * (ceil(M/128) + "
+    set_error("charmlu_gemm_fp8_dev_noalloc: (M + N) * ceil64(K) + %zu * (ceil(M/128) + "
               "ceil(N/128)) = %zu bytes exceeds the static scratch of %zu bytes",
-              need, (size_t)NOALLOC_SCRATCH_BYTES);
+              per_tile, need, (size_t)NOALLOC_SCRATCH_BYTES);
     return CLU_ERR_UNSUPPORTED;
   }
   void *base = nullptr;
@@ -85,7 +89,9 @@ extern "C" int charmlu_gemm_fp8_dev_noalloc(int M, int N, int K, float alpha, co
   if (rc) return rc;
   uint8_t *qa = (uint8_t *)base, *qb = qa + (size_t)M * Kp;
   int *ea = (int *)(qb + (size_t)N * Kp), *eb = ea + tm;
-  if ((rc = quant_rows_fp8(M, K, (int)Kp, 1.0f, dA, lda, qa, ea, s))) return rc;
-  if ((rc = quant_cols_fp8(K, N, (int)Kp, dB, ldb, qb, eb, s))) return rc;
+  float *part = (float *)(eb + tn);
+  if ((rc = quant_rows_fp8(M, K, (int)Kp, 1.0f, dA, lda, qa, ea, part, s))) return rc;
+  if ((rc = quant_cols_fp8(K, N, (int)Kp, dB, ldb, qb, eb, part + FP8_PART_PER_TILE * tm, s)))
+    return rc;
   return fgemm_nt(M, N, (int)Kp, alpha, qa, Kp, ea, qb, Kp, eb, beta, dC, ldc, s);
 }

@@ -216,7 +216,8 @@ int charmlu_gemm_lp_dev(int prec, int M, int N, int K, float alpha, const void *
  * C for FP32 A (M x K), B (K x N), C, where q() rounds to e4m3 after
  * scaling each 128-row tile of A and each 128-column tile of B by the
  * largest power of two that keeps it within +-448; products accumulate in
- * FP32 and are scaled back exactly.  Uses the device workspace. */
+ * FP32, the scales undone exactly by the matrix cores' block scales.  Uses
+ * the device workspace. */
 int charmlu_gemm_fp8_dev(int M, int N, int K, float alpha, const float *dA, long lda,
                          const float *dB, long ldb, float beta, float *dC, long ldc,
                          void *stream);
@@ -228,7 +229,7 @@ int charmlu_gemm_fp8_dev(int M, int N, int K, float alpha, const float *dA, long
  * but the quantized operands and their scales live in a static device
  * array of CHARMLU_NOALLOC_SCRATCH_BYTES that every device using the
  * library holds until the process ends: (M + N) * ceil(K / 64) * 64 +
- * 4 * (ceil(M / 128) + ceil(N / 128)) bytes must fit, else
+ * 36 * (ceil(M / 128) + ceil(N / 128)) bytes must fit, else
  * CHARMLU_ERR_UNSUPPORTED.  It calls no allocating routine and never
  * touches the library's workspace.  Calls on one device share that array:
  * calls that overlap in time on one device race.  Asynchronous. */
