@@ -1267,6 +1267,10 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
       if ((rc = swap_cols(0, n, r0, r1))) return rc;
       break;
     }
+    // early: L21 and the U row of the look-ahead block column (with their
+    // packed operands) need the panel and the first part of the previous
+    // update (which holds this block row), not its second part, which they
+    // run beside
     if ((rc = swap_cols(r1, r1 + la, r1 + la, r1 + la))) return rc;
     if ((rc = u_row(0, la))) return rc;
     // look-ahead block column, then the panel of step t+1 on the panel
@@ -1367,8 +1371,16 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   const int nruns = (nb + RUN - 1) / RUN;
   const bool lp = prec != CHARMLU_PREC_FP32;
   const bool fp8 = prec == CHARMLU_PREC_FP8;
+  // Unpivoted low-precision form: the next step's L21, U row and packed
+  // operands are computed beside the second part of this step's update, so
+  // the packed operands (and FP8 scales) alternate between two buffers by
+  // step parity (the second part of step t still reads step t's)
+  const bool early = lp && nopiv;
+  const size_t nbuf = early ? 2 : 1;
   // K of the low-precision operands, zero-padded (64 bytes per LDS stage row)
   const int Kp = fp8 ? (nb + 63) / 64 * 64 : (nb + 31) / 32 * 32;
+  const size_t pk_bytes = (size_t)(fp8 ? 1 : 2) * n * Kp;  // one packed operand
+  const size_t esc_ints = 2 * (size_t)(n / 128 + 2) * (1 + FP8_PART_PER_TILE);
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_runs(ws, nruns))) return rc;
@@ -1378,15 +1390,12 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     if ((rc = ensure_bytes(&ws.pinv, &ws.pinv_b, sizeof(float) * nb * nb / 4))) return rc;
     if ((rc = ensure_bytes(&ws.ptmp, &ws.ptmp_b, sizeof(float) * nb * nb / 4))) return rc;
     if (lp) {
-      if ((rc = ensure_bytes(&ws.pa16, &ws.pa16_b, (size_t)2 * n * Kp))) return rc;
-      if ((rc = ensure_bytes(&ws.pu16, &ws.pu16_b, (size_t)2 * n * Kp))) return rc;
+      if ((rc = ensure_bytes(&ws.pa16, &ws.pa16_b, nbuf * pk_bytes))) return rc;
+      if ((rc = ensure_bytes(&ws.pu16, &ws.pu16_b, nbuf * pk_bytes))) return rc;
     }
     // per operand: one exponent and FP8_PART_PER_TILE partial maxima per
     // 128-row tile
-    if (fp8 && (rc = ensure_bytes(&ws.esc, &ws.esc_b,
-                                  (sizeof(int) + sizeof(float) * FP8_PART_PER_TILE) * 2 *
-                                      (size_t)(n / 128 + 2))))
-      return rc;
+    if (fp8 && (rc = ensure_bytes(&ws.esc, &ws.esc_b, sizeof(int) * nbuf * esc_ints))) return rc;
     if (nopiv) {
       if ((rc = ensure_bytes(&ws.uinv32, &ws.uinv32_b, sizeof(float) * nb * nb))) return rc;
       if ((rc = ensure_bytes(&ws.l21buf, &ws.l21buf_b, sizeof(float) * nb * n))) return rc;
@@ -1435,8 +1444,12 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   // operand; the 16-bit L21 is packed negated, so the low-precision kernel
   // accumulates onto C (C += (-L21) U).  FP8 row offsets are multiples of
   // 128 (one scale per 128-row tile).
-  auto update = [&](int M, int N, int r_off, const float *L, long ldl, int c0, float *C, int Nt,
-                    hipStream_t st) -> int {
+  // packed operands and FP8 exponents (then partial maxima) of step t
+  auto pa_of = [&](int t) { return (char *)ws.pa16 + (size_t)(t & 1) * (nbuf - 1) * pk_bytes; };
+  auto pu_of = [&](int t) { return (char *)ws.pu16 + (size_t)(t & 1) * (nbuf - 1) * pk_bytes; };
+  auto esc_of = [&](int t) { return (int *)ws.esc + (size_t)(t & 1) * (nbuf - 1) * esc_ints; };
+  auto update = [&](int t, int M, int N, int r_off, const float *L, long ldl, int c0, float *C,
+                    int Nt, hipStream_t st) -> int {
     if (M <= 0 || N <= 0) return CLU_OK;
     const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb, st);
     const size_t ro = (size_t)r_off * Kp;
@@ -1445,15 +1458,15 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       r = sgemm(false, false, M, N, nb, -1.0f, L + (long)r_off * ldl, ldl, ubuf + c0, Nt, 1.0f, C,
                 lda, st);
     else if (fp8)
-      r = fgemm_nt(M, N, Kp, 1.0f, (const uint8_t *)ws.pa16 + ro, Kp,
-                   (const int *)ws.esc + r_off / 128, (const uint8_t *)ws.pu16 + (size_t)c0 * Kp,
-                   Kp, (const int *)ws.esc + (n / 128 + 2) + c0 / 128, 1.0f, C, lda, st);
+      r = fgemm_nt(M, N, Kp, 1.0f, (const uint8_t *)pa_of(t) + ro, Kp, esc_of(t) + r_off / 128,
+                   (const uint8_t *)pu_of(t) + (size_t)c0 * Kp, Kp,
+                   esc_of(t) + (n / 128 + 2) + c0 / 128, 1.0f, C, lda, st);
     else if (prec == CHARMLU_PREC_BF16)
-      r = hgemm_nt<__bf16>(M, N, Kp, 1.0f, (const __bf16 *)ws.pa16 + ro, Kp,
-                           (const __bf16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, st);
+      r = hgemm_nt<__bf16>(M, N, Kp, 1.0f, (const __bf16 *)pa_of(t) + ro, Kp,
+                           (const __bf16 *)pu_of(t) + (size_t)c0 * Kp, Kp, 1.0f, C, lda, st);
     else
-      r = hgemm_nt<_Float16>(M, N, Kp, 1.0f, (const _Float16 *)ws.pa16 + ro, Kp,
-                             (const _Float16 *)ws.pu16 + (size_t)c0 * Kp, Kp, 1.0f, C, lda, st);
+      r = hgemm_nt<_Float16>(M, N, Kp, 1.0f, (const _Float16 *)pa_of(t) + ro, Kp,
+                             (const _Float16 *)pu_of(t) + (size_t)c0 * Kp, Kp, 1.0f, C, lda, st);
     prof_end(pid, st);
     return r;
   };
@@ -1480,6 +1493,12 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     const int Ma = ((int)rows + 127) / 128 * 128;
     return Ma > M ? M : Ma;
   };
+  // the first part always holds the next step's block row when that step's
+  // U row is computed beside the second part
+  auto split_rows_of = [&](int t, int M, int N) -> int {
+    const int Ma = split_rows(t, M, N);
+    return early && Ma < nb ? (nb < M ? nb : M) : Ma;
+  };
   if ((rc = history_prepare(ws.histmx, n, nb, 1 + 2 * prec + (nopiv ? 1 : 0), NB))) {
     fence_streams(ws, su);
     return rc;
@@ -1491,7 +1510,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   for (int t = 0; t < NB; ++t) {
     const int r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
     CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));
-    if (full_pending) {  // the previous update's second part
+    if (full_pending && !early) {  // the previous update's second part
       CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
       full_pending = false;
     }
@@ -1511,6 +1530,10 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if ((rc = swap_cols(0, n, r0, r1))) return rc;
       break;
     }
+    // early: L21 and the U row of the look-ahead block column (with their
+    // packed operands) need the panel and the first part of the previous
+    // update (which holds this block row), not its second part, which they
+    // run beside
     if ((rc = swap_cols(r1, r1 + la, r1 + la, r1 + la))) return rc;
     float *A11 = dA + (long)r0 * lda + r0;
     // L21: in the matrix, or (unpivoted form) A21 inv(U11) in l21buf
@@ -1538,42 +1561,46 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
         // FP8 operands with power-of-two scales per 128-row tile of L21 and
         // per 128-column tile of U12
         const int T = n / 128 + 2;
-        int *escL = (int *)ws.esc, *escU = escL + T;
+        int *escL = esc_of(t), *escU = escL + T;
         float *partL = (float *)(escU + T), *partU = partL + (size_t)FP8_PART_PER_TILE * T;
         if (j0 == 0)
-          e = quant_rows_fp8(Nt, nb, Kp, -1.0f, L21, ldl21, (uint8_t *)ws.pa16, escL, partL, sm);
+          e = quant_rows_fp8(Nt, nb, Kp, -1.0f, L21, ldl21, (uint8_t *)pa_of(t), escL, partL, sm);
         if (!e)
-          e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)ws.pu16 + (size_t)j0 * Kp,
+          e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)pu_of(t) + (size_t)j0 * Kp,
                              escU + j0 / 128, partU + (size_t)FP8_PART_PER_TILE * (j0 / 128), sm);
       } else if (!e && lp) {
         // operands of the trailing update, rounded to 16 bits once per step
         if (prec == CHARMLU_PREC_BF16) {
-          if (j0 == 0) e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, L21, ldl21, (__bf16 *)ws.pa16, sm);
+          if (j0 == 0) e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, L21, ldl21, (__bf16 *)pa_of(t), sm);
           if (!e)
             e = pack_cols<__bf16>(nb, j1 - j0, Kp, ubuf + j0, Nt,
-                                  (__bf16 *)ws.pu16 + (size_t)j0 * Kp, sm);
+                                  (__bf16 *)pu_of(t) + (size_t)j0 * Kp, sm);
         } else {
           if (j0 == 0)
-            e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, L21, ldl21, (_Float16 *)ws.pa16, sm);
+            e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, L21, ldl21, (_Float16 *)pa_of(t), sm);
           if (!e)
             e = pack_cols<_Float16>(nb, j1 - j0, Kp, ubuf + j0, Nt,
-                                    (_Float16 *)ws.pu16 + (size_t)j0 * Kp, sm);
+                                    (_Float16 *)pu_of(t) + (size_t)j0 * Kp, sm);
         }
       }
       prof_end(p, sm);
       return e;
     };
     if ((rc = u_row(0, la))) return rc;
+    if (full_pending) {  // early: the previous update's second part, now
+      CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+      full_pending = false;
+    }
     float *C = A11 + (long)nb * lda + nb;
-    if ((rc = update(Nt, la, 0, L21, ldl21, 0, C, Nt, sm))) return rc;
+    if ((rc = update(t, Nt, la, 0, L21, ldl21, 0, C, Nt, sm))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_la, sm));
     if ((rc = swap_cols(0, n, r0, r1 + la))) return rc;
     if ((rc = u_row(la, Nt))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_urow, sm));  // operands of this step's update ready
     // part 1 beside the panel (enqueued before the panel's launches), part 2
     // on every CU after it
-    const int Ma = Nt > la ? split_rows(t, Nt, Nt - la) : 0;
-    if ((rc = update(Ma, Nt - la, 0, L21, ldl21, la, C + la, Nt, sm))) return rc;
+    const int Ma = Nt > la ? split_rows_of(t, Nt, Nt - la) : 0;
+    if ((rc = update(t, Ma, Nt - la, 0, L21, ldl21, la, C + la, Nt, sm))) return rc;
     CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
     CLU_HIP(hipEventRecord(ws.histmx.ev[2 * (t + 1)], sp));
     if ((rc = panel(t + 1))) return rc;
@@ -1583,7 +1610,8 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       hipStream_t sf = ws.s_full;
       CLU_HIP(hipStreamWaitEvent(sf, ws.ev_urow, 0));
       CLU_HIP(hipStreamWaitEvent(sf, ws.ev_panel, 0));
-      if ((rc = update(Nt - Ma, Nt - la, Ma, L21, ldl21, la, C + (long)Ma * lda + la, Nt, sf)))
+      if ((rc = update(t, Nt - Ma, Nt - la, Ma, L21, ldl21, la, C + (long)Ma * lda + la, Nt,
+                       sf)))
         return rc;
       CLU_HIP(hipEventRecord(ws.ev_full, sf));
       full_pending = true;
