@@ -1351,8 +1351,8 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     set_error("the FP8 update needs a block size that is a multiple of 128 (got %d)", nb);
     return -2;
   }
-  if (nopiv && nb > 1024) {
-    set_error("the unpivoted form needs block size <= 1024 (got %d)", nb);
+  if (nopiv && nb > 2048) {
+    set_error("the unpivoted form needs block size <= 2048 (got %d)", nb);
     return -2;
   }
   if (!dA && n > 0) return -4;

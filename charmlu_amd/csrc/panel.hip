@@ -465,8 +465,9 @@ bool shape_fits(int m, int sw, Cand c, int max_groups, PanelShape *ps) {
 template <typename T>
 bool plan_sw(int m, int sw, int max_groups, bool pivot, PanelShape *ps) {
   if (m <= 0) return false;
-  if (!pivot) {  // one workgroup of 256 threads (the unpivoted strip)
-    if (m > 1024 || sw == PW8 || (sizeof(T) == 8 && sw == PW32)) return false;
+  if (!pivot) {  // one workgroup (the unpivoted strip): 256 threads, 512 above 1024 rows
+    if (m > 2048 || sw == PW8 || (sizeof(T) == 8 && sw == PW32)) return false;
+    if (m > 1024) return shape_fits<T>(m, sw, Cand{4, 512}, 1, ps);
     return shape_fits<T>(m, sw, Cand{m <= 256 ? 1 : m <= 512 ? 2 : 4, 256}, 1, ps);
   }
   if (sw == PW8) {  // tall strips only
@@ -536,11 +537,13 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   const bool multi = G > 1;
   if (!pivot) {
     if (ps.sw == PW) {
-      if (ps.R == 1) CLU_PANEL(1, 256, false, false, PW);
+      if (ps.NT == 512) CLU_PANEL(4, 512, false, false, PW);
+      else if (ps.R == 1) CLU_PANEL(1, 256, false, false, PW);
       else if (ps.R == 2) CLU_PANEL(2, 256, false, false, PW);
       else CLU_PANEL(4, 256, false, false, PW);
     } else if constexpr (sizeof(T) == 4) {
-      if (ps.R == 1) CLU_PANEL(1, 256, false, false, PW32);
+      if (ps.NT == 512) CLU_PANEL(4, 512, false, false, PW32);
+      else if (ps.R == 1) CLU_PANEL(1, 256, false, false, PW32);
       else if (ps.R == 2) CLU_PANEL(2, 256, false, false, PW32);
       else CLU_PANEL(4, 256, false, false, PW32);
     } else {

@@ -174,9 +174,12 @@ def test_panel_plan_known_shapes():
     assert _plan(FP64, 9216, 16, 16)[1][:4] == [16, 4, 256, 9]
     # 8 panel CUs at C3 (FP32, 65536 rows): 8-column strips of 8192 rows each
     assert _plan(FP32, 65536, 8, 8)[1][:4] == [8, 16, 512, 8]
-    # unpivoted strips are one workgroup of at most 1024 rows
+    # unpivoted strips are one workgroup: 256 threads up to 1024 rows, 512
+    # threads up to 2048 (the diagonal block of block size 2048)
     assert _plan(FP32, 1024, 32, 32, pivot=0)[1][:4] == [32, 4, 256, 1]
-    assert _plan(FP32, 1025, 16, 32, pivot=0)[0] == _lib.ERR_UNSUPPORTED
+    assert _plan(FP32, 1025, 32, 32, pivot=0)[1][:4] == [32, 4, 512, 1]
+    assert _plan(FP32, 2048, 16, 32, pivot=0)[1][:4] == [16, 4, 512, 1]
+    assert _plan(FP32, 2049, 16, 32, pivot=0)[0] == _lib.ERR_UNSUPPORTED
     assert _plan(FP64, 512, 32, 32, pivot=0)[0] == _lib.ERR_UNSUPPORTED  # FP64: 16 columns
     assert _plan(FP64, 512, 16, 32, pivot=0)[1][:4] == [16, 2, 256, 1]
     # argument errors
