@@ -352,6 +352,39 @@ int gemm_t<float>(bool ta, bool tb, int M, int N, int K, float alpha, const floa
   return sgemm(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
+// Products with the explicit inverse of a triangular factor, whose entries
+// on the other side of the diagonal are zeros: in q blocks of the
+// triangle's order (4 from order 1024, 2 from 512), each block's K ending
+// at the diagonal, (q + 1) / (2 q) of the full product's flops.  Only zero
+// terms at the end of each sum are dropped.
+static int tri_blocks(int nb) { return nb >= 1024 && nb % 4 == 0 ? 4 : nb >= 512 && nb % 2 == 0 ? 2 : 1; }
+
+// C (nb x N) = X B for X (nb x nb) lower triangular
+template <typename T>
+static int gemm_lower_left(int nb, int N, const T *X, long ldx, const T *B, long ldb, T *C,
+                           long ldc, hipStream_t s) {
+  const int q = tri_blocks(nb), bs = nb / q;
+  for (int i = 0; i < q; ++i) {
+    int rc = gemm_t<T>(false, false, bs, N, (i + 1) * bs, (T)1, X + (long)i * bs * ldx, ldx, B,
+                       ldb, (T)0, C + (long)i * bs * ldc, ldc, s);
+    if (rc) return rc;
+  }
+  return CLU_OK;
+}
+
+// C (M x nb) = A Y for Y (nb x nb) upper triangular
+template <typename T>
+static int gemm_upper_right(int M, int nb, const T *A, long lda, const T *Y, long ldy, T *C,
+                            long ldc, hipStream_t s) {
+  const int q = tri_blocks(nb), bs = nb / q;
+  for (int j = 0; j < q; ++j) {
+    int rc = gemm_t<T>(false, false, M, bs, (j + 1) * bs, (T)1, A, lda, Y + (long)j * bs, ldy,
+                       (T)0, C + (long)j * bs, ldc, s);
+    if (rc) return rc;
+  }
+  return CLU_OK;
+}
+
 // B = L^{-1} B, L unit lower M x M (any M): 256-row diagonal solves plus
 // GEMM updates of the rows below.
 template <typename T>
@@ -1543,8 +1576,8 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     if (nopiv) {
       float *l21 = (float *)ws.l21buf;
       pid = prof_begin(PK_GEMM_OTHER, 2.0 * Nt * nb * nb, sm);
-      if ((rc = sgemm(false, false, Nt, nb, nb, 1.0f, A11 + (long)nb * lda, lda,
-                      (const float *)ws.uinv32, nb, 0.0f, l21, nb, sm)))
+      if ((rc = gemm_upper_right<float>(Nt, nb, A11 + (long)nb * lda, lda,
+                                        (const float *)ws.uinv32, nb, l21, nb, sm)))
         return rc;
       prof_end(pid, sm);
       L21 = l21;
@@ -1555,8 +1588,8 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     auto u_row = [&](int j0, int j1) -> int {
       if (j1 <= j0) return CLU_OK;
       const int p = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
-      int e = sgemm(false, false, nb, j1 - j0, nb, 1.0f, winv_of(t), nb, A11 + nb + j0, lda, 0.0f,
-                    ubuf + j0, Nt, sm);
+      int e = gemm_lower_left<float>(nb, j1 - j0, winv_of(t), nb, A11 + nb + j0, lda, ubuf + j0,
+                                     Nt, sm);
       if (!e && fp8) {
         // FP8 operands with power-of-two scales per 128-row tile of L21 and
         // per 128-column tile of U12
