@@ -183,7 +183,7 @@ int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda,
  * 128-column tile of U12, nb a multiple of 128; FP32 accumulation).  dA: n x n FP32 row-major, overwritten by L and U.  The
  * pivots and info follow charmlu_dgetrf.  prec | CHARMLU_MIXED_NOPIV
  * factors without row interchanges (ipiv[j] = j), for matrices that need
- * none, such as diagonally dominant ones; nb <= 1024 then.  The panel is
+ * none, such as diagonally dominant ones; nb <= 2048 then.  The panel is
  * then the LU of the diagonal block plus L21 = A21 inv(U11).  If every
  * computed |l_ij| < 1, partial pivoting would have chosen the same
  * (diagonal) pivots.  Asynchronous. */
