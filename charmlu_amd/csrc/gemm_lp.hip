@@ -606,7 +606,8 @@ __global__ __launch_bounds__(256) void quant_rows_fp8_kernel(int M, int K, int K
                                                              const float *__restrict__ src,
                                                              long lds, uint8_t *__restrict__ dst,
                                                              const float *__restrict__ part,
-                                                             int *__restrict__ esc, int vec) {
+                                                             int *__restrict__ esc, int vec,
+                                                             float *__restrict__ cpy, long ldc) {
   const int g = blockIdx.x, sl = blockIdx.y, t = threadIdx.x;
   const int r0 = g * 128 + sl * (128 / QSPLIT);
   const int rows = max(0, min(128 / QSPLIT, M - r0));
@@ -620,9 +621,12 @@ __global__ __launch_bounds__(256) void quant_rows_fp8_kernel(int M, int K, int K
         const float4_t f = *reinterpret_cast<const float4_t *>(sp + k);
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = f[q];
+        if (cpy) *reinterpret_cast<float4_t *>(cpy + (long)(r0 + i) * ldc + k) = f;
       } else {
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sp[k + q] : 0.f;
+        if (cpy)
+          for (int q = 0; q < 4 && k + q < K; ++q) cpy[(long)(r0 + i) * ldc + k + q] = v[q];
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sign * ldexpf(v[q], ex) : 0.f;
@@ -639,7 +643,8 @@ __global__ __launch_bounds__(256) void quant_cols_fp8_kernel(int K, int N, int K
                                                              const float *__restrict__ src,
                                                              long lds, uint8_t *__restrict__ dst,
                                                              const float *__restrict__ part,
-                                                             int *__restrict__ esc) {
+                                                             int *__restrict__ esc,
+                                                             float *__restrict__ cpy, long ldc) {
   __shared__ float tile[64][129];
   const int g = blockIdx.x, t = threadIdx.x, c0 = g * 128, k0 = blockIdx.y * 64;
   const int cols = min(128, N - c0);
@@ -648,7 +653,10 @@ __global__ __launch_bounds__(256) void quant_cols_fp8_kernel(int K, int N, int K
   const int c = t & 127, h = t >> 7;  // column, half of the 64-row slab
   for (int e = t; e < 64 * 128; e += 256) {
     const int kk = e >> 7, cc = e & 127;
-    tile[kk][cc] = (k0 + kk < K && cc < cols) ? src[(long)(k0 + kk) * lds + c0 + cc] : 0.f;
+    const bool in = k0 + kk < K && cc < cols;
+    const float v = in ? src[(long)(k0 + kk) * lds + c0 + cc] : 0.f;
+    tile[kk][cc] = v;
+    if (cpy && in) cpy[(long)(k0 + kk) * ldc + c0 + cc] = v;
   }
   __syncthreads();
   if (c < cols) {
@@ -666,20 +674,22 @@ __global__ __launch_bounds__(256) void quant_cols_fp8_kernel(int K, int N, int K
 }
 
 int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds, uint8_t *dst,
-                   int *esc, float *part, hipStream_t s) {
+                   int *esc, float *part, hipStream_t s, float *cpy, long ldc) {
   if (M <= 0) return CLU_OK;
   if (Kp % 64 != 0 || Kp < K) return -1;
-  const int vec = lds % 4 == 0 && ((uintptr_t)src & 15) == 0;
+  // 16-byte source reads (and copy writes) when both are 16-byte aligned
+  const int vec = lds % 4 == 0 && ((uintptr_t)src & 15) == 0 &&
+                  (!cpy || (ldc % 4 == 0 && ((uintptr_t)cpy & 15) == 0));
   const dim3 grid((M + 127) / 128, QSPLIT);
   hipLaunchKernelGGL(absmax_rows_kernel, grid, dim3(256), 0, s, M, K, src, lds, part, vec);
   CLU_LAUNCHED("absmax_rows_kernel");
   hipLaunchKernelGGL(quant_rows_fp8_kernel, grid, dim3(256), 0, s, M, K, Kp, sign, src, lds, dst,
-                     part, esc, vec);
+                     part, esc, vec, cpy, ldc);
   CLU_LAUNCHED("quant_rows_fp8_kernel");
   return CLU_OK;
 }
 int quant_cols_fp8(int K, int N, int Kp, const float *src, long lds, uint8_t *dst, int *esc,
-                   float *part, hipStream_t s) {
+                   float *part, hipStream_t s, float *cpy, long ldc) {
   if (N <= 0) return CLU_OK;
   if (Kp % 64 != 0 || Kp < K) return -1;
   const int tiles = (N + 127) / 128;
@@ -687,7 +697,7 @@ int quant_cols_fp8(int K, int N, int Kp, const float *src, long lds, uint8_t *ds
                      part);
   CLU_LAUNCHED("absmax_cols_kernel");
   hipLaunchKernelGGL(quant_cols_fp8_kernel, dim3(tiles, Kp / 64), dim3(256), 0, s, K, N, Kp, src,
-                     lds, dst, part, esc);
+                     lds, dst, part, esc, cpy, ldc);
   CLU_LAUNCHED("quant_cols_fp8_kernel");
   return CLU_OK;
 }
@@ -702,7 +712,8 @@ template int hgemm_nt<_Float16>(int, int, int, float, const _Float16 *, long, co
 template <typename T16>
 __global__ __launch_bounds__(256) void pack_rows_kernel(int M, int K, int Kp, float scale,
                                                         const float *__restrict__ src, long lds,
-                                                        T16 *__restrict__ dst, int vec) {
+                                                        T16 *__restrict__ dst, int vec,
+                                                        float *__restrict__ cpy, long ldc) {
   // 4 consecutive k per thread (one 8-byte store; a 16-byte load when the
   // source rows are 16-byte aligned), 1024 k per workgroup, rows strided
   // over grid rows: no index division per element
@@ -715,9 +726,12 @@ __global__ __launch_bounds__(256) void pack_rows_kernel(int M, int K, int Kp, fl
       const float4_t f = *reinterpret_cast<const float4_t *>(sr + k0);
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = f[q];
+      if (cpy) *reinterpret_cast<float4_t *>(cpy + (long)i * ldc + k0) = f;
     } else {
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = k0 + q < K ? sr[k0 + q] : 0.f;
+      if (cpy)
+        for (int q = 0; q < 4 && k0 + q < K; ++q) cpy[(long)i * ldc + k0 + q] = v[q];
     }
     union { T16 h[4]; uint2 u; } o;
 #pragma unroll
@@ -731,13 +745,17 @@ __global__ __launch_bounds__(256) void pack_rows_kernel(int M, int K, int Kp, fl
 template <typename T16>
 __global__ __launch_bounds__(256) void pack_cols_kernel(int K, int N, int Kp,
                                                         const float *__restrict__ src, long lds,
-                                                        T16 *__restrict__ dst) {
+                                                        T16 *__restrict__ dst,
+                                                        float *__restrict__ cpy, long ldc) {
   __shared__ float tile[32][33];
   const int k0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int r = ty; r < 32; r += 8) {
     const int k = k0 + r, n = n0 + tx;
-    tile[r][tx] = (k < K && n < N) ? src[(long)k * lds + n] : 0.f;
+    const bool in = k < K && n < N;
+    const float v = in ? src[(long)k * lds + n] : 0.f;
+    tile[r][tx] = v;
+    if (cpy && in) cpy[(long)k * ldc + n] = v;
   }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
@@ -748,28 +766,35 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(int K, int N, int Kp,
 
 template <typename T16>
 int pack_rows(int M, int K, int Kp, float scale, const float *src, long lds, T16 *dst,
-              hipStream_t s) {
+              hipStream_t s, float *cpy, long ldc) {
   if (M <= 0 || Kp <= 0) return CLU_OK;
   if (Kp % 4 != 0 || ((uintptr_t)dst & 7)) return -1;
-  const int vec = lds % 4 == 0 && ((uintptr_t)src & 15) == 0;
+  // 16-byte source reads (and copy writes) when both are 16-byte aligned
+  const int vec = lds % 4 == 0 && ((uintptr_t)src & 15) == 0 &&
+                  (!cpy || (ldc % 4 == 0 && ((uintptr_t)cpy & 15) == 0));
   const dim3 grid((Kp + 1023) / 1024, M < 8192 ? M : 8192);
   hipLaunchKernelGGL((pack_rows_kernel<T16>), grid, dim3(256), 0, s, M, K, Kp, scale, src, lds,
-                     dst, vec);
+                     dst, vec, cpy, ldc);
   CLU_LAUNCHED("pack_rows_kernel");
   return CLU_OK;
 }
 template <typename T16>
-int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s) {
+int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s,
+              float *cpy, long ldc) {
   if (N <= 0 || Kp <= 0) return CLU_OK;
   dim3 grid((N + 31) / 32, (Kp + 31) / 32);
-  hipLaunchKernelGGL((pack_cols_kernel<T16>), grid, dim3(256), 0, s, K, N, Kp, src, lds, dst);
+  hipLaunchKernelGGL((pack_cols_kernel<T16>), grid, dim3(256), 0, s, K, N, Kp, src, lds, dst, cpy,
+                     ldc);
   CLU_LAUNCHED("pack_cols_kernel");
   return CLU_OK;
 }
-template int pack_rows<__bf16>(int, int, int, float, const float *, long, __bf16 *, hipStream_t);
+template int pack_rows<__bf16>(int, int, int, float, const float *, long, __bf16 *, hipStream_t,
+                               float *, long);
 template int pack_rows<_Float16>(int, int, int, float, const float *, long, _Float16 *,
-                                 hipStream_t);
-template int pack_cols<__bf16>(int, int, int, const float *, long, __bf16 *, hipStream_t);
-template int pack_cols<_Float16>(int, int, int, const float *, long, _Float16 *, hipStream_t);
+                                 hipStream_t, float *, long);
+template int pack_cols<__bf16>(int, int, int, const float *, long, __bf16 *, hipStream_t, float *,
+                               long);
+template int pack_cols<_Float16>(int, int, int, const float *, long, _Float16 *, hipStream_t,
+                                 float *, long);
 
 }  // namespace clu

@@ -46,19 +46,21 @@ int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const
 // in 128-row tiles (times sign), or columns of src (K x N) in 128-column
 // tiles stored transposed; dst rows hold Kp bytes (Kp % 64 == 0, zero pad);
 // esc gets one exponent per tile, part (scratch) FP8_PART_PER_TILE floats
-// per tile
+// per tile.  cpy (optional, ld ldc): also receives src itself, in src's
+// layout (the pass that reads src for the operand stores the factor there)
 constexpr int FP8_PART_PER_TILE = 8;
 int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds, uint8_t *dst,
-                   int *esc, float *part, hipStream_t s);
+                   int *esc, float *part, hipStream_t s, float *cpy = nullptr, long ldc = 0);
 int quant_cols_fp8(int K, int N, int Kp, const float *src, long lds, uint8_t *dst, int *esc,
-                   float *part, hipStream_t s);
+                   float *part, hipStream_t s, float *cpy = nullptr, long ldc = 0);
 // dst (M x Kp) = 16-bit src (M x K) padded with zeros; dst (N x Kp) = 16-bit
-// transpose of src (K x N) padded with zeros.
+// transpose of src (K x N) padded with zeros; cpy as for the FP8 quantizers.
 template <typename T16>
 int pack_rows(int M, int K, int Kp, float scale, const float *src, long lds, T16 *dst,
-              hipStream_t s);
+              hipStream_t s, float *cpy = nullptr, long ldc = 0);
 template <typename T16>
-int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s);
+int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s,
+              float *cpy = nullptr, long ldc = 0);
 
 // ---- pivoting -------------------------------------------------------------
 // Row move list of a run of sequential swaps ipiv[k1..k2) (global rows):

@@ -1590,6 +1590,9 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       const int p = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
       int e = gemm_lower_left<float>(nb, j1 - j0, winv_of(t), nb, A11 + nb + j0, lda, ubuf + j0,
                                      Nt, sm);
+      // the pass that reads the U row (and, unpivoted, L21 from its
+      // buffer) for the packed operand also stores it into the factor
+      float *ucp = A11 + nb + j0, *lcp = nopiv ? A11 + (long)nb * lda : nullptr;
       if (!e && fp8) {
         // FP8 operands with power-of-two scales per 128-row tile of L21 and
         // per 128-column tile of U12
@@ -1597,23 +1600,27 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
         int *escL = esc_of(t), *escU = escL + T;
         float *partL = (float *)(escU + T), *partU = partL + (size_t)FP8_PART_PER_TILE * T;
         if (j0 == 0)
-          e = quant_rows_fp8(Nt, nb, Kp, -1.0f, L21, ldl21, (uint8_t *)pa_of(t), escL, partL, sm);
+          e = quant_rows_fp8(Nt, nb, Kp, -1.0f, L21, ldl21, (uint8_t *)pa_of(t), escL, partL, sm,
+                             lcp, lda);
         if (!e)
           e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)pu_of(t) + (size_t)j0 * Kp,
-                             escU + j0 / 128, partU + (size_t)FP8_PART_PER_TILE * (j0 / 128), sm);
+                             escU + j0 / 128, partU + (size_t)FP8_PART_PER_TILE * (j0 / 128), sm,
+                             ucp, lda);
       } else if (!e && lp) {
         // operands of the trailing update, rounded to 16 bits once per step
         if (prec == CHARMLU_PREC_BF16) {
-          if (j0 == 0) e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, L21, ldl21, (__bf16 *)pa_of(t), sm);
+          if (j0 == 0)
+            e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, L21, ldl21, (__bf16 *)pa_of(t), sm, lcp, lda);
           if (!e)
             e = pack_cols<__bf16>(nb, j1 - j0, Kp, ubuf + j0, Nt,
-                                  (__bf16 *)pu_of(t) + (size_t)j0 * Kp, sm);
+                                  (__bf16 *)pu_of(t) + (size_t)j0 * Kp, sm, ucp, lda);
         } else {
           if (j0 == 0)
-            e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, L21, ldl21, (_Float16 *)pa_of(t), sm);
+            e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, L21, ldl21, (_Float16 *)pa_of(t), sm, lcp,
+                                    lda);
           if (!e)
             e = pack_cols<_Float16>(nb, j1 - j0, Kp, ubuf + j0, Nt,
-                                    (_Float16 *)pu_of(t) + (size_t)j0 * Kp, sm);
+                                    (_Float16 *)pu_of(t) + (size_t)j0 * Kp, sm, ucp, lda);
         }
       }
       prof_end(p, sm);
@@ -1649,8 +1656,11 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       CLU_HIP(hipEventRecord(ws.ev_full, sf));
       full_pending = true;
     }
-    if ((rc = convert<float, float>(nb, Nt, ubuf, Nt, A11 + nb, lda, sm))) return rc;
-    if (nopiv && (rc = convert<float, float>(Nt, nb, L21, ldl21, A11 + (long)nb * lda, lda, sm)))
+    // the FP32 update's U row and L21 into the factor storage (the
+    // low-precision forms stored them while packing)
+    if (!lp && (rc = convert<float, float>(nb, Nt, ubuf, Nt, A11 + nb, lda, sm))) return rc;
+    if (!lp && nopiv &&
+        (rc = convert<float, float>(Nt, nb, L21, ldl21, A11 + (long)nb * lda, lda, sm)))
       return rc;
   }
   if (full_pending) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
