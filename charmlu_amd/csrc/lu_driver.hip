@@ -1512,7 +1512,14 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   const bool split = ws.panel_cus > 0;
   const bool force_split = split_forced();
   bool full_pending = false;
-  const double chip_rate = !lp ? 0.6 * 157.3e9 : fp8 ? 530e9 : 450e9;  // flop per ms
+  // the update's rate inside the LU (flop per ms), as measured: FP8 1.0
+  // PFLOP/s at block 2048 and 0.73 at 1024, BF16 and FP16 0.51, FP32 0.6 of
+  // the FP32 peak.  (With 0.53 for FP8 the first part ended long before the
+  // panel and the second, which the next look-ahead update waits for in the
+  // pivoted form, grew: C5 pivoted 547-559 against 602-607 TFLOP/s.  The
+  // margin of 1.1 over the panel: 0.9, 1.3 and 1.6 measured within 1 %.)
+  const double chip_rate =
+      !lp ? 0.6 * 157.3e9 : fp8 ? (nb >= 2048 ? 1000e9 : 730e9) : 510e9;
   auto split_rows = [&](int t, int M, int N) -> int {
     if (!split) return M;
     if (force_split) return forced_split(M);
