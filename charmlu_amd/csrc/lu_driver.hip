@@ -1300,10 +1300,6 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo, 
       if ((rc = swap_cols(0, n, r0, r1))) return rc;
       break;
     }
-    // early: L21 and the U row of the look-ahead block column (with their
-    // packed operands) need the panel and the first part of the previous
-    // update (which holds this block row), not its second part, which they
-    // run beside
     if ((rc = swap_cols(r1, r1 + la, r1 + la, r1 + la))) return rc;
     if ((rc = u_row(0, la))) return rc;
     // look-ahead block column, then the panel of step t+1 on the panel
