@@ -235,10 +235,12 @@ def lp_gemm_only(lib, _lib, torch, n, nb, steps, warmup, prec="bf16"):
     print(json.dumps({"lp_gemm": f"{m}x{m}x{nb} {prec}", **res}), flush=True)
 
 
-def fp64_in_place(clu, lib, _lib, torch, A, b, nb64, n):
+def fp64_in_place(clu, lib, _lib, torch, A, b, nb64, n, g=None):
     """One FP64 LU + solve factoring A itself (no copy: at n = 131072 the
     FP64 matrix and a copy do not both fit beside the other buffers), then A
-    regenerated from its seed for the residual.  Returns (record, x64)."""
+    regenerated from its seed for the residual.  With g, also the
+    transposed solve A^T lam = g from the same factors (untimed).  Returns
+    (record, x64, lam64)."""
     import ctypes
     flops = 2.0 / 3.0 * n ** 3
     x64 = b.clone()
@@ -252,6 +254,10 @@ def fp64_in_place(clu, lib, _lib, torch, A, b, nb64, n):
     prof = (ctypes.c_double * 20)()
     _lib.check(lib.charmlu_profile_read(prof, 20), "charmlu_profile_read")
     lib.charmlu_profile(0)
+    lam64 = None
+    if g is not None:
+        lam64 = g.clone()
+        clu.getrs(A, ipiv, lam64, trans=True)
     _lib.check(lib.charmlu_dgen_matrix(1, clu.lu.SEED_A, n, ctypes.c_void_p(A.data_ptr()), n,
                                        clu.lu._stream_ptr()), "charmlu_dgen_matrix")
     res = clu.residual(A, x64, b)
@@ -260,22 +266,28 @@ def fp64_in_place(clu, lib, _lib, torch, A, b, nb64, n):
             "residual": res["residual"], "info": int(info.item()), "steps": 1,
             "trailing_update_tflops": round(upd, 3),
             "trailing_update_frac": round(upd / FP64_PEAK, 4),
-            "note": "one step factoring A in place (no copy of A), after the mixed runs"}, x64
+            "note": "one step factoring A in place (no copy of A), after the mixed runs"}, x64, lam64
 
 
 def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, fp64_nb=None,
-                fp64_after=False):
+                fp64_after=False, adjoint=False):
     """Mixed-precision LU + FP64 GMRES-IR against the plain FP64 LU on the
     same diagonally dominant system (A U[-0.5,0.5) with a_ii = sum_j |a_ij|,
     b from MatGen).  A mixed step converts A to FP32, factors it with the
     low-precision trailing update and refines x in FP64 to the reference's
     scaled residual <= 1; a FP64 step copies A, factors and solves
     (fp64_after: one step after the mixed runs, factoring A in place).
+    adjoint: after each form's timed run, the adjoint solve of the
+    differentiable solve, A^T lam = g by FP64 GMRES-IR from the same stored
+    low-precision factors (no refactoring; g = b reversed), timed, with its
+    error against the FP64 LU's transposed solve when that runs after.
     Returns the "mixed" object of the JSON line."""
     import ctypes
     flops = 2.0 / 3.0 * n ** 3
     A = clu.gen_matrix(n, kind="diagdom")
     b = clu.gen_rhs(n)
+    g = torch.flip(b, dims=[0]).contiguous() if adjoint else None
+    lams = {}
     out = {"n": n, "block": nb,
            "workload": f"A U[-0.5,0.5) with a_ii = sum_j |a_ij| (diagonally dominant), n={n}"}
     # plain FP64 LU on the same system: the speed reference and the solution
@@ -325,6 +337,8 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
         pivot = mode != "nopiv"
         split = [0.0, 0.0]
 
+        last = {}
+
         def step():
             t_a = time.perf_counter()
             clu.lag2s(A, A32)
@@ -334,6 +348,7 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
             _, st = clu.gesv_ir(A, A32, ipiv, b, x, tol=1.0)
             split[0] += t_b - t_a
             split[1] += time.perf_counter() - t_b
+            last["ipiv"] = ipiv
             return info, st
         for _ in range(warmup):
             step()
@@ -372,6 +387,20 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
                 if rec["max_abs_l"] < 1.0 else "none"
         else:
             rec["pivoting"] = "partial"
+        if adjoint:
+            # the timed run's factors are still in A32 (and ipiv)
+            torch.cuda.synchronize()
+            t_a = time.perf_counter()
+            lam, sta = clu.gesv_ir_multi(A, A32, last["ipiv"], g, trans=True, tol=1.0)
+            torch.cuda.synchronize()
+            rec["adjoint"] = {"ms": round((time.perf_counter() - t_a) * 1e3, 1),
+                              "ir_cycles": sta["cycles"], "ir_iterations": sta["iterations"],
+                              "residual": sta["residual"],
+                              "note": "A^T lam = g from the stored factors by FP64 GMRES-IR "
+                                      "(the backward pass of the mixed differentiable solve; "
+                                      "dA = -lam x^T is rank one per right-hand side and not "
+                                      "formed: at this n it would take 137 GB)"}
+            lams[key] = lam
         if x64 is not None:
             rec["error_vs_fp64"] = float(((x - x64).abs().max() / x64.abs().max()).item())
             rec["speedup_vs_fp64"] = round(rec["tflops"] / out["fp64"]["tflops"], 3)
@@ -381,10 +410,14 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
     if fp64_after and fp64_steps > 0:
         del A32
         torch.cuda.empty_cache()
-        out["fp64"], x64 = fp64_in_place(clu, lib, _lib, torch, A, b, fp64_nb or nb, n)
+        out["fp64"], x64, lam64 = fp64_in_place(clu, lib, _lib, torch, A, b, fp64_nb or nb, n,
+                                                g=g)
         for key, xk in xs.items():
             out[key]["error_vs_fp64"] = float(((xk - x64).abs().max() / x64.abs().max()).item())
             out[key]["speedup_vs_fp64"] = round(out[key]["tflops"] / out["fp64"]["tflops"], 3)
+        for key, lk in lams.items():
+            out[key]["adjoint"]["error_vs_fp64"] = float(
+                ((lk - lam64).abs().max() / lam64.abs().max()).item())
     return out
 
 
@@ -427,7 +460,7 @@ def c5_bench(clu, lib, _lib, torch, nb, fp64_nb):
     traffic per flop: 657 / 553 TFLOP/s against 623 / 522 with 1024 on one
     box), 1024 for the FP64 LU."""
     rec = mixed_bench(clu, lib, _lib, torch, 131072, nb, ["fp8-nopiv", "fp8"], 1, 0, 1, fp64_nb,
-                      fp64_after=True)
+                      fp64_after=True, adjoint=True)
     rec["steps_note"] = "one timed step per form, no warmup (the first call of its shape)"
     return rec
 

@@ -5,7 +5,7 @@
 // (updateLsubBlock, lu.C:247-252): one thread per element of A, rows
 // contiguous across lanes.  dgemv replaces cblas_dgemv(RowMajor, NoTrans /
 // Trans, ...) (recvXvec, benchmark.C:494-496): 16 lanes per row for the
-// row-major no-transpose product, one lane per column for the transpose.
+// row-major no-transpose product, two columns per lane for the transpose.
 // Negative increments follow the CBLAS convention (the vector is walked from
 // its last stored element).
 #include "common.h"
@@ -81,24 +81,65 @@ __global__ __launch_bounds__(256) void dgemv_n_kernel(int M, int N, double alpha
   }
 }
 
-// y = alpha * A^T x + beta * y, A is M x N row-major, y has N entries
-__global__ __launch_bounds__(256) void dgemv_t_kernel(int M, int N, double alpha,
+// y = alpha * A^T x + beta * y, A is M x N row-major, y has N entries.
+// 128 columns per workgroup, two per lane (16-byte loads when the rows are
+// 16-byte aligned); the 8 waves take every 8th row, four rows in flight per
+// lane, and their partial sums are added in a fixed order through LDS (one
+// lane per column with one load in flight streamed A at a fraction of HBM
+// speed: the adjoint's transposed products dominated its refinement).
+__global__ __launch_bounds__(512) void dgemv_t_kernel(int M, int N, double alpha,
                                                       const double *__restrict__ A, long lda,
                                                       const double *__restrict__ x, long incx,
                                                       double beta, double *__restrict__ y,
                                                       long incy) {
-  __shared__ double part[4][64];
+  __shared__ double part[8][2][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = blockIdx.x * 64 + lane;
-  double s = 0.0;
-  if (j < N)
-    for (int k = wave; k < M; k += 4) s += A[(long)k * lda + j] * x[vidx(k, M, incx)];
-  part[wave][lane] = s;
+  const int c0 = blockIdx.x * 128 + 2 * lane;
+  const bool vec = ((lda | N) & 1) == 0 && ((unsigned long)A & 15) == 0;
+  double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
+  if (c0 < N) {
+    if (vec) {
+      int k = wave;
+      for (; k + 24 < M; k += 32) {
+        double2 a[4];
+        double xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a[u] = *reinterpret_cast<const double2 *>(A + (long)(k + 8 * u) * lda + c0);
+          xv[u] = x[vidx(k + 8 * u, M, incx)];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          sx[u] += a[u].x * xv[u];
+          sy[u] += a[u].y * xv[u];
+        }
+      }
+      for (; k < M; k += 8) {
+        const double2 a0 = *reinterpret_cast<const double2 *>(A + (long)k * lda + c0);
+        const double x0 = x[vidx(k, M, incx)];
+        sx[0] += a0.x * x0;
+        sy[0] += a0.y * x0;
+      }
+    } else {
+      for (int k = wave; k < M; k += 8) {
+        const double x0 = x[vidx(k, M, incx)];
+        sx[0] += A[(long)k * lda + c0] * x0;
+        if (c0 + 1 < N) sy[0] += A[(long)k * lda + c0 + 1] * x0;
+      }
+    }
+  }
+  part[wave][0][lane] = (sx[0] + sx[1]) + (sx[2] + sx[3]);
+  part[wave][1][lane] = (sy[0] + sy[1]) + (sy[2] + sy[3]);
   __syncthreads();
-  if (wave == 0 && j < N) {
-    const double t = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
-    double *yp = y + vidx(j, N, incy);
-    *yp = beta == 0.0 ? alpha * t : alpha * t + beta * *yp;
+  if (wave < 2) {
+    const int c = c0 + wave;
+    if (c < N) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < 8; ++w) t += part[w][wave][lane];
+      double *yp = y + vidx(c, N, incy);
+      *yp = beta == 0.0 ? alpha * t : alpha * t + beta * *yp;
+    }
   }
 }
 
@@ -109,7 +150,7 @@ int dgemv(bool trans, int M, int N, double alpha, const double *A, long lda, con
     hipLaunchKernelGGL(dgemv_n_kernel, dim3((M + 15) / 16), dim3(256), 0, s, M, N, alpha, A, lda,
                        x, incx, beta, y, incy);
   else
-    hipLaunchKernelGGL(dgemv_t_kernel, dim3((N + 63) / 64), dim3(256), 0, s, M, N, alpha, A, lda,
+    hipLaunchKernelGGL(dgemv_t_kernel, dim3((N + 127) / 128), dim3(512), 0, s, M, N, alpha, A, lda,
                        x, incx, beta, y, incy);
   CLU_LAUNCHED("dgemv_kernel");
   return CLU_OK;

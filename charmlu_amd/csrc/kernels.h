@@ -165,6 +165,9 @@ int blk_gemv(bool trans, int m, const T *G, long ldg, const T *x, T *y, hipStrea
 template <typename T>
 int gemv_sub(bool trans, int M, int K, const T *A, long lda, const T *x, T *y,
              hipStream_t s);
+// in-place transpose of nblk contiguous S x S blocks (S a multiple of 32)
+template <typename T>
+int blk_transpose(int S, int nblk, T *G, hipStream_t s);
 
 // ---- generation, residual, conversion -------------------------------------
 int gen_uniform(uint64_t seed, long row0, long col0, long m, long n, double *A,

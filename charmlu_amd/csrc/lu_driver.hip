@@ -554,9 +554,11 @@ static int apply_runs(int nruns, const int2 *moves, const int *counts, T *A, lon
 // 8192 and more pass S = 1024: a vector solve is two launches per block, at
 // n = 65536 its 1024 launches of 256-row blocks spent a third of the solve
 // between kernels, and the refinement corrects the extra rounding.
+// The transposed solve keeps the inverses transposed (one in-place pass
+// after inverting), so both directions apply them by rows.
 // inv_ready: the inverses of the diagonal blocks are still in the workspace
-// from the previous call with the same factors and S (the refinement's
-// repeated solves)
+// from the previous call with the same factors, S and trans (the
+// refinement's repeated solves)
 template <typename T>
 static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const int *dipiv, T *dB,
                       long ldb, hipStream_t s, bool inv_ready = false, int S = 256) {
@@ -600,6 +602,9 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
     if (last > 0 && (rc = tri_inv<T>(up, !up, last, 1, dLU + (long)nfull * S * (lda + 1), lda, 0,
                                      Inv + (size_t)nfull * S * S, S, 0, tbuf, s)))
       return rc;
+    // (a shorter last block's inverse fills the top left of its S x S slot,
+    // which the transpose maps onto itself)
+    if (trans == CHARMLU_TRANS && (rc = blk_transpose<T>(S, nblk, Inv, s))) return rc;
   }
   const bool vec = nrhs == 1 && ldb == 1;
   auto interchanges = [&](bool inverse) -> int {
@@ -611,7 +616,8 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
                          2L * RUN);
     return e;
   };
-  // dst = op(G) src for the inverse G of one diagonal block
+  // dst = op(G) src for the inverse G of one diagonal block (G^T stored for
+  // the transposed solve: pass tr = false there)
   auto diag = [&](bool tr, int m, const T *G, const T *src, long lds, T *dst, long ldd) -> int {
     if (vec) return blk_gemv<T>(tr, m, G, S, src, dst, s);
     return gemm_t<T>(tr, false, m, nrhs, m, (T)1, G, S, src, lds, (T)0, dst, ldd, s);
@@ -650,7 +656,7 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
   // X = P^T W
   for (int k = 0; k < nblk; ++k) {
     const int x0 = k * S, m = n - x0 < S ? n - x0 : S;
-    if ((rc = diag(true, m, Uinv + (size_t)k * S * S, dB + (long)x0 * ldb, ldb,
+    if ((rc = diag(false, m, Uinv + (size_t)k * S * S, dB + (long)x0 * ldb, ldb,
                    Y + (long)x0 * ldy, ldy)))
       return rc;
     if ((rc = update(true, n - x0 - m, m, F + (long)x0 * lda + x0 + m, Y + (long)x0 * ldy, ldy,
@@ -659,7 +665,7 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
   }
   for (int k = nblk - 1; k >= 0; --k) {
     const int x0 = k * S, m = n - x0 < S ? n - x0 : S;
-    if ((rc = diag(true, m, Linv + (size_t)k * S * S, Y + (long)x0 * ldy, ldy,
+    if ((rc = diag(false, m, Linv + (size_t)k * S * S, Y + (long)x0 * ldy, ldy,
                    dB + (long)x0 * ldb, ldb)))
       return rc;
     if ((rc = update(true, x0, m, F + (long)x0 * lda, dB + (long)x0 * ldb, ldb, Y, ldy)))

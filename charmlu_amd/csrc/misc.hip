@@ -107,34 +107,70 @@ __global__ void residual_init_kernel(double *out) {
   if (threadIdx.x < 5) out[threadIdx.x] = 0.0;
 }
 
-// one wave per row
+// 16 lanes per row (16 rows per workgroup); unit-stride 16-byte aligned
+// rows take 16-byte loads with four partial sums, so each lane keeps four
+// loads in flight (one 8-byte load chain per lane of a wave per row streamed
+// A at 5.0 TB/s)
 __global__ __launch_bounds__(256) void residual_kernel(int n, const double *__restrict__ A,
                                                        long lda, const double *__restrict__ x,
                                                        const double *__restrict__ b,
                                                        double *__restrict__ r,
                                                        double *__restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  const long i = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= n) return;
-  double s = 0.0, as = 0.0, am = 0.0;
-  const double *Ai = A + i * lda;
-  for (int j = lane; j < n; j += 64) {
-    const double a = Ai[j];
-    s += a * x[j];
-    as += fabs(a);
-    am = fmax(am, fabs(a));
+  const long i = (long)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int sub = threadIdx.x & 15;
+  double s[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0}, am = 0.0;
+  const bool vec = ((lda | n) & 1) == 0 && (((unsigned long)A | (unsigned long)x) & 15) == 0;
+  if (i < n) {
+    const double *Ai = A + i * lda;
+    if (vec) {
+      const double2 *a2 = reinterpret_cast<const double2 *>(Ai);
+      const double2 *x2 = reinterpret_cast<const double2 *>(x);
+      const int n2 = n / 2;
+      int k = sub;
+      for (; k + 48 < n2; k += 64) {
+        double2 p[4], q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          p[u] = a2[k + 16 * u];
+          q[u] = x2[k + 16 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          s[u] += p[u].x * q[u].x + p[u].y * q[u].y;
+          as[u] += fabs(p[u].x) + fabs(p[u].y);
+          am = fmax(am, fmax(fabs(p[u].x), fabs(p[u].y)));
+        }
+      }
+      for (; k < n2; k += 16) {
+        const double2 p0 = a2[k], q0 = x2[k];
+        s[0] += p0.x * q0.x + p0.y * q0.y;
+        as[0] += fabs(p0.x) + fabs(p0.y);
+        am = fmax(am, fmax(fabs(p0.x), fabs(p0.y)));
+      }
+    } else {
+      for (int j = sub; j < n; j += 16) {
+        const double a = Ai[j];
+        s[0] += a * x[j];
+        as[0] += fabs(a);
+        am = fmax(am, fabs(a));
+      }
+    }
   }
-  s = wave_sum(s);
-  as = wave_sum(as);
-  am = wave_max(am);
-  if (lane == 0) {
-    const double ri = b[i] - s;
+  double st = (s[0] + s[1]) + (s[2] + s[3]), ast = (as[0] + as[1]) + (as[2] + as[3]);
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) {
+    st += __shfl_xor(st, off, 16);
+    ast += __shfl_xor(ast, off, 16);
+    am = fmax(am, __shfl_xor(am, off, 16));
+  }
+  if (i < n && sub == 0) {
+    const double ri = b[i] - st;
     if (r) r[i] = ri;
     atomic_max_nonneg(&out[0], am);
     atomic_max_nonneg(&out[1], fabs(b[i]));
     atomic_max_nonneg(&out[2], fabs(x[i]));
     atomic_max_nonneg(&out[3], fabs(ri));
-    atomic_max_nonneg(&out[4], as);
+    atomic_max_nonneg(&out[4], ast);
   }
 }
 
@@ -143,30 +179,69 @@ int residual(int n, const double *A, long lda, const double *x, const double *b,
   hipLaunchKernelGGL(residual_init_kernel, dim3(1), dim3(64), 0, s, out);
   CLU_LAUNCHED("residual_init_kernel");
   if (n <= 0) return CLU_OK;
-  hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, A, lda,
-                     x, b, r, out);
+  hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, s, n, A,
+                     lda, x, b, r, out);
   CLU_LAUNCHED("residual_kernel");
   return CLU_OK;
 }
 
 // out[0] = max_ij |a_ij|, out[1] = max_j sum_i |a_ij| (the infinity norm
 // of A^T): one thread per column, rows in order (coalesced across lanes).
-__global__ __launch_bounds__(256) void abs_colsum_kernel(int n, const double *__restrict__ A,
+// 128 columns per workgroup as in the transposed product: two columns per
+// lane, the 8 waves over every 8th row, four rows in flight per lane, the
+// column sums added over the waves in a fixed order
+__global__ __launch_bounds__(512) void abs_colsum_kernel(int n, const double *__restrict__ A,
                                                          long lda, double *__restrict__ out) {
-  const long j = (long)blockIdx.x * 256 + threadIdx.x;
-  double cs = 0.0, cm = 0.0;
-  if (j < n) {
-    for (long i = 0; i < n; ++i) {
-      const double a = fabs(A[i * lda + j]);
-      cs += a;
-      cm = fmax(cm, a);
+  __shared__ double part[8][2][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c0 = blockIdx.x * 128 + 2 * lane;
+  const bool vec = ((lda | n) & 1) == 0 && ((unsigned long)A & 15) == 0;
+  double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0}, cm = 0.0;
+  if (c0 < n) {
+    if (vec) {
+      int k = wave;
+      for (; k + 24 < n; k += 32) {
+        double2 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          a[u] = *reinterpret_cast<const double2 *>(A + (long)(k + 8 * u) * lda + c0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          sx[u] += fabs(a[u].x);
+          sy[u] += fabs(a[u].y);
+          cm = fmax(cm, fmax(fabs(a[u].x), fabs(a[u].y)));
+        }
+      }
+      for (; k < n; k += 8) {
+        const double2 a0 = *reinterpret_cast<const double2 *>(A + (long)k * lda + c0);
+        sx[0] += fabs(a0.x);
+        sy[0] += fabs(a0.y);
+        cm = fmax(cm, fmax(fabs(a0.x), fabs(a0.y)));
+      }
+    } else {
+      for (int k = wave; k < n; k += 8) {
+        const double a0 = fabs(A[(long)k * lda + c0]);
+        sx[0] += a0;
+        cm = fmax(cm, a0);
+        if (c0 + 1 < n) {
+          const double a1 = fabs(A[(long)k * lda + c0 + 1]);
+          sy[0] += a1;
+          cm = fmax(cm, a1);
+        }
+      }
     }
   }
-  cs = wave_max(cs);
+  part[wave][0][lane] = (sx[0] + sx[1]) + (sx[2] + sx[3]);
+  part[wave][1][lane] = (sy[0] + sy[1]) + (sy[2] + sy[3]);
   cm = wave_max(cm);
-  if ((threadIdx.x & 63) == 0) {
-    atomic_max_nonneg(&out[0], cm);
-    atomic_max_nonneg(&out[1], cs);
+  if (lane == 0) atomic_max_nonneg(&out[0], cm);
+  __syncthreads();
+  if (wave < 2) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) t += part[w][wave][lane];
+    t = wave_max(c0 + wave < n ? t : 0.0);
+    if (lane == 0) atomic_max_nonneg(&out[1], t);
   }
 }
 
@@ -174,7 +249,7 @@ int abs_colsum(int n, const double *A, long lda, double *out, hipStream_t s) {
   hipLaunchKernelGGL(residual_init_kernel, dim3(1), dim3(64), 0, s, out);
   CLU_LAUNCHED("residual_init_kernel");
   if (n <= 0) return CLU_OK;
-  hipLaunchKernelGGL(abs_colsum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, A,
+  hipLaunchKernelGGL(abs_colsum_kernel, dim3((unsigned)((n + 127) / 128)), dim3(512), 0, s, n, A,
                      lda, out);
   CLU_LAUNCHED("abs_colsum_kernel");
   return CLU_OK;

@@ -331,6 +331,40 @@ template int tri_inv<double>(bool, bool, int, int, const double *, long, long, d
 template int tri_inv<float>(bool, bool, int, int, const float *, long, long, float *, long, long,
                             float *, hipStream_t);
 
+// In-place transpose of nblk contiguous S x S blocks (S a multiple of 32):
+// each workgroup swaps one pair of 32 x 32 tiles (i, j) and (j, i), i <= j,
+// through LDS.  The transposed solve keeps its diagonal-block inverses
+// transposed, so that applying them reads rows.
+template <typename T>
+__global__ __launch_bounds__(256) void blk_transpose_kernel(int S, T *__restrict__ G) {
+  const int ti = blockIdx.x, tj = blockIdx.y;
+  if (ti > tj) return;
+  __shared__ T a[32][33], b[32][33];
+  T *B = G + (size_t)blockIdx.z * S * S;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    a[r][tx] = B[(long)(ti * 32 + r) * S + tj * 32 + tx];
+    b[r][tx] = B[(long)(tj * 32 + r) * S + ti * 32 + tx];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    B[(long)(ti * 32 + r) * S + tj * 32 + tx] = b[tx][r];
+    B[(long)(tj * 32 + r) * S + ti * 32 + tx] = a[tx][r];
+  }
+}
+
+template <typename T>
+int blk_transpose(int S, int nblk, T *G, hipStream_t s) {
+  if (S <= 0 || nblk <= 0) return CLU_OK;
+  if (S % 32 != 0 || nblk > 65535) return -1;
+  hipLaunchKernelGGL((blk_transpose_kernel<T>), dim3(S / 32, S / 32, nblk), dim3(256), 0, s, S,
+                     G);
+  CLU_LAUNCHED("blk_transpose_kernel");
+  return CLU_OK;
+}
+template int blk_transpose<double>(int, int, double *, hipStream_t);
+template int blk_transpose<float>(int, int, float *, hipStream_t);
+
 // y = op(G) x for one m x m diagonal-block inverse G (m <= 256, ld ldg).
 // No transpose: one wave per row (all of a lane's loads issued before the
 // shuffle reduction); transpose: one lane per output, rows split over the 4
@@ -490,21 +524,64 @@ __global__ __launch_bounds__(256) void gemv_sub_nv_kernel(int M, int K, const T 
   if (row < M && sub == 0) y[row] = (T)((double)y[row] - s);
 }
 
-// y[j] -= sum_k A[k][j] x[k]; one lane per column j, K split over 4 waves
-template <typename T>
-__global__ __launch_bounds__(256) void gemv_sub_t_kernel(int M, int K, const T *__restrict__ A,
-                                                         long lda, const T *__restrict__ x,
-                                                         T *__restrict__ y) {
-  __shared__ double part[4][64];
+// y[j] -= sum_k A[k][j] x[k] (j < M, k < K).  128 columns per workgroup,
+// two per lane (one 2-element load per row); the 16 waves take every 16th
+// row, 8 rows in flight per lane, and their partial sums are added in a
+// fixed order through LDS.  (One lane per column with one 4-byte load in
+// flight read a 1024-row strip of FP32 factors at about 1 TB/s.)
+template <typename T, bool VEC>
+__global__ __launch_bounds__(1024) void gemv_sub_t_kernel(int M, int K, const T *__restrict__ A,
+                                                          long lda, const T *__restrict__ x,
+                                                          T *__restrict__ y) {
+  typedef T v2_t __attribute__((ext_vector_type(2)));
+  __shared__ double part[16][2][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = blockIdx.x * 64 + lane;
-  double s = 0.0;
-  if (j < M)
-    for (int k = wave; k < K; k += 4) s += (double)A[(long)k * lda + j] * (double)x[k];
-  part[wave][lane] = s;
+  const int c0 = blockIdx.x * 128 + 2 * lane;
+  double s0[8], s1[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) s0[u] = s1[u] = 0.0;
+  if (c0 < M) {
+    int k = wave;
+    if (VEC) {
+      for (; k + 112 < K; k += 128) {
+        v2_t a[8];
+        T xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          a[u] = *reinterpret_cast<const v2_t *>(A + (long)(k + 16 * u) * lda + c0);
+          xv[u] = x[k + 16 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          s0[u] += (double)a[u][0] * (double)xv[u];
+          s1[u] += (double)a[u][1] * (double)xv[u];
+        }
+      }
+      for (; k < K; k += 16) {
+        const v2_t a0 = *reinterpret_cast<const v2_t *>(A + (long)k * lda + c0);
+        s0[0] += (double)a0[0] * (double)x[k];
+        s1[0] += (double)a0[1] * (double)x[k];
+      }
+    } else {
+      for (; k < K; k += 16) {
+        const double xk = (double)x[k];
+        s0[0] += (double)A[(long)k * lda + c0] * xk;
+        if (c0 + 1 < M) s1[0] += (double)A[(long)k * lda + c0 + 1] * xk;
+      }
+    }
+  }
+  part[wave][0][lane] = ((s0[0] + s0[1]) + (s0[2] + s0[3])) + ((s0[4] + s0[5]) + (s0[6] + s0[7]));
+  part[wave][1][lane] = ((s1[0] + s1[1]) + (s1[2] + s1[3])) + ((s1[4] + s1[5]) + (s1[6] + s1[7]));
   __syncthreads();
-  if (wave == 0 && j < M)
-    y[j] = (T)((double)y[j] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])));
+  if (wave < 2) {
+    const int c = c0 + wave;
+    if (c < M) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < 16; ++w) t += part[w][wave][lane];
+      y[c] = (T)((double)y[c] - t);
+    }
+  }
 }
 
 template <typename T>
@@ -519,9 +596,12 @@ int gemv_sub(bool trans, int M, int K, const T *A, long lda, const T *x, T *y, h
   } else if (!trans) {
     hipLaunchKernelGGL((gemv_sub_n_kernel<T>), dim3((M + 15) / 16), dim3(256), 0, s, M, K, A,
                        lda, x, y);
+  } else if (lda % 2 == 0 && M % 2 == 0 && ((uintptr_t)A % (2 * sizeof(T))) == 0) {
+    hipLaunchKernelGGL((gemv_sub_t_kernel<T, true>), dim3((M + 127) / 128), dim3(1024), 0, s, M,
+                       K, A, lda, x, y);
   } else {
-    hipLaunchKernelGGL((gemv_sub_t_kernel<T>), dim3((M + 63) / 64), dim3(256), 0, s, M, K, A,
-                       lda, x, y);
+    hipLaunchKernelGGL((gemv_sub_t_kernel<T, false>), dim3((M + 127) / 128), dim3(1024), 0, s, M,
+                       K, A, lda, x, y);
   }
   CLU_LAUNCHED("gemv_sub_kernel");
   return CLU_OK;

@@ -256,3 +256,27 @@ def test_gesv_ir_transposed_multi_rhs_vs_oracle(dev, orc):
         print(f"trans={trans}: {st}, vs oracle {err:.2e}")
         assert err < 1e-10 and st["residual"] <= 1.0
         assert np.all(X[:, 2].cpu().numpy() == 0.0)
+
+
+@pytest.mark.parametrize("n,block", [(1024, 256), (1088, 64), (999, 37)])
+def test_gesv_ir_transposed_norms(dev, orc, n, block):
+    """The transposed refinement's scaled residuals use max|a_ij|, the
+    largest column sum ||A^T||_inf and max|b| (abs_colsum on the device):
+    their ratio residual / residual_ns = ||A^T||_inf max|x| /
+    (max|a| max|x| + max|b|) is free of the residual's rounding, so it pins
+    those maxima against numpy on the same A, b and returned x (even n, a
+    partial last group of 128 columns, odd n)."""
+    import torch
+    A = dev.gen_matrix(n, seed=41)
+    b = dev.gen_rhs(n, seed=43)
+    S = dev.lag2s(A)
+    ipiv, info = dev.getrf_mixed(S, block, "bf16")
+    assert int(info.item()) == 0
+    x, st = dev.gesv_ir_multi(A, S, ipiv, b, trans=True, tol=1.0)
+    An, bn, xn = A.cpu().numpy(), b.cpu().numpy(), x.cpu().numpy()
+    xmax = np.abs(xn).max()
+    want = np.abs(An).sum(axis=0).max() * xmax / (np.abs(An).max() * xmax + np.abs(bn).max())
+    assert st["residual"] > 0 and st["residual"] <= 1.0
+    assert np.isclose(st["residual"] / st["residual_ns"], want, rtol=1e-12, atol=0)
+    assert np.abs(An.T @ xn - bn).max() <= 16 * n * 2.0 ** -53 * (
+        np.abs(An).max() * xmax + np.abs(bn).max())

@@ -282,6 +282,16 @@ def test_dgemv_trans_and_colmajor(dev):
     assert _lib.load().charmlu_cblas_dgemv(_lib.COL_MAJOR, _lib.NO_TRANS, 23, 37, 1.0, dp(A), 23,
                                            dp(x), 1, 0.0, dp(y2), 1) == 0
     assert np.allclose(y2, A.T @ x, atol=1e-12)
+    # even shapes take the two-columns-per-lane 16-byte path: a partial last
+    # group of 128 columns, rows not a multiple of the 32-row unroll, and a
+    # negative x increment
+    for M, N, incx in [(203, 262, 1), (1100, 128, -1), (64, 2, 1)]:
+        A = rng.standard_normal((M, N)); x = rng.standard_normal(M); y = rng.standard_normal(N)
+        xs = x[::-1] if incx < 0 else x
+        ref = -1.5 * A.T @ xs + 2.0 * y
+        assert _lib.load().charmlu_cblas_dgemv(_lib.ROW_MAJOR, _lib.TRANS, M, N, -1.5, dp(A), N,
+                                               dp(x), incx, 2.0, dp(y), 1) == 0
+        assert np.allclose(y, ref, atol=1e-12), (M, N, incx)
 
 
 # ---------------------------------------------------------------------------
@@ -422,8 +432,8 @@ def test_getrs_transpose_and_multi_rhs(dev, orc):
     assert np.allclose(bt.cpu().numpy(), np.linalg.solve(Ah.T, Bh[:, 0]), rtol=1e-9, atol=1e-9)
 
 
-def test_residual_kernel_matches_oracle(dev, orc):
-    n = 1000
+@pytest.mark.parametrize("n", [1000, 999, 37])
+def test_residual_kernel_matches_oracle(dev, orc, n):
     A = dev.gen_matrix(n, seed=3)
     x = dev.gen_rhs(n, seed=17)
     b = dev.gen_rhs(n, seed=19)
