@@ -305,7 +305,8 @@ template <int BM, int BN, int NST, int OCC, bool CINIT, bool VEC>
 __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kernel(
     int M, int N, int K, float alpha, const uint8_t *__restrict__ A, long lda,
     const uint8_t *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
-    int tiles_m, int tiles_n, const int *__restrict__ escA, const int *__restrict__ escB) {
+    int tiles_m, int tiles_n, const int *__restrict__ escA, const int *__restrict__ escB, int ks,
+    const int *__restrict__ escA1, const int *__restrict__ escB1) {
   constexpr int NT = (BM / 64) * (BN / 64) * 64;
   constexpr int BK = 64;  // K per stage (bytes per row)
   constexpr int WGN = BN / 64;
@@ -332,8 +333,12 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
   const int lr = lane & 31, lh = lane >> 5;
   const bool full = m0 + BM <= M && n0 + BN <= N;
   // E8M0 block scales of the two operands (2^(s - 127) undoes 2^esc); a
-  // wave's 64 rows and 64 columns each lie in one 128-row scale tile
-  const int sa = 127 - escA[(m0 + wm0) >> 7], sb = 127 - escB[(n0 + wn0) >> 7];
+  // wave's 64 rows and 64 columns each lie in one 128-row scale tile.  K
+  // stages from ks on belong to a second product with scales of their own
+  // (escA1, escB1: two updates of C summed in one pass over it)
+  const int sa0 = 127 - escA[(m0 + wm0) >> 7], sb0 = 127 - escB[(n0 + wn0) >> 7];
+  const int sa1 = escA1 ? 127 - escA1[(m0 + wm0) >> 7] : sa0;
+  const int sb1 = escB1 ? 127 - escB1[(n0 + wn0) >> 7] : sb0;
 
   // The matrix-core op's first operand is the B tile (32 C columns), its
   // second the A tile (32 C rows): element r of block (i, j) is C[row][col]
@@ -419,6 +424,7 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
     __asm__ volatile("" ::: "memory");
     if (kc + NST - 1 < nk) issue(kc + NST - 1);
     const char *st = smem + (kc % NST) * STAGE;
+    const int sa = kc < ks ? sa0 : sa1, sb = kc < ks ? sb0 : sb1;
     int8v_t a[2], b[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -470,11 +476,13 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
 template <int BMv, int BNv, int NS, int OC, bool CI, bool V>
 static void fgemm_launch(int M, int N, int K, float alpha, const uint8_t *A, long lda,
                          const uint8_t *B, long ldb, float beta, float *C, long ldc,
-                         hipStream_t s, const int *escA, const int *escB) {
+                         hipStream_t s, const int *escA, const int *escB, int ks,
+                         const int *escA1, const int *escB1) {
   constexpr int NT = (BMv / 64) * (BNv / 64) * 64;
   const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
   hipLaunchKernelGGL((fgemm_nt_kernel<BMv, BNv, NS, OC, CI, V>), dim3(tm * tn), dim3(NT), 0, s, M,
-                     N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB);
+                     N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB, ks, escA1,
+                     escB1);
 }
 
 // FP8 operands with per-tile power-of-two scales; the large accumulate-into-C
@@ -487,29 +495,34 @@ static void fgemm_launch(int M, int N, int K, float alpha, const uint8_t *A, lon
 // CU with or without a start offset between them, measured slower (678-689).
 int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
              const uint8_t *B, long ldb, const int *escB, float beta, float *C, long ldc,
-             hipStream_t s) {
+             hipStream_t s, int kseg, const int *escA1, const int *escB1) {
   if (M <= 0 || N <= 0) return CLU_OK;
   if (K % 64 != 0 || K <= 0 || lda % 16 != 0 || ldb % 16 != 0 || ((uintptr_t)A & 15) ||
       ((uintptr_t)B & 15) || !escA || !escB)
     return -1;
+  // a second scale set from K index kseg on (both or neither)
+  if (kseg % 64 != 0 || kseg < 0 || kseg > K || (escA1 == nullptr) != (escB1 == nullptr) ||
+      (escA1 && kseg == 0))
+    return -1;
+  const int ks = escA1 ? kseg / 64 : K / 64;
   const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;
   const bool cinit = alpha == 1.f && beta == 1.f;
   const bool large = (long)M * N >= (1L << 25);
   if (cinit && vec && K >= 1024 && large)
     fgemm_launch<128, 256, 3, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA,
-                                             escB);
+                                             escB, ks, escA1, escB1);
   else if (cinit && vec)
     fgemm_launch<128, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA,
-                                             escB);
+                                             escB, ks, escA1, escB1);
   else if (cinit)
     fgemm_launch<128, 128, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                              escA, escB);
+                                              escA, escB, ks, escA1, escB1);
   else if (vec)
     fgemm_launch<128, 128, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                              escA, escB);
+                                              escA, escB, ks, escA1, escB1);
   else
     fgemm_launch<128, 128, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                               escA, escB);
+                                               escA, escB, ks, escA1, escB1);
   CLU_LAUNCHED("fgemm_nt_kernel");
   return CLU_OK;
 }
@@ -600,14 +613,15 @@ CLU_DEV int tile_exponent_of(const float *part, int g) {
   return tile_exponent(mx);
 }
 
-// dst (M x Kp bytes) = fp8(sign * src * 2^e_g) for rows 128 g + 16 s .. + 15
-// of 128-row tile g, zero for k >= K; esc[g] = e_g
+// dst (M x Kp bytes, rows ldd apart) = fp8(sign * src * 2^e_g) for rows
+// 128 g + 16 s .. + 15 of 128-row tile g, zero for k >= K; esc[g] = e_g
 __global__ __launch_bounds__(256) void quant_rows_fp8_kernel(int M, int K, int Kp, float sign,
                                                              const float *__restrict__ src,
                                                              long lds, uint8_t *__restrict__ dst,
                                                              const float *__restrict__ part,
                                                              int *__restrict__ esc, int vec,
-                                                             float *__restrict__ cpy, long ldc) {
+                                                             float *__restrict__ cpy, long ldc,
+                                                             long ldd) {
   const int g = blockIdx.x, sl = blockIdx.y, t = threadIdx.x;
   const int r0 = g * 128 + sl * (128 / QSPLIT);
   const int rows = max(0, min(128 / QSPLIT, M - r0));
@@ -630,21 +644,22 @@ __global__ __launch_bounds__(256) void quant_rows_fp8_kernel(int M, int K, int K
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = k + q < K ? sign * ldexpf(v[q], ex) : 0.f;
-      *reinterpret_cast<unsigned *>(dst + (long)(r0 + i) * Kp + k) =
+      *reinterpret_cast<unsigned *>(dst + (long)(r0 + i) * ldd + k) =
           quant4(v[0], v[1], v[2], v[3]);
     }
   }
 }
 
-// dst (N x Kp bytes) = fp8(src^T * 2^e_g) for the columns of 128-column tile
-// g of src (K x N), zero for k >= K; esc[g] = e_g.  Workgroup (g, j)
+// dst (N x Kp bytes, rows ldd apart) = fp8(src^T * 2^e_g) for the columns of
+// 128-column tile g of src (K x N), zero for k >= K; esc[g] = e_g.  Workgroup (g, j)
 // transposes the 64-row slab j through LDS.
 __global__ __launch_bounds__(256) void quant_cols_fp8_kernel(int K, int N, int Kp,
                                                              const float *__restrict__ src,
                                                              long lds, uint8_t *__restrict__ dst,
                                                              const float *__restrict__ part,
                                                              int *__restrict__ esc,
-                                                             float *__restrict__ cpy, long ldc) {
+                                                             float *__restrict__ cpy, long ldc,
+                                                             long ldd) {
   __shared__ float tile[64][129];
   const int g = blockIdx.x, t = threadIdx.x, c0 = g * 128, k0 = blockIdx.y * 64;
   const int cols = min(128, N - c0);
@@ -667,16 +682,17 @@ __global__ __launch_bounds__(256) void quant_cols_fp8_kernel(int K, int N, int K
       w[q] = quant4(ldexpf(tile[kk][c], ex), ldexpf(tile[kk + 1][c], ex),
                     ldexpf(tile[kk + 2][c], ex), ldexpf(tile[kk + 3][c], ex));
     }
-    uint4 *dp = reinterpret_cast<uint4 *>(dst + (long)(c0 + c) * Kp + k0 + 32 * h);
+    uint4 *dp = reinterpret_cast<uint4 *>(dst + (long)(c0 + c) * ldd + k0 + 32 * h);
     dp[0] = make_uint4(w[0], w[1], w[2], w[3]);
     dp[1] = make_uint4(w[4], w[5], w[6], w[7]);
   }
 }
 
 int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds, uint8_t *dst,
-                   int *esc, float *part, hipStream_t s, float *cpy, long ldc) {
+                   int *esc, float *part, hipStream_t s, float *cpy, long ldc, long ldd) {
   if (M <= 0) return CLU_OK;
-  if (Kp % 64 != 0 || Kp < K) return -1;
+  if (ldd == 0) ldd = Kp;
+  if (Kp % 64 != 0 || Kp < K || ldd < Kp || ldd % 4 != 0 || ((uintptr_t)dst & 3)) return -1;
   // 16-byte source reads (and copy writes) when both are 16-byte aligned
   const int vec = lds % 4 == 0 && ((uintptr_t)src & 15) == 0 &&
                   (!cpy || (ldc % 4 == 0 && ((uintptr_t)cpy & 15) == 0));
@@ -684,20 +700,21 @@ int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds,
   hipLaunchKernelGGL(absmax_rows_kernel, grid, dim3(256), 0, s, M, K, src, lds, part, vec);
   CLU_LAUNCHED("absmax_rows_kernel");
   hipLaunchKernelGGL(quant_rows_fp8_kernel, grid, dim3(256), 0, s, M, K, Kp, sign, src, lds, dst,
-                     part, esc, vec, cpy, ldc);
+                     part, esc, vec, cpy, ldc, ldd);
   CLU_LAUNCHED("quant_rows_fp8_kernel");
   return CLU_OK;
 }
 int quant_cols_fp8(int K, int N, int Kp, const float *src, long lds, uint8_t *dst, int *esc,
-                   float *part, hipStream_t s, float *cpy, long ldc) {
+                   float *part, hipStream_t s, float *cpy, long ldc, long ldd) {
   if (N <= 0) return CLU_OK;
-  if (Kp % 64 != 0 || Kp < K) return -1;
+  if (ldd == 0) ldd = Kp;
+  if (Kp % 64 != 0 || Kp < K || ldd < Kp || ldd % 16 != 0 || ((uintptr_t)dst & 15)) return -1;
   const int tiles = (N + 127) / 128;
   hipLaunchKernelGGL(absmax_cols_kernel, dim3(tiles, QSPLIT), dim3(256), 0, s, K, N, src, lds,
                      part);
   CLU_LAUNCHED("absmax_cols_kernel");
   hipLaunchKernelGGL(quant_cols_fp8_kernel, dim3(tiles, Kp / 64), dim3(256), 0, s, K, N, Kp, src,
-                     lds, dst, part, esc, cpy, ldc);
+                     lds, dst, part, esc, cpy, ldc, ldd);
   CLU_LAUNCHED("quant_cols_fp8_kernel");
   return CLU_OK;
 }
@@ -707,13 +724,14 @@ template int hgemm_nt<__bf16>(int, int, int, float, const __bf16 *, long, const 
 template int hgemm_nt<_Float16>(int, int, int, float, const _Float16 *, long, const _Float16 *,
                                 long, float, float *, long, hipStream_t);
 
-// dst (M x Kp, row-major) = scale * src (M x K) rounded to 16 bits, zero for
-// k >= K (scale = -1 folds the update's minus sign into the operand)
+// dst (M x Kp, rows ldd apart) = scale * src (M x K) rounded to 16 bits, zero
+// for k >= K (scale = -1 folds the update's minus sign into the operand)
 template <typename T16>
 __global__ __launch_bounds__(256) void pack_rows_kernel(int M, int K, int Kp, float scale,
                                                         const float *__restrict__ src, long lds,
                                                         T16 *__restrict__ dst, int vec,
-                                                        float *__restrict__ cpy, long ldc) {
+                                                        float *__restrict__ cpy, long ldc,
+                                                        long ldd) {
   // 4 consecutive k per thread (one 8-byte store; a 16-byte load when the
   // source rows are 16-byte aligned), 1024 k per workgroup, rows strided
   // over grid rows: no index division per element
@@ -736,17 +754,19 @@ __global__ __launch_bounds__(256) void pack_rows_kernel(int M, int K, int Kp, fl
     union { T16 h[4]; uint2 u; } o;
 #pragma unroll
     for (int q = 0; q < 4; ++q) o.h[q] = Lp<T16>::from(scale * v[q]);
-    *reinterpret_cast<uint2 *>(dst + (long)i * Kp + k0) = o.u;
+    *reinterpret_cast<uint2 *>(dst + (long)i * ldd + k0) = o.u;
   }
 }
 
-// dst (N x Kp, row-major) = src (K x N)^T rounded to 16 bits, zero for k >= K;
-// 32 x 32 tiles through LDS so both sides are read and written along rows
+// dst (N x Kp, rows ldd apart) = src (K x N)^T rounded to 16 bits, zero for
+// k >= K; 32 x 32 tiles through LDS so both sides are read and written along
+// rows
 template <typename T16>
 __global__ __launch_bounds__(256) void pack_cols_kernel(int K, int N, int Kp,
                                                         const float *__restrict__ src, long lds,
                                                         T16 *__restrict__ dst,
-                                                        float *__restrict__ cpy, long ldc) {
+                                                        float *__restrict__ cpy, long ldc,
+                                                        long ldd) {
   __shared__ float tile[32][33];
   const int k0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -760,41 +780,44 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(int K, int N, int Kp,
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
     const int n = n0 + r, k = k0 + tx;
-    if (n < N && k < Kp) dst[(long)n * Kp + k] = Lp<T16>::from(tile[tx][r]);
+    if (n < N && k < Kp) dst[(long)n * ldd + k] = Lp<T16>::from(tile[tx][r]);
   }
 }
 
 template <typename T16>
 int pack_rows(int M, int K, int Kp, float scale, const float *src, long lds, T16 *dst,
-              hipStream_t s, float *cpy, long ldc) {
+              hipStream_t s, float *cpy, long ldc, long ldd) {
   if (M <= 0 || Kp <= 0) return CLU_OK;
-  if (Kp % 4 != 0 || ((uintptr_t)dst & 7)) return -1;
+  if (ldd == 0) ldd = Kp;
+  if (Kp % 4 != 0 || ldd < Kp || ldd % 4 != 0 || ((uintptr_t)dst & 7)) return -1;
   // 16-byte source reads (and copy writes) when both are 16-byte aligned
   const int vec = lds % 4 == 0 && ((uintptr_t)src & 15) == 0 &&
                   (!cpy || (ldc % 4 == 0 && ((uintptr_t)cpy & 15) == 0));
   const dim3 grid((Kp + 1023) / 1024, M < 8192 ? M : 8192);
   hipLaunchKernelGGL((pack_rows_kernel<T16>), grid, dim3(256), 0, s, M, K, Kp, scale, src, lds,
-                     dst, vec, cpy, ldc);
+                     dst, vec, cpy, ldc, ldd);
   CLU_LAUNCHED("pack_rows_kernel");
   return CLU_OK;
 }
 template <typename T16>
 int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s,
-              float *cpy, long ldc) {
+              float *cpy, long ldc, long ldd) {
   if (N <= 0 || Kp <= 0) return CLU_OK;
+  if (ldd == 0) ldd = Kp;
+  if (ldd < Kp) return -1;
   dim3 grid((N + 31) / 32, (Kp + 31) / 32);
   hipLaunchKernelGGL((pack_cols_kernel<T16>), grid, dim3(256), 0, s, K, N, Kp, src, lds, dst, cpy,
-                     ldc);
+                     ldc, ldd);
   CLU_LAUNCHED("pack_cols_kernel");
   return CLU_OK;
 }
 template int pack_rows<__bf16>(int, int, int, float, const float *, long, __bf16 *, hipStream_t,
-                               float *, long);
+                               float *, long, long);
 template int pack_rows<_Float16>(int, int, int, float, const float *, long, _Float16 *,
-                                 hipStream_t, float *, long);
+                                 hipStream_t, float *, long, long);
 template int pack_cols<__bf16>(int, int, int, const float *, long, __bf16 *, hipStream_t, float *,
-                               long);
+                               long, long);
 template int pack_cols<_Float16>(int, int, int, const float *, long, _Float16 *, hipStream_t,
-                                 float *, long);
+                                 float *, long, long);
 
 }  // namespace clu

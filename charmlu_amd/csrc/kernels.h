@@ -38,29 +38,37 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
              float beta, float *C, long ldc, hipStream_t s);
 // C = alpha * A * B^T + beta * C with FP8 e4m3 operands (A: M x K, B:
 // N x K, K a multiple of 64) carrying power-of-two scales per 128-row tile
-// (escA[M/128], escB[N/128]: the stored values are v * 2^esc)
+// (escA[M/128], escB[N/128]: the stored values are v * 2^esc).  With escA1
+// and escB1, K indices from kseg on (a multiple of 64) carry those scales
+// instead: two products of their own scaling summed in one pass over C.
 int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
              const uint8_t *B, long ldb, const int *escB, float beta, float *C, long ldc,
-             hipStream_t s);
+             hipStream_t s, int kseg = 0, const int *escA1 = nullptr,
+             const int *escB1 = nullptr);
 // FP8 quantization with per-tile power-of-two scales: rows of src (M x K)
 // in 128-row tiles (times sign), or columns of src (K x N) in 128-column
 // tiles stored transposed; dst rows hold Kp bytes (Kp % 64 == 0, zero pad);
 // esc gets one exponent per tile, part (scratch) FP8_PART_PER_TILE floats
 // per tile.  cpy (optional, ld ldc): also receives src itself, in src's
-// layout (the pass that reads src for the operand stores the factor there)
+// layout (the pass that reads src for the operand stores the factor there).
+// ldd: distance between dst rows (0: Kp; larger: dst is one K segment of a
+// wider operand).
 constexpr int FP8_PART_PER_TILE = 8;
 int quant_rows_fp8(int M, int K, int Kp, float sign, const float *src, long lds, uint8_t *dst,
-                   int *esc, float *part, hipStream_t s, float *cpy = nullptr, long ldc = 0);
+                   int *esc, float *part, hipStream_t s, float *cpy = nullptr, long ldc = 0,
+                   long ldd = 0);
 int quant_cols_fp8(int K, int N, int Kp, const float *src, long lds, uint8_t *dst, int *esc,
-                   float *part, hipStream_t s, float *cpy = nullptr, long ldc = 0);
+                   float *part, hipStream_t s, float *cpy = nullptr, long ldc = 0,
+                   long ldd = 0);
 // dst (M x Kp) = 16-bit src (M x K) padded with zeros; dst (N x Kp) = 16-bit
-// transpose of src (K x N) padded with zeros; cpy as for the FP8 quantizers.
+// transpose of src (K x N) padded with zeros; cpy and ldd as for the FP8
+// quantizers.
 template <typename T16>
 int pack_rows(int M, int K, int Kp, float scale, const float *src, long lds, T16 *dst,
-              hipStream_t s, float *cpy = nullptr, long ldc = 0);
+              hipStream_t s, float *cpy = nullptr, long ldc = 0, long ldd = 0);
 template <typename T16>
 int pack_cols(int K, int N, int Kp, const float *src, long lds, T16 *dst, hipStream_t s,
-              float *cpy = nullptr, long ldc = 0);
+              float *cpy = nullptr, long ldc = 0, long ldd = 0);
 
 // ---- pivoting -------------------------------------------------------------
 // Row move list of a run of sequential swaps ipiv[k1..k2) (global rows):

@@ -469,6 +469,12 @@ static bool split_forced() {
   const char *e = getenv("CHARMLU_FORCE_SPLIT");
   return e && e[0] == '1';
 }
+// the unpivoted low-precision form's paired steps (CHARMLU_PAIRED=0: one
+// trailing update per step, as the other forms)
+static bool pairs_enabled() {
+  const char *e = getenv("CHARMLU_PAIRED");
+  return !(e && e[0] == '0');
+}
 static int forced_split(int M) {
   const int Ma = (M / 2 + 127) / 128 * 128;
   return Ma > M ? M : Ma;
@@ -1411,7 +1417,10 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   // the packed operands (and FP8 scales) alternate between two buffers by
   // step parity (the second part of step t still reads step t's)
   const bool early = lp && nopiv;
-  const size_t nbuf = early ? 2 : 1;
+  // ... and its steps run in pairs that update the rest of the matrix once
+  // (below; CHARMLU_PAIRED=0 keeps one update per step)
+  const bool paired = early && pairs_enabled();
+  const size_t nbuf = paired ? 4 : early ? 2 : 1;
   // K of the low-precision operands, zero-padded (64 bytes per LDS stage row)
   const int Kp = fp8 ? (nb + 63) / 64 * 64 : (nb + 31) / 32 * 32;
   const size_t pk_bytes = (size_t)(fp8 ? 1 : 2) * n * Kp;  // one packed operand
@@ -1545,8 +1554,206 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     fence_streams(ws, su);
     return rc;
   }
+
+  // ---- unpivoted low-precision form in pairs of steps ---------------------
+  // Steps a = 2p and b = 2p + 1 update the rest of the matrix (rows and
+  // columns from (a + 2) nb on) together, K = 2 nb: their packed operands lie
+  // side by side in one buffer per pair (K segment 0: step a, 1: step b), so
+  // the FP32 C is read and written once per two steps (the update's C
+  // traffic is as large as its K loop's time at K = nb).  Step a updates only
+  // what step b needs: block column b and block row b (K = nb).  The pair's
+  // update does block columns a + 2 and a + 3 and block rows a + 2 and a + 3
+  // first (the next pair's panels and U rows need them); the rest of it runs
+  // beside the next pair's two panels on the update stream's CUs and after
+  // them on every CU, split by rows, and every element of it is updated by
+  // the same launch shape wherever the split falls.
+  const long ld2 = 2L * Kp;
+  const int T = n / 128 + 2;
+  auto PA = [&](int p) { return (char *)ws.pa16 + (size_t)(p & 1) * 2 * pk_bytes; };
+  auto PU = [&](int p) { return (char *)ws.pu16 + (size_t)(p & 1) * 2 * pk_bytes; };
+  auto ESC = [&](int p, int sg) { return (int *)ws.esc + (size_t)(2 * (p & 1) + sg) * esc_ints; };
+  // C (M x N from global row gr, column gc) += segments [s0, s1) of pair p
+  auto upd2 = [&](int p, int s0, int s1, int gr, int gc, int M, int N, hipStream_t st) -> int {
+    if (M <= 0 || N <= 0) return CLU_OK;
+    const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb * (s1 - s0), st);
+    float *C = dA + (long)gr * lda + gc;
+    const size_t ao = (size_t)gr * ld2 + (size_t)s0 * Kp, bo = (size_t)gc * ld2 + (size_t)s0 * Kp;
+    const int K = (s1 - s0) * Kp;
+    int r;
+    if (fp8) {
+      const bool two = s1 - s0 == 2;
+      r = fgemm_nt(M, N, K, 1.0f, (const uint8_t *)PA(p) + ao, ld2, ESC(p, s0) + gr / 128,
+                   (const uint8_t *)PU(p) + bo, ld2, ESC(p, s0) + T + gc / 128, 1.0f, C, lda, st,
+                   two ? Kp : 0, two ? ESC(p, 1) + gr / 128 : nullptr,
+                   two ? ESC(p, 1) + T + gc / 128 : nullptr);
+    } else if (prec == CHARMLU_PREC_BF16) {
+      r = hgemm_nt<__bf16>(M, N, K, 1.0f, (const __bf16 *)PA(p) + ao, ld2,
+                           (const __bf16 *)PU(p) + bo, ld2, 1.0f, C, lda, st);
+    } else {
+      r = hgemm_nt<_Float16>(M, N, K, 1.0f, (const _Float16 *)PA(p) + ao, ld2,
+                             (const _Float16 *)PU(p) + bo, ld2, 1.0f, C, lda, st);
+    }
+    prof_end(pid, st);
+    return r;
+  };
+  // L21 = A21 inv(U11) of step t into l21buf
+  auto l21_of = [&](int t) -> int {
+    const int r0 = t * nb, Nt = n - r0 - nb;
+    float *A11 = dA + (long)r0 * lda + r0;
+    const int pid = prof_begin(PK_GEMM_OTHER, 2.0 * Nt * nb * nb, sm);
+    int e = gemm_upper_right<float>(Nt, nb, A11 + (long)nb * lda, lda, (const float *)ws.uinv32,
+                                    nb, (float *)ws.l21buf, nb, sm);
+    prof_end(pid, sm);
+    return e;
+  };
+  // U row of step t, columns [j0, j1) after its block, packed into K segment
+  // t & 1 of pair t / 2 (global rows and columns; FP8 scales per global
+  // 128-row tile), with j0 == 0 also its L21; both stored into the factor by
+  // the same passes
+  auto u_row2 = [&](int t, int j0, int j1) -> int {
+    const int p = t >> 1, sg = t & 1, r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
+    if (j1 <= j0) return CLU_OK;
+    float *A11 = dA + (long)r0 * lda + r0;
+    const int pid = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
+    int e = gemm_lower_left<float>(nb, j1 - j0, winv_of(t), nb, A11 + nb + j0, lda, ubuf + j0,
+                                   Nt, sm);
+    float *ucp = A11 + nb + j0, *lcp = A11 + (long)nb * lda;
+    const float *l21 = (const float *)ws.l21buf;
+    const size_t ro = (size_t)r1 * ld2 + (size_t)sg * Kp;
+    const size_t co = (size_t)(r1 + j0) * ld2 + (size_t)sg * Kp;
+    if (!e && fp8) {
+      int *escL = ESC(p, sg), *escU = escL + T;
+      float *partL = (float *)(escU + T), *partU = partL + (size_t)FP8_PART_PER_TILE * T;
+      if (j0 == 0)
+        e = quant_rows_fp8(Nt, nb, Kp, -1.0f, l21, nb, (uint8_t *)PA(p) + ro, escL + r1 / 128,
+                           partL + (size_t)FP8_PART_PER_TILE * (r1 / 128), sm, lcp, lda, ld2);
+      if (!e)
+        e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)PU(p) + co,
+                           escU + (r1 + j0) / 128,
+                           partU + (size_t)FP8_PART_PER_TILE * ((r1 + j0) / 128), sm, ucp, lda,
+                           ld2);
+    } else if (!e && prec == CHARMLU_PREC_BF16) {
+      if (j0 == 0)
+        e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, l21, nb, (__bf16 *)PA(p) + ro, sm, lcp, lda,
+                              ld2);
+      if (!e)
+        e = pack_cols<__bf16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (__bf16 *)PU(p) + co, sm, ucp, lda,
+                              ld2);
+    } else if (!e) {
+      if (j0 == 0)
+        e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, l21, nb, (_Float16 *)PA(p) + ro, sm, lcp, lda,
+                                ld2);
+      if (!e)
+        e = pack_cols<_Float16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (_Float16 *)PU(p) + co, sm, ucp,
+                                lda, ld2);
+    }
+    prof_end(pid, sm);
+    return e;
+  };
+  // rows of a K = 2 nb update of N columns that fill the duration of the
+  // panel of step s on the update stream's CUs (the rate: the paired
+  // update's inside the LU, as measured: FP8 1.13 PFLOP/s at block 2048 and
+  // 0.93 at 1024, BF16 0.63)
+  const double pair_rate = fp8 ? (nb >= 2048 ? 1130e9 : 930e9) : 630e9;
+  auto fill_rows = [&](int s, int M, int N) -> int {
+    if (M <= 0) return 0;
+    if (!split) return M;
+    if (force_split) return forced_split(M);
+    double p_ms = s < (int)ws.histmx.ms.size() ? ws.histmx.ms[s] : -1.0;
+    if (p_ms <= 0.0) p_ms = nb / 1024.0 * 3.5;
+    const double rate = pair_rate * (ws.ncu - ws.panel_cus) / ws.ncu;
+    const double rows = 1.1 * p_ms * rate / (4.0 * N * nb);
+    if (rows >= M) return M;
+    const int r = ((int)rows + 127) / 128 * 128;
+    return r > M ? M : r;
+  };
+  auto paired_steps = [&]() -> int {
+    int rc;
+    if ((rc = panel(0))) return rc;
+    CLU_HIP(hipEventRecord(ws.ev_panel, sp));
+    // the rest of pair yp's update: rows and columns [y0, n), ydone rows of
+    // it enqueued; y_full: its last rows run on the all-CU stream (ev_full)
+    int yp = -1, y0 = n, ydone = 0;
+    bool y_full = false;
+    auto y_part = [&](int rows, hipStream_t st) -> int {
+      const int M = n - y0 - ydone;
+      rows = rows < M ? rows : M;
+      if (rows <= 0) return CLU_OK;
+      const int e = upd2(yp, 0, 2, y0 + ydone, y0, rows, n - y0, st);
+      ydone += rows;
+      return e;
+    };
+    // the panel of step s, after what ev_la marks
+    auto panel_step = [&](int s) -> int {
+      CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
+      CLU_HIP(hipEventRecord(ws.histmx.ev[2 * s], sp));
+      const int e = panel(s);
+      if (e) return e;
+      CLU_HIP(hipEventRecord(ws.histmx.ev[2 * s + 1], sp));
+      CLU_HIP(hipEventRecord(ws.ev_panel, sp));
+      return CLU_OK;
+    };
+    for (int a = 0; a < NB; a += 2) {
+      const int p = a / 2, b = a + 1;
+      const int r1a = (a + 1) * nb, Nta = n - r1a;
+      CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));  // panel(a)
+      if (Nta <= 0) break;
+      // step a: L21, the U row of block column b, block column b (K = nb)
+      if ((rc = l21_of(a))) return rc;
+      if ((rc = u_row2(a, 0, nb))) return rc;
+      if ((rc = upd2(p, 0, 1, r1a, r1a, Nta, nb, sm))) return rc;
+      CLU_HIP(hipEventRecord(ws.ev_la, sm));
+      if ((rc = panel_step(b))) return rc;
+      // the rest of step a's U row and block row b, then rows of the previous
+      // pair's rest beside panel(b)
+      if ((rc = u_row2(a, nb, Nta))) return rc;
+      if ((rc = upd2(p, 0, 1, r1a, r1a + nb, nb, Nta - nb, sm))) return rc;
+      if (yp >= 0 && (rc = y_part(fill_rows(b, n - y0 - ydone, n - y0), sm))) return rc;
+      CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));  // panel(b)
+      // the previous pair's remaining rows on every CU (no panel runs until
+      // the next update waits for them), beside step b's L21 and U row
+      if (yp >= 0 && ydone < n - y0) {
+        hipStream_t sf = ws.s_full;
+        CLU_HIP(hipStreamWaitEvent(sf, ws.ev_panel, 0));
+        if ((rc = y_part(n - y0 - ydone, sf))) return rc;
+        CLU_HIP(hipEventRecord(ws.ev_full, sf));
+        y_full = true;
+      }
+      const int r1b = r1a + nb, Ntb = n - r1b;
+      if (Ntb <= 0) continue;
+      // step b: L21 and the U row of block column a + 2, then the pair's
+      // update of block column a + 2 (K = 2 nb), which the next panel needs
+      if ((rc = l21_of(b))) return rc;
+      if ((rc = u_row2(b, 0, nb))) return rc;
+      if (y_full) {
+        CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+        y_full = false;
+      }
+      if ((rc = upd2(p, 0, 2, r1b, r1b, Ntb, nb, sm))) return rc;
+      CLU_HIP(hipEventRecord(ws.ev_la, sm));
+      if (b + 1 < NB && (rc = panel_step(b + 1))) return rc;
+      if ((rc = u_row2(b, nb, Ntb))) return rc;
+      // block column a + 3, then block rows a + 2 and a + 3 to the right
+      const int w2 = Ntb - nb < nb ? Ntb - nb : nb, h2 = nb + w2;
+      if ((rc = upd2(p, 0, 2, r1b, r1b + nb, Ntb, w2, sm))) return rc;
+      if ((rc = upd2(p, 0, 2, r1b, r1b + h2, h2, Ntb - h2, sm))) return rc;
+      // the rest, from its first rows beside panel(a + 2)
+      yp = p;
+      y0 = r1b + h2;
+      ydone = 0;
+      if (y0 < n && (rc = y_part(fill_rows(b + 1, n - y0, n - y0), sm))) return rc;
+    }
+    if (y_full) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+    if (yp >= 0 && ydone < n - y0 && (rc = y_part(n - y0 - ydone, sm))) return rc;
+    if ((rc = finish_info(dinfo, ws.abortw, sm))) return rc;
+    CLU_HIP(hipEventRecord(ws.ev_end, sm));
+    CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
+    return CLU_OK;
+  };
+
   rc = [&]() -> int {  // the steps; an error returns from here to the fence below
   int rc;
+  if (paired) return paired_steps();
   if ((rc = panel(0))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_panel, sp));
   for (int t = 0; t < NB; ++t) {

@@ -454,17 +454,21 @@ def test_fp32_first_strip_matches_restatement(dev, orc, n):
     assert np.array_equal(G, S), err
 
 
-@pytest.mark.parametrize("prec,pivot", [("bf16", True), ("fp8", True), ("fp32", True),
-                                        ("bf16", False), ("fp8", False)])
-def test_forced_split_is_bitwise_unsplit_mixed(dev, orc, prec, pivot):
+@pytest.mark.parametrize("prec,pivot,n,nb", [("bf16", True, 2048, 512), ("fp8", True, 2048, 512),
+                                             ("fp32", True, 2048, 512),
+                                             ("bf16", False, 2048, 512),
+                                             ("fp8", False, 2048, 512),
+                                             ("bf16", False, 4096, 256),
+                                             ("fp8", False, 3840, 256)])
+def test_forced_split_is_bitwise_unsplit_mixed(dev, orc, prec, pivot, n, nb):
     """The trailing update's split (rows beside the next panel, the rest on
     every CU after it) computes every tile as the unsplit launch does: with
     the split forced to half the rows at every step, the factors and pivots
     are bit-identical (the FP8 half starts at a multiple of 128 rows, so its
-    per-tile scales line up)."""
+    per-tile scales line up).  The unpivoted cases of 15 and 16 steps split
+    the rest of each pair's update three ways."""
     import os
     import torch
-    n, nb = 2048, 512
     A = (orc.gen_uniform if pivot else orc.gen_diagdom)(orc.SEED_A + 5, n)
     runs = []
     for forced in (False, True):
@@ -480,3 +484,45 @@ def test_forced_split_is_bitwise_unsplit_mixed(dev, orc, prec, pivot):
         runs.append((A32.cpu().numpy(), ipiv.cpu().numpy()))
     assert np.array_equal(runs[0][1], runs[1][1])
     assert np.array_equal(runs[0][0], runs[1][0])
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp8"])
+@pytest.mark.parametrize("n,nb", [(4096, 256), (2560, 512), (3072, 1024), (1536, 512)])
+def test_paired_steps_match_per_step_form(dev, orc, prec, n, nb):
+    """The unpivoted low-precision form updates the rest of the matrix once
+    per two steps (K = 2 nb, the two steps' packed operands side by side,
+    FP8 with a scale set per step); CHARMLU_PAIRED=0 updates it once per
+    step.  Both factor within the update's roundoff bound of the FP64
+    matrix, agree to a few units of it, and refine to the FP64 oracle's x
+    (even and odd step counts, 3 to 16 steps)."""
+    import os
+    import torch
+    A = orc.gen_diagdom(orc.SEED_A + 7, n)
+    b = orc.gen_rhs(n)
+    dA = torch.from_numpy(A).cuda()
+    unit = 2.0 ** -4 if prec == "fp8" else UNIT[prec]
+    out = {}
+    for paired in (True, False):
+        A32 = dA.float()
+        if not paired:
+            os.environ["CHARMLU_PAIRED"] = "0"
+        try:
+            print(f"{prec} n={n} nb={nb} paired={paired}", flush=True)
+            ipiv, info = dev.getrf_mixed(A32, nb, prec, pivot=False)
+            torch.cuda.synchronize()
+        finally:
+            os.environ.pop("CHARMLU_PAIRED", None)
+        assert int(info.item()) == 0
+        LU = A32.cpu().numpy()
+        be = _backward_error(A, LU, np.arange(n))
+        assert be < 8 * unit * (n // nb + 1) * max(1, nb // 512), (paired, be)
+        out[paired] = (LU, ipiv, A32, be)
+    diff = np.abs(out[True][0] - out[False][0]).max() / np.abs(out[False][0]).max()
+    assert diff < 64 * unit, diff
+    x_ref = orc.lu_solve_problem(n, nb, A=A.copy())["x"]
+    _, ipiv, A32, be = out[True]
+    x, st = dev.gesv_ir(dA, A32, ipiv, torch.from_numpy(b).cuda(), tol=1.0)
+    err = np.abs(x.cpu().numpy() - x_ref).max() / np.abs(x_ref).max()
+    print(f"paired {prec} n={n} nb={nb}: be {be:.2e} (per step {out[False][3]:.2e}), "
+          f"diff {diff:.2e}, {st}, err {err:.2e}")
+    assert st["residual"] <= 1.0 and err < 1e-10
