@@ -1683,7 +1683,9 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       ydone += rows;
       return e;
     };
-    // the panel of step s, after what ev_la marks
+    // the panel of step s, after what ev_la marks (the work that runs beside
+    // it is enqueued before its launches, so that the host's enqueueing of
+    // the panel's many small launches does not hold it back)
     auto panel_step = [&](int s) -> int {
       CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
       CLU_HIP(hipEventRecord(ws.histmx.ev[2 * s], sp));
@@ -1703,12 +1705,12 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if ((rc = u_row2(a, 0, nb))) return rc;
       if ((rc = upd2(p, 0, 1, r1a, r1a, Nta, nb, sm))) return rc;
       CLU_HIP(hipEventRecord(ws.ev_la, sm));
-      if ((rc = panel_step(b))) return rc;
-      // the rest of step a's U row and block row b, then rows of the previous
-      // pair's rest beside panel(b)
+      // beside panel(b): the rest of step a's U row, block row b, then rows
+      // of the previous pair's rest
       if ((rc = u_row2(a, nb, Nta))) return rc;
       if ((rc = upd2(p, 0, 1, r1a, r1a + nb, nb, Nta - nb, sm))) return rc;
       if (yp >= 0 && (rc = y_part(fill_rows(b, n - y0 - ydone, n - y0), sm))) return rc;
+      if ((rc = panel_step(b))) return rc;
       CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));  // panel(b)
       // the previous pair's remaining rows on every CU (no panel runs until
       // the next update waits for them), beside step b's L21 and U row
@@ -1731,17 +1733,18 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       }
       if ((rc = upd2(p, 0, 2, r1b, r1b, Ntb, nb, sm))) return rc;
       CLU_HIP(hipEventRecord(ws.ev_la, sm));
-      if (b + 1 < NB && (rc = panel_step(b + 1))) return rc;
+      // beside panel(a + 2): the rest of step b's U row, block column a + 3,
+      // block rows a + 2 and a + 3 to the right, then the first rows of the
+      // rest
       if ((rc = u_row2(b, nb, Ntb))) return rc;
-      // block column a + 3, then block rows a + 2 and a + 3 to the right
       const int w2 = Ntb - nb < nb ? Ntb - nb : nb, h2 = nb + w2;
       if ((rc = upd2(p, 0, 2, r1b, r1b + nb, Ntb, w2, sm))) return rc;
       if ((rc = upd2(p, 0, 2, r1b, r1b + h2, h2, Ntb - h2, sm))) return rc;
-      // the rest, from its first rows beside panel(a + 2)
       yp = p;
       y0 = r1b + h2;
       ydone = 0;
       if (y0 < n && (rc = y_part(fill_rows(b + 1, n - y0, n - y0), sm))) return rc;
+      if (b + 1 < NB && (rc = panel_step(b + 1))) return rc;
     }
     if (y_full) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
     if (yp >= 0 && ydone < n - y0 && (rc = y_part(n - y0 - ydone, sm))) return rc;
