@@ -306,7 +306,7 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
     int M, int N, int K, float alpha, const uint8_t *__restrict__ A, long lda,
     const uint8_t *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
     int tiles_m, int tiles_n, const int *__restrict__ escA, const int *__restrict__ escB, int ks,
-    const int *__restrict__ escA1, const int *__restrict__ escB1) {
+    int nseg, long estride) {
   constexpr int NT = (BM / 64) * (BN / 64) * 64;
   constexpr int BK = 64;  // K per stage (bytes per row)
   constexpr int WGN = BN / 64;
@@ -334,11 +334,17 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
   const bool full = m0 + BM <= M && n0 + BN <= N;
   // E8M0 block scales of the two operands (2^(s - 127) undoes 2^esc); a
   // wave's 64 rows and 64 columns each lie in one 128-row scale tile.  K
-  // stages from ks on belong to a second product with scales of their own
-  // (escA1, escB1: two updates of C summed in one pass over it)
-  const int sa0 = 127 - escA[(m0 + wm0) >> 7], sb0 = 127 - escB[(n0 + wn0) >> 7];
-  const int sa1 = escA1 ? 127 - escA1[(m0 + wm0) >> 7] : sa0;
-  const int sb1 = escB1 ? 127 - escB1[(n0 + wn0) >> 7] : sb0;
+  // runs over nseg segments of ks stages, products of their own scaling (up
+  // to 4 updates of C summed in one pass over it), whose exponents lie
+  // estride apart
+  int sav[4], sbv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const long o = (q < nseg ? q : 0) * estride;
+    sav[q] = 127 - escA[o + ((m0 + wm0) >> 7)];
+    sbv[q] = 127 - escB[o + ((n0 + wn0) >> 7)];
+  }
+  int seg = 0, seg_end = ks;
 
   // The matrix-core op's first operand is the B tile (32 C columns), its
   // second the A tile (32 C rows): element r of block (i, j) is C[row][col]
@@ -424,7 +430,12 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
     __asm__ volatile("" ::: "memory");
     if (kc + NST - 1 < nk) issue(kc + NST - 1);
     const char *st = smem + (kc % NST) * STAGE;
-    const int sa = kc < ks ? sa0 : sa1, sb = kc < ks ? sb0 : sb1;
+    if (kc == seg_end) {
+      ++seg;
+      seg_end += ks;
+    }
+    const int sa = seg == 0 ? sav[0] : seg == 1 ? sav[1] : seg == 2 ? sav[2] : sav[3];
+    const int sb = seg == 0 ? sbv[0] : seg == 1 ? sbv[1] : seg == 2 ? sbv[2] : sbv[3];
     int8v_t a[2], b[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -476,13 +487,13 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
 template <int BMv, int BNv, int NS, int OC, bool CI, bool V>
 static void fgemm_launch(int M, int N, int K, float alpha, const uint8_t *A, long lda,
                          const uint8_t *B, long ldb, float beta, float *C, long ldc,
-                         hipStream_t s, const int *escA, const int *escB, int ks,
-                         const int *escA1, const int *escB1) {
+                         hipStream_t s, const int *escA, const int *escB, int ks, int nseg,
+                         long estride) {
   constexpr int NT = (BMv / 64) * (BNv / 64) * 64;
   const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
   hipLaunchKernelGGL((fgemm_nt_kernel<BMv, BNv, NS, OC, CI, V>), dim3(tm * tn), dim3(NT), 0, s, M,
-                     N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB, ks, escA1,
-                     escB1);
+                     N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB, ks, nseg,
+                     estride);
 }
 
 // FP8 operands with per-tile power-of-two scales; the large accumulate-into-C
@@ -495,34 +506,33 @@ static void fgemm_launch(int M, int N, int K, float alpha, const uint8_t *A, lon
 // CU with or without a start offset between them, measured slower (678-689).
 int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
              const uint8_t *B, long ldb, const int *escB, float beta, float *C, long ldc,
-             hipStream_t s, int kseg, const int *escA1, const int *escB1) {
+             hipStream_t s, int kseg, int nseg, long estride) {
   if (M <= 0 || N <= 0) return CLU_OK;
   if (K % 64 != 0 || K <= 0 || lda % 16 != 0 || ldb % 16 != 0 || ((uintptr_t)A & 15) ||
       ((uintptr_t)B & 15) || !escA || !escB)
     return -1;
-  // a second scale set from K index kseg on (both or neither)
-  if (kseg % 64 != 0 || kseg < 0 || kseg > K || (escA1 == nullptr) != (escB1 == nullptr) ||
-      (escA1 && kseg == 0))
+  // nseg segments of kseg K indices each, with their own scales
+  if (nseg < 1 || nseg > 4 || (nseg > 1 && (kseg <= 0 || kseg % 64 != 0 || kseg * nseg != K)))
     return -1;
-  const int ks = escA1 ? kseg / 64 : K / 64;
+  const int ks = nseg > 1 ? kseg / 64 : K / 64;
   const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;
   const bool cinit = alpha == 1.f && beta == 1.f;
   const bool large = (long)M * N >= (1L << 25);
   if (cinit && vec && K >= 1024 && large)
     fgemm_launch<128, 256, 3, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA,
-                                             escB, ks, escA1, escB1);
+                                             escB, ks, nseg, estride);
   else if (cinit && vec)
     fgemm_launch<128, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA,
-                                             escB, ks, escA1, escB1);
+                                             escB, ks, nseg, estride);
   else if (cinit)
     fgemm_launch<128, 128, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                              escA, escB, ks, escA1, escB1);
+                                              escA, escB, ks, nseg, estride);
   else if (vec)
     fgemm_launch<128, 128, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                              escA, escB, ks, escA1, escB1);
+                                              escA, escB, ks, nseg, estride);
   else
     fgemm_launch<128, 128, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                               escA, escB, ks, escA1, escB1);
+                                               escA, escB, ks, nseg, estride);
   CLU_LAUNCHED("fgemm_nt_kernel");
   return CLU_OK;
 }

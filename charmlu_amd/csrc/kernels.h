@@ -38,13 +38,13 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
              float beta, float *C, long ldc, hipStream_t s);
 // C = alpha * A * B^T + beta * C with FP8 e4m3 operands (A: M x K, B:
 // N x K, K a multiple of 64) carrying power-of-two scales per 128-row tile
-// (escA[M/128], escB[N/128]: the stored values are v * 2^esc).  With escA1
-// and escB1, K indices from kseg on (a multiple of 64) carry those scales
-// instead: two products of their own scaling summed in one pass over C.
+// (escA[M/128], escB[N/128]: the stored values are v * 2^esc).  With nseg
+// > 1 (at most 4), K is nseg segments of kseg indices (a multiple of 64),
+// segment q scaled by escA[q * estride + ...] and escB[q * estride + ...]:
+// up to 4 products of their own scaling summed in one pass over C.
 int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
              const uint8_t *B, long ldb, const int *escB, float beta, float *C, long ldc,
-             hipStream_t s, int kseg = 0, const int *escA1 = nullptr,
-             const int *escB1 = nullptr);
+             hipStream_t s, int kseg = 0, int nseg = 1, long estride = 0);
 // FP8 quantization with per-tile power-of-two scales: rows of src (M x K)
 // in 128-row tiles (times sign), or columns of src (K x N) in 128-column
 // tiles stored transposed; dst rows hold Kp bytes (Kp % 64 == 0, zero pad);

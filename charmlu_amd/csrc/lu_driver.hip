@@ -469,11 +469,13 @@ static bool split_forced() {
   const char *e = getenv("CHARMLU_FORCE_SPLIT");
   return e && e[0] == '1';
 }
-// the unpivoted low-precision form's paired steps (CHARMLU_PAIRED=0: one
-// trailing update per step, as the other forms)
-static bool pairs_enabled() {
-  const char *e = getenv("CHARMLU_PAIRED");
-  return !(e && e[0] == '0');
+// steps per group of the unpivoted low-precision form (CHARMLU_GROUP: 1 for
+// one trailing update per step, as the other forms, up to 4; default
+// group_default)
+static int group_steps(int group_default) {
+  const char *e = getenv("CHARMLU_GROUP");
+  if (!e || e[0] < '1' || e[0] > '4') return group_default;
+  return e[0] - '0';
 }
 static int forced_split(int M) {
   const int Ma = (M / 2 + 127) / 128 * 128;
@@ -1417,10 +1419,13 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   // the packed operands (and FP8 scales) alternate between two buffers by
   // step parity (the second part of step t still reads step t's)
   const bool early = lp && nopiv;
-  // ... and its steps run in pairs that update the rest of the matrix once
-  // (below; CHARMLU_PAIRED=0 keeps one update per step)
-  const bool paired = early && pairs_enabled();
-  const size_t nbuf = paired ? 4 : early ? 2 : 1;
+  // ... and its steps run in groups of G that update the rest of the matrix
+  // once (below).  G = 4 measured best (at n = 65536, block 1024, FP8 / BF16:
+  // 465 / 400, 475 / 409, 479 / 418 TFLOP/s with G = 2, 3, 4; n = 131072,
+  // block 2048, FP8: 790, 804, 817)
+  const int G = early ? group_steps(4) : 1;
+  const bool grouped = G > 1;
+  const size_t nbuf = grouped ? 2 * G : early ? 2 : 1;
   // K of the low-precision operands, zero-padded (64 bytes per LDS stage row)
   const int Kp = fp8 ? (nb + 63) / 64 * 64 : (nb + 31) / 32 * 32;
   const size_t pk_bytes = (size_t)(fp8 ? 1 : 2) * n * Kp;  // one packed operand
@@ -1555,44 +1560,42 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     return rc;
   }
 
-  // ---- unpivoted low-precision form in pairs of steps ---------------------
-  // Steps a = 2p and b = 2p + 1 update the rest of the matrix (rows and
-  // columns from (a + 2) nb on) together, K = 2 nb: their packed operands lie
-  // side by side in one buffer per pair (K segment 0: step a, 1: step b), so
-  // the FP32 C is read and written once per two steps (the update's C
-  // traffic is as large as its K loop's time at K = nb).  Step a updates only
-  // what step b needs: block column b and block row b (K = nb).  The pair's
-  // update does block columns a + 2 and a + 3 and block rows a + 2 and a + 3
-  // first (the next pair's panels and U rows need them); the rest of it runs
-  // beside the next pair's two panels on the update stream's CUs and after
-  // them on every CU, split by rows, and every element of it is updated by
-  // the same launch shape wherever the split falls.
-  const long ld2 = 2L * Kp;
+  // ---- unpivoted low-precision form in groups of steps --------------------
+  // The G steps a = G p .. a + G - 1 of group p update the rest of the matrix
+  // (rows and columns from (a + G) nb on) together, K = G nb: their packed
+  // operands lie side by side in one buffer per group (K segment i: step
+  // a + i), so the FP32 C is read and written once per G steps (the update's
+  // C traffic takes as long as its K loop at K = nb).  Inside the group each
+  // step first brings the next block column and block row up to date with
+  // the group's steps so far (K = (i + 1) nb), which the next panel and U
+  // row need.  The group's update does the next group's G block columns and
+  // block rows first; the rest of it runs beside the next group's panels on
+  // the update stream's CUs and after them on every CU, split by rows, and
+  // every element of it is updated by the same launch shape wherever the
+  // split falls.
+  const long ldg = (long)G * Kp;
   const int T = n / 128 + 2;
-  auto PA = [&](int p) { return (char *)ws.pa16 + (size_t)(p & 1) * 2 * pk_bytes; };
-  auto PU = [&](int p) { return (char *)ws.pu16 + (size_t)(p & 1) * 2 * pk_bytes; };
-  auto ESC = [&](int p, int sg) { return (int *)ws.esc + (size_t)(2 * (p & 1) + sg) * esc_ints; };
-  // C (M x N from global row gr, column gc) += segments [s0, s1) of pair p
-  auto upd2 = [&](int p, int s0, int s1, int gr, int gc, int M, int N, hipStream_t st) -> int {
+  auto PA = [&](int p) { return (char *)ws.pa16 + (size_t)(p & 1) * G * pk_bytes; };
+  auto PU = [&](int p) { return (char *)ws.pu16 + (size_t)(p & 1) * G * pk_bytes; };
+  auto ESC = [&](int p, int sg) { return (int *)ws.esc + (size_t)(G * (p & 1) + sg) * esc_ints; };
+  // C (M x N from global row gr, column gc) += segments [0, ns) of group p
+  auto updg = [&](int p, int ns, int gr, int gc, int M, int N, hipStream_t st) -> int {
     if (M <= 0 || N <= 0) return CLU_OK;
-    const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb * (s1 - s0), st);
+    const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb * ns, st);
     float *C = dA + (long)gr * lda + gc;
-    const size_t ao = (size_t)gr * ld2 + (size_t)s0 * Kp, bo = (size_t)gc * ld2 + (size_t)s0 * Kp;
-    const int K = (s1 - s0) * Kp;
+    const size_t ao = (size_t)gr * ldg, bo = (size_t)gc * ldg;
+    const int K = ns * Kp;
     int r;
-    if (fp8) {
-      const bool two = s1 - s0 == 2;
-      r = fgemm_nt(M, N, K, 1.0f, (const uint8_t *)PA(p) + ao, ld2, ESC(p, s0) + gr / 128,
-                   (const uint8_t *)PU(p) + bo, ld2, ESC(p, s0) + T + gc / 128, 1.0f, C, lda, st,
-                   two ? Kp : 0, two ? ESC(p, 1) + gr / 128 : nullptr,
-                   two ? ESC(p, 1) + T + gc / 128 : nullptr);
-    } else if (prec == CHARMLU_PREC_BF16) {
-      r = hgemm_nt<__bf16>(M, N, K, 1.0f, (const __bf16 *)PA(p) + ao, ld2,
-                           (const __bf16 *)PU(p) + bo, ld2, 1.0f, C, lda, st);
-    } else {
-      r = hgemm_nt<_Float16>(M, N, K, 1.0f, (const _Float16 *)PA(p) + ao, ld2,
-                             (const _Float16 *)PU(p) + bo, ld2, 1.0f, C, lda, st);
-    }
+    if (fp8)
+      r = fgemm_nt(M, N, K, 1.0f, (const uint8_t *)PA(p) + ao, ldg, ESC(p, 0) + gr / 128,
+                   (const uint8_t *)PU(p) + bo, ldg, ESC(p, 0) + T + gc / 128, 1.0f, C, lda, st,
+                   Kp, ns, (long)esc_ints);
+    else if (prec == CHARMLU_PREC_BF16)
+      r = hgemm_nt<__bf16>(M, N, K, 1.0f, (const __bf16 *)PA(p) + ao, ldg,
+                           (const __bf16 *)PU(p) + bo, ldg, 1.0f, C, lda, st);
+    else
+      r = hgemm_nt<_Float16>(M, N, K, 1.0f, (const _Float16 *)PA(p) + ao, ldg,
+                             (const _Float16 *)PU(p) + bo, ldg, 1.0f, C, lda, st);
     prof_end(pid, st);
     return r;
   };
@@ -1607,11 +1610,11 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     return e;
   };
   // U row of step t, columns [j0, j1) after its block, packed into K segment
-  // t & 1 of pair t / 2 (global rows and columns; FP8 scales per global
+  // t % G of group t / G (global rows and columns; FP8 scales per global
   // 128-row tile), with j0 == 0 also its L21; both stored into the factor by
   // the same passes
   auto u_row2 = [&](int t, int j0, int j1) -> int {
-    const int p = t >> 1, sg = t & 1, r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
+    const int p = t / G, sg = t % G, r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
     if (j1 <= j0) return CLU_OK;
     float *A11 = dA + (long)r0 * lda + r0;
     const int pid = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
@@ -1619,67 +1622,72 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
                                    Nt, sm);
     float *ucp = A11 + nb + j0, *lcp = A11 + (long)nb * lda;
     const float *l21 = (const float *)ws.l21buf;
-    const size_t ro = (size_t)r1 * ld2 + (size_t)sg * Kp;
-    const size_t co = (size_t)(r1 + j0) * ld2 + (size_t)sg * Kp;
+    const size_t ro = (size_t)r1 * ldg + (size_t)sg * Kp;
+    const size_t co = (size_t)(r1 + j0) * ldg + (size_t)sg * Kp;
     if (!e && fp8) {
       int *escL = ESC(p, sg), *escU = escL + T;
       float *partL = (float *)(escU + T), *partU = partL + (size_t)FP8_PART_PER_TILE * T;
       if (j0 == 0)
         e = quant_rows_fp8(Nt, nb, Kp, -1.0f, l21, nb, (uint8_t *)PA(p) + ro, escL + r1 / 128,
-                           partL + (size_t)FP8_PART_PER_TILE * (r1 / 128), sm, lcp, lda, ld2);
+                           partL + (size_t)FP8_PART_PER_TILE * (r1 / 128), sm, lcp, lda, ldg);
       if (!e)
         e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)PU(p) + co,
                            escU + (r1 + j0) / 128,
                            partU + (size_t)FP8_PART_PER_TILE * ((r1 + j0) / 128), sm, ucp, lda,
-                           ld2);
+                           ldg);
     } else if (!e && prec == CHARMLU_PREC_BF16) {
       if (j0 == 0)
         e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, l21, nb, (__bf16 *)PA(p) + ro, sm, lcp, lda,
-                              ld2);
+                              ldg);
       if (!e)
         e = pack_cols<__bf16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (__bf16 *)PU(p) + co, sm, ucp, lda,
-                              ld2);
+                              ldg);
     } else if (!e) {
       if (j0 == 0)
         e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, l21, nb, (_Float16 *)PA(p) + ro, sm, lcp, lda,
-                                ld2);
+                                ldg);
       if (!e)
         e = pack_cols<_Float16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (_Float16 *)PU(p) + co, sm, ucp,
-                                lda, ld2);
+                                lda, ldg);
     }
     prof_end(pid, sm);
     return e;
   };
-  // rows of a K = 2 nb update of N columns that fill the duration of the
-  // panel of step s on the update stream's CUs (the rate: the paired
-  // update's inside the LU, as measured: FP8 1.13 PFLOP/s at block 2048 and
-  // 0.93 at 1024, BF16 0.63)
-  const double pair_rate = fp8 ? (nb >= 2048 ? 1130e9 : 930e9) : 630e9;
-  auto fill_rows = [&](int s, int M, int N) -> int {
+  // rows of a K = ns nb update of N columns that fill the duration of the
+  // panel of step s on the update stream's CUs.  The rate at K: 0.8 of the
+  // standalone kernel's, which at 64512^2 spends a fixed 6.9 (FP8) or 6.5
+  // (16-bit) ms on C plus 3.9 or 7.3 ms per 1024 of K (measured inside the
+  // LU: FP8 0.93-0.96 PFLOP/s at K = 2048 and 1.13-1.23 at 4096, BF16 0.64
+  // at 2048)
+  auto fill_rows = [&](int s, int ns, int M, int N) -> int {
     if (M <= 0) return 0;
     if (!split) return M;
     if (force_split) return forced_split(M);
     double p_ms = s < (int)ws.histmx.ms.size() ? ws.histmx.ms[s] : -1.0;
     if (p_ms <= 0.0) p_ms = nb / 1024.0 * 3.5;
-    const double rate = pair_rate * (ws.ncu - ws.panel_cus) / ws.ncu;
-    const double rows = 1.1 * p_ms * rate / (4.0 * N * nb);
+    const double kk = (double)ns * nb / 1024.0;
+    const double k_rate = 0.8 * 2.0 * 64512.0 * 64512.0 * 1024.0 * kk /
+                          (fp8 ? 6.9 + 3.9 * kk : 6.5 + 7.3 * kk);  // flop per ms
+    const double rate = k_rate * (ws.ncu - ws.panel_cus) / ws.ncu;
+    const double rows = 1.1 * p_ms * rate / (2.0 * N * ns * nb);
     if (rows >= M) return M;
     const int r = ((int)rows + 127) / 128 * 128;
     return r > M ? M : r;
   };
-  auto paired_steps = [&]() -> int {
+  auto grouped_steps = [&]() -> int {
     int rc;
     if ((rc = panel(0))) return rc;
     CLU_HIP(hipEventRecord(ws.ev_panel, sp));
-    // the rest of pair yp's update: rows and columns [y0, n), ydone rows of
-    // it enqueued; y_full: its last rows run on the all-CU stream (ev_full)
-    int yp = -1, y0 = n, ydone = 0;
+    // the rest of group yp's update (yg segments): rows and columns [y0, n),
+    // ydone rows of it enqueued; y_full: its last rows run on the all-CU
+    // stream (ev_full)
+    int yp = -1, yg = 0, y0 = n, ydone = 0;
     bool y_full = false;
     auto y_part = [&](int rows, hipStream_t st) -> int {
       const int M = n - y0 - ydone;
       rows = rows < M ? rows : M;
       if (rows <= 0) return CLU_OK;
-      const int e = upd2(yp, 0, 2, y0 + ydone, y0, rows, n - y0, st);
+      const int e = updg(yp, yg, y0 + ydone, y0, rows, n - y0, st);
       ydone += rows;
       return e;
     };
@@ -1695,56 +1703,58 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       CLU_HIP(hipEventRecord(ws.ev_panel, sp));
       return CLU_OK;
     };
-    for (int a = 0; a < NB; a += 2) {
-      const int p = a / 2, b = a + 1;
-      const int r1a = (a + 1) * nb, Nta = n - r1a;
-      CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));  // panel(a)
-      if (Nta <= 0) break;
-      // step a: L21, the U row of block column b, block column b (K = nb)
-      if ((rc = l21_of(a))) return rc;
-      if ((rc = u_row2(a, 0, nb))) return rc;
-      if ((rc = upd2(p, 0, 1, r1a, r1a, Nta, nb, sm))) return rc;
-      CLU_HIP(hipEventRecord(ws.ev_la, sm));
-      // beside panel(b): the rest of step a's U row, block row b, then rows
-      // of the previous pair's rest
-      if ((rc = u_row2(a, nb, Nta))) return rc;
-      if ((rc = upd2(p, 0, 1, r1a, r1a + nb, nb, Nta - nb, sm))) return rc;
-      if (yp >= 0 && (rc = y_part(fill_rows(b, n - y0 - ydone, n - y0), sm))) return rc;
-      if ((rc = panel_step(b))) return rc;
-      CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));  // panel(b)
-      // the previous pair's remaining rows on every CU (no panel runs until
-      // the next update waits for them), beside step b's L21 and U row
-      if (yp >= 0 && ydone < n - y0) {
-        hipStream_t sf = ws.s_full;
-        CLU_HIP(hipStreamWaitEvent(sf, ws.ev_panel, 0));
-        if ((rc = y_part(n - y0 - ydone, sf))) return rc;
-        CLU_HIP(hipEventRecord(ws.ev_full, sf));
-        y_full = true;
+    for (int a = 0; a < NB; a += G) {
+      const int p = a / G, gl = NB - a < G ? NB - a : G;
+      for (int i = 0; i < gl; ++i) {
+        const int s = a + i, r1 = (s + 1) * nb, Nt = n - r1;
+        CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));  // panel(s)
+        if (Nt <= 0) break;
+        // the previous group's remaining rows on every CU once the group's
+        // last panel is done (no panel runs until the next update waits for
+        // them), beside that step's L21 and U row
+        if (i == gl - 1 && yp >= 0 && ydone < n - y0) {
+          hipStream_t sf = ws.s_full;
+          CLU_HIP(hipStreamWaitEvent(sf, ws.ev_panel, 0));
+          if ((rc = y_part(n - y0 - ydone, sf))) return rc;
+          CLU_HIP(hipEventRecord(ws.ev_full, sf));
+          y_full = true;
+        }
+        if ((rc = l21_of(s))) return rc;
+        if ((rc = u_row2(s, 0, nb))) return rc;
+        if (i < gl - 1) {
+          // block column s + 1 with the group's steps a .. s, then beside
+          // panel(s + 1): the rest of the U row, block row s + 1 likewise,
+          // and rows of the previous group's rest
+          if ((rc = updg(p, i + 1, r1, r1, Nt, nb, sm))) return rc;
+          CLU_HIP(hipEventRecord(ws.ev_la, sm));
+          if ((rc = u_row2(s, nb, Nt))) return rc;
+          if ((rc = updg(p, i + 1, r1, r1 + nb, nb, Nt - nb, sm))) return rc;
+          if (yp >= 0 && (rc = y_part(fill_rows(s + 1, yg, n - y0 - ydone, n - y0), sm)))
+            return rc;
+          if ((rc = panel_step(s + 1))) return rc;
+          continue;
+        }
+        // the group's last step: its update of the rest (gl segments), the
+        // next panel's block column first
+        if (y_full) {
+          CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+          y_full = false;
+        }
+        const int W = Nt < G * nb ? Nt : G * nb;  // the next group's blocks
+        if ((rc = updg(p, gl, r1, r1, Nt, nb, sm))) return rc;
+        CLU_HIP(hipEventRecord(ws.ev_la, sm));
+        // beside panel(s + 1): the rest of the U row, the next group's other
+        // block columns and its block rows, then the first rows of the rest
+        if ((rc = u_row2(s, nb, Nt))) return rc;
+        if ((rc = updg(p, gl, r1, r1 + nb, Nt, W - nb, sm))) return rc;
+        if ((rc = updg(p, gl, r1, r1 + W, W, Nt - W, sm))) return rc;
+        yp = p;
+        yg = gl;
+        y0 = r1 + W;
+        ydone = 0;
+        if (y0 < n && (rc = y_part(fill_rows(s + 1, yg, n - y0, n - y0), sm))) return rc;
+        if (s + 1 < NB && (rc = panel_step(s + 1))) return rc;
       }
-      const int r1b = r1a + nb, Ntb = n - r1b;
-      if (Ntb <= 0) continue;
-      // step b: L21 and the U row of block column a + 2, then the pair's
-      // update of block column a + 2 (K = 2 nb), which the next panel needs
-      if ((rc = l21_of(b))) return rc;
-      if ((rc = u_row2(b, 0, nb))) return rc;
-      if (y_full) {
-        CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
-        y_full = false;
-      }
-      if ((rc = upd2(p, 0, 2, r1b, r1b, Ntb, nb, sm))) return rc;
-      CLU_HIP(hipEventRecord(ws.ev_la, sm));
-      // beside panel(a + 2): the rest of step b's U row, block column a + 3,
-      // block rows a + 2 and a + 3 to the right, then the first rows of the
-      // rest
-      if ((rc = u_row2(b, nb, Ntb))) return rc;
-      const int w2 = Ntb - nb < nb ? Ntb - nb : nb, h2 = nb + w2;
-      if ((rc = upd2(p, 0, 2, r1b, r1b + nb, Ntb, w2, sm))) return rc;
-      if ((rc = upd2(p, 0, 2, r1b, r1b + h2, h2, Ntb - h2, sm))) return rc;
-      yp = p;
-      y0 = r1b + h2;
-      ydone = 0;
-      if (y0 < n && (rc = y_part(fill_rows(b + 1, n - y0, n - y0), sm))) return rc;
-      if (b + 1 < NB && (rc = panel_step(b + 1))) return rc;
     }
     if (y_full) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
     if (yp >= 0 && ydone < n - y0 && (rc = y_part(n - y0 - ydone, sm))) return rc;
@@ -1756,7 +1766,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
 
   rc = [&]() -> int {  // the steps; an error returns from here to the fence below
   int rc;
-  if (paired) return paired_steps();
+  if (grouped) return grouped_steps();
   if ((rc = panel(0))) return rc;
   CLU_HIP(hipEventRecord(ws.ev_panel, sp));
   for (int t = 0; t < NB; ++t) {

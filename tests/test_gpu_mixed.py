@@ -486,15 +486,20 @@ def test_forced_split_is_bitwise_unsplit_mixed(dev, orc, prec, pivot, n, nb):
     assert np.array_equal(runs[0][0], runs[1][0])
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp8"])
-@pytest.mark.parametrize("n,nb", [(4096, 256), (2560, 512), (3072, 1024), (1536, 512)])
-def test_paired_steps_match_per_step_form(dev, orc, prec, n, nb):
+GROUP_CASES = [(prec, g, n, nb) for prec in ("bf16", "fp8") for g in (2, 3, 4)
+               for n, nb in ((4096, 256), (2560, 512))] + [
+    ("fp16", 2, 1536, 512), ("fp16", 3, 4096, 256), ("fp8", 2, 3072, 1024),
+    ("fp8", 3, 3072, 1024), ("bf16", 2, 1536, 512)]
+
+
+@pytest.mark.parametrize("prec,g,n,nb", GROUP_CASES)
+def test_grouped_steps_match_per_step_form(dev, orc, prec, g, n, nb):
     """The unpivoted low-precision form updates the rest of the matrix once
-    per two steps (K = 2 nb, the two steps' packed operands side by side,
-    FP8 with a scale set per step); CHARMLU_PAIRED=0 updates it once per
-    step.  Both factor within the update's roundoff bound of the FP64
+    per g steps (K = g nb, the steps' packed operands side by side, FP8 with
+    a scale set per step; CHARMLU_GROUP=g); CHARMLU_GROUP=1 updates it once
+    per step.  Both factor within the update's roundoff bound of the FP64
     matrix, agree to a few units of it, and refine to the FP64 oracle's x
-    (even and odd step counts, 3 to 16 steps)."""
+    (3 to 16 steps, groups cut short at the end)."""
     import os
     import torch
     A = orc.gen_diagdom(orc.SEED_A + 7, n)
@@ -502,27 +507,26 @@ def test_paired_steps_match_per_step_form(dev, orc, prec, n, nb):
     dA = torch.from_numpy(A).cuda()
     unit = 2.0 ** -4 if prec == "fp8" else UNIT[prec]
     out = {}
-    for paired in (True, False):
+    for group in (g, 1):
         A32 = dA.float()
-        if not paired:
-            os.environ["CHARMLU_PAIRED"] = "0"
+        os.environ["CHARMLU_GROUP"] = str(group)
         try:
-            print(f"{prec} n={n} nb={nb} paired={paired}", flush=True)
+            print(f"{prec} n={n} nb={nb} group={group}", flush=True)
             ipiv, info = dev.getrf_mixed(A32, nb, prec, pivot=False)
             torch.cuda.synchronize()
         finally:
-            os.environ.pop("CHARMLU_PAIRED", None)
+            os.environ.pop("CHARMLU_GROUP", None)
         assert int(info.item()) == 0
         LU = A32.cpu().numpy()
         be = _backward_error(A, LU, np.arange(n))
-        assert be < 8 * unit * (n // nb + 1) * max(1, nb // 512), (paired, be)
-        out[paired] = (LU, ipiv, A32, be)
-    diff = np.abs(out[True][0] - out[False][0]).max() / np.abs(out[False][0]).max()
+        assert be < 8 * unit * (n // nb + 1) * max(1, nb // 512), (group, be)
+        out[group] = (LU, ipiv, A32, be)
+    diff = np.abs(out[g][0] - out[1][0]).max() / np.abs(out[1][0]).max()
     assert diff < 64 * unit, diff
     x_ref = orc.lu_solve_problem(n, nb, A=A.copy())["x"]
-    _, ipiv, A32, be = out[True]
+    _, ipiv, A32, be = out[g]
     x, st = dev.gesv_ir(dA, A32, ipiv, torch.from_numpy(b).cuda(), tol=1.0)
     err = np.abs(x.cpu().numpy() - x_ref).max() / np.abs(x_ref).max()
-    print(f"paired {prec} n={n} nb={nb}: be {be:.2e} (per step {out[False][3]:.2e}), "
+    print(f"group {g} {prec} n={n} nb={nb}: be {be:.2e} (per step {out[1][3]:.2e}), "
           f"diff {diff:.2e}, {st}, err {err:.2e}")
     assert st["residual"] <= 1.0 and err < 1e-10
