@@ -331,10 +331,14 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
     A32 = torch.empty((n, n), dtype=torch.float32, device="cuda")
     x = torch.empty_like(b)
     for key in precs:
-        # "<prec>" factors with partial pivoting, "<prec>-nopiv" without
-        # interchanges (the matrix is diagonally dominant)
+        # "<prec>" factors with partial pivoting (low-precision updates:
+        # speculative identity pivots, kept while every panel's multipliers
+        # stay below 1/2), "<prec>-xchg" with partial pivoting exchanging
+        # candidate rows at every panel (CHARMLU_SPECULATE=0), "<prec>-nopiv"
+        # without interchanges (the matrix is diagonally dominant)
         prec, _, mode = key.partition("-")
         pivot = mode != "nopiv"
+        env = {"CHARMLU_SPECULATE": "0"} if mode == "xchg" else {}
         split = [0.0, 0.0]
 
         last = {}
@@ -342,8 +346,17 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
         def step():
             t_a = time.perf_counter()
             clu.lag2s(A, A32)
-            ipiv, info = clu.getrf_mixed(A32, nb, prec, pivot=pivot)
-            torch.cuda.synchronize()
+            saved = {k: os.environ.get(k) for k in env}
+            os.environ.update(env)
+            try:
+                ipiv, info = clu.getrf_mixed(A32, nb, prec, pivot=pivot)
+                torch.cuda.synchronize()
+            finally:
+                for k, v in saved.items():
+                    if v is None:
+                        os.environ.pop(k, None)
+                    else:
+                        os.environ[k] = v
             t_b = time.perf_counter()
             _, st = clu.gesv_ir(A, A32, ipiv, b, x, tol=1.0)
             split[0] += t_b - t_a
@@ -376,15 +389,23 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
                "panel_stream_ms": round(p[8] / steps, 1),
                "interchange_ms": round(p[11] / steps, 1),
                "u_row_ms": round(p[14] / steps, 1)}
+        # partial pivoting picks the diagonal pivots iff every computed
+        # multiplier is below 1 in magnitude
+        mx = 0.0
+        for r in range(0, n, 4096):  # row blocks (one kernel over n^2 entries overflows)
+            mx = max(mx, float(A32[r:r + 4096].tril(r - 1).abs().max().item()))
+        rec["max_abs_l"] = mx
+        identity = bool((last["ipiv"] == torch.arange(n, device="cuda",
+                                                       dtype=last["ipiv"].dtype)).all().item())
         if not pivot:
-            # partial pivoting picks the same (diagonal) pivots iff every
-            # computed multiplier is below 1 in magnitude
-            mx = 0.0
-            for r in range(0, n, 4096):  # row blocks (one kernel over n^2 entries overflows)
-                mx = max(mx, float(A32[r:r + 4096].tril(r - 1).abs().max().item()))
-            rec["max_abs_l"] = mx
             rec["pivoting"] = "none (max |l_ij| < 1: partial pivoting would agree)" \
-                if rec["max_abs_l"] < 1.0 else "none"
+                if mx < 1.0 else "none"
+        elif mode == "xchg":
+            rec["pivoting"] = "partial (candidate rows exchanged at every panel)"
+        elif prec != "fp32" and identity:
+            rec["pivoting"] = ("partial (speculative: every panel eliminated without "
+                               "interchanges had all multipliers below 1/2, so partial "
+                               "pivoting keeps every diagonal row)")
         else:
             rec["pivoting"] = "partial"
         if adjoint:
@@ -453,13 +474,15 @@ def small_case(clu, torch, n, nb, steps=5):
 
 def c5_bench(clu, lib, _lib, torch, nb, fp64_nb):
     """C5: the FP8 form at n = 131072 (FP64 A 137 GB + FP32 factors 69 GB in
-    HBM), unpivoted and pivoted, one step each after no warmup, then one
+    HBM), unpivoted, pivoted (speculative identity pivots) and pivoted with
+    exchanging panels, one step each after no warmup, then one
     FP64 LU + solve of the same system in place for the error and speed
     comparison.  Block 2048 for the FP8 forms (their update reads and writes
     the FP32 trailing matrix once per step, so a wider block halves that
     traffic per flop: 657 / 553 TFLOP/s against 623 / 522 with 1024 on one
     box), 1024 for the FP64 LU."""
-    rec = mixed_bench(clu, lib, _lib, torch, 131072, nb, ["fp8-nopiv", "fp8"], 1, 0, 1, fp64_nb,
+    rec = mixed_bench(clu, lib, _lib, torch, 131072, nb, ["fp8-nopiv", "fp8", "fp8-xchg"], 1, 0,
+                      1, fp64_nb,
                       fp64_after=True, adjoint=True)
     rec["steps_note"] = "one timed step per form, no warmup (the first call of its shape)"
     return rec
@@ -482,7 +505,7 @@ def main():
     ap.add_argument("--mixed-n", type=int, default=65536)
     ap.add_argument("--mixed-nb", type=int, default=1024)
     ap.add_argument("--mixed-fp64-nb", type=int, default=1024)
-    ap.add_argument("--mixed-precs", default="bf16,fp16,bf16-nopiv")
+    ap.add_argument("--mixed-precs", default="bf16,fp16,bf16-xchg,bf16-nopiv")
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--mixed-fp64-steps", type=int, default=2)
     ap.add_argument("--no-mixed", action="store_true")

@@ -191,6 +191,11 @@ int abs_colsum(int n, const double *A, long lda, double *out, hipStream_t s);
 template <typename TD, typename TS>
 int convert(long m, long n, const TS *src, long lds, TD *dst, long ldd, hipStream_t s);
 int fill_int(int *p, int v, long n, hipStream_t s);
+// *flag = 1 unless a panel eliminated without interchanges has every
+// multiplier (strictly lower nb x nb A11, M x nb L21) below thr in magnitude
+// and every |u_jj| above tiny (flag cleared first)
+int spec_check(int M, int nb, const float *L21, long ld21, const float *A11, long lda,
+               float thr, float tiny, int *flag, hipStream_t s);
 // out[4 g + {0..3}] = {XCC id, HW_ID, start, end (100 MHz clock)} of each of
 // `groups` one-per-CU workgroups that sleep `sleeps` x 8128 cycles
 int residency_probe(int groups, int sleeps, unsigned long long *out, hipStream_t s);

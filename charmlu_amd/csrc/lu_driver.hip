@@ -145,6 +145,12 @@ struct Workspace {
   // unpivoted mixed form: inv(U11) and L21 = A21 inv(U11) before it is copied back
   void *uinv32 = nullptr, *l21buf = nullptr;
   size_t uinv32_b = 0, l21buf_b = 0;
+  // speculative identity pivots: the diagonal block before its unpivoted
+  // elimination, the check's flag on the device and in pinned host memory
+  void *specbak = nullptr;
+  size_t specbak_b = 0;
+  int *spec_flag = nullptr, *spec_hflag = nullptr;
+  hipEvent_t ev_chk = nullptr;
   // FP8 update: power-of-two exponents per 128-row tile of L21 and of U12^T
   void *esc = nullptr;
   size_t esc_b = 0;
@@ -269,6 +275,9 @@ static int get_ws(Workspace **out) {
     CLU_HIP(hipEventCreateWithFlags(&w.ev_urow, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_ula, hipEventDisableTiming));
     CLU_HIP(hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming));
+    CLU_HIP(hipEventCreateWithFlags(&w.ev_chk, hipEventDisableTiming));
+    CLU_HIP(hipMalloc(&w.spec_flag, 64));
+    CLU_HIP(hipHostMalloc((void **)&w.spec_hflag, 64, hipHostMallocDefault));
     // panel candidate records: 2 parities x 64 workgroups x 136 words of 8
     // bytes, zeroed once; their sequence numbers only ever grow (pw.epoch),
     // so a record left by an earlier call can never look current
@@ -468,6 +477,12 @@ static int factor_panel(T *A, long lda, int n, int c0, int w, int pc0, int pcw, 
 static bool split_forced() {
   const char *e = getenv("CHARMLU_FORCE_SPLIT");
   return e && e[0] == '1';
+}
+// speculative identity pivots for the pivoted low-precision form
+// (CHARMLU_SPECULATE=0: every panel exchanges candidate rows)
+static bool speculate_enabled() {
+  const char *e = getenv("CHARMLU_SPECULATE");
+  return !(e && e[0] == '0');
 }
 // steps per group of the unpivoted low-precision form (CHARMLU_GROUP: 1 for
 // one trailing update per step, as the other forms, up to 4; default
@@ -757,9 +772,9 @@ static int release_ws(Workspace &w) {
   hipStream_t ss[4] = {w.s_panel, w.s_main, w.s_full, w.s_pgemm};
   for (hipStream_t st : ss)
     if (st) CLU_HIP(hipStreamDestroy(st));
-  hipEvent_t ev[] = {w.ev_pg_in, w.ev_pg_out, w.ev_panel, w.ev_la,  w.ev_start, w.ev_end,
-                     w.ev_urow,  w.ev_ula,    w.ev_full,  w.ev_fence[0], w.ev_fence[1],
-                     w.ev_fence[2]};
+  hipEvent_t ev[] = {w.ev_pg_in, w.ev_pg_out, w.ev_panel,    w.ev_la,       w.ev_start,
+                     w.ev_end,   w.ev_urow,   w.ev_ula,      w.ev_full,     w.ev_fence[0],
+                     w.ev_fence[1], w.ev_fence[2], w.ev_chk};
   for (hipEvent_t e : ev)
     if (e) CLU_HIP(hipEventDestroy(e));
   for (PanelHistory *h : {&w.hist64, &w.histmx})
@@ -767,9 +782,10 @@ static int release_ws(Workspace &w) {
   void *bufs[] = {w.slots,  w.abortw, w.pmoves, w.pcount, w.smoves, w.scount, w.stage,
                   w.winv,   w.ubuf,   w.winv32, w.ubuf32, w.pa16,   w.pu16,   w.uinv32,
                   w.l21buf, w.esc,    w.irbuf,  w.sinv,   w.sy,     w.smv,    w.tinv,
-                  w.pinv,   w.ptmp};
+                  w.pinv,   w.ptmp,   w.specbak, w.spec_flag};
   for (void *b : bufs)
     if (b) CLU_HIP(hipFree(b));
+  if (w.spec_hflag) CLU_HIP(hipHostFree(w.spec_hflag));
   w = Workspace();
   return CLU_OK;
 }
@@ -1423,8 +1439,13 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   // once (below).  G = 4 measured best (at n = 65536, block 1024, FP8 / BF16:
   // 465 / 400, 475 / 409, 479 / 418 TFLOP/s with G = 2, 3, 4; n = 131072,
   // block 2048, FP8: 790, 804, 817)
-  const int G = early ? group_steps(4) : 1;
+  // The pivoted low-precision form runs the grouped unpivoted schedule as
+  // long as its panels show that partial pivoting would keep every diagonal
+  // row (below; its unpivoted panels take nb <= 2048)
+  const bool spec_req = lp && !nopiv && nb <= 2048 && speculate_enabled();
+  const int G = (early || spec_req) ? group_steps(4) : 1;
   const bool grouped = G > 1;
+  const bool spec = grouped && !nopiv;
   const size_t nbuf = grouped ? 2 * G : early ? 2 : 1;
   // K of the low-precision operands, zero-padded (64 bytes per LDS stage row)
   const int Kp = fp8 ? (nb + 63) / 64 * 64 : (nb + 31) / 32 * 32;
@@ -1445,10 +1466,12 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     // per operand: one exponent and FP8_PART_PER_TILE partial maxima per
     // 128-row tile
     if (fp8 && (rc = ensure_bytes(&ws.esc, &ws.esc_b, sizeof(int) * nbuf * esc_ints))) return rc;
-    if (nopiv) {
+    if (nopiv || spec) {
       if ((rc = ensure_bytes(&ws.uinv32, &ws.uinv32_b, sizeof(float) * nb * nb))) return rc;
       if ((rc = ensure_bytes(&ws.l21buf, &ws.l21buf_b, sizeof(float) * nb * n))) return rc;
     }
+    if (spec && (rc = ensure_bytes(&ws.specbak, &ws.specbak_b, sizeof(float) * nb * nb)))
+      return rc;
   }
   float *ubuf = (float *)ws.ubuf32;
   // inv(L11) of step t, by step parity: the panel of step t+1 writes its
@@ -1463,11 +1486,11 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   const int NB = n / nb;
   auto moves_of = [&](int t) { return ws.smoves + (size_t)(t & 1) * nruns * 2 * RUN; };
   auto counts_of = [&](int t) { return ws.scount + (size_t)(t & 1) * nruns; };
-  auto panel = [&](int t) -> int {
+  auto panel = [&](int t, bool unpiv = false) -> int {
     const int r0 = t * nb;
     const int pid = prof_begin(PK_PANEL, (double)(n - r0) * nb, sp);
     int r;
-    if (nopiv) {
+    if (nopiv || unpiv) {
       // without interchanges the panel splits into the LU of the diagonal
       // block (here) and L21 = A21 inv(U11), one GEMM over all rows below
       // that the update stream runs with all its CUs at the start of step t
@@ -1560,216 +1583,11 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     return rc;
   }
 
-  // ---- unpivoted low-precision form in groups of steps --------------------
-  // The G steps a = G p .. a + G - 1 of group p update the rest of the matrix
-  // (rows and columns from (a + G) nb on) together, K = G nb: their packed
-  // operands lie side by side in one buffer per group (K segment i: step
-  // a + i), so the FP32 C is read and written once per G steps (the update's
-  // C traffic takes as long as its K loop at K = nb).  Inside the group each
-  // step first brings the next block column and block row up to date with
-  // the group's steps so far (K = (i + 1) nb), which the next panel and U
-  // row need.  The group's update does the next group's G block columns and
-  // block rows first; the rest of it runs beside the next group's panels on
-  // the update stream's CUs and after them on every CU, split by rows, and
-  // every element of it is updated by the same launch shape wherever the
-  // split falls.
-  const long ldg = (long)G * Kp;
-  const int T = n / 128 + 2;
-  auto PA = [&](int p) { return (char *)ws.pa16 + (size_t)(p & 1) * G * pk_bytes; };
-  auto PU = [&](int p) { return (char *)ws.pu16 + (size_t)(p & 1) * G * pk_bytes; };
-  auto ESC = [&](int p, int sg) { return (int *)ws.esc + (size_t)(G * (p & 1) + sg) * esc_ints; };
-  // C (M x N from global row gr, column gc) += segments [0, ns) of group p
-  auto updg = [&](int p, int ns, int gr, int gc, int M, int N, hipStream_t st) -> int {
-    if (M <= 0 || N <= 0) return CLU_OK;
-    const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb * ns, st);
-    float *C = dA + (long)gr * lda + gc;
-    const size_t ao = (size_t)gr * ldg, bo = (size_t)gc * ldg;
-    const int K = ns * Kp;
-    int r;
-    if (fp8)
-      r = fgemm_nt(M, N, K, 1.0f, (const uint8_t *)PA(p) + ao, ldg, ESC(p, 0) + gr / 128,
-                   (const uint8_t *)PU(p) + bo, ldg, ESC(p, 0) + T + gc / 128, 1.0f, C, lda, st,
-                   Kp, ns, (long)esc_ints);
-    else if (prec == CHARMLU_PREC_BF16)
-      r = hgemm_nt<__bf16>(M, N, K, 1.0f, (const __bf16 *)PA(p) + ao, ldg,
-                           (const __bf16 *)PU(p) + bo, ldg, 1.0f, C, lda, st);
-    else
-      r = hgemm_nt<_Float16>(M, N, K, 1.0f, (const _Float16 *)PA(p) + ao, ldg,
-                             (const _Float16 *)PU(p) + bo, ldg, 1.0f, C, lda, st);
-    prof_end(pid, st);
-    return r;
-  };
-  // L21 = A21 inv(U11) of step t into l21buf
-  auto l21_of = [&](int t) -> int {
-    const int r0 = t * nb, Nt = n - r0 - nb;
-    float *A11 = dA + (long)r0 * lda + r0;
-    const int pid = prof_begin(PK_GEMM_OTHER, 2.0 * Nt * nb * nb, sm);
-    int e = gemm_upper_right<float>(Nt, nb, A11 + (long)nb * lda, lda, (const float *)ws.uinv32,
-                                    nb, (float *)ws.l21buf, nb, sm);
-    prof_end(pid, sm);
-    return e;
-  };
-  // U row of step t, columns [j0, j1) after its block, packed into K segment
-  // t % G of group t / G (global rows and columns; FP8 scales per global
-  // 128-row tile), with j0 == 0 also its L21; both stored into the factor by
-  // the same passes
-  auto u_row2 = [&](int t, int j0, int j1) -> int {
-    const int p = t / G, sg = t % G, r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
-    if (j1 <= j0) return CLU_OK;
-    float *A11 = dA + (long)r0 * lda + r0;
-    const int pid = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
-    int e = gemm_lower_left<float>(nb, j1 - j0, winv_of(t), nb, A11 + nb + j0, lda, ubuf + j0,
-                                   Nt, sm);
-    float *ucp = A11 + nb + j0, *lcp = A11 + (long)nb * lda;
-    const float *l21 = (const float *)ws.l21buf;
-    const size_t ro = (size_t)r1 * ldg + (size_t)sg * Kp;
-    const size_t co = (size_t)(r1 + j0) * ldg + (size_t)sg * Kp;
-    if (!e && fp8) {
-      int *escL = ESC(p, sg), *escU = escL + T;
-      float *partL = (float *)(escU + T), *partU = partL + (size_t)FP8_PART_PER_TILE * T;
-      if (j0 == 0)
-        e = quant_rows_fp8(Nt, nb, Kp, -1.0f, l21, nb, (uint8_t *)PA(p) + ro, escL + r1 / 128,
-                           partL + (size_t)FP8_PART_PER_TILE * (r1 / 128), sm, lcp, lda, ldg);
-      if (!e)
-        e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)PU(p) + co,
-                           escU + (r1 + j0) / 128,
-                           partU + (size_t)FP8_PART_PER_TILE * ((r1 + j0) / 128), sm, ucp, lda,
-                           ldg);
-    } else if (!e && prec == CHARMLU_PREC_BF16) {
-      if (j0 == 0)
-        e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, l21, nb, (__bf16 *)PA(p) + ro, sm, lcp, lda,
-                              ldg);
-      if (!e)
-        e = pack_cols<__bf16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (__bf16 *)PU(p) + co, sm, ucp, lda,
-                              ldg);
-    } else if (!e) {
-      if (j0 == 0)
-        e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, l21, nb, (_Float16 *)PA(p) + ro, sm, lcp, lda,
-                                ldg);
-      if (!e)
-        e = pack_cols<_Float16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (_Float16 *)PU(p) + co, sm, ucp,
-                                lda, ldg);
-    }
-    prof_end(pid, sm);
-    return e;
-  };
-  // rows of a K = ns nb update of N columns that fill the duration of the
-  // panel of step s on the update stream's CUs.  The rate at K: 0.8 of the
-  // standalone kernel's, which at 64512^2 spends a fixed 6.9 (FP8) or 6.5
-  // (16-bit) ms on C plus 3.9 or 7.3 ms per 1024 of K (measured inside the
-  // LU: FP8 0.93-0.96 PFLOP/s at K = 2048 and 1.13-1.23 at 4096, BF16 0.64
-  // at 2048)
-  auto fill_rows = [&](int s, int ns, int M, int N) -> int {
-    if (M <= 0) return 0;
-    if (!split) return M;
-    if (force_split) return forced_split(M);
-    double p_ms = s < (int)ws.histmx.ms.size() ? ws.histmx.ms[s] : -1.0;
-    if (p_ms <= 0.0) p_ms = nb / 1024.0 * 3.5;
-    const double kk = (double)ns * nb / 1024.0;
-    const double k_rate = 0.8 * 2.0 * 64512.0 * 64512.0 * 1024.0 * kk /
-                          (fp8 ? 6.9 + 3.9 * kk : 6.5 + 7.3 * kk);  // flop per ms
-    const double rate = k_rate * (ws.ncu - ws.panel_cus) / ws.ncu;
-    const double rows = 1.1 * p_ms * rate / (2.0 * N * ns * nb);
-    if (rows >= M) return M;
-    const int r = ((int)rows + 127) / 128 * 128;
-    return r > M ? M : r;
-  };
-  auto grouped_steps = [&]() -> int {
-    int rc;
-    if ((rc = panel(0))) return rc;
-    CLU_HIP(hipEventRecord(ws.ev_panel, sp));
-    // the rest of group yp's update (yg segments): rows and columns [y0, n),
-    // ydone rows of it enqueued; y_full: its last rows run on the all-CU
-    // stream (ev_full)
-    int yp = -1, yg = 0, y0 = n, ydone = 0;
-    bool y_full = false;
-    auto y_part = [&](int rows, hipStream_t st) -> int {
-      const int M = n - y0 - ydone;
-      rows = rows < M ? rows : M;
-      if (rows <= 0) return CLU_OK;
-      const int e = updg(yp, yg, y0 + ydone, y0, rows, n - y0, st);
-      ydone += rows;
-      return e;
-    };
-    // the panel of step s, after what ev_la marks (the work that runs beside
-    // it is enqueued before its launches, so that the host's enqueueing of
-    // the panel's many small launches does not hold it back)
-    auto panel_step = [&](int s) -> int {
-      CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
-      CLU_HIP(hipEventRecord(ws.histmx.ev[2 * s], sp));
-      const int e = panel(s);
-      if (e) return e;
-      CLU_HIP(hipEventRecord(ws.histmx.ev[2 * s + 1], sp));
-      CLU_HIP(hipEventRecord(ws.ev_panel, sp));
-      return CLU_OK;
-    };
-    for (int a = 0; a < NB; a += G) {
-      const int p = a / G, gl = NB - a < G ? NB - a : G;
-      for (int i = 0; i < gl; ++i) {
-        const int s = a + i, r1 = (s + 1) * nb, Nt = n - r1;
-        CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));  // panel(s)
-        if (Nt <= 0) break;
-        // the previous group's remaining rows on every CU once the group's
-        // last panel is done (no panel runs until the next update waits for
-        // them), beside that step's L21 and U row
-        if (i == gl - 1 && yp >= 0 && ydone < n - y0) {
-          hipStream_t sf = ws.s_full;
-          CLU_HIP(hipStreamWaitEvent(sf, ws.ev_panel, 0));
-          if ((rc = y_part(n - y0 - ydone, sf))) return rc;
-          CLU_HIP(hipEventRecord(ws.ev_full, sf));
-          y_full = true;
-        }
-        if ((rc = l21_of(s))) return rc;
-        if ((rc = u_row2(s, 0, nb))) return rc;
-        if (i < gl - 1) {
-          // block column s + 1 with the group's steps a .. s, then beside
-          // panel(s + 1): the rest of the U row, block row s + 1 likewise,
-          // and rows of the previous group's rest
-          if ((rc = updg(p, i + 1, r1, r1, Nt, nb, sm))) return rc;
-          CLU_HIP(hipEventRecord(ws.ev_la, sm));
-          if ((rc = u_row2(s, nb, Nt))) return rc;
-          if ((rc = updg(p, i + 1, r1, r1 + nb, nb, Nt - nb, sm))) return rc;
-          if (yp >= 0 && (rc = y_part(fill_rows(s + 1, yg, n - y0 - ydone, n - y0), sm)))
-            return rc;
-          if ((rc = panel_step(s + 1))) return rc;
-          continue;
-        }
-        // the group's last step: its update of the rest (gl segments), the
-        // next panel's block column first
-        if (y_full) {
-          CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
-          y_full = false;
-        }
-        const int W = Nt < G * nb ? Nt : G * nb;  // the next group's blocks
-        if ((rc = updg(p, gl, r1, r1, Nt, nb, sm))) return rc;
-        CLU_HIP(hipEventRecord(ws.ev_la, sm));
-        // beside panel(s + 1): the rest of the U row, the next group's other
-        // block columns and its block rows, then the first rows of the rest
-        if ((rc = u_row2(s, nb, Nt))) return rc;
-        if ((rc = updg(p, gl, r1, r1 + nb, Nt, W - nb, sm))) return rc;
-        if ((rc = updg(p, gl, r1, r1 + W, W, Nt - W, sm))) return rc;
-        yp = p;
-        yg = gl;
-        y0 = r1 + W;
-        ydone = 0;
-        if (y0 < n && (rc = y_part(fill_rows(s + 1, yg, n - y0, n - y0), sm))) return rc;
-        if (s + 1 < NB && (rc = panel_step(s + 1))) return rc;
-      }
-    }
-    if (y_full) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
-    if (yp >= 0 && ydone < n - y0 && (rc = y_part(n - y0 - ydone, sm))) return rc;
-    if ((rc = finish_info(dinfo, ws.abortw, sm))) return rc;
-    CLU_HIP(hipEventRecord(ws.ev_end, sm));
-    CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
-    return CLU_OK;
-  };
-
-  rc = [&]() -> int {  // the steps; an error returns from here to the fence below
+  // the steps from t0 on, one trailing update per step (panel(t0) enqueued
+  // and ev_panel recorded)
+  auto per_step = [&](int t0) -> int {
   int rc;
-  if (grouped) return grouped_steps();
-  if ((rc = panel(0))) return rc;
-  CLU_HIP(hipEventRecord(ws.ev_panel, sp));
-  for (int t = 0; t < NB; ++t) {
+  for (int t = t0; t < NB; ++t) {
     const int r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
     CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));
     if (full_pending && !early) {  // the previous update's second part
@@ -1897,6 +1715,293 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   CLU_HIP(hipEventRecord(ws.ev_end, sm));
   CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
   return CLU_OK;
+  };
+
+  // ---- unpivoted low-precision form in groups of steps --------------------
+  // The G steps a = G p .. a + G - 1 of group p update the rest of the matrix
+  // (rows and columns from (a + G) nb on) together, K = G nb: their packed
+  // operands lie side by side in one buffer per group (K segment i: step
+  // a + i), so the FP32 C is read and written once per G steps (the update's
+  // C traffic takes as long as its K loop at K = nb).  Inside the group each
+  // step first brings the next block column and block row up to date with
+  // the group's steps so far (K = (i + 1) nb), which the next panel and U
+  // row need.  The group's update does the next group's G block columns and
+  // block rows first; the rest of it runs beside the next group's panels on
+  // the update stream's CUs and after them on every CU, split by rows, and
+  // every element of it is updated by the same launch shape wherever the
+  // split falls.
+  const long ldg = (long)G * Kp;
+  const int T = n / 128 + 2;
+  auto PA = [&](int p) { return (char *)ws.pa16 + (size_t)(p & 1) * G * pk_bytes; };
+  auto PU = [&](int p) { return (char *)ws.pu16 + (size_t)(p & 1) * G * pk_bytes; };
+  auto ESC = [&](int p, int sg) { return (int *)ws.esc + (size_t)(G * (p & 1) + sg) * esc_ints; };
+  // C (M x N from global row gr, column gc) += segments [0, ns) of group p
+  auto updg = [&](int p, int ns, int gr, int gc, int M, int N, hipStream_t st) -> int {
+    if (M <= 0 || N <= 0) return CLU_OK;
+    const int pid = prof_begin(PK_GEMM_BIG, 2.0 * M * N * nb * ns, st);
+    float *C = dA + (long)gr * lda + gc;
+    const size_t ao = (size_t)gr * ldg, bo = (size_t)gc * ldg;
+    const int K = ns * Kp;
+    int r;
+    if (fp8)
+      r = fgemm_nt(M, N, K, 1.0f, (const uint8_t *)PA(p) + ao, ldg, ESC(p, 0) + gr / 128,
+                   (const uint8_t *)PU(p) + bo, ldg, ESC(p, 0) + T + gc / 128, 1.0f, C, lda, st,
+                   Kp, ns, (long)esc_ints);
+    else if (prec == CHARMLU_PREC_BF16)
+      r = hgemm_nt<__bf16>(M, N, K, 1.0f, (const __bf16 *)PA(p) + ao, ldg,
+                           (const __bf16 *)PU(p) + bo, ldg, 1.0f, C, lda, st);
+    else
+      r = hgemm_nt<_Float16>(M, N, K, 1.0f, (const _Float16 *)PA(p) + ao, ldg,
+                             (const _Float16 *)PU(p) + bo, ldg, 1.0f, C, lda, st);
+    prof_end(pid, st);
+    return r;
+  };
+  // L21 = A21 inv(U11) of step t into l21buf
+  auto l21_of = [&](int t) -> int {
+    const int r0 = t * nb, Nt = n - r0 - nb;
+    float *A11 = dA + (long)r0 * lda + r0;
+    const int pid = prof_begin(PK_GEMM_OTHER, 2.0 * Nt * nb * nb, sm);
+    int e = gemm_upper_right<float>(Nt, nb, A11 + (long)nb * lda, lda, (const float *)ws.uinv32,
+                                    nb, (float *)ws.l21buf, nb, sm);
+    prof_end(pid, sm);
+    return e;
+  };
+  // U row of step t, columns [j0, j1) after its block, packed into K segment
+  // t % G of group t / G (global rows and columns; FP8 scales per global
+  // 128-row tile), with j0 == 0 also its L21; both stored into the factor by
+  // the same passes
+  auto u_row2 = [&](int t, int j0, int j1) -> int {
+    const int p = t / G, sg = t % G, r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
+    if (j1 <= j0) return CLU_OK;
+    float *A11 = dA + (long)r0 * lda + r0;
+    const int pid = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
+    int e = gemm_lower_left<float>(nb, j1 - j0, winv_of(t), nb, A11 + nb + j0, lda, ubuf + j0,
+                                   Nt, sm);
+    float *ucp = A11 + nb + j0, *lcp = A11 + (long)nb * lda;
+    const float *l21 = (const float *)ws.l21buf;
+    const size_t ro = (size_t)r1 * ldg + (size_t)sg * Kp;
+    const size_t co = (size_t)(r1 + j0) * ldg + (size_t)sg * Kp;
+    if (!e && fp8) {
+      int *escL = ESC(p, sg), *escU = escL + T;
+      float *partL = (float *)(escU + T), *partU = partL + (size_t)FP8_PART_PER_TILE * T;
+      if (j0 == 0)
+        e = quant_rows_fp8(Nt, nb, Kp, -1.0f, l21, nb, (uint8_t *)PA(p) + ro, escL + r1 / 128,
+                           partL + (size_t)FP8_PART_PER_TILE * (r1 / 128), sm, lcp, lda, ldg);
+      if (!e)
+        e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)PU(p) + co,
+                           escU + (r1 + j0) / 128,
+                           partU + (size_t)FP8_PART_PER_TILE * ((r1 + j0) / 128), sm, ucp, lda,
+                           ldg);
+    } else if (!e && prec == CHARMLU_PREC_BF16) {
+      if (j0 == 0)
+        e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, l21, nb, (__bf16 *)PA(p) + ro, sm, lcp, lda,
+                              ldg);
+      if (!e)
+        e = pack_cols<__bf16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (__bf16 *)PU(p) + co, sm, ucp, lda,
+                              ldg);
+    } else if (!e) {
+      if (j0 == 0)
+        e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, l21, nb, (_Float16 *)PA(p) + ro, sm, lcp, lda,
+                                ldg);
+      if (!e)
+        e = pack_cols<_Float16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (_Float16 *)PU(p) + co, sm, ucp,
+                                lda, ldg);
+    }
+    prof_end(pid, sm);
+    return e;
+  };
+  // rows of a K = ns nb update of N columns that fill the duration of the
+  // panel of step s on the update stream's CUs.  The rate at K: 0.8 of the
+  // standalone kernel's, which at 64512^2 spends a fixed 6.9 (FP8) or 6.5
+  // (16-bit) ms on C plus 3.9 or 7.3 ms per 1024 of K (measured inside the
+  // LU: FP8 0.93-0.96 PFLOP/s at K = 2048 and 1.13-1.23 at 4096, BF16 0.64
+  // at 2048)
+  auto fill_rows = [&](int s, int ns, int M, int N) -> int {
+    if (M <= 0) return 0;
+    if (!split) return M;
+    if (force_split) return forced_split(M);
+    double p_ms = s < (int)ws.histmx.ms.size() ? ws.histmx.ms[s] : -1.0;
+    if (p_ms <= 0.0) p_ms = nb / 1024.0 * 3.5;
+    const double kk = (double)ns * nb / 1024.0;
+    const double k_rate = 0.8 * 2.0 * 64512.0 * 64512.0 * 1024.0 * kk /
+                          (fp8 ? 6.9 + 3.9 * kk : 6.5 + 7.3 * kk);  // flop per ms
+    const double rate = k_rate * (ws.ncu - ws.panel_cus) / ws.ncu;
+    const double rows = 1.1 * p_ms * rate / (2.0 * N * ns * nb);
+    if (rows >= M) return M;
+    const int r = ((int)rows + 127) / 128 * 128;
+    return r > M ? M : r;
+  };
+  // Speculative identity pivots (the pivoted form on this schedule): each
+  // panel is eliminated without interchanges, from a copy of its diagonal
+  // block, and checked before anything uses it.  If every multiplier of the
+  // elimination is below 1/2 in magnitude (and no pivot is tiny), partial
+  // pivoting keeps every diagonal row: the diagonal candidate exceeds every
+  // other one in its column at least twice over, and the pivoted schedule
+  // computes the same candidates with other roundings of the same
+  // operations, which move them by a small multiple of the update's unit
+  // roundoff.  Otherwise the step falls back: the rest of the matrix gets
+  // what the group's schedule still owes it, the diagonal block is restored,
+  // and the exchanging panel and one update per step take over from that
+  // step on.  The check costs one host round trip per step (the host waits
+  // for its flag before it enqueues the step's U row).
+  constexpr float SPEC_THR = 0.5f;
+  const float tiny32 = 100.0f * 1.1920929e-07f;  // the panel's singular-pivot bound
+  auto gpanel = [&](int s) -> int {
+    if (spec) {
+      const int r0 = s * nb;
+      int e = convert<float, float>(nb, nb, dA + (long)r0 * lda + r0, lda, (float *)ws.specbak, nb,
+                                    sp);
+      if (e) return e;
+    }
+    return panel(s, true);
+  };
+  auto grouped_steps = [&]() -> int {
+    int rc;
+    if ((rc = gpanel(0))) return rc;
+    CLU_HIP(hipEventRecord(ws.ev_panel, sp));
+    // the rest of group yp's update (yg segments): rows and columns [y0, n),
+    // ydone rows of it enqueued; y_full: its last rows run on the all-CU
+    // stream (ev_full)
+    int yp = -1, yg = 0, y0 = n, ydone = 0;
+    bool y_full = false;
+    auto y_part = [&](int rows, hipStream_t st) -> int {
+      const int M = n - y0 - ydone;
+      rows = rows < M ? rows : M;
+      if (rows <= 0) return CLU_OK;
+      const int e = updg(yp, yg, y0 + ydone, y0, rows, n - y0, st);
+      ydone += rows;
+      return e;
+    };
+    // the panel of step s, after what ev_la marks (the work that runs beside
+    // it is enqueued before its launches, so that the host's enqueueing of
+    // the panel's many small launches does not hold it back)
+    auto panel_step = [&](int s) -> int {
+      CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
+      CLU_HIP(hipEventRecord(ws.histmx.ev[2 * s], sp));
+      const int e = gpanel(s);
+      if (e) return e;
+      CLU_HIP(hipEventRecord(ws.histmx.ev[2 * s + 1], sp));
+      CLU_HIP(hipEventRecord(ws.ev_panel, sp));
+      return CLU_OK;
+    };
+    // the fall back at step s (index i of group p)
+    auto fallback = [&](int s, int i, int p) -> int {
+      int e;
+      const int r0 = s * nb, r1 = r0 + nb, Nt = n - r1;
+      if (y_full) {
+        CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+        y_full = false;
+      }
+      if (yp >= 0 && ydone < n - y0 && (e = y_part(n - y0 - ydone, sm))) return e;
+      if (i > 0 && Nt > 0 && (e = updg(p, i, r1, r1, Nt, Nt, sm))) return e;
+      float *A11 = dA + (long)r0 * lda + r0;
+      if ((e = convert<float, float>(nb, nb, (const float *)ws.specbak, nb, A11, lda, sm)))
+        return e;
+      if ((e = fill_int(dinfo, 0, 1, sm))) return e;  // a tiny pivot of the elimination
+      CLU_HIP(hipEventRecord(ws.ev_la, sm));
+      CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
+      if ((e = panel(s))) return e;
+      CLU_HIP(hipEventRecord(ws.ev_panel, sp));
+      full_pending = false;
+      return per_step(s);
+    };
+    // the check of step s's elimination (L21 in l21buf unless it is the last)
+    auto spec_ok = [&](int s, int Nt) -> int {
+      const int r0 = s * nb;
+      int e = spec_check(Nt, nb, (const float *)ws.l21buf, nb, dA + (long)r0 * lda + r0, lda,
+                         SPEC_THR, tiny32, ws.spec_flag, sm);
+      if (e) return -1;
+      if (hipMemcpyAsync(ws.spec_hflag, ws.spec_flag, sizeof(int), hipMemcpyDeviceToHost, sm) !=
+              hipSuccess ||
+          hipEventRecord(ws.ev_chk, sm) != hipSuccess || hipEventSynchronize(ws.ev_chk) != hipSuccess)
+        return -1;
+      return *ws.spec_hflag == 0 ? 1 : 0;
+    };
+    for (int a = 0; a < NB; a += G) {
+      const int p = a / G, gl = NB - a < G ? NB - a : G;
+      for (int i = 0; i < gl; ++i) {
+        const int s = a + i, r1 = (s + 1) * nb, Nt = n - r1;
+        CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));  // panel(s)
+        if (Nt <= 0) {
+          if (spec) {
+            const int ok = spec_ok(s, 0);
+            if (ok < 0) {
+              set_error("the check of a speculative panel failed to run");
+              return CLU_ERR_HIP;
+            }
+            if (!ok) return fallback(s, i, p);
+          }
+          break;
+        }
+        // the previous group's remaining rows on every CU once the group's
+        // last panel is done (no panel runs until the next update waits for
+        // them), beside that step's L21 and U row
+        if (i == gl - 1 && yp >= 0 && ydone < n - y0) {
+          hipStream_t sf = ws.s_full;
+          CLU_HIP(hipStreamWaitEvent(sf, ws.ev_panel, 0));
+          if ((rc = y_part(n - y0 - ydone, sf))) return rc;
+          CLU_HIP(hipEventRecord(ws.ev_full, sf));
+          y_full = true;
+        }
+        if ((rc = l21_of(s))) return rc;
+        if (spec) {
+          const int ok = spec_ok(s, Nt);
+          if (ok < 0) {
+            set_error("the check of a speculative panel failed to run");
+            return CLU_ERR_HIP;
+          }
+          if (!ok) return fallback(s, i, p);
+        }
+        if ((rc = u_row2(s, 0, nb))) return rc;
+        if (i < gl - 1) {
+          // block column s + 1 with the group's steps a .. s, then beside
+          // panel(s + 1): the rest of the U row, block row s + 1 likewise,
+          // and rows of the previous group's rest
+          if ((rc = updg(p, i + 1, r1, r1, Nt, nb, sm))) return rc;
+          CLU_HIP(hipEventRecord(ws.ev_la, sm));
+          if ((rc = u_row2(s, nb, Nt))) return rc;
+          if ((rc = updg(p, i + 1, r1, r1 + nb, nb, Nt - nb, sm))) return rc;
+          if (yp >= 0 && (rc = y_part(fill_rows(s + 1, yg, n - y0 - ydone, n - y0), sm)))
+            return rc;
+          if ((rc = panel_step(s + 1))) return rc;
+          continue;
+        }
+        // the group's last step: its update of the rest (gl segments), the
+        // next panel's block column first
+        if (y_full) {
+          CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+          y_full = false;
+        }
+        const int W = Nt < G * nb ? Nt : G * nb;  // the next group's blocks
+        if ((rc = updg(p, gl, r1, r1, Nt, nb, sm))) return rc;
+        CLU_HIP(hipEventRecord(ws.ev_la, sm));
+        // beside panel(s + 1): the rest of the U row, the next group's other
+        // block columns and its block rows, then the first rows of the rest
+        if ((rc = u_row2(s, nb, Nt))) return rc;
+        if ((rc = updg(p, gl, r1, r1 + nb, Nt, W - nb, sm))) return rc;
+        if ((rc = updg(p, gl, r1, r1 + W, W, Nt - W, sm))) return rc;
+        yp = p;
+        yg = gl;
+        y0 = r1 + W;
+        ydone = 0;
+        if (y0 < n && (rc = y_part(fill_rows(s + 1, yg, n - y0, n - y0), sm))) return rc;
+        if (s + 1 < NB && (rc = panel_step(s + 1))) return rc;
+      }
+    }
+    if (y_full) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+    if (yp >= 0 && ydone < n - y0 && (rc = y_part(n - y0 - ydone, sm))) return rc;
+    if ((rc = finish_info(dinfo, ws.abortw, sm))) return rc;
+    CLU_HIP(hipEventRecord(ws.ev_end, sm));
+    CLU_HIP(hipStreamWaitEvent(su, ws.ev_end, 0));
+    return CLU_OK;
+  };
+
+  rc = [&]() -> int {  // the steps; an error returns from here to the fence below
+  int rc;
+  if (grouped) return grouped_steps();
+  if ((rc = panel(0))) return rc;
+  CLU_HIP(hipEventRecord(ws.ev_panel, sp));
+  return per_step(0);
   }();
   if (rc) fence_streams(ws, su);
   return rc;

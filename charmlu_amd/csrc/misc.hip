@@ -301,6 +301,40 @@ __global__ void fill_int_kernel(int *p, int v, long n) {
   if (i < n) p[i] = v;
 }
 
+// Check of a panel eliminated without interchanges: *flag = 1 unless every
+// multiplier (the strictly lower part of the nb x nb diagonal block, in the
+// matrix, and the M x nb block L21 below it) is below thr in magnitude and
+// every diagonal entry of U11 above tiny; NaN fails.  Workgroup i takes row
+// i of the diagonal block, then row i - nb of L21.
+__global__ __launch_bounds__(256) void spec_check_kernel(int M, int nb,
+                                                         const float *__restrict__ L21, long ld21,
+                                                         const float *__restrict__ A11, long lda,
+                                                         float thr, float tiny,
+                                                         int *__restrict__ flag) {
+  const long i = blockIdx.x;
+  bool bad = false;
+  if (i < nb) {
+    const float *r = A11 + i * lda;
+    for (int j = threadIdx.x; j < i; j += 256) bad = bad || !(fabsf(r[j]) < thr);
+    if (threadIdx.x == 0) bad = bad || !(fabsf(r[i]) > tiny);
+  } else if (i - nb < M) {
+    const float *r = L21 + (i - nb) * ld21;
+    for (int j = threadIdx.x; j < nb; j += 256) bad = bad || !(fabsf(r[j]) < thr);
+  }
+  if (bad) *flag = 1;
+}
+
+int spec_check(int M, int nb, const float *L21, long ld21, const float *A11, long lda,
+               float thr, float tiny, int *flag, hipStream_t s) {
+  int rc = fill_int(flag, 0, 1, s);
+  if (rc) return rc;
+  if (nb <= 0) return CLU_OK;
+  hipLaunchKernelGGL(spec_check_kernel, dim3((unsigned)(nb + (M > 0 ? M : 0))), dim3(256), 0, s,
+                     M, nb, L21, ld21, A11, lda, thr, tiny, flag);
+  CLU_LAUNCHED("spec_check_kernel");
+  return CLU_OK;
+}
+
 int fill_int(int *p, int v, long n, hipStream_t s) {
   if (n <= 0) return CLU_OK;
   hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);

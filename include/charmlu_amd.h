@@ -180,13 +180,23 @@ int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda,
  * factored in FP32; the trailing update (updateMatrix, lu.C:26-43) runs with
  * its operands rounded to prec (PREC_FP32, PREC_BF16, PREC_FP16, or
  * PREC_FP8: e4m3 with a power-of-two scale per 128-row tile of L21 and per
- * 128-column tile of U12, nb a multiple of 128; FP32 accumulation).  dA: n x n FP32 row-major, overwritten by L and U.  The
- * pivots and info follow charmlu_dgetrf.  prec | CHARMLU_MIXED_NOPIV
- * factors without row interchanges (ipiv[j] = j), for matrices that need
- * none, such as diagonally dominant ones; nb <= 2048 then.  The panel is
- * then the LU of the diagonal block plus L21 = A21 inv(U11).  If every
- * computed |l_ij| < 1, partial pivoting would have chosen the same
- * (diagonal) pivots.  Asynchronous. */
+ * 128-column tile of U12, nb a multiple of 128; FP32 accumulation).  dA: n
+ * x n FP32 row-major, overwritten by L and U.  The pivots and info follow
+ * charmlu_dgetrf.  prec | CHARMLU_MIXED_NOPIV factors without row
+ * interchanges (ipiv[j] = j), for matrices that need none, such as
+ * diagonally dominant ones; nb <= 2048 then.  The panel is then the LU of
+ * the diagonal block plus L21 = A21 inv(U11).  If every computed |l_ij| <
+ * 1, partial pivoting would have chosen the same (diagonal) pivots.
+ * With partial pivoting and a BF16, FP16 or FP8 update (nb <= 2048), each
+ * panel is first eliminated that way and kept if every multiplier is below
+ * 1/2 in magnitude and no pivot is tiny (partial pivoting then keeps every
+ * diagonal row); at the first panel that fails, the diagonal block is
+ * restored and the factorization continues with exchanging panels (the
+ * pivots are partial pivoting's either way; the check makes the host wait
+ * once per panel, so the call returns when the last panel has been
+ * checked; CHARMLU_SPECULATE=0 exchanges at every panel).  The unpivoted
+ * low-precision schedule updates the rest of the matrix once per group of
+ * up to 4 steps (CHARMLU_GROUP).  Asynchronous otherwise. */
 int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipiv,
                          int *dinfo, void *stream);
 

@@ -530,3 +530,102 @@ def test_grouped_steps_match_per_step_form(dev, orc, prec, g, n, nb):
     print(f"group {g} {prec} n={n} nb={nb}: be {be:.2e} (per step {out[1][3]:.2e}), "
           f"diff {diff:.2e}, {st}, err {err:.2e}")
     assert st["residual"] <= 1.0 and err < 1e-10
+
+
+def _getrf_env(dev, A, nb, prec, env, pivot=True):
+    import os
+    import torch
+    A32 = torch.from_numpy(np.ascontiguousarray(A, dtype=np.float32)).cuda()
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        ipiv, info = dev.getrf_mixed(A32, nb, prec, pivot=pivot)
+        torch.cuda.synchronize()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return A32, ipiv.cpu().numpy(), int(info.item())
+
+
+@pytest.mark.parametrize("prec,n,nb", [("bf16", 2048, 256), ("fp8", 4096, 512),
+                                       ("fp16", 1536, 512), ("bf16", 2560, 512)])
+def test_speculative_pivots_on_diagdom(dev, orc, prec, n, nb):
+    """The pivoted low-precision form eliminates each panel without
+    interchanges and keeps the result when every multiplier is below 1/2
+    (partial pivoting then keeps every diagonal row).  On a diagonally
+    dominant matrix: the identity pivots, as the exchanging panels find
+    (CHARMLU_SPECULATE=0), |l_ij| < 1/2, the update's backward-error bound,
+    and the refined x within 1e-10 of the FP64 oracle's."""
+    import torch
+    A = orc.gen_diagdom(orc.SEED_A + 11, n)
+    S, ipiv, info = _getrf_env(dev, A, nb, prec, {})
+    X, ipiv_x, info_x = _getrf_env(dev, A, nb, prec, {"CHARMLU_SPECULATE": "0"})
+    assert info == info_x == 0
+    assert np.array_equal(ipiv, np.arange(n)) and np.array_equal(ipiv_x, np.arange(n))
+    LU = S.cpu().numpy()
+    assert np.abs(np.tril(LU, -1)).max() < 0.5
+    unit = 2.0 ** -4 if prec == "fp8" else UNIT[prec]
+    be = _backward_error(A, LU, ipiv)
+    assert be < 8 * unit * (n // nb + 1) * max(1, nb // 512), be
+    b = orc.gen_rhs(n)
+    x_ref = orc.lu_solve_problem(n, nb, A=A.copy())["x"]
+    dA = torch.from_numpy(A).cuda()
+    x, st = dev.gesv_ir(dA, S, torch.from_numpy(ipiv).cuda(), torch.from_numpy(b).cuda(), tol=1.0)
+    err = np.abs(x.cpu().numpy() - x_ref).max() / np.abs(x_ref).max()
+    print(f"speculative {prec} n={n} nb={nb}: be {be:.2e} {st} err {err:.2e}")
+    assert st["residual"] <= 1.0 and err < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_speculative_falls_back_at_the_first_panel(dev, orc, prec):
+    """A uniform matrix needs interchanges in its first column: the
+    speculative elimination of panel 0 fails its check, the diagonal block
+    is restored and the exchanging schedule runs from step 0, so factors
+    and pivots are bit-identical to CHARMLU_SPECULATE=0."""
+    n, nb = 2048, 256
+    A = orc.gen_uniform(orc.SEED_A + 13, n)
+    S, ipiv, info = _getrf_env(dev, A, nb, prec, {})
+    X, ipiv_x, info_x = _getrf_env(dev, A, nb, prec, {"CHARMLU_SPECULATE": "0"})
+    assert info == info_x == 0
+    assert not np.array_equal(ipiv, np.arange(n))
+    assert np.array_equal(ipiv, ipiv_x)
+    assert np.array_equal(S.cpu().numpy(), X.cpu().numpy())
+
+
+@pytest.mark.parametrize("prec,k", [("bf16", 3), ("bf16", 5), ("bf16", 2), ("bf16", 7),
+                                    ("fp8", 7)])
+def test_speculative_falls_back_mid_factorization(dev, orc, prec, k):
+    """Diagonally dominant except the trailing block from block k on (its
+    diagonal shrunk to 1e-3), whose columns need interchanges: panels 0 ..
+    k-1 keep the identity (their multipliers are tiny), panel k fails the
+    check and the group's owed updates are applied before the exchanging
+    schedule takes over (k = 3: the fourth step of a group of 4, k = 5 and
+    2: the second and third of one, k = 7: the last panel of the matrix;
+    FP8 only there, whose factors of a random block precondition poorly).
+    A partial-pivoting factor: identity pivots before block k, |l_ij| <= 1,
+    the update's backward-error bound, and refinement to the FP64 oracle's
+    x."""
+    import torch
+    n, nb = 2048, 256
+    A = orc.gen_diagdom(orc.SEED_A + 17, n)
+    idx = np.arange(k * nb, n)
+    A[idx, idx] = 1e-3
+    S, ipiv, info = _getrf_env(dev, A, nb, prec, {})
+    assert info == 0
+    assert np.array_equal(ipiv[:k * nb], np.arange(k * nb))
+    assert not np.array_equal(ipiv[k * nb:], np.arange(k * nb, n))
+    LU = S.cpu().numpy()
+    assert np.abs(np.tril(LU, -1)).max() <= 1.0 + 1e-6
+    unit = 2.0 ** -4 if prec == "fp8" else UNIT[prec]
+    be = _backward_error(A, LU, ipiv)
+    assert be < 64 * unit * np.sqrt(n), be
+    b = orc.gen_rhs(n)
+    x_ref = orc.lu_solve_problem(n, nb, A=A.copy())["x"]
+    dA = torch.from_numpy(A).cuda()
+    x, st = dev.gesv_ir(dA, S, torch.from_numpy(ipiv).cuda(), torch.from_numpy(b).cuda(), tol=1.0)
+    err = np.abs(x.cpu().numpy() - x_ref).max() / np.abs(x_ref).max()
+    print(f"fallback {prec} k={k}: be {be:.2e} {st} err {err:.2e}")
+    assert st["residual"] <= 1.0 and err < 1e-10
