@@ -1578,7 +1578,9 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     const int Ma = split_rows(t, M, N);
     return early && Ma < nb ? (nb < M ? nb : M) : Ma;
   };
-  if ((rc = history_prepare(ws.histmx, n, nb, 1 + 2 * prec + (nopiv ? 1 : 0), NB))) {
+  // (the speculative schedule's panels are the unpivoted ones: their
+  // durations, not those of the exchanging panels, size its updates)
+  if ((rc = history_prepare(ws.histmx, n, nb, 1 + 2 * prec + ((nopiv || spec) ? 1 : 0), NB))) {
     fence_streams(ws, su);
     return rc;
   }
