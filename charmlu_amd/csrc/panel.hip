@@ -514,6 +514,107 @@ int panel_plan(int m, int w, int max_groups, bool pivot, PanelShape *ps) {
 template int panel_plan<double>(int, int, int, bool, PanelShape *);
 template int panel_plan<float>(int, int, int, bool, PanelShape *);
 
+// Leaf of the recursive panel without interchanges: rows [j0, j0 + m) x
+// columns [j0, j0 + w), w <= 32.  Without pivoting only the w x w diagonal
+// block needs the column-by-column elimination; every row below is the
+// solution x of x U = a with that block's U, independent of the others.
+// One workgroup of 1024 threads: wave 0 eliminates the diagonal block in
+// LDS (a row per lane, no workgroup barrier per column), then every thread
+// solves rows below (one at a time, U read from LDS), and the diagonal
+// block goes back with the pivots (the identity) and the first tiny pivot.
+// (The exchanging strip, used for this before, spends about 2 us per
+// column on a workgroup barrier and the pivot row through LDS for every
+// row: 70 us for 32 columns of a 1024-row leaf.)  As in that strip, a zero
+// pivot leaves its column as it is.
+template <typename T>
+__global__ __launch_bounds__(1024) void leaf_nopiv_kernel(T *__restrict__ A, long lda, int j0,
+                                                          int m, int w, int *__restrict__ ipiv,
+                                                          int *info) {
+  constexpr int W = 32;
+  __shared__ __attribute__((aligned(16))) T D[W][W];
+  const int t = threadIdx.x;
+  for (int e = t; e < W * W; e += 1024) {
+    const int i = e / W, j = e % W;
+    D[i][j] = (i < w && j < w) ? A[(long)(j0 + i) * lda + j0 + j] : (i == j ? (T)1 : (T)0);
+  }
+  __syncthreads();
+  if (t < 64) {
+    // wave 0: lane i < W holds row i; at column c lane c publishes its
+    // (finished) row c, every lane below it reads it back and eliminates
+    const int i = t;
+    T r[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) r[j] = i < W ? D[i][j] : (T)0;
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      if (i == c) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) D[c][j] = r[j];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      T u[W];
+#pragma unroll
+      for (int j = c; j < W; ++j) u[j] = D[c][j];
+      if (i > c && i < W && u[c] != (T)0) {
+        const T l = r[c] / u[c];
+        r[c] = l;
+#pragma unroll
+        for (int j = c + 1; j < W; ++j) r[j] -= l * u[j];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (i < W) {
+#pragma unroll
+      for (int j = 0; j < W; ++j) D[i][j] = r[j];
+    }
+  }
+  __syncthreads();
+  for (int row = w + t; row < m; row += 1024) {
+    // (U is read again for every row: kept in registers across rows it
+    // would take 528 of them)
+    __asm__ volatile("" ::: "memory");
+    T *ar = A + (long)(j0 + row) * lda + j0;
+    T x[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) x[j] = j < w ? ar[j] : (T)0;
+    // right-looking substitution: x_k /= u_kk, then x_j -= x_k u_kj, row k
+    // of U read as whole 16-byte pieces from its first one holding j > k
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      constexpr int V = 16 / sizeof(T);
+      T uk[W];
+#pragma unroll
+      for (int q = k / V * V; q < W; q += V) {
+        typedef T vec_t __attribute__((ext_vector_type(V)));
+        const vec_t v = *reinterpret_cast<const vec_t *>(&D[k][q]);
+#pragma unroll
+        for (int e = 0; e < V; ++e) uk[q + e] = v[e];
+      }
+      if (uk[k] != (T)0) x[k] = x[k] / uk[k];
+#pragma unroll
+      for (int j = k + 1; j < W; ++j) x[j] -= x[k] * uk[j];
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      if (j < w) ar[j] = x[j];
+  }
+  for (int e = t; e < w * w; e += 1024) {
+    const int i = e / w, j = e % w;
+    A[(long)(j0 + i) * lda + j0 + j] = D[i][j];
+  }
+  if (t < w) ipiv[j0 + t] = j0 + t;
+  if (t == 0) {
+    const T tiny = (T)100 * (sizeof(T) == 8 ? (T)2.220446049250313e-16 : (T)1.1920929e-07);
+    for (int c = 0; c < w; ++c)
+      if (fabs(D[c][c]) <= tiny) {
+        if (*info == 0) *info = j0 + c + 1;
+        break;
+      }
+  }
+}
+
 template <typename T>
 int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *ipiv,
                PanelWork &pw, hipStream_t s, bool pivot) {
@@ -535,6 +636,15 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
                      A, lda, j0, m, w, pc0, pc1, ipiv, recs, seq0, pw.info, pw.abortw)
   // the instantiated shapes: (R, NT, multi-workgroup, pivoting, strip width)
   const bool multi = G > 1;
+  if constexpr (sizeof(T) == 4) {  // (the FP64 factorization always exchanges)
+    if (!pivot && w <= 32) {
+      hipLaunchKernelGGL((leaf_nopiv_kernel<T>), dim3(1), dim3(1024), 0, s, A, lda, j0, m, w,
+                         ipiv, pw.info);
+      CLU_LAUNCHED("leaf_nopiv_kernel");
+      pw.epoch += (unsigned)w;
+      return CLU_OK;
+    }
+  }
   if (!pivot) {
     if (ps.sw == PW) {
       if (ps.NT == 512) CLU_PANEL(4, 512, false, false, PW);
