@@ -146,6 +146,62 @@ def cpu_baseline(n: int, nb: int, timeout: float = 300.0):
     return out
 
 
+def block_dgemm(lib, _lib, torch, nb, lu_tflops, trials=10):
+    """The reference's single-block check (testdgemm and "Percent of DGEMM",
+    benchmark.C:297-341,374-381): C -= A B for one b x b block, the mean of
+    10 timed calls after one untimed.  On the device through
+    charmlu_dgemm_dev (operands resident, each call timed to its
+    completion, as the reference times one call), and through the
+    host-pointer CBLAS shim charmlu_cblas_dgemm (the platformBlas.h
+    drop-in: its copies to and from the device included).  percent_of_dgemm
+    is the reference's ratio: the LU's rate over the device block rate (a
+    single b x b product is far too small to fill the GPU, so the ratio
+    exceeds 100 %; the trailing update's rate is the roofline object)."""
+    import ctypes
+    import numpy as np
+    g = torch.Generator(device="cpu").manual_seed(0)
+    Ah, Bh, Ch = ((torch.rand(nb, nb, generator=g, dtype=torch.float64) - 0.5) for _ in range(3))
+    A, B, C = Ah.cuda(), Bh.cuda(), Ch.cuda()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    flops = 2.0 * nb ** 3
+
+    def dev_call():
+        _lib.check(lib.charmlu_dgemm_dev(_lib.NO_TRANS, _lib.NO_TRANS, nb, nb, nb, -1.0, p(A), nb,
+                                         p(B), nb, 1.0, p(C), nb, st), "charmlu_dgemm_dev")
+        torch.cuda.synchronize()
+
+    dev_call()
+    t = []
+    for _ in range(trials):
+        t0 = time.perf_counter()
+        dev_call()
+        t.append(time.perf_counter() - t0)
+    dev_s = sum(t) / trials
+    An, Bn, Cn = (np.ascontiguousarray(x.numpy()) for x in (Ah, Bh, Ch))
+    dp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+
+    def host_call():
+        rc = lib.charmlu_cblas_dgemm(_lib.ROW_MAJOR, _lib.NO_TRANS, _lib.NO_TRANS, nb, nb, nb,
+                                     -1.0, dp(An), nb, dp(Bn), nb, 1.0, dp(Cn), nb)
+        if rc != 0:
+            raise RuntimeError(f"charmlu_cblas_dgemm returned {rc}")
+
+    host_call()
+    t = []
+    for _ in range(trials):
+        t0 = time.perf_counter()
+        host_call()
+        t.append(time.perf_counter() - t0)
+    host_s = sum(t) / trials
+    dev_tf = flops / dev_s / 1e12
+    return {"b": nb, "device_ms": round(dev_s * 1e3, 4), "device_tflops": round(dev_tf, 3),
+            "host_shim_ms": round(host_s * 1e3, 4),
+            "host_shim_tflops": round(flops / host_s / 1e12, 4),
+            "percent_of_dgemm": round(lu_tflops / dev_tf * 100.0, 1),
+            "note": "one b x b x b dgemm (C -= A B) per call, mean of 10 calls"}
+
+
 def gemm_only(lib, _lib, torch, n, nb, steps, warmup):
     """The step-0 trailing update C -= L U (M = N = n - nb, K = nb) on the
     factorization's own row-major layout, timed with events on the stream."""
@@ -679,6 +735,7 @@ def main():
         "row_interchange_gbs": round(swap_bytes / (sw_ms * 1e-3) / 1e9, 1) if sw_ms > 0 else None,
     }
     out["c1_gpu"] = small_case(clu, torch, 4096, 128)
+    out["block_dgemm"] = block_dgemm(lib, _lib, torch, nb, out["value"])
     if not args.no_mixed:
         del A, A0
         torch.cuda.empty_cache()

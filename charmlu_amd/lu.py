@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import time
 
 import numpy as np
 
@@ -323,6 +324,31 @@ def hpl_gflops(n: int, seconds: float) -> float:
     return (2.0 / 3.0 * n ** 3 + 1.5 * n ** 2) / seconds / 1e9
 
 
+def testdgemm(b: int, trials: int = 10) -> float:
+    """Seconds of one b x b x b block product C -= A B on the device
+    (charmlu_dgemm_dev), the mean of ``trials`` calls each timed to its
+    completion after one untimed call: the reference's testdgemm
+    (benchmark.C:297-341), which times one cblas_dgemm of the block size."""
+    torch = _torch()
+    g = torch.Generator(device="cpu").manual_seed(0)
+    A, B, C = ((torch.rand(b, b, generator=g, dtype=torch.float64) - 0.5).cuda() for _ in range(3))
+    lib = _lib.load()
+
+    def call():
+        check(lib.charmlu_dgemm_dev(_lib.NO_TRANS, _lib.NO_TRANS, b, b, b, -1.0, _ptr(A), b,
+                                    _ptr(B), b, 1.0, _ptr(C), b, _stream_ptr()),
+              "charmlu_dgemm_dev")
+        torch.cuda.synchronize()
+
+    call()
+    total = 0.0
+    for _ in range(trials):
+        t0 = time.perf_counter()
+        call()
+        total += time.perf_counter() - t0
+    return total / trials
+
+
 def run(cfg: LUConfig, seed_a: int = SEED_A, seed_b: int = SEED_B, kind: str = "uniform",
         verbose: bool = True):
     """Generate, factor + solve (timed, benchmark.C:242-245 -> 343-349),
@@ -350,9 +376,18 @@ def run(cfg: LUConfig, seed_a: int = SEED_A, seed_b: int = SEED_B, kind: str = "
                gflops_hpl=hpl_gflops(n, seconds),
                tflops_eff=(2.0 / 3.0 * n ** 3) / seconds / 1e12 if seconds > 0 else math.inf,
                ipiv=ipiv, x=x, **res)
+    # the block-dgemm line and the LU's rate against it (benchmark.C:374-381;
+    # one device stands for the reference's cores)
+    dgemm_s = testdgemm(b)
+    out["dgemm_ms"] = dgemm_s * 1e3
+    out["dgemm_gflops"] = 2.0 * b ** 3 / dgemm_s / 1e9
+    out["percent_of_dgemm"] = out["gflops_hpl"] / out["dgemm_gflops"] * 100.0
     if verbose:
         print(f"RESULT procs: 1 device\tblock size: {b}\tTime(s): {seconds:f}")
         print(f"HPL flop rate: {out['gflops_hpl']:f} GFlops")
+        print(f"The dgemm {b} x {b} takes {out['dgemm_ms']:g} ms and achieves "
+              f"{out['dgemm_gflops']:g} GFlop/sec")
+        print(f"Percent of DGEMM is: {out['percent_of_dgemm']:g}%")
         print(f"epsilon = {EPS:e}\nresidual = {res['residual']:f}")
         if res["residual"] > 16:
             print("=== WARNING: Scaled residual is greater than 16 - OUT OF SPEC ===")

@@ -770,3 +770,18 @@ def test_panel_stream_resident_workgroups_other_masks(dev, tmp_path, cus):
     print(f"{cus} panel CUs: {g0} workgroups, {late0} late; {g1} workgroups, {late1} late")
     assert g0 == {16: 16, 24: 24, 48: 32}[cus]
     assert late0 == 0 and late1 > 0
+
+
+def test_run_prints_the_reference_result_lines(dev, capsys):
+    """lu.run, the reference driver's run on the device: its result lines
+    (benchmark.C:343-349, 374-381, 385-400) with the block-dgemm check
+    (testdgemm, benchmark.C:297-341) and the residual under 16."""
+    from charmlu_amd import lu
+    cfg = dev.LUConfig.from_argv(["1024", "128", "100"])
+    out = lu.run(cfg)
+    text = capsys.readouterr().out
+    assert "RESULT procs: 1 device\tblock size: 128" in text
+    assert "The dgemm 128 x 128 takes" in text and "GFlop/sec" in text
+    assert "Percent of DGEMM is:" in text
+    assert out["dgemm_gflops"] > 0 and out["percent_of_dgemm"] > 0
+    assert out["residual"] < 16 and out["info"] == 0
