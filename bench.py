@@ -326,7 +326,7 @@ def fp64_in_place(clu, lib, _lib, torch, A, b, nb64, n, g=None):
 
 
 def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, fp64_nb=None,
-                fp64_after=False, adjoint=False):
+                fp64_after=False, adjoint=False, nb_of=None):
     """Mixed-precision LU + FP64 GMRES-IR against the plain FP64 LU on the
     same diagonally dominant system (A U[-0.5,0.5) with a_ii = sum_j |a_ij|,
     b from MatGen).  A mixed step converts A to FP32, factors it with the
@@ -391,7 +391,9 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
         # speculative identity pivots, kept while every panel's multipliers
         # stay below 1/2), "<prec>-xchg" with partial pivoting exchanging
         # candidate rows at every panel (CHARMLU_SPECULATE=0), "<prec>-nopiv"
-        # without interchanges (the matrix is diagonally dominant)
+        # without interchanges (the matrix is diagonally dominant); nb_of:
+        # a block size of its own for some of them
+        nbk = (nb_of or {}).get(key, nb)
         prec, _, mode = key.partition("-")
         pivot = mode != "nopiv"
         env = {"CHARMLU_SPECULATE": "0"} if mode == "xchg" else {}
@@ -405,7 +407,7 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
             saved = {k: os.environ.get(k) for k in env}
             os.environ.update(env)
             try:
-                ipiv, info = clu.getrf_mixed(A32, nb, prec, pivot=pivot)
+                ipiv, info = clu.getrf_mixed(A32, nbk, prec, pivot=pivot)
                 torch.cuda.synchronize()
             finally:
                 for k, v in saved.items():
@@ -434,7 +436,7 @@ def mixed_bench(clu, lib, _lib, torch, n, nb, precs, steps, warmup, fp64_steps, 
         lib.charmlu_profile(0)
         p = list(prof)
         res = clu.residual(A, x, b)
-        rec = {"tflops": round(flops / sec / 1e12, 3), "ms": round(sec * 1e3, 1),
+        rec = {"block": nbk, "tflops": round(flops / sec / 1e12, 3), "ms": round(sec * 1e3, 1),
                "residual": res["residual"], "residual_ns": res["residual_ns"],
                "info": int(info.item()), "passed": bool(res["residual"] < 16),
                "ir_cycles": st["cycles"], "ir_iterations": st["iterations"],
@@ -528,18 +530,18 @@ def small_case(clu, torch, n, nb, steps=5):
             "residual": res["residual"], "info": int(info.item())}
 
 
-def c5_bench(clu, lib, _lib, torch, nb, fp64_nb):
+def c5_bench(clu, lib, _lib, torch, nb, fp64_nb, xchg_nb=2048):
     """C5: the FP8 form at n = 131072 (FP64 A 137 GB + FP32 factors 69 GB in
     HBM), unpivoted, pivoted (speculative identity pivots) and pivoted with
     exchanging panels, one step each after no warmup, then one
     FP64 LU + solve of the same system in place for the error and speed
-    comparison.  Block 2048 for the FP8 forms (their update reads and writes
-    the FP32 trailing matrix once per step, so a wider block halves that
-    traffic per flop: 657 / 553 TFLOP/s against 623 / 522 with 1024 on one
-    box), 1024 for the FP64 LU."""
+    comparison.  Block 1024 for the grouped forms (groups of 4 steps: one
+    update per group at K = 4096; 893 against 833 TFLOP/s with block 2048,
+    829 with 512), 2048 for the exchanging form (one update per step, whose
+    FP32 C traffic per flop halves with the block: 614 against 550), 1024
+    for the FP64 LU."""
     rec = mixed_bench(clu, lib, _lib, torch, 131072, nb, ["fp8-nopiv", "fp8", "fp8-xchg"], 1, 0,
-                      1, fp64_nb,
-                      fp64_after=True, adjoint=True)
+                      1, fp64_nb, fp64_after=True, adjoint=True, nb_of={"fp8-xchg": xchg_nb})
     rec["steps_note"] = "one timed step per form, no warmup (the first call of its shape)"
     return rec
 
@@ -567,8 +569,11 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-c5", action="store_true",
                     help="skip the FP8 form at n=131072 (C5) in the default run")
-    ap.add_argument("--c5-nb", type=int, default=2048,
-                    help="block size of the FP8 forms at n=131072 (the FP64 LU uses --mixed-fp64-nb)")
+    ap.add_argument("--c5-nb", type=int, default=1024,
+                    help="block size of the grouped FP8 forms at n=131072 (the FP64 LU uses "
+                         "--mixed-fp64-nb)")
+    ap.add_argument("--c5-xchg-nb", type=int, default=2048,
+                    help="block size of the exchanging FP8 form at n=131072")
     ap.add_argument("--c5", action="store_true",
                     help="only the FP8 form at n=131072 (block --c5-nb, diagonally dominant; "
                          "FP64 A 137 GB + FP32 factors 69 GB in HBM), one step each")
@@ -606,7 +611,8 @@ def main():
                      args.lp_prec)
         return
     if args.c5:
-        print(json.dumps({"c5": c5_bench(clu, lib, _lib, torch, args.c5_nb, args.mixed_fp64_nb)}),
+        print(json.dumps({"c5": c5_bench(clu, lib, _lib, torch, args.c5_nb, args.mixed_fp64_nb,
+                            args.c5_xchg_nb)}),
               flush=True)
         return
     if args.mixed_only:
@@ -744,7 +750,8 @@ def main():
                                    args.mixed_fp64_steps, args.mixed_fp64_nb)
     if not args.no_mixed and not args.no_c5:
         torch.cuda.empty_cache()
-        out["c5"] = c5_bench(clu, lib, _lib, torch, args.c5_nb, args.mixed_fp64_nb)
+        out["c5"] = c5_bench(clu, lib, _lib, torch, args.c5_nb, args.mixed_fp64_nb,
+                            args.c5_xchg_nb)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, nb)
     print(json.dumps(out), flush=True)
