@@ -1470,7 +1470,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if ((rc = ensure_bytes(&ws.uinv32, &ws.uinv32_b, sizeof(float) * nb * nb))) return rc;
       if ((rc = ensure_bytes(&ws.l21buf, &ws.l21buf_b, sizeof(float) * nb * n))) return rc;
     }
-    if (spec && (rc = ensure_bytes(&ws.specbak, &ws.specbak_b, sizeof(float) * nb * nb)))
+    if (spec && (rc = ensure_bytes(&ws.specbak, &ws.specbak_b, sizeof(float) * 4 * nb * nb)))
       return rc;
   }
   float *ubuf = (float *)ws.ubuf32;
@@ -1758,57 +1758,77 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
     prof_end(pid, st);
     return r;
   };
-  // L21 = A21 inv(U11) of step t into l21buf
-  auto l21_of = [&](int t) -> int {
-    const int r0 = t * nb, Nt = n - r0 - nb;
+  // rows [row0, row0 + rows) of L21 = A21 inv(U11) of step t into the same
+  // rows of l21buf
+  auto l21_rows = [&](int t, int row0, int rows) -> int {
+    if (rows <= 0) return CLU_OK;
+    const int r0 = t * nb;
     float *A11 = dA + (long)r0 * lda + r0;
-    const int pid = prof_begin(PK_GEMM_OTHER, 2.0 * Nt * nb * nb, sm);
-    int e = gemm_upper_right<float>(Nt, nb, A11 + (long)nb * lda, lda, (const float *)ws.uinv32,
-                                    nb, (float *)ws.l21buf, nb, sm);
+    const int pid = prof_begin(PK_GEMM_OTHER, 2.0 * rows * nb * nb, sm);
+    int e = gemm_upper_right<float>(rows, nb, A11 + (long)(nb + row0) * lda, lda,
+                                    (const float *)ws.uinv32, nb,
+                                    (float *)ws.l21buf + (size_t)row0 * nb, nb, sm);
     prof_end(pid, sm);
     return e;
   };
-  // U row of step t, columns [j0, j1) after its block, packed into K segment
-  // t % G of group t / G (global rows and columns; FP8 scales per global
-  // 128-row tile), with j0 == 0 also its L21; both stored into the factor by
-  // the same passes
-  auto u_row2 = [&](int t, int j0, int j1) -> int {
+  // U row of step t, columns [j0, j1) after its block, into ubuf
+  auto urow_gemm = [&](int t, int j0, int j1) -> int {
+    const int r0 = t * nb, Nt = n - r0 - nb;
+    float *A11 = dA + (long)r0 * lda + r0;
+    return gemm_lower_left<float>(nb, j1 - j0, winv_of(t), nb, A11 + nb + j0, lda, ubuf + j0, Nt,
+                                  sm);
+  };
+  // those columns packed into K segment t % G of group t / G (global columns;
+  // FP8 scales per global 128-column tile), with copy also stored into the
+  // factor by the same pass
+  auto pack_u = [&](int t, int j0, int j1, bool copy) -> int {
     const int p = t / G, sg = t % G, r0 = t * nb, r1 = r0 + nb, Nt = n - r1;
     if (j1 <= j0) return CLU_OK;
-    float *A11 = dA + (long)r0 * lda + r0;
-    const int pid = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
-    int e = gemm_lower_left<float>(nb, j1 - j0, winv_of(t), nb, A11 + nb + j0, lda, ubuf + j0,
-                                   Nt, sm);
-    float *ucp = A11 + nb + j0, *lcp = A11 + (long)nb * lda;
-    const float *l21 = (const float *)ws.l21buf;
-    const size_t ro = (size_t)r1 * ldg + (size_t)sg * Kp;
+    float *ucp = copy ? dA + (long)r0 * lda + r0 + nb + j0 : nullptr;
     const size_t co = (size_t)(r1 + j0) * ldg + (size_t)sg * Kp;
-    if (!e && fp8) {
-      int *escL = ESC(p, sg), *escU = escL + T;
-      float *partL = (float *)(escU + T), *partU = partL + (size_t)FP8_PART_PER_TILE * T;
-      if (j0 == 0)
-        e = quant_rows_fp8(Nt, nb, Kp, -1.0f, l21, nb, (uint8_t *)PA(p) + ro, escL + r1 / 128,
-                           partL + (size_t)FP8_PART_PER_TILE * (r1 / 128), sm, lcp, lda, ldg);
-      if (!e)
-        e = quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)PU(p) + co,
-                           escU + (r1 + j0) / 128,
-                           partU + (size_t)FP8_PART_PER_TILE * ((r1 + j0) / 128), sm, ucp, lda,
-                           ldg);
-    } else if (!e && prec == CHARMLU_PREC_BF16) {
-      if (j0 == 0)
-        e = pack_rows<__bf16>(Nt, nb, Kp, -1.0f, l21, nb, (__bf16 *)PA(p) + ro, sm, lcp, lda,
-                              ldg);
-      if (!e)
-        e = pack_cols<__bf16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (__bf16 *)PU(p) + co, sm, ucp, lda,
-                              ldg);
-    } else if (!e) {
-      if (j0 == 0)
-        e = pack_rows<_Float16>(Nt, nb, Kp, -1.0f, l21, nb, (_Float16 *)PA(p) + ro, sm, lcp, lda,
-                                ldg);
-      if (!e)
-        e = pack_cols<_Float16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (_Float16 *)PU(p) + co, sm, ucp,
-                                lda, ldg);
+    if (fp8) {
+      int *escU = ESC(p, sg) + T;
+      float *partU = (float *)(escU + T) + (size_t)FP8_PART_PER_TILE * T;
+      return quant_cols_fp8(nb, j1 - j0, Kp, ubuf + j0, Nt, (uint8_t *)PU(p) + co,
+                            escU + (r1 + j0) / 128,
+                            partU + (size_t)FP8_PART_PER_TILE * ((r1 + j0) / 128), sm, ucp, lda,
+                            ldg);
     }
+    if (prec == CHARMLU_PREC_BF16)
+      return pack_cols<__bf16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (__bf16 *)PU(p) + co, sm, ucp, lda,
+                               ldg);
+    return pack_cols<_Float16>(nb, j1 - j0, Kp, ubuf + j0, Nt, (_Float16 *)PU(p) + co, sm, ucp,
+                               lda, ldg);
+  };
+  // rows [row0, row0 + rows) of step t's L21 (l21buf) packed into its K
+  // segment (global rows; FP8 scales per global 128-row tile), with copy
+  // also stored into the factor
+  auto pack_l = [&](int t, int row0, int rows, bool copy) -> int {
+    const int p = t / G, sg = t % G, r0 = t * nb, r1 = r0 + nb;
+    if (rows <= 0) return CLU_OK;
+    const float *l21 = (const float *)ws.l21buf + (size_t)row0 * nb;
+    float *lcp = copy ? dA + (long)(r1 + row0) * lda + r0 : nullptr;
+    const size_t ro = (size_t)(r1 + row0) * ldg + (size_t)sg * Kp;
+    if (fp8) {
+      int *escL = ESC(p, sg);
+      float *partL = (float *)(escL + 2 * T);
+      return quant_rows_fp8(rows, nb, Kp, -1.0f, l21, nb, (uint8_t *)PA(p) + ro,
+                            escL + (r1 + row0) / 128,
+                            partL + (size_t)FP8_PART_PER_TILE * ((r1 + row0) / 128), sm, lcp, lda,
+                            ldg);
+    }
+    if (prec == CHARMLU_PREC_BF16)
+      return pack_rows<__bf16>(rows, nb, Kp, -1.0f, l21, nb, (__bf16 *)PA(p) + ro, sm, lcp, lda,
+                               ldg);
+    return pack_rows<_Float16>(rows, nb, Kp, -1.0f, l21, nb, (_Float16 *)PA(p) + ro, sm, lcp, lda,
+                               ldg);
+  };
+  // U row columns [j0, j1) of step t, packed and stored into the factor
+  auto u_row2 = [&](int t, int j0, int j1) -> int {
+    if (j1 <= j0) return CLU_OK;
+    const int pid = prof_begin(PK_TRSM, (double)(j1 - j0) * nb * (nb - 1), sm);
+    int e = urow_gemm(t, j0, j1);
+    if (!e) e = pack_u(t, j0, j1, true);
     prof_end(pid, sm);
     return e;
   };
@@ -1848,11 +1868,16 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   // for its flag before it enqueues the step's U row).
   constexpr float SPEC_THR = 0.5f;
   const float tiny32 = 100.0f * 1.1920929e-07f;  // the panel's singular-pivot bound
+  // copies of diagonal blocks, by step parity: before the block's
+  // speculative panel (slot 0) and before its first update of the step
+  // before (slot 2; a failed check of that step restores it)
+  auto bak = [&](int slot, int s) {
+    return (float *)ws.specbak + (size_t)(slot + (s & 1)) * nb * nb;
+  };
   auto gpanel = [&](int s) -> int {
     if (spec) {
       const int r0 = s * nb;
-      int e = convert<float, float>(nb, nb, dA + (long)r0 * lda + r0, lda, (float *)ws.specbak, nb,
-                                    sp);
+      int e = convert<float, float>(nb, nb, dA + (long)r0 * lda + r0, lda, bak(0, s), nb, sp);
       if (e) return e;
     }
     return panel(s, true);
@@ -1887,9 +1912,16 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       return CLU_OK;
     };
     // the fall back at step s (index i of group p)
-    auto fallback = [&](int s, int i, int p) -> int {
+    auto fallback = [&](int s, int i, int p, bool next_ran) -> int {
       int e;
       const int r0 = s * nb, r1 = r0 + nb, Nt = n - r1;
+      if (next_ran) {
+        // panel(s + 1) eliminated its diagonal block after that block's
+        // update by step s: back to the block before that update
+        CLU_HIP(hipStreamWaitEvent(sm, ws.ev_panel, 0));
+        float *A22 = dA + (long)r1 * lda + r1;
+        if ((e = convert<float, float>(nb, nb, bak(2, s + 1), nb, A22, lda, sm))) return e;
+      }
       if (y_full) {
         CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
         y_full = false;
@@ -1897,8 +1929,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       if (yp >= 0 && ydone < n - y0 && (e = y_part(n - y0 - ydone, sm))) return e;
       if (i > 0 && Nt > 0 && (e = updg(p, i, r1, r1, Nt, Nt, sm))) return e;
       float *A11 = dA + (long)r0 * lda + r0;
-      if ((e = convert<float, float>(nb, nb, (const float *)ws.specbak, nb, A11, lda, sm)))
-        return e;
+      if ((e = convert<float, float>(nb, nb, bak(0, s), nb, A11, lda, sm))) return e;
       if ((e = fill_int(dinfo, 0, 1, sm))) return e;  // a tiny pivot of the elimination
       CLU_HIP(hipEventRecord(ws.ev_la, sm));
       CLU_HIP(hipStreamWaitEvent(sp, ws.ev_la, 0));
@@ -1907,17 +1938,24 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
       full_pending = false;
       return per_step(s);
     };
-    // the check of step s's elimination (L21 in l21buf unless it is the last)
-    auto spec_ok = [&](int s, int Nt) -> int {
+    // the check of step s's elimination (L21 in l21buf unless it is the
+    // last), enqueued; then its verdict (1 keep, 0 fall back, -1 error)
+    auto spec_enqueue = [&](int s, int Nt) -> int {
       const int r0 = s * nb;
       int e = spec_check(Nt, nb, (const float *)ws.l21buf, nb, dA + (long)r0 * lda + r0, lda,
                          SPEC_THR, tiny32, ws.spec_flag, sm);
-      if (e) return -1;
-      if (hipMemcpyAsync(ws.spec_hflag, ws.spec_flag, sizeof(int), hipMemcpyDeviceToHost, sm) !=
-              hipSuccess ||
-          hipEventRecord(ws.ev_chk, sm) != hipSuccess || hipEventSynchronize(ws.ev_chk) != hipSuccess)
-        return -1;
+      if (e) return e;
+      CLU_HIP(hipMemcpyAsync(ws.spec_hflag, ws.spec_flag, sizeof(int), hipMemcpyDeviceToHost, sm));
+      CLU_HIP(hipEventRecord(ws.ev_chk, sm));
+      return CLU_OK;
+    };
+    auto spec_verdict = [&]() -> int {
+      if (hipEventSynchronize(ws.ev_chk) != hipSuccess) return -1;
       return *ws.spec_hflag == 0 ? 1 : 0;
+    };
+    auto spec_ok = [&](int s, int Nt) -> int {
+      if (spec_enqueue(s, Nt)) return -1;
+      return spec_verdict();
     };
     for (int a = 0; a < NB; a += G) {
       const int p = a / G, gl = NB - a < G ? NB - a : G;
@@ -1931,7 +1969,7 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
               set_error("the check of a speculative panel failed to run");
               return CLU_ERR_HIP;
             }
-            if (!ok) return fallback(s, i, p);
+            if (!ok) return fallback(s, i, p, false);
           }
           break;
         }
@@ -1945,49 +1983,75 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
           CLU_HIP(hipEventRecord(ws.ev_full, sf));
           y_full = true;
         }
-        if ((rc = l21_of(s))) return rc;
+        const bool last = i == gl - 1;
+        const int ns = last ? gl : i + 1;  // K segments of block column s + 1's update
+        // Between panel(s) and panel(s + 1) only what the next panel needs:
+        // the first block row of L21, the U row of block column s + 1 (both
+        // packed, their copies into the factor after the check) and the
+        // diagonal block of s + 1 brought up to date (its copy first, for a
+        // failed check)
+        int pid = prof_begin(PK_TRSM, (double)nb * nb * (nb - 1), sm);
+        if ((rc = l21_rows(s, 0, nb))) return rc;
+        if ((rc = urow_gemm(s, 0, nb))) return rc;
+        if ((rc = pack_u(s, 0, nb, false))) return rc;
+        if ((rc = pack_l(s, 0, nb, false))) return rc;
+        prof_end(pid, sm);
+        if (last && y_full) {  // the previous group's rest holds the next block
+          CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
+          y_full = false;
+        }
+        if (spec && (rc = convert<float, float>(nb, nb, dA + (long)r1 * lda + r1, lda,
+                                                bak(2, s + 1), nb, sm)))
+          return rc;
+        if ((rc = updg(p, ns, r1, r1, nb, nb, sm))) return rc;
+        CLU_HIP(hipEventRecord(ws.ev_la, sm));
+        // beside panel(s + 1): the rest of L21, then (speculative) the check
+        if ((rc = l21_rows(s, nb, Nt - nb))) return rc;
+        bool panel_done = false;
         if (spec) {
-          const int ok = spec_ok(s, Nt);
+          if ((rc = spec_enqueue(s, Nt))) return rc;
+          if (s + 1 < NB && (rc = panel_step(s + 1))) return rc;
+          panel_done = true;
+          const int ok = spec_verdict();
           if (ok < 0) {
             set_error("the check of a speculative panel failed to run");
             return CLU_ERR_HIP;
           }
-          if (!ok) return fallback(s, i, p);
+          if (!ok) return fallback(s, i, p, s + 1 < NB);
         }
-        if ((rc = u_row2(s, 0, nb))) return rc;
-        if (i < gl - 1) {
-          // block column s + 1 with the group's steps a .. s, then beside
-          // panel(s + 1): the rest of the U row, block row s + 1 likewise,
-          // and rows of the previous group's rest
-          if ((rc = updg(p, i + 1, r1, r1, Nt, nb, sm))) return rc;
-          CLU_HIP(hipEventRecord(ws.ev_la, sm));
-          if ((rc = u_row2(s, nb, Nt))) return rc;
+        // the step's factor copies, the rest of its L21 and of block column
+        // s + 1, the rest of its U row
+        pid = prof_begin(PK_TRSM, (double)(Nt - nb) * nb * nb, sm);
+        if ((rc = convert<float, float>(nb, nb, (const float *)ws.l21buf, nb,
+                                        dA + (long)r1 * lda + s * nb, lda, sm)))
+          return rc;
+        if ((rc = convert<float, float>(nb, nb, ubuf, Nt, dA + (long)(s * nb) * lda + r1, lda,
+                                        sm)))
+          return rc;
+        if ((rc = pack_l(s, nb, Nt - nb, true))) return rc;
+        prof_end(pid, sm);
+        if ((rc = updg(p, ns, r1 + nb, r1, Nt - nb, nb, sm))) return rc;
+        if ((rc = u_row2(s, nb, Nt))) return rc;
+        if (!last) {
+          // block row s + 1 with the group's steps a .. s, then rows of the
+          // previous group's rest
           if ((rc = updg(p, i + 1, r1, r1 + nb, nb, Nt - nb, sm))) return rc;
           if (yp >= 0 && (rc = y_part(fill_rows(s + 1, yg, n - y0 - ydone, n - y0), sm)))
             return rc;
-          if ((rc = panel_step(s + 1))) return rc;
-          continue;
+        } else {
+          // the group's update of the rest (gl segments): the next group's
+          // other block columns and its block rows, then the first rows of
+          // the rest
+          const int W = Nt < G * nb ? Nt : G * nb;
+          if ((rc = updg(p, gl, r1, r1 + nb, Nt, W - nb, sm))) return rc;
+          if ((rc = updg(p, gl, r1, r1 + W, W, Nt - W, sm))) return rc;
+          yp = p;
+          yg = gl;
+          y0 = r1 + W;
+          ydone = 0;
+          if (y0 < n && (rc = y_part(fill_rows(s + 1, yg, n - y0, n - y0), sm))) return rc;
         }
-        // the group's last step: its update of the rest (gl segments), the
-        // next panel's block column first
-        if (y_full) {
-          CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
-          y_full = false;
-        }
-        const int W = Nt < G * nb ? Nt : G * nb;  // the next group's blocks
-        if ((rc = updg(p, gl, r1, r1, Nt, nb, sm))) return rc;
-        CLU_HIP(hipEventRecord(ws.ev_la, sm));
-        // beside panel(s + 1): the rest of the U row, the next group's other
-        // block columns and its block rows, then the first rows of the rest
-        if ((rc = u_row2(s, nb, Nt))) return rc;
-        if ((rc = updg(p, gl, r1, r1 + nb, Nt, W - nb, sm))) return rc;
-        if ((rc = updg(p, gl, r1, r1 + W, W, Nt - W, sm))) return rc;
-        yp = p;
-        yg = gl;
-        y0 = r1 + W;
-        ydone = 0;
-        if (y0 < n && (rc = y_part(fill_rows(s + 1, yg, n - y0, n - y0), sm))) return rc;
-        if (s + 1 < NB && (rc = panel_step(s + 1))) return rc;
+        if (!panel_done && s + 1 < NB && (rc = panel_step(s + 1))) return rc;
       }
     }
     if (y_full) CLU_HIP(hipStreamWaitEvent(sm, ws.ev_full, 0));
