@@ -335,15 +335,10 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
   // E8M0 block scales of the two operands (2^(s - 127) undoes 2^esc); a
   // wave's 64 rows and 64 columns each lie in one 128-row scale tile.  K
   // runs over nseg segments of ks stages, products of their own scaling (up
-  // to 4 updates of C summed in one pass over it), whose exponents lie
-  // estride apart
-  int sav[4], sbv[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const long o = (q < nseg ? q : 0) * estride;
-    sav[q] = 127 - escA[o + ((m0 + wm0) >> 7)];
-    sbv[q] = 127 - escB[o + ((n0 + wn0) >> 7)];
-  }
+  // to 16 updates of C summed in one pass over it), whose exponents lie
+  // estride apart; the current segment's pair is reloaded at each boundary
+  const int ta = (m0 + wm0) >> 7, tb = (n0 + wn0) >> 7;
+  int sa = 127 - escA[ta], sb = 127 - escB[tb];
   int seg = 0, seg_end = ks;
 
   // The matrix-core op's first operand is the B tile (32 C columns), its
@@ -430,12 +425,12 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
     __asm__ volatile("" ::: "memory");
     if (kc + NST - 1 < nk) issue(kc + NST - 1);
     const char *st = smem + (kc % NST) * STAGE;
-    if (kc == seg_end) {
+    if (kc == seg_end && seg + 1 < nseg) {
       ++seg;
       seg_end += ks;
+      sa = 127 - escA[seg * estride + ta];
+      sb = 127 - escB[seg * estride + tb];
     }
-    const int sa = seg == 0 ? sav[0] : seg == 1 ? sav[1] : seg == 2 ? sav[2] : sav[3];
-    const int sb = seg == 0 ? sbv[0] : seg == 1 ? sbv[1] : seg == 2 ? sbv[2] : sbv[3];
     int8v_t a[2], b[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -512,7 +507,7 @@ int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const
       ((uintptr_t)B & 15) || !escA || !escB)
     return -1;
   // nseg segments of kseg K indices each, with their own scales
-  if (nseg < 1 || nseg > 4 || (nseg > 1 && (kseg <= 0 || kseg % 64 != 0 || kseg * nseg != K)))
+  if (nseg < 1 || nseg > 16 || (nseg > 1 && (kseg <= 0 || kseg % 64 != 0 || kseg * nseg != K)))
     return -1;
   const int ks = nseg > 1 ? kseg / 64 : K / 64;
   const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;

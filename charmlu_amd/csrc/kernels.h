@@ -39,9 +39,9 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
 // C = alpha * A * B^T + beta * C with FP8 e4m3 operands (A: M x K, B:
 // N x K, K a multiple of 64) carrying power-of-two scales per 128-row tile
 // (escA[M/128], escB[N/128]: the stored values are v * 2^esc).  With nseg
-// > 1 (at most 4), K is nseg segments of kseg indices (a multiple of 64),
+// > 1 (at most 16), K is nseg segments of kseg indices (a multiple of 64),
 // segment q scaled by escA[q * estride + ...] and escB[q * estride + ...]:
-// up to 4 products of their own scaling summed in one pass over C.
+// up to 16 products of their own scaling summed in one pass over C.
 int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
              const uint8_t *B, long ldb, const int *escB, float beta, float *C, long ldc,
              hipStream_t s, int kseg = 0, int nseg = 1, long estride = 0);

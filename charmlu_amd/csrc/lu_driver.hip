@@ -485,12 +485,15 @@ static bool speculate_enabled() {
   return !(e && e[0] == '0');
 }
 // steps per group of the unpivoted low-precision form (CHARMLU_GROUP: 1 for
-// one trailing update per step, as the other forms, up to 4; default
+// one trailing update per step, as the other forms, up to 16; default
 // group_default)
 static int group_steps(int group_default) {
   const char *e = getenv("CHARMLU_GROUP");
-  if (!e || e[0] < '1' || e[0] > '4') return group_default;
-  return e[0] - '0';
+  if (!e) return group_default;
+  char *end = nullptr;
+  const long g = strtol(e, &end, 10);
+  if (end == e || *end != 0 || g < 1 || g > 16) return group_default;
+  return (int)g;
 }
 static int forced_split(int M) {
   const int Ma = (M / 2 + 127) / 128 * 128;
@@ -1436,14 +1439,14 @@ int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipi
   // step parity (the second part of step t still reads step t's)
   const bool early = lp && nopiv;
   // ... and its steps run in groups of G that update the rest of the matrix
-  // once (below).  G = 4 measured best (at n = 65536, block 1024, FP8 / BF16:
-  // 465 / 400, 475 / 409, 479 / 418 TFLOP/s with G = 2, 3, 4; n = 131072,
-  // block 2048, FP8: 790, 804, 817)
+  // once (below).  Measured (block 1024, pivoted with speculative identity
+  // pivots), G = 4, 8, 12, 16: n = 65536 BF16 444, 458, 467, 451 and FP8
+  // 537, 552, 561, 565 TFLOP/s; n = 131072 FP8 891, 952, 966, 975
   // The pivoted low-precision form runs the grouped unpivoted schedule as
   // long as its panels show that partial pivoting would keep every diagonal
   // row (below; its unpivoted panels take nb <= 2048)
   const bool spec_req = lp && !nopiv && nb <= 2048 && speculate_enabled();
-  const int G = (early || spec_req) ? group_steps(4) : 1;
+  const int G = (early || spec_req) ? group_steps(fp8 ? 16 : 12) : 1;
   const bool grouped = G > 1;
   const bool spec = grouped && !nopiv;
   const size_t nbuf = grouped ? 2 * G : early ? 2 : 1;

@@ -196,7 +196,8 @@ int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda,
  * once per panel, so the call returns when the last panel has been
  * checked; CHARMLU_SPECULATE=0 exchanges at every panel).  The unpivoted
  * low-precision schedule updates the rest of the matrix once per group of
- * up to 4 steps (CHARMLU_GROUP).  Asynchronous otherwise. */
+ * steps (CHARMLU_GROUP, 1 to 16; 16 for FP8, 12 otherwise).  Asynchronous
+ * otherwise. */
 int charmlu_sgetrf_mixed(int n, int nb, int prec, float *dA, long lda, int *dipiv,
                          int *dinfo, void *stream);
 

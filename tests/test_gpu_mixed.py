@@ -489,7 +489,8 @@ def test_forced_split_is_bitwise_unsplit_mixed(dev, orc, prec, pivot, n, nb):
 GROUP_CASES = [(prec, g, n, nb) for prec in ("bf16", "fp8") for g in (2, 3, 4)
                for n, nb in ((4096, 256), (2560, 512))] + [
     ("fp16", 2, 1536, 512), ("fp16", 3, 4096, 256), ("fp8", 2, 3072, 1024),
-    ("fp8", 3, 3072, 1024), ("bf16", 2, 1536, 512)]
+    ("fp8", 3, 3072, 1024), ("bf16", 2, 1536, 512), ("fp8", 8, 4096, 256),
+    ("bf16", 8, 4096, 256), ("fp8", 6, 2560, 512), ("fp8", 7, 3840, 256)]
 
 
 @pytest.mark.parametrize("prec,g,n,nb", GROUP_CASES)
@@ -499,7 +500,7 @@ def test_grouped_steps_match_per_step_form(dev, orc, prec, g, n, nb):
     a scale set per step; CHARMLU_GROUP=g); CHARMLU_GROUP=1 updates it once
     per step.  Both factor within the update's roundoff bound of the FP64
     matrix, agree to a few units of it, and refine to the FP64 oracle's x
-    (3 to 16 steps, groups cut short at the end)."""
+    (3 to 16 steps, groups of 2 to 8, cut short at the end)."""
     import os
     import torch
     A = orc.gen_diagdom(orc.SEED_A + 7, n)
