@@ -31,6 +31,7 @@
 // are the instruction's E8M0 block scales (2^(s - 127)), so products come
 // out unscaled and C is used as it is.
 #include <hip/hip_bf16.h>
+#include <stdlib.h>
 
 #include "common.h"
 #include "kernels.h"
@@ -60,27 +61,34 @@ template <> struct Lp<_Float16> {
 typedef int int8v_t __attribute__((ext_vector_type(8)));
 typedef int int4v_t __attribute__((ext_vector_type(4)));
 
+// s_waitcnt vmcnt(N): on gfx9 the counter's low 4 bits are bits 3:0 of the
+// immediate and its high 2 bits are bits 15:14 (expcnt and lgkmcnt left at
+// their maxima, i.e. not waited for)
 template <int N>
 CLU_DEV void wait_vm_lp() {
-  static_assert(N >= 0 && N < 16, "vmcnt");
-  __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8));
+  static_assert(N >= 0 && N < 64, "vmcnt");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
 }
 
-template <typename T16, int BM, int BN, int NST, int OCC, bool CINIT, bool VEC>
-__global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kernel(
+template <typename T16, int BM, int BN, int WM, int WN, int NST, int OCC, bool CINIT, bool VEC,
+          bool EDGE = true, int GMT = 0>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, OCC) void hgemm_nt_kernel(
     int M, int N, int K, float alpha, const T16 *__restrict__ A, long lda,
     const T16 *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
     int tiles_m, int tiles_n) {
-  // BM x BN C tile, one wave per 64 x 64 block of it (BM / 64 x BN / 64
-  // waves).  A stage holds 64 bytes of K per row: 32 16-bit values.
-  constexpr int NT = (BM / 64) * (BN / 64) * 64;  // threads
+  // BM x BN C tile, one wave per WM x WN block of it (WM / 16 x WN / 16
+  // blocks of 16 x 16).  A stage holds 64 bytes of K per row: 32 16-bit
+  // values.  EDGE = false: every tile lies inside C.
+  constexpr int NT = (BM / WM) * (BN / WN) * 64;  // threads
   constexpr int EPC = 16 / sizeof(T16);  // elements per 16-byte chunk
   constexpr int BK = 4 * EPC;
-  constexpr int WGN = BN / 64;
+  constexpr int WGN = BN / WN;
+  constexpr int BI = WN / 16, BJ = WM / 16;  // 16 x 16 blocks per wave (columns, rows)
   constexpr int HALF = BM * 64;          // bytes of the A operand's stage
   constexpr int STAGE = (BM + BN) * 64;
   constexpr int CA = BM * 4 / NT, CB = BN * 4 / NT;  // 16-byte chunks per thread
-  static_assert(BM % 64 == 0 && BN % 64 == 0 && CA >= 1 && CB >= 1, "tile");
+  static_assert(BM % WM == 0 && BN % WN == 0 && WM % 16 == 0 && WN % 16 == 0, "tile");
+  static_assert(CA >= 1 && CB >= 1 && CA * NT == BM * 4 && CB * NT == BN * 4, "staging");
   __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
   typedef typename Lp<T16>::vec_t vec_t;
 
@@ -95,7 +103,7 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
     const int bid = blockIdx.x, nblk = gridDim.x, per = nblk / 8;
     int lid = bid;
     if (bid < per * 8) lid = (bid % 8) * per + bid / 8;
-    constexpr int GM = 512 / BM;
+    constexpr int GM = GMT > 0 ? GMT : 512 / BM;
     const int group_size = GM * tiles_n, group = lid / group_size, first_m = group * GM;
     const int gm = min(tiles_m - first_m, GM), in_group = lid - group * group_size;
     tm = first_m + in_group % gm;
@@ -103,24 +111,24 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
   }
   const int m0 = tm * BM, n0 = tn * BN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm0 = (wave / WGN) * 64, wn0 = (wave % WGN) * 64;
+  const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
   const int lr = lane & 15, lq = lane >> 4;
-  const bool full = m0 + BM <= M && n0 + BN <= N;
+  const bool full = !EDGE || (m0 + BM <= M && n0 + BN <= N);
 
   // The MFMA's first operand is the B tile (16 C columns), its second the A
   // tile (16 C rows), so D[r][c] of block (i, j) is C[row][col] with
   // row = m0 + wm0 + 16 j + lr and col = n0 + wn0 + 16 i + 4 lq + r: each
   // lane owns 4 consecutive columns of one row (one 16-byte access).
-  float4_t acc[4][4];
+  float4_t acc[BI][BJ];
   auto crow = [&](int j) { return m0 + wm0 + 16 * j + lr; };
   auto ccol = [&](int i) { return n0 + wn0 + 16 * i + 4 * lq; };
   if (CINIT) {
     // C += A B^T: C itself is the initial accumulator, requested before the
     // first operand stage so its latency overlaps the K loop
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < BJ; ++j) {
         const int row = crow(j), col = ccol(i);
         if (VEC && full) {
           const float4_t *cp4 = reinterpret_cast<const float4_t *>(C + (long)row * ldc + col);
@@ -133,9 +141,9 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
       }
   } else {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+      for (int j = 0; j < BJ; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
   }
 
   // this thread's chunks of A and of B per stage (rows past M or N are
@@ -169,15 +177,19 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
       __builtin_amdgcn_global_load_lds((gbl_ptr_t)(gb[r] + kk), (lds_ptr_t)(st + loffb[r]), 16, 0,
                                        0);
   };
-  int aoff[4], boff[4];
+  int aoff[BJ], boff[BI];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int m = wm0 + 16 * i + lr, n = wn0 + 16 * i + lr;
-    aoff[i] = m * 64 + ((lq ^ lds_swz(m)) << 4);
+  for (int j = 0; j < BJ; ++j) {
+    const int m = wm0 + 16 * j + lr;
+    aoff[j] = m * 64 + ((lq ^ lds_swz(m)) << 4);
+  }
+#pragma unroll
+  for (int i = 0; i < BI; ++i) {
+    const int n = wn0 + 16 * i + lr;
     boff[i] = HALF + n * 64 + ((lq ^ lds_swz(n)) << 4);
   }
 
-  static_assert(NST >= 2 && NST <= 5, "stages");
+  static_assert(NST >= 2 && NST <= 5 && (CA + CB) * (NST - 2) < 64, "stages");
 #pragma unroll
   for (int st = 0; st < NST - 1; ++st)
     if (st < nk) issue(st);
@@ -196,23 +208,23 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
     __asm__ volatile("" ::: "memory");
     if (kc + NST - 1 < nk) issue(kc + NST - 1);
     const char *st = smem + (kc % NST) * STAGE;
-    vec_t a[4], b[4];
+    vec_t a[BJ], b[BI];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const vec_t *>(st + aoff[i]);
+    for (int j = 0; j < BJ; ++j) a[j] = *reinterpret_cast<const vec_t *>(st + aoff[j]);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const vec_t *>(st + boff[j]);
+    for (int i = 0; i < BI; ++i) b[i] = *reinterpret_cast<const vec_t *>(st + boff[i]);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = Lp<T16>::mfma(b[i], a[j], acc[i][j]);
+      for (int j = 0; j < BJ; ++j) acc[i][j] = Lp<T16>::mfma(b[i], a[j], acc[i][j]);
   }
   wait_vm_lp<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
 
   // epilogue: CINIT stores the accumulators; otherwise C = alpha acc + beta C
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < BI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < BJ; ++j) {
       const int row = crow(j), col = ccol(i);
       float *cp = C + (long)row * ldc + col;
       if (VEC && full) {
@@ -235,13 +247,29 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void hgemm_nt_kern
     }
 }
 
-template <typename T16, int BMv, int BNv, int NS, int OC, bool CI, bool V>
+template <typename T16, int BMv, int BNv, int NS, int OC, bool CI, bool V, int WMv = 64,
+          int WNv = 64, bool E = true, int GMv = 0>
 static void hgemm_launch(int M, int N, int K, float alpha, const T16 *A, long lda, const T16 *B,
                          long ldb, float beta, float *C, long ldc, hipStream_t s) {
-  constexpr int NT = (BMv / 64) * (BNv / 64) * 64;
-  const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
-  hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, BNv, NS, OC, CI, V>), dim3(tm * tn), dim3(NT), 0,
-                     s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
+  constexpr int NT = (BMv / WMv) * (BNv / WNv) * 64;
+  const int tm = E ? (M + BMv - 1) / BMv : M / BMv, tn = E ? (N + BNv - 1) / BNv : N / BNv;
+  if (tm <= 0 || tn <= 0) return;
+  hipLaunchKernelGGL((hgemm_nt_kernel<T16, BMv, BNv, WMv, WNv, NS, OC, CI, V, E, GMv>),
+                     dim3(tm * tn), dim3(NT), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                     tm, tn);
+}
+
+// Large accumulate-into-C updates take 256 x 256 tiles over the whole tiles
+// of C, walked in groups of 4 tile rows, and 128 x 128 tiles for the rest
+// (FP8: four waves of 128 x 128 per tile, the accumulators in the AGPRs;
+// 16-bit: 16 waves of 64 x 64) unless CHARMLU_LP_WIDE=0 (read at each call)
+// selects one launch of 256 x 256 (16-bit) or 128 x 256 (FP8) tiles of
+// 64 x 64 per wave walked in groups of 2.  Each element of C receives the
+// same products in the same order from either, so the choice does not
+// change C.
+static bool lp_wide() {
+  const char *e = getenv("CHARMLU_LP_WIDE");
+  return !(e && *e == '0');
 }
 // 128 x 128 tiles, 2 stages, 4 workgroups per CU (measured at 65280^2 x 256
 // BF16: 212 TFLOP/s against 188 with 4 stages and 2 per CU, 198 with 64 x
@@ -262,6 +290,26 @@ static void hgemm_launch2(bool cinit, bool vec, int M, int N, int K, float alpha
   // 507 against 433 inside the LU at n = 65536.  Only while the grid keeps
   // every CU busy with them (at least 2^25 C elements).
   const bool large = (long)M * N >= (1L << 25);
+  if (cinit && vec && K >= 1024 && large && lp_wide()) {
+    // whole 256 x 256 tiles walked in groups of 4 tile rows per XCD, then
+    // the rows below and the columns beside them with 128 x 128 tiles (the
+    // same per-element products in the same order).  Kernel time at
+    // 53248^2 x 12288 BF16 (the update of a group of 12 steps of 1024):
+    // 1010 TFLOP/s against 980 with groups of 2 tile rows over all tiles;
+    // four waves of 128 x 128 instead of 16 of 64 x 64: 980 (4 stages) and
+    // 968 (3 stages); at 61440^2 x 4096 928 against 886, at 64512^2 x 1024
+    // 619 against 596.
+    const int Mw = M / 256 * 256, Nw = N / 256 * 256;
+    hgemm_launch<T16, 256, 256, 4, 1, true, true, 64, 64, false, 4>(Mw, Nw, K, alpha, A, lda, B,
+                                                                    ldb, beta, C, ldc, s);
+    if (M > Mw)
+      hgemm_launch<T16, 128, 128, 3, 3, true, true>(M - Mw, N, K, alpha, A + (long)Mw * lda, lda,
+                                                    B, ldb, beta, C + (long)Mw * ldc, ldc, s);
+    if (N > Nw && Mw > 0)
+      hgemm_launch<T16, 128, 128, 3, 3, true, true>(Mw, N - Nw, K, alpha, A, lda,
+                                                    B + (long)Nw * ldb, ldb, beta, C + Nw, ldc, s);
+    return;
+  }
   if (cinit && vec && K >= 1024 && large) {
     hgemm_launch<T16, 256, 256, 4, 1, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
     return;
@@ -299,21 +347,26 @@ int hgemm_nt(int M, int N, int K, float alpha, const T16 *A, long lda, const T16
 // quantized with power-of-two scales per 128-row tile (escA[g]: rows 128 g ..
 // of A were multiplied by 2^escA[g] before rounding; escB likewise per 128
 // rows of B, i.e. per 128 columns of C).  BM x BN C tile, one wave per
-// 64 x 64 block (2 x 2 blocks of 32 x 32), K in stages of 64 bytes per row
-// staged as in hgemm_nt_kernel.
-template <int BM, int BN, int NST, int OCC, bool CINIT, bool VEC>
-__global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kernel(
+// WM x WN block of it (WM / 32 x WN / 32 blocks of 32 x 32), K in stages of
+// 64 bytes per row staged as in hgemm_nt_kernel.  EDGE = false: every tile
+// lies inside C (the caller launches only whole tiles).
+template <int BM, int BN, int WM, int WN, int NST, int OCC, bool CINIT, bool VEC, bool EDGE,
+          int GMT = 0>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, OCC) void fgemm_nt_kernel(
     int M, int N, int K, float alpha, const uint8_t *__restrict__ A, long lda,
     const uint8_t *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
     int tiles_m, int tiles_n, const int *__restrict__ escA, const int *__restrict__ escB, int ks,
     int nseg, long estride) {
-  constexpr int NT = (BM / 64) * (BN / 64) * 64;
+  constexpr int NT = (BM / WM) * (BN / WN) * 64;
   constexpr int BK = 64;  // K per stage (bytes per row)
-  constexpr int WGN = BN / 64;
+  constexpr int WGN = BN / WN;
+  constexpr int BI = WN / 32, BJ = WM / 32;  // 32 x 32 blocks per wave (columns, rows)
   constexpr int HALF = BM * 64;
   constexpr int STAGE = (BM + BN) * 64;
   constexpr int CA = BM * 4 / NT, CB = BN * 4 / NT;
-  static_assert(BM % 64 == 0 && BN % 64 == 0 && CA >= 1 && CB >= 1, "tile");
+  static_assert(BM % WM == 0 && BN % WN == 0 && WM % 32 == 0 && WN % 32 == 0, "tile");
+  static_assert(128 % WM == 0 && 128 % WN == 0, "a wave's rows and columns lie in one scale tile");
+  static_assert(CA >= 1 && CB >= 1 && CA * NT == BM * 4 && CB * NT == BN * 4, "staging");
   __shared__ __attribute__((aligned(1024))) char smem[NST * STAGE];
 
   int tm, tn;  // XCD-aware tile order, as in hgemm_nt_kernel
@@ -321,7 +374,7 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
     const int bid = blockIdx.x, nblk = gridDim.x, per = nblk / 8;
     int lid = bid;
     if (bid < per * 8) lid = (bid % 8) * per + bid / 8;
-    constexpr int GM = 512 / BM;
+    constexpr int GM = GMT > 0 ? GMT : 512 / BM;
     const int group_size = GM * tiles_n, group = lid / group_size, first_m = group * GM;
     const int gm = min(tiles_m - first_m, GM), in_group = lid - group * group_size;
     tm = first_m + in_group % gm;
@@ -329,13 +382,13 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
   }
   const int m0 = tm * BM, n0 = tn * BN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm0 = (wave / WGN) * 64, wn0 = (wave % WGN) * 64;
+  const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
   const int lr = lane & 31, lh = lane >> 5;
-  const bool full = m0 + BM <= M && n0 + BN <= N;
+  const bool full = !EDGE || (m0 + BM <= M && n0 + BN <= N);
   // E8M0 block scales of the two operands (2^(s - 127) undoes 2^esc); a
-  // wave's 64 rows and 64 columns each lie in one 128-row scale tile.  K
-  // runs over nseg segments of ks stages, products of their own scaling (up
-  // to 16 updates of C summed in one pass over it), whose exponents lie
+  // wave's rows and columns each lie in one 128-row scale tile.  K runs
+  // over nseg segments of ks stages, products of their own scaling (up to
+  // 16 updates of C summed in one pass over it), whose exponents lie
   // estride apart; the current segment's pair is reloaded at each boundary
   const int ta = (m0 + wm0) >> 7, tb = (n0 + wn0) >> 7;
   int sa = 127 - escA[ta], sb = 127 - escB[tb];
@@ -345,13 +398,13 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
   // second the A tile (32 C rows): element r of block (i, j) is C[row][col]
   // with row = m0 + wm0 + 32 j + lr and col = n0 + wn0 + 32 i + 8 (r / 4) +
   // 4 lh + r % 4, so elements 4 s .. 4 s + 3 are 4 consecutive columns.
-  float16_t acc[2][2];
+  float16_t acc[BI][BJ];
   auto crow = [&](int j) { return m0 + wm0 + 32 * j + lr; };
   auto ccol = [&](int i, int s4) { return n0 + wn0 + 32 * i + 8 * s4 + 4 * lh; };
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < BJ; ++j)
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) {
         float4_t v = {0.f, 0.f, 0.f, 0.f};
@@ -401,23 +454,29 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
                                        0);
   };
   // a lane's operand: the two 16-byte chunks 2 lh and 2 lh + 1 of its row
-  int aoff[2][2], boff[2][2];
+  int aoff[BJ][2], boff[BI][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int m = wm0 + 32 * i + lr, n = wn0 + 32 * i + lr;
-      aoff[i][h] = m * 64 + (((2 * lh + h) ^ lds_swz(m)) << 4);
+    for (int j = 0; j < BJ; ++j) {
+      const int m = wm0 + 32 * j + lr;
+      aoff[j][h] = m * 64 + (((2 * lh + h) ^ lds_swz(m)) << 4);
+    }
+#pragma unroll
+    for (int i = 0; i < BI; ++i) {
+      const int n = wn0 + 32 * i + lr;
       boff[i][h] = HALF + n * 64 + (((2 * lh + h) ^ lds_swz(n)) << 4);
     }
+  }
 
-  static_assert(NST >= 2 && NST <= 4, "stages");
+  static_assert(NST >= 2 && NST <= 5 && (CA + CB) * (NST - 2) < 64, "stages");
 #pragma unroll
   for (int st = 0; st < NST - 1; ++st)
     if (st < nk) issue(st);
   for (int kc = 0; kc < nk; ++kc) {
     const int ahead = nk - 1 - kc;
-    if (NST >= 4 && ahead >= 2) wait_vm_lp<(CA + CB) * (NST >= 4 ? 2 : 0)>();
+    if (NST >= 5 && ahead >= 3) wait_vm_lp<(CA + CB) * (NST >= 5 ? 3 : 0)>();
+    else if (NST >= 4 && ahead >= 2) wait_vm_lp<(CA + CB) * (NST >= 4 ? 2 : 0)>();
     else if (NST >= 3 && ahead >= 1) wait_vm_lp<(CA + CB) * (NST >= 3 ? 1 : 0)>();
     else wait_vm_lp<0>();
     __asm__ volatile("" ::: "memory");
@@ -431,29 +490,32 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
       sa = 127 - escA[seg * estride + ta];
       sb = 127 - escB[seg * estride + tb];
     }
-    int8v_t a[2], b[2];
+    int8v_t a[BJ], b[BI];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int4v_t a0 = *reinterpret_cast<const int4v_t *>(st + aoff[i][0]);
-      const int4v_t a1 = *reinterpret_cast<const int4v_t *>(st + aoff[i][1]);
+    for (int j = 0; j < BJ; ++j) {
+      const int4v_t a0 = *reinterpret_cast<const int4v_t *>(st + aoff[j][0]);
+      const int4v_t a1 = *reinterpret_cast<const int4v_t *>(st + aoff[j][1]);
+      a[j] = (int8v_t){a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    }
+#pragma unroll
+    for (int i = 0; i < BI; ++i) {
       const int4v_t b0 = *reinterpret_cast<const int4v_t *>(st + boff[i][0]);
       const int4v_t b1 = *reinterpret_cast<const int4v_t *>(st + boff[i][1]);
-      a[i] = (int8v_t){a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
       b[i] = (int8v_t){b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < BJ; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(b[i], a[j], acc[i][j], 0, 0,
                                                                     0, sb, 0, sa);
   }
   wait_vm_lp<0>();  // no direct-to-LDS write may be in flight when the workgroup exits
 
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < BJ; ++j)
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) {
         const int row = crow(j), col = ccol(i, s4);
@@ -479,26 +541,35 @@ __global__ __launch_bounds__((BM / 64) * (BN / 64) * 64, OCC) void fgemm_nt_kern
       }
 }
 
-template <int BMv, int BNv, int NS, int OC, bool CI, bool V>
+template <int BMv, int BNv, int WMv, int WNv, int NS, int OC, bool CI, bool V, bool E = true,
+          int GMv = 0>
 static void fgemm_launch(int M, int N, int K, float alpha, const uint8_t *A, long lda,
                          const uint8_t *B, long ldb, float beta, float *C, long ldc,
                          hipStream_t s, const int *escA, const int *escB, int ks, int nseg,
                          long estride) {
-  constexpr int NT = (BMv / 64) * (BNv / 64) * 64;
-  const int tm = (M + BMv - 1) / BMv, tn = (N + BNv - 1) / BNv;
-  hipLaunchKernelGGL((fgemm_nt_kernel<BMv, BNv, NS, OC, CI, V>), dim3(tm * tn), dim3(NT), 0, s, M,
-                     N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA, escB, ks, nseg,
-                     estride);
+  constexpr int NT = (BMv / WMv) * (BNv / WNv) * 64;
+  const int tm = E ? (M + BMv - 1) / BMv : M / BMv, tn = E ? (N + BNv - 1) / BNv : N / BNv;
+  if (tm <= 0 || tn <= 0) return;
+  hipLaunchKernelGGL((fgemm_nt_kernel<BMv, BNv, WMv, WNv, NS, OC, CI, V, E, GMv>), dim3(tm * tn),
+                     dim3(NT), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, escA,
+                     escB, ks, nseg, estride);
 }
 
-// FP8 operands with per-tile power-of-two scales; the large accumulate-into-C
-// updates (the mixed LU's trailing update at block 1024) take 128 x 256
-// tiles on 3 stages with 2 workgroups per CU (at most 128 VGPRs: four waves
-// per SIMD), everything else 128 x 128 tiles on 2 stages with 4 per CU.  At
-// 64512^2 x 1024: 781-787 TFLOP/s standalone (749 with two
+
+// FP8 operands with per-tile power-of-two scales.  The large accumulate-
+// into-C updates (the mixed LU's trailing update) take the wide tiles on
+// 3 stages below K = 4096 and 4 from there, walked in groups of 4 tile rows
+// (1024 rows of A) per XCD; kernel time at 57344^2 x 16384 (the update of a
+// group of 16 steps of 1024): 2007 TFLOP/s against 1770 with 128 x 256
+// tiles of 64 x 64 per wave (3 stages, 2 workgroups per CU), 1984 with 3
+// stages, 1937 with groups of 2 tile rows; at 61440^2 x 4096 1615 against
+// 1512; at 64512^2 x 1024 842 against 833 (4 stages: 763).  The other
+// calls take 128 x 128 tiles on 2 stages with 4 per CU.  Before the
+// block-scaled instruction: 749 TFLOP/s at 64512^2 x 1024 with two
 // v_mfma_f32_16x16x32_fp8_fp8 per 16-byte operand read and the scales
-// applied to C before and after the K loop); persistent workgroups, two per
-// CU with or without a start offset between them, measured slower (678-689).
+// applied to C before and after the K loop; persistent workgroups, two per
+// CU with or without a start offset between them, measured slower (678-689
+// against 781 for 128 x 256 tiles at the time).
 int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const int *escA,
              const uint8_t *B, long ldb, const int *escB, float beta, float *C, long ldc,
              hipStream_t s, int kseg, int nseg, long estride) {
@@ -513,21 +584,41 @@ int fgemm_nt(int M, int N, int K, float alpha, const uint8_t *A, long lda, const
   const bool vec = ldc % 4 == 0 && ((uintptr_t)C & 15) == 0;
   const bool cinit = alpha == 1.f && beta == 1.f;
   const bool large = (long)M * N >= (1L << 25);
-  if (cinit && vec && K >= 1024 && large)
-    fgemm_launch<128, 256, 3, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA,
-                                             escB, ks, nseg, estride);
+  if (cinit && vec && K >= 1024 && large && lp_wide()) {
+    // whole 256 x 256 tiles, then the rows below them and the columns beside
+    // them with 128 x 128 tiles (an element's products are accumulated in
+    // the same order by either kernel, so the split does not change C);
+    // the scale tiles of the edges start 2 Mw / 256 (2 Nw / 256) tiles in
+    const int Mw = M / 256 * 256, Nw = N / 256 * 256;
+    if (K >= 4096)
+      fgemm_launch<256, 256, 128, 128, 4, 1, true, true, false, 4>(
+          Mw, Nw, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA, escB, ks, nseg, estride);
+    else
+      fgemm_launch<256, 256, 128, 128, 3, 1, true, true, false, 4>(
+          Mw, Nw, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA, escB, ks, nseg, estride);
+    if (M > Mw)
+      fgemm_launch<128, 128, 64, 64, 2, 4, true, true>(
+          M - Mw, N, K, alpha, A + (long)Mw * lda, lda, B, ldb, beta, C + (long)Mw * ldc, ldc, s,
+          escA + Mw / 128, escB, ks, nseg, estride);
+    if (N > Nw && Mw > 0)
+      fgemm_launch<128, 128, 64, 64, 2, 4, true, true>(
+          Mw, N - Nw, K, alpha, A, lda, B + (long)Nw * ldb, ldb, beta, C + Nw, ldc, s, escA,
+          escB + Nw / 128, ks, nseg, estride);
+  } else if (cinit && vec && K >= 1024 && large)
+    fgemm_launch<128, 256, 64, 64, 3, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                     s, escA, escB, ks, nseg, estride);
   else if (cinit && vec)
-    fgemm_launch<128, 128, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, escA,
-                                             escB, ks, nseg, estride);
+    fgemm_launch<128, 128, 64, 64, 2, 4, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                     s, escA, escB, ks, nseg, estride);
   else if (cinit)
-    fgemm_launch<128, 128, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                              escA, escB, ks, nseg, estride);
+    fgemm_launch<128, 128, 64, 64, 2, 4, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                      s, escA, escB, ks, nseg, estride);
   else if (vec)
-    fgemm_launch<128, 128, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                              escA, escB, ks, nseg, estride);
+    fgemm_launch<128, 128, 64, 64, 2, 4, false, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                      s, escA, escB, ks, nseg, estride);
   else
-    fgemm_launch<128, 128, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s,
-                                               escA, escB, ks, nseg, estride);
+    fgemm_launch<128, 128, 64, 64, 2, 4, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C,
+                                                       ldc, s, escA, escB, ks, nseg, estride);
   CLU_LAUNCHED("fgemm_nt_kernel");
   return CLU_OK;
 }

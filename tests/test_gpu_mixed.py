@@ -533,6 +533,69 @@ def test_grouped_steps_match_per_step_form(dev, orc, prec, g, n, nb):
     assert st["residual"] <= 1.0 and err < 1e-10
 
 
+@pytest.mark.parametrize("prec", ["fp8", "bf16", "fp16"])
+@pytest.mark.parametrize("M,N,K", [(6144, 5900, 1024), (5950, 6400, 2048), (6400, 6144, 1024)])
+def test_wide_tiles_match_narrow(dev, prec, M, N, K):
+    """The large accumulate-into-C updates take 256 x 256 tiles of four
+    128 x 128 waves for their whole tiles and 128 x 128 tiles for the rows
+    below and the columns beside them; CHARMLU_LP_WIDE=0 takes tiles of
+    64 x 64 per wave everywhere.  Each element of C receives the same
+    products in the same order from both, so C is bit-identical (ragged N,
+    ragged M, and no edge)."""
+    import ctypes
+    import os
+    import torch
+    from charmlu_amd import _lib
+    torch.manual_seed(11)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    C0 = torch.rand(M, N, device="cuda")
+    if prec == "fp8":
+        A = torch.rand(M, K, device="cuda") - 0.5
+        B = torch.rand(K, N, device="cuda") - 0.5
+    else:
+        dt = torch.bfloat16 if prec == "bf16" else torch.float16
+        A = (torch.rand(M, K, device="cuda") - 0.5).to(dt)
+        B = (torch.rand(N, K, device="cuda") - 0.5).to(dt)
+    out = {}
+    for wide in ("1", "0"):
+        C = C0.clone()
+        os.environ["CHARMLU_LP_WIDE"] = wide
+        try:
+            print(f"{prec} update {M}x{N}x{K} wide={wide}", flush=True)
+            if prec == "fp8":
+                _fp8_gemm(dev, A, B, 1.0, 1.0, C)
+            else:
+                rc = _lib.load().charmlu_gemm_lp_dev(
+                    _lib.PREC_BF16 if prec == "bf16" else _lib.PREC_FP16, M, N, K, 1.0,
+                    ctypes.c_void_p(A.data_ptr()), K, ctypes.c_void_p(B.data_ptr()), K, 1.0,
+                    ctypes.c_void_p(C.data_ptr()), N, s)
+                assert rc == 0
+                torch.cuda.synchronize()
+        finally:
+            os.environ.pop("CHARMLU_LP_WIDE", None)
+        out[wide] = C
+    assert not torch.equal(out["1"], C0)
+    assert torch.equal(out["1"], out["0"])
+
+
+@pytest.mark.parametrize("prec,n,nb,pivot", [("fp8", 8192, 1024, True), ("fp8", 8192, 1024, False),
+                                             ("fp8", 7168, 512, True), ("bf16", 8192, 1024, True),
+                                             ("fp16", 7168, 512, False)])
+def test_wide_tiles_in_the_lu(dev, orc, prec, n, nb, pivot):
+    """The low-precision LU with groups of 2 steps (rest updates of K = 2 nb,
+    FP8 with one scale pair per step, over up to (n - 2 nb)^2 elements)
+    gives bit-identical factors and pivots with the wide update tiles and
+    without (ragged edges: test_wide_tiles_match_narrow)."""
+    A = orc.gen_diagdom(orc.SEED_A + 19, n)
+    W, ipw, infw = _getrf_env(dev, A, nb, prec, {"CHARMLU_GROUP": "2", "CHARMLU_LP_WIDE": "1"},
+                              pivot=pivot)
+    X, ipx, infx = _getrf_env(dev, A, nb, prec, {"CHARMLU_GROUP": "2", "CHARMLU_LP_WIDE": "0"},
+                              pivot=pivot)
+    assert infw == infx == 0
+    assert np.array_equal(ipw, ipx)
+    assert np.array_equal(W.cpu().numpy(), X.cpu().numpy())
+
+
 def _getrf_env(dev, A, nb, prec, env, pivot=True):
     import os
     import torch
