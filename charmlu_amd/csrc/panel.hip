@@ -514,14 +514,32 @@ int panel_plan(int m, int w, int max_groups, bool pivot, PanelShape *ps) {
 template int panel_plan<double>(int, int, int, bool, PanelShape *);
 template int panel_plan<float>(int, int, int, bool, PanelShape *);
 
+// Right-looking substitution for x U = a with the leaf's U in LDS (D): x_k
+// /= u_kk, then x_j -= x_k u_kj, row k of U read as whole 16-byte pieces
+// from its first one holding j > k.
+#define CLU_LEAF_SOLVE(x)                                                       \
+  _Pragma("unroll") for (int k = 0; k < W; ++k) {                               \
+    constexpr int V = 16 / sizeof(T);                                           \
+    T uk[W];                                                                    \
+    _Pragma("unroll") for (int q = k / V * V; q < W; q += V) {                  \
+      typedef T vec_t __attribute__((ext_vector_type(V)));                      \
+      const vec_t v = *reinterpret_cast<const vec_t *>(&D[k][q]);               \
+      _Pragma("unroll") for (int e = 0; e < V; ++e) uk[q + e] = v[e];           \
+    }                                                                           \
+    if (uk[k] != (T)0) x[k] = x[k] / uk[k];                                     \
+    _Pragma("unroll") for (int j = k + 1; j < W; ++j) x[j] -= x[k] * uk[j];     \
+  }
+
 // Leaf of the recursive panel without interchanges: rows [j0, j0 + m) x
 // columns [j0, j0 + w), w <= 32.  Without pivoting only the w x w diagonal
 // block needs the column-by-column elimination; every row below is the
 // solution x of x U = a with that block's U, independent of the others.
 // One workgroup of 1024 threads: wave 0 eliminates the diagonal block in
 // LDS (a row per lane, no workgroup barrier per column), then every thread
-// solves rows below (one at a time, U read from LDS), and the diagonal
-// block goes back with the pivots (the identity) and the first tiny pivot.
+// solves rows below (one per thread, U read from LDS; the strip staged
+// through LDS, 1024 rows per pass: 16-byte aligned rows only), and the
+// diagonal block goes back with the pivots (the identity) and the first
+// tiny pivot.
 // (The exchanging strip, used for this before, spends about 2 us per
 // column on a workgroup barrier and the pivot row through LDS for every
 // row: 70 us for 32 columns of a 1024-row leaf.)  As in that strip, a zero
@@ -530,8 +548,10 @@ template <typename T>
 __global__ __launch_bounds__(1024) void leaf_nopiv_kernel(T *__restrict__ A, long lda, int j0,
                                                           int m, int w, int *__restrict__ ipiv,
                                                           int *info) {
-  constexpr int W = 32;
+  constexpr int W = 32, SP = W + 1;  // SP: row stride of the staged strip
+  static_assert(sizeof(T) == 4, "the staged strip holds 1024 rows of 32 floats");
   __shared__ __attribute__((aligned(16))) T D[W][W];
+  __shared__ T S[1024 * SP];
   const int t = threadIdx.x;
   for (int e = t; e < W * W; e += 1024) {
     const int i = e / W, j = e % W;
@@ -571,34 +591,56 @@ __global__ __launch_bounds__(1024) void leaf_nopiv_kernel(T *__restrict__ A, lon
     }
   }
   __syncthreads();
-  for (int row = w + t; row < m; row += 1024) {
-    // (U is read again for every row: kept in registers across rows it
-    // would take 528 of them)
-    __asm__ volatile("" ::: "memory");
-    T *ar = A + (long)(j0 + row) * lda + j0;
-    T x[W];
+  typedef T vec4_t __attribute__((ext_vector_type(4)));
+  {
+    // rows below the block in passes of up to 1024 (the caller guarantees
+    // 16-byte aligned rows): the strip moves between memory and LDS in
+    // 16-byte pieces, 8 lanes per 128-byte row (a lane reading its own row
+    // from memory touches 64 rows' cache lines per access: 47 us for a
+    // 1024-row leaf against 28.5 staged), and each thread solves one row
+    // from LDS (the odd row stride puts the lanes' rows in distinct banks)
+    for (int r0 = w; r0 < m; r0 += 1024) {
+      const int nr = min(1024, m - r0);
+      for (int c = t; c < nr * 8; c += 1024) {
+        const int rr = c >> 3, q = (c & 7) * 4;
+        const T *src = A + (long)(j0 + r0 + rr) * lda + j0 + q;
+        vec4_t v = {(T)0, (T)0, (T)0, (T)0};
+        if (q + 4 <= w) {
+          v = *reinterpret_cast<const vec4_t *>(src);
+        } else {
 #pragma unroll
-    for (int j = 0; j < W; ++j) x[j] = j < w ? ar[j] : (T)0;
-    // right-looking substitution: x_k /= u_kk, then x_j -= x_k u_kj, row k
-    // of U read as whole 16-byte pieces from its first one holding j > k
+          for (int e = 0; e < 4; ++e)
+            if (q + e < w) v[e] = src[e];
+        }
 #pragma unroll
-    for (int k = 0; k < W; ++k) {
-      constexpr int V = 16 / sizeof(T);
-      T uk[W];
-#pragma unroll
-      for (int q = k / V * V; q < W; q += V) {
-        typedef T vec_t __attribute__((ext_vector_type(V)));
-        const vec_t v = *reinterpret_cast<const vec_t *>(&D[k][q]);
-#pragma unroll
-        for (int e = 0; e < V; ++e) uk[q + e] = v[e];
+        for (int e = 0; e < 4; ++e) S[rr * SP + q + e] = v[e];
       }
-      if (uk[k] != (T)0) x[k] = x[k] / uk[k];
+      __syncthreads();
+      if (t < nr) {
+        __asm__ volatile("" ::: "memory");
+        T x[W];
 #pragma unroll
-      for (int j = k + 1; j < W; ++j) x[j] -= x[k] * uk[j];
+        for (int j = 0; j < W; ++j) x[j] = S[t * SP + j];
+        CLU_LEAF_SOLVE(x)
+#pragma unroll
+        for (int j = 0; j < W; ++j) S[t * SP + j] = x[j];
+      }
+      __syncthreads();
+      for (int c = t; c < nr * 8; c += 1024) {
+        const int rr = c >> 3, q = (c & 7) * 4;
+        T *dst = A + (long)(j0 + r0 + rr) * lda + j0 + q;
+        const vec4_t v = {S[rr * SP + q], S[rr * SP + q + 1], S[rr * SP + q + 2],
+                          S[rr * SP + q + 3]};
+        if (q + 4 <= w) {
+          *reinterpret_cast<vec4_t *>(dst) = v;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (q + e < w) dst[e] = v[e];
+        }
+      }
+      __syncthreads();  // before the next pass refills S
     }
-#pragma unroll
-    for (int j = 0; j < W; ++j)
-      if (j < w) ar[j] = x[j];
   }
   for (int e = t; e < w * w; e += 1024) {
     const int i = e / w, j = e % w;
@@ -614,6 +656,8 @@ __global__ __launch_bounds__(1024) void leaf_nopiv_kernel(T *__restrict__ A, lon
       }
   }
 }
+
+#undef CLU_LEAF_SOLVE
 
 template <typename T>
 int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *ipiv,
@@ -637,7 +681,7 @@ int panel_base(T *A, long lda, int j0, int m_end, int w, int pc0, int pc1, int *
   // the instantiated shapes: (R, NT, multi-workgroup, pivoting, strip width)
   const bool multi = G > 1;
   if constexpr (sizeof(T) == 4) {  // (the FP64 factorization always exchanges)
-    if (!pivot && w <= 32) {
+    if (!pivot && w <= 32 && lda % 4 == 0 && j0 % 4 == 0 && ((uintptr_t)A & 15) == 0) {
       hipLaunchKernelGGL((leaf_nopiv_kernel<T>), dim3(1), dim3(1024), 0, s, A, lda, j0, m, w,
                          ipiv, pw.info);
       CLU_LAUNCHED("leaf_nopiv_kernel");
