@@ -159,6 +159,11 @@ struct Workspace {
   // solve: inverses of the diagonal blocks of L and U, the intermediate y
   void *sinv = nullptr, *sy = nullptr, *smv = nullptr;  // solve: inverses, Y, move lists
   size_t sinv_b = 0, sy_b = 0, smv_b = 0;
+  // one-launch vector solves (trsv_tiles): published values with their
+  // sequence numbers (zeroed when allocated), the last number used
+  void *stag = nullptr;
+  size_t stag_b = 0;
+  unsigned trsv_seq = 0;
   // scratch of the triangular inverses (products of the doubling steps)
   void *tinv = nullptr;
   size_t tinv_b = 0;
@@ -495,6 +500,12 @@ static int group_steps(int group_default) {
   if (end == e || *end != 0 || g < 1 || g > 16) return group_default;
   return (int)g;
 }
+// one-launch triangles for the solve of one right-hand side
+// (CHARMLU_TRSV_TILES=0: a pair of launches per diagonal block instead)
+static bool trsv_tiles_enabled() {
+  const char *e = getenv("CHARMLU_TRSV_TILES");
+  return !(e && e[0] == '0');
+}
 static int forced_split(int M) {
   const int Ma = (M / 2 + 127) / 128 * 128;
   return Ma > M ? M : Ma;
@@ -656,6 +667,31 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
     return gemm_t<T>(ta, false, M, nrhs, K, (T)-1, Ablk, lda, X, ldx, (T)1, D, ldd, s);
   };
   const T *F = dLU;
+  constexpr int VW = sizeof(T) / 4;
+  if (vec && trans == CHARMLU_NO_TRANS && n % S == 0 && lda % (16 / sizeof(T)) == 0 &&
+      ((uintptr_t)dLU & 15) == 0 && trsv_tiles_enabled()) {
+    // both triangles in one launch each (trsv_tiles): L y = P b, then U x = y
+    unsigned long long *tag;
+    unsigned seq;
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      const size_t had = ws.stag_b, need = 2 * sizeof(unsigned long long) * VW * (size_t)n;
+      if ((rc = ensure_bytes(&ws.stag, &ws.stag_b, need))) return rc;
+      if (ws.stag_b != had || ws.trsv_seq >= 0xfffffff0u) {
+        if ((rc = fill_int((int *)ws.stag, 0, (long)(ws.stag_b / sizeof(int)), s))) return rc;
+        ws.trsv_seq = 0;
+      }
+      seq = ws.trsv_seq + 1;
+      ws.trsv_seq += 2;
+      tag = (unsigned long long *)ws.stag;
+    }
+    unsigned *abortw = (unsigned *)ws.abortw + 8;
+    unsigned long long *tagx = tag, *tagt = tag + (size_t)VW * n;
+    if ((rc = interchanges(false))) return rc;
+    if ((rc = trsv_tiles<T>(false, n, S, F, lda, Linv, dB, Y, tagx, tagt, seq, abortw, s)))
+      return rc;
+    return trsv_tiles<T>(true, n, S, F, lda, Uinv, Y, dB, tagx, tagt, seq + 1, abortw, s);
+  }
   if (trans == CHARMLU_NO_TRANS) {
     if ((rc = interchanges(false))) return rc;
     // L y = P b: y into Y, B is the running right-hand side
@@ -785,7 +821,7 @@ static int release_ws(Workspace &w) {
   void *bufs[] = {w.slots,  w.abortw, w.pmoves, w.pcount, w.smoves, w.scount, w.stage,
                   w.winv,   w.ubuf,   w.winv32, w.ubuf32, w.pa16,   w.pu16,   w.uinv32,
                   w.l21buf, w.esc,    w.irbuf,  w.sinv,   w.sy,     w.smv,    w.tinv,
-                  w.pinv,   w.ptmp,   w.specbak, w.spec_flag};
+                  w.pinv,   w.ptmp,   w.specbak, w.spec_flag, w.stag};
   for (void *b : bufs)
     if (b) CLU_HIP(hipFree(b));
   if (w.spec_hflag) CLU_HIP(hipHostFree(w.spec_hflag));
@@ -2246,6 +2282,19 @@ static int gmres_ir(bool trans, int n, const double *dA, long lda, const float *
     CLU_HIP(hipMemcpyAsync(dh, y.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
     if ((rc = basis_axpy(k, n, 1.0, V, n, dh, dx, s))) return rc;
     if ((rc = check(&ref, &ns, &rmax))) return rc;
+  }
+  {
+    // a one-launch block solve that gave up a wait left NaN in its rows and
+    // its sequence number in the abort word (check() has synchronized s)
+    Workspace *wsp;
+    if ((rc = get_ws(&wsp))) return rc;
+    unsigned ab = 0;
+    CLU_HIP(hipMemcpy(&ab, (unsigned *)wsp->abortw + 8, sizeof(ab), hipMemcpyDeviceToHost));
+    if (ab != 0) {
+      CLU_HIP(hipMemset((unsigned *)wsp->abortw + 8, 0, sizeof(ab)));
+      set_error("block solve: a wait for published values gave up (call %u)", ab);
+      return CLU_ERR_HIP;
+    }
   }
   if (stats) {
     stats[0] = outer;

@@ -611,4 +611,202 @@ template int gemv_sub<double>(bool, int, int, const double *, long, const double
 template int gemv_sub<float>(bool, int, int, const float *, long, const float *, float *,
                              hipStream_t);
 
+// ---------------------------------------------------------------------------
+// trsv_tiles: one triangle of the block solve of one right-hand side in one
+// launch (forwardSolve / backwardSolve with their offDiagSolve updates,
+// lu.ci:387-458, for a single vector).
+//
+// The launch-per-block form (blk_gemv, then gemv_sub, for every diagonal
+// block) spends most of a large solve in the gaps and ramps of its 2 n / S
+// dependent launches.  Here workgroup q owns TRSV_ROWS = 64 rows (S / 64
+// tiles per diagonal block of S rows), counted from the top for the forward
+// (unit lower) triangle and from the bottom for the backward (upper) one.
+// It streams its rows of every off-diagonal block column in solve order, each
+// as soon as the solution of that block is published, keeping FP64 partial
+// sums per lane; then it publishes t = rhs - sum for its rows, reads the t of
+// its whole block once every tile of the block has published, and computes
+// its rows of x = inv(D) t from the stored inverse of the diagonal block D.
+//
+// Published values travel as 64-bit words, (sequence number << 32) | 32 bits
+// of payload (two words for a double), stored and loaded with relaxed
+// agent-scope atomics: a reader accepts a block only when every word carries
+// the call's sequence number, so no flags, fences or clearing are needed
+// between calls.  A workgroup waits only for tiles of earlier blocks (lower
+// workgroup indices) and for the other tiles of its own block (adjacent
+// indices); workgroups are dispatched in index order, so every wait ends once
+// S / 64 workgroups fit on the device at once.  Each wait is still bounded:
+// after TRSV_SPIN rounds the workgroup stores the sequence number into the
+// abort word and writes NaN to its rows (the tiles that need them then stop
+// at their first round), so a lost value shows up as NaN and as the abort
+// word, never as a hang.
+constexpr int TRSV_ROWS = 64;
+constexpr int TRSV_SPIN = 1 << 22;
+
+template <typename T>
+CLU_DEV void tag_store(unsigned long long *tag, long i, T v, unsigned seq) {
+  constexpr int VW = sizeof(T) / 4;
+  unsigned u[VW];
+  __builtin_memcpy(u, &v, sizeof(T));
+  const unsigned long long hi = (unsigned long long)seq << 32;
+#pragma unroll
+  for (int k = 0; k < VW; ++k) st_agent(tag + i * VW + k, hi | u[k]);
+}
+
+// Read the S values published at tag into dst (LDS).  Polls one value per
+// tile until each carries seq, then reads all S and checks every word, again
+// until all do.  Returns false (the same in every thread) when it gave up.
+template <typename T, int S>
+CLU_DEV bool tag_wait(const unsigned long long *tag, unsigned seq, T *dst, unsigned *abortw) {
+  constexpr int VW = sizeof(T) / 4;
+  constexpr int NTILE = S / TRSV_ROWS;
+  const int t = threadIdx.x;
+  bool full = false;
+  for (int spin = 0;; ++spin) {
+    bool ok = true;
+    if (!full) {
+      if (t < NTILE) {
+        const long i = (long)t * TRSV_ROWS + TRSV_ROWS - 1;
+#pragma unroll
+        for (int k = 0; k < VW; ++k) ok = ok && (ld_agent(tag + i * VW + k) >> 32) == seq;
+      }
+    } else {
+      for (int i = t; i < S; i += blockDim.x) {
+        unsigned u[VW];
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+          const unsigned long long w = ld_agent(tag + (long)i * VW + k);
+          ok = ok && (w >> 32) == seq;
+          u[k] = (unsigned)w;
+        }
+        T v;
+        __builtin_memcpy(&v, u, sizeof(T));
+        dst[i] = v;
+      }
+    }
+    if (__syncthreads_and(ok)) {
+      if (full) return true;
+      full = true;
+      continue;
+    }
+    const bool quit = spin >= TRSV_SPIN || (t == 0 && ld_agent(abortw) == seq);
+    if (__syncthreads_or(quit)) {
+      if (t == 0) st_agent(abortw, seq);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+template <typename T, int S, bool UPPER>
+__global__ __launch_bounds__(256) void trsv_tiles_kernel(
+    int n, const T *__restrict__ F, long lda, const T *__restrict__ Inv,
+    const T *__restrict__ rhs, T *__restrict__ out, unsigned long long *tagx,
+    unsigned long long *tagt, unsigned seq, unsigned *abortw) {
+  constexpr int VW = sizeof(T) / 4;
+  constexpr int VE = 16 / sizeof(T);          // elements per 16-byte load
+  constexpr int NV = S / (64 * VE);           // loads per lane per row
+  constexpr int RB = NV >= 16 ? 1 : 16 / NV;  // rows per batch: 16 loads in flight
+  static_assert(NV >= 1 && 16 % RB == 0, "tile shape");
+  typedef T vec_t __attribute__((ext_vector_type(VE)));
+  __shared__ __attribute__((aligned(16))) T xs[S];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ntile = n / TRSV_ROWS, nblk = n / S;
+  const int q = UPPER ? ntile - 1 - (int)blockIdx.x : (int)blockIdx.x;
+  const int r0 = q * TRSV_ROWS, blk = r0 / S;
+  const int rw = r0 + wave * 16;  // this wave's 16 rows
+  double acc[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+
+  // acc[r] += sum over this lane's columns c of M[rw + r][c] * xs[c], the
+  // columns of M starting at the block's first column (row stride ldm)
+  auto dots = [&](const T *M, long ldm) {
+    vec_t xv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) xv[v] = *reinterpret_cast<const vec_t *>(&xs[(v * 64 + lane) * VE]);
+#pragma unroll
+    for (int b = 0; b < 16; b += RB) {
+      vec_t a[RB][NV];
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          a[r][v] = *reinterpret_cast<const vec_t *>(M + (long)(rw + b + r) * ldm +
+                                                     (v * 64 + lane) * VE);
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+          for (int e = 0; e < VE; ++e) acc[b + r] += (double)a[r][v][e] * (double)xv[v][e];
+    }
+  };
+  // lane r (< 16) ends with the sum of row rw + r
+  auto reduce = [&]() {
+    double mine = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const double s = wave_sum(acc[r]);
+      if (lane == r) mine = s;
+      acc[r] = 0.0;
+    }
+    return mine;
+  };
+
+  bool ok = true;
+  const int nj = UPPER ? nblk - 1 - blk : blk;
+  for (int jj = 0; jj < nj && ok; ++jj) {
+    const int j = UPPER ? nblk - 1 - jj : jj;
+    ok = tag_wait<T, S>(tagx + (long)j * S * VW, seq, xs, abortw);
+    if (ok) dots(F + (long)j * S, lda);
+    __syncthreads();  // every lane has its x values before xs is refilled
+  }
+  const T nan = __builtin_nan("");
+  double s = reduce();
+  if (lane < 16) {
+    const int r = rw + lane;
+    tag_store<T>(tagt, r, ok ? (T)((double)rhs[r] - s) : nan, seq);
+  }
+  if (ok) ok = tag_wait<T, S>(tagt + (long)blk * S * VW, seq, xs, abortw);
+  // row r of block blk's inverse is at Inv + r * S (blocks of S x S, ld S)
+  if (ok) dots(Inv, S);
+  s = reduce();
+  if (lane < 16) {
+    const int r = rw + lane;
+    const T xr = ok ? (T)s : nan;
+    out[r] = xr;
+    tag_store<T>(tagx, r, xr, seq);
+  }
+}
+
+template <typename T>
+int trsv_tiles(bool upper, int n, int S, const T *F, long lda, const T *Inv, const T *rhs, T *out,
+               unsigned long long *tagx, unsigned long long *tagt, unsigned seq,
+               unsigned *abortw, hipStream_t s) {
+  constexpr int VE = 16 / sizeof(T);
+  if (n <= 0 || (S != 256 && S != 1024) || n % S || lda % VE ||
+      ((uintptr_t)F & 15) || ((uintptr_t)Inv & 15))
+    return -1;
+  const dim3 grid(n / TRSV_ROWS), block(256);
+#define CLU_TRSV(SS, UP)                                                                     \
+  hipLaunchKernelGGL((trsv_tiles_kernel<T, SS, UP>), grid, block, 0, s, n, F, lda, Inv, rhs, \
+                     out, tagx, tagt, seq, abortw)
+  if (S == 256) {
+    if (upper) CLU_TRSV(256, true);
+    else CLU_TRSV(256, false);
+  } else {
+    if (upper) CLU_TRSV(1024, true);
+    else CLU_TRSV(1024, false);
+  }
+#undef CLU_TRSV
+  CLU_LAUNCHED("trsv_tiles_kernel");
+  return CLU_OK;
+}
+template int trsv_tiles<double>(bool, int, int, const double *, long, const double *,
+                                const double *, double *, unsigned long long *,
+                                unsigned long long *, unsigned, unsigned *, hipStream_t);
+template int trsv_tiles<float>(bool, int, int, const float *, long, const float *, const float *,
+                               float *, unsigned long long *, unsigned long long *, unsigned,
+                               unsigned *, hipStream_t);
+
 }  // namespace clu

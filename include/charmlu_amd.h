@@ -169,7 +169,13 @@ int charmlu_dgetrf(int n, int nb, double *dA, long lda, int *dipiv, int *dinfo,
 /* Solve op(A) X = B with the factors of charmlu_dgetrf: the row
  * interchanges, then forward and backward block substitution
  * (forwardSolve/backwardSolve/offDiagSolve, lu.ci:387-458).
- * trans: NO_TRANS or TRANS.  B is n x nrhs row-major, overwritten by X. */
+ * trans: NO_TRANS or TRANS.  B is n x nrhs row-major, overwritten by X.
+ * One right-hand side (nrhs = 1, ldb = 1) with NO_TRANS and n a multiple of
+ * the solve's diagonal block (256) runs each triangle as one launch whose
+ * workgroups wait for each other's published values; such a wait is bounded,
+ * and one that gives up leaves NaN in the rows that depend on it (and makes
+ * charmlu_dgesv_ir / _ex return CHARMLU_ERR_HIP).  CHARMLU_TRSV_TILES=0
+ * selects two launches per diagonal block instead. */
 int charmlu_dgetrs(int trans, int n, int nrhs, const double *dLU, long lda,
                    const int *dipiv, double *dB, long ldb, void *stream);
 
