@@ -618,8 +618,9 @@ template int gemv_sub<float>(bool, int, int, const float *, long, const float *,
 //
 // The launch-per-block form (blk_gemv, then gemv_sub, for every diagonal
 // block) spends most of a large solve in the gaps and ramps of its 2 n / S
-// dependent launches.  Here workgroup q owns TRSV_ROWS = 64 rows (S / 64
-// tiles per diagonal block of S rows), counted from the top for the forward
+// dependent launches.  Here workgroup q owns a tile of TR = 16, 32 or 64
+// rows (WR = TR / 4 per wave; S / TR tiles per diagonal block of S rows),
+// counted from the top for the forward
 // (unit lower) triangle and from the bottom for the backward (upper) one.
 // It streams its rows of every off-diagonal block column in solve order, each
 // as soon as the solution of that block is published, keeping FP64 partial
@@ -634,12 +635,11 @@ template int gemv_sub<float>(bool, int, int, const float *, long, const float *,
 // between calls.  A workgroup waits only for tiles of earlier blocks (lower
 // workgroup indices) and for the other tiles of its own block (adjacent
 // indices); workgroups are dispatched in index order, so every wait ends once
-// S / 64 workgroups fit on the device at once.  Each wait is still bounded:
+// S / TR workgroups fit on the device at once.  Each wait is still bounded:
 // after TRSV_SPIN rounds the workgroup stores the sequence number into the
 // abort word and writes NaN to its rows (the tiles that need them then stop
 // at their first round), so a lost value shows up as NaN and as the abort
 // word, never as a hang.
-constexpr int TRSV_ROWS = 64;
 constexpr int TRSV_SPIN = 1 << 22;
 
 template <typename T>
@@ -655,17 +655,18 @@ CLU_DEV void tag_store(unsigned long long *tag, long i, T v, unsigned seq) {
 // Read the S values published at tag into dst (LDS).  Polls one value per
 // tile until each carries seq, then reads all S and checks every word, again
 // until all do.  Returns false (the same in every thread) when it gave up.
-template <typename T, int S>
+template <typename T, int S, int TR>
 CLU_DEV bool tag_wait(const unsigned long long *tag, unsigned seq, T *dst, unsigned *abortw) {
   constexpr int VW = sizeof(T) / 4;
-  constexpr int NTILE = S / TRSV_ROWS;
+  constexpr int NTILE = S / TR;
+  static_assert(NTILE <= 256, "one polling thread per tile");
   const int t = threadIdx.x;
   bool full = false;
   for (int spin = 0;; ++spin) {
     bool ok = true;
     if (!full) {
       if (t < NTILE) {
-        const long i = (long)t * TRSV_ROWS + TRSV_ROWS - 1;
+        const long i = (long)t * TR + TR - 1;
 #pragma unroll
         for (int k = 0; k < VW; ++k) ok = ok && (ld_agent(tag + i * VW + k) >> 32) == seq;
       }
@@ -697,7 +698,7 @@ CLU_DEV bool tag_wait(const unsigned long long *tag, unsigned seq, T *dst, unsig
   }
 }
 
-template <typename T, int S, bool UPPER>
+template <typename T, int S, bool UPPER, int WR>
 __global__ __launch_bounds__(256) void trsv_tiles_kernel(
     int n, const T *__restrict__ F, long lda, const T *__restrict__ Inv,
     const T *__restrict__ rhs, T *__restrict__ out, unsigned long long *tagx,
@@ -705,18 +706,20 @@ __global__ __launch_bounds__(256) void trsv_tiles_kernel(
   constexpr int VW = sizeof(T) / 4;
   constexpr int VE = 16 / sizeof(T);          // elements per 16-byte load
   constexpr int NV = S / (64 * VE);           // loads per lane per row
-  constexpr int RB = NV >= 16 ? 1 : 16 / NV;  // rows per batch: 16 loads in flight
-  static_assert(NV >= 1 && 16 % RB == 0, "tile shape");
+  constexpr int TR = 4 * WR;                  // rows per tile (WR per wave)
+  constexpr int RB0 = NV >= 16 ? 1 : 16 / NV;
+  constexpr int RB = RB0 < WR ? RB0 : WR;     // rows per batch: up to 16 loads in flight
+  static_assert(NV >= 1 && WR % RB == 0 && S % TR == 0, "tile shape");
   typedef T vec_t __attribute__((ext_vector_type(VE)));
   __shared__ __attribute__((aligned(16))) T xs[S];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ntile = n / TRSV_ROWS, nblk = n / S;
+  const int ntile = n / TR, nblk = n / S;
   const int q = UPPER ? ntile - 1 - (int)blockIdx.x : (int)blockIdx.x;
-  const int r0 = q * TRSV_ROWS, blk = r0 / S;
-  const int rw = r0 + wave * 16;  // this wave's 16 rows
-  double acc[16];
+  const int r0 = q * TR, blk = r0 / S;
+  const int rw = r0 + wave * WR;  // this wave's WR rows
+  double acc[WR];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+  for (int r = 0; r < WR; ++r) acc[r] = 0.0;
 
   // acc[r] += sum over this lane's columns c of M[rw + r][c] * xs[c], the
   // columns of M starting at the block's first column (row stride ldm)
@@ -725,7 +728,7 @@ __global__ __launch_bounds__(256) void trsv_tiles_kernel(
 #pragma unroll
     for (int v = 0; v < NV; ++v) xv[v] = *reinterpret_cast<const vec_t *>(&xs[(v * 64 + lane) * VE]);
 #pragma unroll
-    for (int b = 0; b < 16; b += RB) {
+    for (int b = 0; b < WR; b += RB) {
       vec_t a[RB][NV];
 #pragma unroll
       for (int r = 0; r < RB; ++r)
@@ -741,11 +744,11 @@ __global__ __launch_bounds__(256) void trsv_tiles_kernel(
           for (int e = 0; e < VE; ++e) acc[b + r] += (double)a[r][v][e] * (double)xv[v][e];
     }
   };
-  // lane r (< 16) ends with the sum of row rw + r
+  // lane r (< WR) ends with the sum of row rw + r
   auto reduce = [&]() {
     double mine = 0.0;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
+    for (int r = 0; r < WR; ++r) {
       const double s = wave_sum(acc[r]);
       if (lane == r) mine = s;
       acc[r] = 0.0;
@@ -757,21 +760,21 @@ __global__ __launch_bounds__(256) void trsv_tiles_kernel(
   const int nj = UPPER ? nblk - 1 - blk : blk;
   for (int jj = 0; jj < nj && ok; ++jj) {
     const int j = UPPER ? nblk - 1 - jj : jj;
-    ok = tag_wait<T, S>(tagx + (long)j * S * VW, seq, xs, abortw);
+    ok = tag_wait<T, S, TR>(tagx + (long)j * S * VW, seq, xs, abortw);
     if (ok) dots(F + (long)j * S, lda);
     __syncthreads();  // every lane has its x values before xs is refilled
   }
   const T nan = __builtin_nan("");
   double s = reduce();
-  if (lane < 16) {
+  if (lane < WR) {
     const int r = rw + lane;
     tag_store<T>(tagt, r, ok ? (T)((double)rhs[r] - s) : nan, seq);
   }
-  if (ok) ok = tag_wait<T, S>(tagt + (long)blk * S * VW, seq, xs, abortw);
+  if (ok) ok = tag_wait<T, S, TR>(tagt + (long)blk * S * VW, seq, xs, abortw);
   // row r of block blk's inverse is at Inv + r * S (blocks of S x S, ld S)
   if (ok) dots(Inv, S);
   s = reduce();
-  if (lane < 16) {
+  if (lane < WR) {
     const int r = rw + lane;
     const T xr = ok ? (T)s : nan;
     out[r] = xr;
@@ -787,17 +790,30 @@ int trsv_tiles(bool upper, int n, int S, const T *F, long lda, const T *Inv, con
   if (n <= 0 || (S != 256 && S != 1024) || n % S || lda % VE ||
       ((uintptr_t)F & 15) || ((uintptr_t)Inv & 15))
     return -1;
-  const dim3 grid(n / TRSV_ROWS), block(256);
-#define CLU_TRSV(SS, UP)                                                                     \
-  hipLaunchKernelGGL((trsv_tiles_kernel<T, SS, UP>), grid, block, 0, s, n, F, lda, Inv, rhs, \
-                     out, tagx, tagt, seq, abortw)
-  if (S == 256) {
-    if (upper) CLU_TRSV(256, true);
-    else CLU_TRSV(256, false);
-  } else {
-    if (upper) CLU_TRSV(1024, true);
-    else CLU_TRSV(1024, false);
+  // rows per tile (CHARMLU_TRSV_ROWS: 16, 32 or 64)
+  int tr = 32;
+  if (const char *e = getenv("CHARMLU_TRSV_ROWS")) {
+    const int v = atoi(e);
+    if (v == 16 || v == 32 || v == 64) tr = v;
   }
+  const dim3 grid(n / tr), block(256);
+#define CLU_TRSV(SS, UP, WW)                                                                     \
+  hipLaunchKernelGGL((trsv_tiles_kernel<T, SS, UP, WW>), grid, block, 0, s, n, F, lda, Inv, rhs, \
+                     out, tagx, tagt, seq, abortw)
+#define CLU_TRSV_W(SS, UP) \
+  do {                     \
+    if (tr == 16) CLU_TRSV(SS, UP, 4);      \
+    else if (tr == 32) CLU_TRSV(SS, UP, 8); \
+    else CLU_TRSV(SS, UP, 16);              \
+  } while (0)
+  if (S == 256) {
+    if (upper) CLU_TRSV_W(256, true);
+    else CLU_TRSV_W(256, false);
+  } else {
+    if (upper) CLU_TRSV_W(1024, true);
+    else CLU_TRSV_W(1024, false);
+  }
+#undef CLU_TRSV_W
 #undef CLU_TRSV
   CLU_LAUNCHED("trsv_tiles_kernel");
   return CLU_OK;
