@@ -174,14 +174,15 @@ template <typename T>
 int gemv_sub(bool trans, int M, int K, const T *A, long lda, const T *x, T *y,
              hipStream_t s);
 // one triangle of the block solve of one right-hand side in one launch:
-// out = inv(op) rhs for the unit lower (upper = false) or upper triangle of
-// the n x n factors F, with the inverses of the S x S diagonal blocks in Inv
-// (S = 256 or 1024, n a multiple of S); tagx / tagt hold VW n published words
+// out = op(T)^{-1} rhs, op(T) a triangle of the n x n factors F (trans: its
+// transpose), solved from the last block up when upper is set (op(T) upper
+// triangular), with the inverses of op's S x S diagonal blocks in Inv (S =
+// 256 or 1024, n a multiple of S); tagx / tagt hold VW n published words
 // each, seq is new for every call, abortw receives seq if a wait gave up
 template <typename T>
-int trsv_tiles(bool upper, int n, int S, const T *F, long lda, const T *Inv, const T *rhs, T *out,
-               unsigned long long *tagx, unsigned long long *tagt, unsigned seq,
-               unsigned *abortw, hipStream_t s);
+int trsv_tiles(bool upper, bool trans, int n, int S, const T *F, long lda, const T *Inv,
+               const T *rhs, T *out, unsigned long long *tagx, unsigned long long *tagt,
+               unsigned seq, unsigned *abortw, hipStream_t s);
 // in-place transpose of nblk contiguous S x S blocks (S a multiple of 32)
 template <typename T>
 int blk_transpose(int S, int nblk, T *G, hipStream_t s);

@@ -668,9 +668,10 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
   };
   const T *F = dLU;
   constexpr int VW = sizeof(T) / 4;
-  if (vec && trans == CHARMLU_NO_TRANS && n % S == 0 && lda % (16 / sizeof(T)) == 0 &&
-      ((uintptr_t)dLU & 15) == 0 && trsv_tiles_enabled()) {
-    // both triangles in one launch each (trsv_tiles): L y = P b, then U x = y
+  if (vec && n % S == 0 && lda % (16 / sizeof(T)) == 0 && ((uintptr_t)dLU & 15) == 0 &&
+      trsv_tiles_enabled()) {
+    // both triangles in one launch each (trsv_tiles): L y = P b, then U x = y;
+    // transposed: U^T z = b, then L^T w = z, x = P^T w
     unsigned long long *tag;
     unsigned seq;
     {
@@ -687,10 +688,15 @@ static int getrs_impl(int trans, int n, int nrhs, const T *dLU, long lda, const 
     }
     unsigned *abortw = (unsigned *)ws.abortw + 8;
     unsigned long long *tagx = tag, *tagt = tag + (size_t)VW * n;
-    if ((rc = interchanges(false))) return rc;
-    if ((rc = trsv_tiles<T>(false, n, S, F, lda, Linv, dB, Y, tagx, tagt, seq, abortw, s)))
+    const bool tr = trans == CHARMLU_TRANS;
+    if (!tr && (rc = interchanges(false))) return rc;
+    if ((rc = trsv_tiles<T>(false, tr, n, S, F, lda, tr ? Uinv : Linv, dB, Y, tagx, tagt, seq,
+                            abortw, s)))
       return rc;
-    return trsv_tiles<T>(true, n, S, F, lda, Uinv, Y, dB, tagx, tagt, seq + 1, abortw, s);
+    if ((rc = trsv_tiles<T>(true, tr, n, S, F, lda, tr ? Linv : Uinv, Y, dB, tagx, tagt, seq + 1,
+                            abortw, s)))
+      return rc;
+    return tr ? interchanges(true) : CLU_OK;
   }
   if (trans == CHARMLU_NO_TRANS) {
     if ((rc = interchanges(false))) return rc;

@@ -698,7 +698,7 @@ CLU_DEV bool tag_wait(const unsigned long long *tag, unsigned seq, T *dst, unsig
   }
 }
 
-template <typename T, int S, bool UPPER, int WR>
+template <typename T, int S, bool UPPER, bool TRANS, int WR>
 __global__ __launch_bounds__(256) void trsv_tiles_kernel(
     int n, const T *__restrict__ F, long lda, const T *__restrict__ Inv,
     const T *__restrict__ rhs, T *__restrict__ out, unsigned long long *tagx,
@@ -744,6 +744,43 @@ __global__ __launch_bounds__(256) void trsv_tiles_kernel(
           for (int e = 0; e < VE; ++e) acc[b + r] += (double)a[r][v][e] * (double)xv[v][e];
     }
   };
+  // TRANS: the tile's rows of op(F) are columns r0 .. r0 + TR - 1 of F; lane
+  // group rg of CQ lanes reads one row of F's S-row block (CQ 16-byte pieces
+  // of TR columns), NRG rows at a time, and keeps VE column sums; the row
+  // groups' sums are added in LDS in a fixed order at the end.
+  constexpr int CQ = TR / VE, NRG = 256 / CQ;
+  static_assert(TR % VE == 0 && S % NRG == 0, "transposed tile shape");
+  __shared__ double red[TRANS ? NRG : 1][TRANS ? TR : 1];
+  const int cq = threadIdx.x % CQ, rg = threadIdx.x / CQ;
+  double acct[VE];
+#pragma unroll
+  for (int e = 0; e < VE; ++e) acct[e] = 0.0;
+  auto dots_t = [&](const T *M) {  // M: row 0 of the block, column r0
+    constexpr int U = 8;            // rows in flight per lane
+    static_assert((S / NRG) % U == 0, "transposed unroll");
+    for (int k0 = rg; k0 < S; k0 += U * NRG) {
+      vec_t a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        a[u] = *reinterpret_cast<const vec_t *>(M + (long)(k0 + u * NRG) * lda + cq * VE);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double z = (double)xs[k0 + u * NRG];
+#pragma unroll
+        for (int e = 0; e < VE; ++e) acct[e] += (double)a[u][e] * z;
+      }
+    }
+  };
+  // lane r (< WR) of each wave ends with the sum of row rw + r
+  auto reduce_t = [&]() {
+#pragma unroll
+    for (int e = 0; e < VE; ++e) red[rg][cq * VE + e] = acct[e];
+    __syncthreads();
+    double mine = 0.0;
+    if (lane < WR)
+      for (int g2 = 0; g2 < NRG; ++g2) mine += red[g2][wave * WR + lane];
+    return mine;
+  };
   // lane r (< WR) ends with the sum of row rw + r
   auto reduce = [&]() {
     double mine = 0.0;
@@ -761,11 +798,16 @@ __global__ __launch_bounds__(256) void trsv_tiles_kernel(
   for (int jj = 0; jj < nj && ok; ++jj) {
     const int j = UPPER ? nblk - 1 - jj : jj;
     ok = tag_wait<T, S, TR>(tagx + (long)j * S * VW, seq, xs, abortw);
-    if (ok) dots(F + (long)j * S, lda);
+    if (ok) {
+      if constexpr (TRANS) dots_t(F + (long)j * S * lda + r0);
+      else dots(F + (long)j * S, lda);
+    }
     __syncthreads();  // every lane has its x values before xs is refilled
   }
   const T nan = __builtin_nan("");
-  double s = reduce();
+  double s;
+  if constexpr (TRANS) s = reduce_t();
+  else s = reduce();
   if (lane < WR) {
     const int r = rw + lane;
     tag_store<T>(tagt, r, ok ? (T)((double)rhs[r] - s) : nan, seq);
@@ -783,45 +825,39 @@ __global__ __launch_bounds__(256) void trsv_tiles_kernel(
 }
 
 template <typename T>
-int trsv_tiles(bool upper, int n, int S, const T *F, long lda, const T *Inv, const T *rhs, T *out,
-               unsigned long long *tagx, unsigned long long *tagt, unsigned seq,
-               unsigned *abortw, hipStream_t s) {
+int trsv_tiles(bool upper, bool trans, int n, int S, const T *F, long lda, const T *Inv,
+               const T *rhs, T *out, unsigned long long *tagx, unsigned long long *tagt,
+               unsigned seq, unsigned *abortw, hipStream_t s) {
   constexpr int VE = 16 / sizeof(T);
+  constexpr int WR = 8;  // 32-row tiles (16 rows measured the same, 64 slower)
   if (n <= 0 || (S != 256 && S != 1024) || n % S || lda % VE ||
       ((uintptr_t)F & 15) || ((uintptr_t)Inv & 15))
     return -1;
-  // rows per tile (CHARMLU_TRSV_ROWS: 16, 32 or 64)
-  int tr = 32;
-  if (const char *e = getenv("CHARMLU_TRSV_ROWS")) {
-    const int v = atoi(e);
-    if (v == 16 || v == 32 || v == 64) tr = v;
-  }
-  const dim3 grid(n / tr), block(256);
-#define CLU_TRSV(SS, UP, WW)                                                                     \
-  hipLaunchKernelGGL((trsv_tiles_kernel<T, SS, UP, WW>), grid, block, 0, s, n, F, lda, Inv, rhs, \
-                     out, tagx, tagt, seq, abortw)
-#define CLU_TRSV_W(SS, UP) \
-  do {                     \
-    if (tr == 16) CLU_TRSV(SS, UP, 4);      \
-    else if (tr == 32) CLU_TRSV(SS, UP, 8); \
-    else CLU_TRSV(SS, UP, 16);              \
+  const dim3 grid(n / (4 * WR)), block(256);
+#define CLU_TRSV(SS, UP, TT)                                                                    \
+  hipLaunchKernelGGL((trsv_tiles_kernel<T, SS, UP, TT, WR>), grid, block, 0, s, n, F, lda, Inv, \
+                     rhs, out, tagx, tagt, seq, abortw)
+#define CLU_TRSV_T(SS, UP)   \
+  do {                       \
+    if (trans) CLU_TRSV(SS, UP, true); \
+    else CLU_TRSV(SS, UP, false);      \
   } while (0)
   if (S == 256) {
-    if (upper) CLU_TRSV_W(256, true);
-    else CLU_TRSV_W(256, false);
+    if (upper) CLU_TRSV_T(256, true);
+    else CLU_TRSV_T(256, false);
   } else {
-    if (upper) CLU_TRSV_W(1024, true);
-    else CLU_TRSV_W(1024, false);
+    if (upper) CLU_TRSV_T(1024, true);
+    else CLU_TRSV_T(1024, false);
   }
-#undef CLU_TRSV_W
+#undef CLU_TRSV_T
 #undef CLU_TRSV
   CLU_LAUNCHED("trsv_tiles_kernel");
   return CLU_OK;
 }
-template int trsv_tiles<double>(bool, int, int, const double *, long, const double *,
+template int trsv_tiles<double>(bool, bool, int, int, const double *, long, const double *,
                                 const double *, double *, unsigned long long *,
                                 unsigned long long *, unsigned, unsigned *, hipStream_t);
-template int trsv_tiles<float>(bool, int, int, const float *, long, const float *, const float *,
+template int trsv_tiles<float>(bool, bool, int, int, const float *, long, const float *, const float *,
                                float *, unsigned long long *, unsigned long long *, unsigned,
                                unsigned *, hipStream_t);
 
