@@ -281,9 +281,51 @@ __global__ void convert_kernel(long m, long n, const TS *__restrict__ src, long 
   }
 }
 
+// The same for rows of 16-byte-aligned groups of four: workgroup b converts
+// up to 4096 elements of one row (one division per workgroup, not per
+// element), four per thread per pass, the four passes' loads issued first.
+template <typename TD, typename TS>
+__global__ __launch_bounds__(256) void convert_rows_kernel(long n, long chunks,
+                                                           const TS *__restrict__ src, long lds,
+                                                           TD *__restrict__ dst, long ldd) {
+  typedef TS sv __attribute__((ext_vector_type(4)));
+  typedef TD dv __attribute__((ext_vector_type(4)));
+  const long b = blockIdx.x, i = b / chunks, c0 = (b - i * chunks) * 4096;
+  const TS *sr = src + i * lds;
+  TD *dr = dst + i * ldd;
+  sv v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const long j = c0 + (long)(u * 256 + threadIdx.x) * 4;
+    if (j < n) v[u] = *reinterpret_cast<const sv *>(sr + j);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const long j = c0 + (long)(u * 256 + threadIdx.x) * 4;
+    if (j < n) {
+      dv o;
+      o.x = (TD)v[u].x;
+      o.y = (TD)v[u].y;
+      o.z = (TD)v[u].z;
+      o.w = (TD)v[u].w;
+      *reinterpret_cast<dv *>(dr + j) = o;
+    }
+  }
+}
+
 template <typename TD, typename TS>
 int convert(long m, long n, const TS *src, long lds, TD *dst, long ldd, hipStream_t s) {
   if (m <= 0 || n <= 0) return CLU_OK;
+  if (n % 4 == 0 && lds % 4 == 0 && ldd % 4 == 0 && (uintptr_t)src % (4 * sizeof(TS)) == 0 &&
+      (uintptr_t)dst % (4 * sizeof(TD)) == 0) {
+    const long chunks = (n + 4095) / 4096;
+    if (m * chunks < (1L << 31)) {
+      hipLaunchKernelGGL((convert_rows_kernel<TD, TS>), dim3((unsigned)(m * chunks)), dim3(256), 0,
+                         s, n, chunks, src, lds, dst, ldd);
+      CLU_LAUNCHED("convert_rows_kernel");
+      return CLU_OK;
+    }
+  }
   long blocks = (m * n + 255) / 256;
   if (blocks > 65536L * 16) blocks = 65536L * 16;
   hipLaunchKernelGGL((convert_kernel<TD, TS>), dim3((unsigned)blocks), dim3(256), 0, s, m, n,
